@@ -1,0 +1,182 @@
+/*
+ * spectral_hip.h -- C ABI of the MI355X (gfx950) spectral Navier-Stokes hot path.
+ *
+ * Plain C: device pointers, sizes and an opaque stream.  `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).  Every entry point
+ * returns 0 on success or a negative SH_E* code; sh_last_error() gives the
+ * message of the last failure on the calling thread.  Launches are
+ * stream-ordered and asynchronous unless the comment says "synchronous".
+ *
+ * Data layout everywhere is the reference's: interleaved complex
+ * (complex<double> = "z", complex<float> = "c"), element (i, j, k) of an
+ * Nx x Ny x Nz field at n = i + Nx*j + Nx*Ny*k (src/Scalar.cpp:682-690).
+ * A "vector" argument is three such fields; a `v3` pointer means the three
+ * fields are contiguous (component d at v3 + d*Nx*Ny*Nz elements).
+ *
+ * Supported FFT lengths per axis: 8 12 16 24 32 48 64 96 128 256 512.
+ * Sign convention (src/FFT1D.cpp:72-101): dir = -1 gives exp(-2 pi i jk/N)
+ * (forward), dir = +1 gives exp(+2 pi i jk/N), unnormalised.
+ */
+#ifndef SPECTRAL_HIP_H
+#define SPECTRAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SH_OK = 0,
+    SH_EINVAL = -1,      /* bad argument (null pointer, size, direction) */
+    SH_EUNSUPPORTED = -2, /* FFT length not in the supported list */
+    SH_EHIP = -3,        /* HIP runtime error */
+    SH_ENOMEM = -4       /* device allocation failed */
+};
+
+/* precision tags */
+enum { SH_F64 = 64, SH_F32 = 32 };
+
+/* 3D transform flags */
+enum {
+    SH_ZERO_NYQUIST = 1, /* Scalar::zeroHighModes after the transform (src/Scalar.cpp:336-370) */
+    SH_NORMALIZE = 2     /* divide by Nx*Ny*Nz (src/Scalar.cpp:223-227) */
+};
+
+/* ------------------------------------------------------------------ runtime */
+/* Select the device and build per-device tables.  Optional: every entry
+ * point initialises the current device on first use. */
+int sh_init(int device);
+const char* sh_last_error(void);
+int sh_device_count(int* count);
+/* Pooled device allocation (replaces upcxx::new_array/delete_array,
+ * src/Scalar.cpp:95, 117, 38-40).  Freed blocks are cached for reuse. */
+int sh_malloc(void** ptr, size_t bytes);
+int sh_free(void* ptr);
+/* Release every cached (free) pool block back to HIP. */
+int sh_pool_trim(void);
+/* synchronous copies */
+int sh_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int sh_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* stream-ordered device copy (the deep copies of src/Scalar.cpp:52-66) */
+int sh_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int sh_memset(void* dst, int value, size_t bytes, void* stream);
+int sh_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------------ FFT */
+/* Batched strided 1D C2C FFT in place: line b starts at data + b*dist and
+ * holds n elements at `stride`.  Replaces FFT1D::transform(ptr, DIR)
+ * (src/FFT1D.H:21-35, src/FFT1D.cpp:95-101), batched. */
+int sh_fft1d_z2z(void* data, int n, long stride, long dist, long batch, int dir, void* stream);
+int sh_fft1d_c2c(void* data, int n, long stride, long dist, long batch, int dir, void* stream);
+
+/* 3D C2C FFT in place.  Replaces Scalar::transform(DIR) (src/Scalar.cpp:400-646):
+ * the slab transposes of the reference are not needed with the whole field
+ * in one device's memory.  flags: SH_ZERO_NYQUIST | SH_NORMALIZE.
+ *   Scalar::forward() == sh_fft3d_z2z(f, .., -1, SH_ZERO_NYQUIST, ..)
+ *   Scalar::inverse() == sh_fft3d_z2z(f, .., +1, SH_NORMALIZE, ..) */
+int sh_fft3d_z2z(void* data, int nx, int ny, int nz, int dir, int flags, void* stream);
+int sh_fft3d_c2c(void* data, int nx, int ny, int nz, int dir, int flags, void* stream);
+/* the same transform on `count` fields spaced `field_stride` elements apart */
+int sh_fft3d_many_z2z(void* data, int count, long field_stride, int nx, int ny, int nz, int dir,
+                      int flags, void* stream);
+int sh_fft3d_many_c2c(void* data, int count, long field_stride, int nx, int ny, int nz, int dir,
+                      int flags, void* stream);
+
+/* ------------------------------------------------------------------ spectral operators (complex128)
+ * Fourier-space inputs; wavenumbers from Scalar::freq (src/Scalar.cpp:198-206). */
+/* Scalar::zeroHighModes (src/Scalar.cpp:336-370) */
+int sh_zero_high_z(void* f, int nx, int ny, int nz, void* stream);
+/* Scalar::gradient: v_d = i k_d f (src/Scalar.cpp:245-276) */
+int sh_gradient_z(const void* f, void* vx, void* vy, void* vz, int nx, int ny, int nz, void* stream);
+/* Scalar::laplacian: -|k|^2 f (src/Scalar.cpp:278-306) */
+int sh_laplacian_z(const void* f, void* out, int nx, int ny, int nz, void* stream);
+/* Scalar::laplacian_inverse(a, b): f /= (a - b|k|^2) where |k|^2 != 0, in place
+ * (src/Scalar.cpp:308-334) */
+int sh_laplacian_inverse_z(void* f, double a, double b, int nx, int ny, int nz, void* stream);
+/* Vector::divergence: sum_d i k_d v_d (src/Vector.cpp:124-153) */
+int sh_divergence_z(const void* vx, const void* vy, const void* vz, void* out, int nx, int ny,
+                    int nz, void* stream);
+/* Vector::curl: i k x v (src/Vector.cpp:92-122) */
+int sh_curl_z(const void* vx, const void* vy, const void* vz, void* ox, void* oy, void* oz,
+              int nx, int ny, int nz, void* stream);
+/* Vector::laplacian: -|k|^2 v_d (src/Vector.cpp:155-181) */
+int sh_vlaplacian_z(const void* vx, const void* vy, const void* vz, void* ox, void* oy, void* oz,
+                    int nx, int ny, int nz, void* stream);
+/* Pointwise complex cross product of physical fields, out = u x w
+ * (the loop of src/Vector.cpp:79-84). */
+int sh_cross_pointwise_z(const void* ux, const void* uy, const void* uz, const void* wx,
+                         const void* wy, const void* wz, void* ox, void* oy, void* oz, long n,
+                         void* stream);
+/* Vector::cross (src/Vector.cpp:66-90): out = forward(inverse(a) x inverse(b)).
+ * a3, b3, out3: 3 contiguous Fourier fields; work: 6 fields of scratch. */
+int sh_cross_z(const void* a3, const void* b3, void* out3, void* work6, int nx, int ny, int nz,
+               void* stream);
+
+/* ------------------------------------------------------------------ elementwise (Operators, src/Operators.cpp:9-99) */
+/* out = x * y (Scalar * Scalar) */
+int sh_mul_z(void* out, const void* x, const void* y, long n, void* stream);
+/* out = x + y, out = x - y */
+int sh_add_z(void* out, const void* x, const void* y, long n, void* stream);
+int sh_sub_z(void* out, const void* x, const void* y, long n, void* stream);
+/* out = alpha * x (double * Scalar / double * Vector) */
+int sh_scale_z(void* out, double alpha, const void* x, long n, void* stream);
+/* Scalar::average: mean |f| (src/Scalar.cpp:15-22).  Synchronous. */
+int sh_average_z(const void* f, long n, double* result, void* stream);
+/* sum of |f|^2 over n elements, FP64 accumulation.  Synchronous. */
+int sh_sumsq_z(const void* f, long n, double* result, void* stream);
+
+/* ------------------------------------------------------------------ initial conditions and diagnostics */
+/* counter splitmix64 complex field, Re/Im i.i.d. U[-1,1] (element e: draws 2e, 2e+1) */
+int sh_prng_complex_z(void* out, long n, uint64_t seed, void* stream);
+/* Taylor-Green velocity (physical, complex128, 3 contiguous fields) on x_i = 2 pi i / N */
+int sh_taylor_green_physical_z(void* v3, int nx, int ny, int nz, void* stream);
+/* random solenoidal Fourier velocity, E(k) ~ k^4 exp(-2 (k/k0)^2), scaled to u_rms */
+int sh_solenoidal_ic_z(void* v3, int nx, int ny, int nz, uint64_t seed, double k0, double urms,
+                       void* stream);
+/* shell energy spectrum E(k) of a Fourier velocity (FP64 sums), nbins host doubles.
+ * Also returns total energy 0.5<|u|^2> and dissipation 2 nu sum k^2 E if the
+ * pointers are non-null.  Synchronous. */
+int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek, int nbins,
+                  double* energy, double* dissipation, void* stream);
+
+/* ------------------------------------------------------------------ Navier-Stokes fractional step (exec/ns.cpp:132-217) */
+typedef struct sh_ns_s* sh_ns_t;
+/* precision: SH_F64 (plain method) or SH_F32 (FP32 solver; twiddles, CN
+ * factors and reductions stay FP64).  fused: 1 = the fused kernel schedule,
+ * 0 = one kernel per reference operation. */
+int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, int precision,
+                 int fused, void* stream);
+int sh_ns_destroy(sh_ns_t handle);
+/* V0: 3 contiguous Fourier fields, complex128 device memory */
+int sh_ns_set_initial(sh_ns_t handle, const void* V0);
+/* exec/ns.cpp:132-153: Pi0, Forward-Euler V1, omega1, cross0 */
+int sh_ns_startup(sh_ns_t handle);
+/* nsteps iterations of exec/ns.cpp:167-217 */
+int sh_ns_steps(sh_ns_t handle, int nsteps);
+/* current velocity V1 into 3 contiguous complex128 device fields */
+int sh_ns_get_velocity(sh_ns_t handle, void* out_v3);
+int sh_ns_get_vorticity(sh_ns_t handle, void* out_v3);
+/* div.average() of the last step (exec/ns.cpp:209-210).  Synchronous. */
+int sh_ns_div_average(sh_ns_t handle, double* result);
+/* Stokes limit switch (nonlinear term off), for closed-form checks */
+int sh_ns_set_nonlinear(sh_ns_t handle, int on);
+/* device bytes held by the solver state */
+int sh_ns_state_bytes(sh_ns_t handle, size_t* bytes);
+
+/* ------------------------------------------------------------------ kernel timing
+ * When enabled, every launch of the named kernel classes is bracketed with
+ * HIP events on its own stream; totals are read after a synchronize. */
+int sh_prof_enable(int on);
+int sh_prof_reset(void);
+/* name: kernel class, e.g. "fft_x", "fft_y", "fft_z", "ns_fourier", "ns_cross_z".
+ * Synchronous. */
+int sh_prof_query(const char* name, double* total_ms, long* launches);
+/* newline-separated list of kernel classes seen so far */
+int sh_prof_names(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,337 @@
+// fft_core.hpp -- register/LDS line FFT for gfx950.
+//
+// A line of N complex values is owned by TPL threads.  The transform is a
+// Stockham autosort over radices R0..R3 (N = R0*R1*R2*R3): in stage s each
+// thread runs ceil(N/R_s / TPL) radix-R_s butterflies held in registers, and
+// the data move between stages through LDS (re and im in separate arrays).
+// Stage s, butterfly j (Ns = R0*...*R_{s-1}):
+//     in : positions j + r*N/R_s,                       r = 0..R_s-1
+//     twiddle: exp(sign 2 pi i (j mod Ns) r / (Ns R_s))  (from the N-entry table)
+//     out: positions (j/Ns)*Ns*R_s + (j mod Ns) + r*Ns
+// so the first stage reads, and the last stage writes, position
+// tid + b*TPL + r*N/R: consecutive threads touch consecutive elements, which
+// makes the global loads/stores of the contiguous pass coalesced with no
+// extra LDS round trip.
+//
+// The in-register DFTs use exact-symmetric constants (cos/sin of 2 pi k/32
+// evaluated in extended precision and rounded once); the inter-stage
+// twiddles come from a host-built table (twiddles(), long double).
+#pragma once
+
+#include "sh_common.hpp"
+
+namespace sh {
+
+// cos(2 pi k / 32), sin(2 pi k / 32), k = 0..15
+constexpr double kCos32[16] = {1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452,
+                               0.7071067811865476, 0.5555702330196022, 0.3826834323650898,
+                               0.19509032201612828, 0.0, -0.19509032201612828,
+                               -0.3826834323650898, -0.5555702330196022, -0.7071067811865476,
+                               -0.8314696123025452, -0.9238795325112867, -0.9807852804032304};
+constexpr double kSin32[16] = {0.0, 0.19509032201612828, 0.3826834323650898, 0.5555702330196022,
+                               0.7071067811865476, 0.8314696123025452, 0.9238795325112867,
+                               0.9807852804032304, 1.0, 0.9807852804032304, 0.9238795325112867,
+                               0.8314696123025452, 0.7071067811865476, 0.5555702330196022,
+                               0.3826834323650898, 0.19509032201612828};
+constexpr double kSqrtHalf = 0.7071067811865476;
+constexpr double kSqrt3Half = 0.8660254037844386;
+
+// a * exp(SIGN 2 pi i K / R), R | 32, 0 <= K < R/2
+template <typename T, int R, int K, int SIGN>
+__device__ __forceinline__ C2<T> twc(C2<T> a) {
+    static_assert(32 % R == 0, "radix must divide 32");
+    if constexpr (K == 0) {
+        return a;
+    } else if constexpr (4 * K == R) {  // exp(SIGN i pi/2) = SIGN i
+        return SIGN > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+    } else if constexpr (8 * K == R || 8 * K == 3 * R) {
+        // c = +-h, s = SIGN*h: (x + i y)(c + i s)
+        constexpr T h = T(kSqrtHalf);
+        constexpr int sc = (8 * K == R) ? 1 : -1;
+        if constexpr (sc > 0) {
+            return SIGN > 0 ? mk<T>(h * (a.x - a.y), h * (a.x + a.y))
+                            : mk<T>(h * (a.x + a.y), h * (a.y - a.x));
+        } else {
+            return SIGN > 0 ? mk<T>(-h * (a.x + a.y), h * (a.x - a.y))
+                            : mk<T>(h * (a.y - a.x), -h * (a.x + a.y));
+        }
+    } else {
+        constexpr int m = K * (32 / R);
+        constexpr T c = T(kCos32[m]);
+        constexpr T s = T(SIGN) * T(kSin32[m]);
+        return mk<T>(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+template <typename T, int R, int SIGN, int K>
+__device__ __forceinline__ void dit_combine(C2<T>* x, const C2<T>* e, const C2<T>* o) {
+    if constexpr (K < R / 2) {
+        const C2<T> t = twc<T, R, K, SIGN>(o[K]);
+        x[K] = cadd<T>(e[K], t);
+        x[K + R / 2] = csub<T>(e[K], t);
+        dit_combine<T, R, SIGN, K + 1>(x, e, o);
+    }
+}
+
+// In-register DFT of size R: X[q] = sum_r x[r] exp(SIGN 2 pi i r q / R), natural order.
+template <typename T, int R, int SIGN>
+struct Dft {
+    __device__ static __forceinline__ void run(C2<T>* x) {
+        if constexpr (R == 1) {
+        } else if constexpr (R == 2) {
+            const C2<T> a = x[0], b = x[1];
+            x[0] = cadd<T>(a, b);
+            x[1] = csub<T>(a, b);
+        } else if constexpr (R == 3) {
+            constexpr T h = T(SIGN) * T(kSqrt3Half);
+            const C2<T> t = cadd<T>(x[1], x[2]);
+            const C2<T> m = mk<T>(x[0].x - T(0.5) * t.x, x[0].y - T(0.5) * t.y);
+            const C2<T> e = csub<T>(x[1], x[2]);
+            const C2<T> d = mk<T>(-h * e.y, h * e.x);
+            x[0] = cadd<T>(x[0], t);
+            x[1] = cadd<T>(m, d);
+            x[2] = csub<T>(m, d);
+        } else if constexpr (R == 4) {
+            const C2<T> a = cadd<T>(x[0], x[2]), b = csub<T>(x[0], x[2]);
+            const C2<T> c = cadd<T>(x[1], x[3]), e = csub<T>(x[1], x[3]);
+            const C2<T> d = SIGN > 0 ? mk<T>(-e.y, e.x) : mk<T>(e.y, -e.x);
+            x[0] = cadd<T>(a, c);
+            x[2] = csub<T>(a, c);
+            x[1] = cadd<T>(b, d);
+            x[3] = csub<T>(b, d);
+        } else {
+            static_assert(R % 2 == 0 && 32 % R == 0, "unsupported radix");
+            C2<T> e[R / 2], o[R / 2];
+#pragma unroll
+            for (int i = 0; i < R / 2; i++) {
+                e[i] = x[2 * i];
+                o[i] = x[2 * i + 1];
+            }
+            Dft<T, R / 2, SIGN>::run(e);
+            Dft<T, R / 2, SIGN>::run(o);
+            dit_combine<T, R, SIGN, 0>(x, e, o);
+        }
+    }
+};
+
+template <int N_, int TPL_, int R0, int R1 = 1, int R2 = 1, int R3 = 1>
+struct FftPlan {
+    static constexpr int N = N_;
+    static constexpr int TPL = TPL_;
+    static constexpr int S = (R1 == 1) ? 1 : (R2 == 1) ? 2 : (R3 == 1) ? 3 : 4;
+    static constexpr int radix(int s) { return s == 0 ? R0 : s == 1 ? R1 : s == 2 ? R2 : R3; }
+    static constexpr int ns(int s) {
+        int p = 1;
+        for (int i = 0; i < s; i++) p *= radix(i);
+        return p;
+    }
+    static constexpr int nbf(int s) { return N / radix(s); }
+    static constexpr int nb(int s) { return (nbf(s) + TPL - 1) / TPL; }
+    static constexpr bool full(int s) { return nbf(s) % TPL == 0; }
+    static constexpr int slots(int s) { return nb(s) * radix(s); }
+    static constexpr int emax() {
+        int e = 0;
+        for (int s = 0; s < S; s++) e = slots(s) > e ? slots(s) : e;
+        return e;
+    }
+    static constexpr int E = emax();
+    static_assert(R0 * R1 * R2 * R3 == N, "radices must multiply to N");
+};
+
+template <class P, int s, int SIGN, typename T>
+__device__ __forceinline__ void stage_bfly(C2<T>* x, int tid, const C2<T>* __restrict__ tw) {
+    constexpr int R = P::radix(s), Ns = P::ns(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+            C2<T>* v = x + b * R;
+            if constexpr (Ns > 1) {
+                const int k = j % Ns;
+                constexpr int step = P::N / (Ns * R);
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    C2<T> w = tw[k * r * step];
+                    if constexpr (SIGN > 0) w.y = -w.y;
+                    v[r] = cmul<T>(v[r], w);
+                }
+            }
+            Dft<T, R, SIGN>::run(v);
+        }
+    }
+}
+
+template <class P, int s, typename T, class Idx>
+__device__ __forceinline__ void lds_put(const C2<T>* x, int tid, T* re, T* im, const Idx& idx) {
+    constexpr int R = P::radix(s), Ns = P::ns(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+            const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int o = idx(base + r * Ns);
+                re[o] = x[b * R + r].x;
+                im[o] = x[b * R + r].y;
+            }
+        }
+    }
+}
+
+template <class P, int s, typename T, class Idx>
+__device__ __forceinline__ void lds_get(C2<T>* x, int tid, const T* re, const T* im, const Idx& idx) {
+    constexpr int R = P::radix(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int o = idx(j + r * (P::N / R));
+                x[b * R + r] = mk<T>(re[o], im[o]);
+            }
+        }
+    }
+}
+
+// Whole-line transform.  On entry x holds the stage-0 inputs (see for_in),
+// on exit the outputs (see for_out).  If the LDS arrays were used before the
+// call, the caller must __syncthreads() first.
+template <class P, int SIGN, typename T, class Idx>
+__device__ __forceinline__ void line_fft(C2<T>* x, int tid, T* re, T* im, const Idx& idx,
+                                         const C2<T>* __restrict__ tw) {
+    stage_bfly<P, 0, SIGN, T>(x, tid, tw);
+    if constexpr (P::S > 1) {
+        lds_put<P, 0, T>(x, tid, re, im, idx);
+        __syncthreads();
+        lds_get<P, 1, T>(x, tid, re, im, idx);
+        stage_bfly<P, 1, SIGN, T>(x, tid, tw);
+    }
+    if constexpr (P::S > 2) {
+        __syncthreads();
+        lds_put<P, 1, T>(x, tid, re, im, idx);
+        __syncthreads();
+        lds_get<P, 2, T>(x, tid, re, im, idx);
+        stage_bfly<P, 2, SIGN, T>(x, tid, tw);
+    }
+    if constexpr (P::S > 3) {
+        __syncthreads();
+        lds_put<P, 2, T>(x, tid, re, im, idx);
+        __syncthreads();
+        lds_get<P, 3, T>(x, tid, re, im, idx);
+        stage_bfly<P, 3, SIGN, T>(x, tid, tw);
+    }
+}
+
+// f(slot, pos) for every stage-0 input this thread owns
+template <class P, class F>
+__device__ __forceinline__ void for_in(int tid, F&& f) {
+    constexpr int R = P::radix(0), NB = P::nb(0);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(0) || j < P::nbf(0)) {
+#pragma unroll
+            for (int r = 0; r < R; r++) f(b * R + r, j + r * (P::N / R));
+        }
+    }
+}
+
+// f(slot, pos) for every final output this thread owns
+template <class P, class F>
+__device__ __forceinline__ void for_out(int tid, F&& f) {
+    constexpr int s = P::S - 1;
+    constexpr int R = P::radix(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+#pragma unroll
+            for (int r = 0; r < R; r++) f(b * R + r, j + r * (P::N / R));
+        }
+    }
+}
+
+// Plans used per length.  TPL = N / max radix; the tuple is (plan, lines per
+// block in the contiguous pass, columns per block in the strided pass).
+using P8 = FftPlan<8, 1, 8>;
+using P12 = FftPlan<12, 3, 4, 3>;
+using P16 = FftPlan<16, 4, 4, 4>;
+using P24 = FftPlan<24, 3, 8, 3>;
+using P32 = FftPlan<32, 4, 8, 4>;
+using P48 = FftPlan<48, 12, 4, 4, 3>;
+using P64 = FftPlan<64, 8, 8, 8>;
+using P96 = FftPlan<96, 12, 8, 4, 3>;
+using P128 = FftPlan<128, 8, 16, 8>;
+using P256 = FftPlan<256, 16, 16, 16>;
+using P512 = FftPlan<512, 64, 8, 8, 8>;
+
+#define SH_FOR_EACH_FFT_SIZE(X) \
+    X(8, P8, 64, 16)            \
+    X(12, P12, 64, 16)          \
+    X(16, P16, 64, 16)          \
+    X(24, P24, 64, 16)          \
+    X(32, P32, 64, 16)          \
+    X(48, P48, 16, 16)          \
+    X(64, P64, 32, 16)          \
+    X(96, P96, 16, 16)          \
+    X(128, P128, 32, 16)        \
+    X(256, P256, 16, 16)        \
+    X(512, P512, 4, 8)
+
+// Plans for the fused solver kernels: at most 9 complex values per thread
+// (several fields are held at once), and first radix == last radix, so the
+// output slot->position map of a transform equals the input map of the next
+// one and a thread can chain an inverse transform, a pointwise product and a
+// forward transform without an LDS reshuffle.
+using Q12 = FftPlan<12, 4, 2, 3, 2>;
+using Q24 = FftPlan<24, 8, 2, 3, 2, 2>;
+using Q32 = FftPlan<32, 8, 4, 2, 4>;
+using Q48 = FftPlan<48, 12, 4, 3, 4>;
+using Q96 = FftPlan<96, 24, 4, 2, 3, 4>;
+using Q128 = FftPlan<128, 16, 8, 2, 8>;
+using Q256 = FftPlan<256, 32, 8, 4, 8>;
+
+#define SH_FOR_EACH_FUSED_SIZE(X) \
+    X(8, P8, 16)                  \
+    X(12, Q12, 16)                \
+    X(16, P16, 16)                \
+    X(24, Q24, 16)                \
+    X(32, Q32, 16)                \
+    X(48, Q48, 16)                \
+    X(64, P64, 16)                \
+    X(96, Q96, 16)                \
+    X(128, Q128, 16)              \
+    X(256, Q256, 16)              \
+    X(512, P512, 8)
+
+// epilogue of a pass: optional scale and Nyquist zeroing (src/Scalar.cpp:336-370)
+template <typename T>
+struct Epi {
+    T scale;
+    int flags;   // 1: scale, 2: zero Nyquist
+    int a_n;     // contiguous pass: line -> (a = line % a_n, b = line / a_n)
+    int za, zb;  // Nyquist index for coordinates a and b (strided: a = column, b = outer)
+};
+
+// batched passes (fft3d.hip); dir = -1 forward, +1 backward
+template <typename T>
+int pass_contig(C2<T>* data, int n, long nlines, long ldist, int count, long fstride, int dir,
+                const Epi<T>& ep, hipStream_t st);
+template <typename T>
+int pass_strided(C2<T>* data, int n, int ncols, long cdist, int nouter, long odist, long es,
+                 int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st);
+
+inline bool fft_size_supported(int n) {
+    switch (n) {
+#define SH_CASE(n_, P_, L_, C_) case n_:
+        SH_FOR_EACH_FFT_SIZE(SH_CASE)
+#undef SH_CASE
+        return true;
+        default:
+            return false;
+    }
+}
+
+}  // namespace sh

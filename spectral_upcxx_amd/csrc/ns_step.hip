@@ -1,0 +1,710 @@
+// ns_step.hip -- the fractional step of exec/ns.cpp (AB2 on the nonlinear and
+// pressure terms, Crank-Nicolson on the viscous term), two schedules:
+//
+//  * op-by-op (fused = 0, FP64): the reference's sequence of Scalar/Vector
+//    operations (exec/ns.cpp:167-217), one kernel per operation, with the
+//    temporaries the reference allocates per step kept resident instead.
+//
+//  * fused (fused = 1, FP64 or FP32): five kernels per step.
+//      A  x-pass inverse of V and of omega = i k x V (curl formed on load)
+//      B  y-pass inverse of the six fields
+//      C  z-pass inverse of the six fields, complex pointwise u x omega,
+//         z-pass forward of the three products -- all in registers
+//      D  y-pass forward of the products
+//      E  x-pass forward of the products + Nyquist zeroing + the whole
+//         Fourier-space update (F, L, G = lap^-1 div(V+L+F), V - grad G,
+//         CN divide) + |div V| partial sums
+//    Per step this moves 51 field passes (816 N^3 bytes in FP64) against the
+//    reference's 9 3D FFTs + ~20 array sweeps.  Every operation is mode-local
+//    except the transforms, so the fusion changes no arithmetic beyond the
+//    rounding of a different (equally exact) operation order.
+//
+// State (Fourier space): V1, N0 (= cross0), and six work fields W.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fft_core.hpp"
+
+namespace sh {
+
+template <typename T>
+int fft3d(void* data, int count, long fstride, int nx, int ny, int nz, int dir, int flags,
+          void* stream);
+int reduce_z(const void* f, long n, int mode, double* result, hipStream_t st);
+
+// ---------------------------------------------------------------- fused kernels
+template <typename T>
+struct StepCoef {
+    T Kh;          // K/2                   (exec/ns.cpp:171)
+    T c1;          // nu*K/2                (exec/ns.cpp:174)
+    double c1d;    // nu*K/2 in FP64 for the CN factor (exec/ns.cpp:186)
+    T inv_n3;      // 1/(Nx Ny Nz)          (src/Scalar.cpp:223-227)
+};
+
+template <int N>
+struct LdsLine {
+    static constexpr int LS = N + N / 16 + 1;
+};
+
+// A: inverse x-pass of V (3 fields) and omega = i k x V (3 fields) into W.
+template <typename T, class P, int LPB>
+__global__ __launch_bounds__(P::TPL* LPB * 6) void k_ns_inv_x(const C2<T>* __restrict__ V,
+                                                              C2<T>* __restrict__ W,
+                                                              const C2<T>* __restrict__ tw,
+                                                              int ny, int nz, long F) {
+    constexpr int N = P::N;
+    constexpr int LS = LdsLine<N>::LS;
+    __shared__ T sre[6 * LPB * LS];
+    __shared__ T sim[6 * LPB * LS];
+    const int tid = threadIdx.x % P::TPL;
+    const int ll = threadIdx.x / P::TPL;  // 0 .. 6*LPB-1
+    const int g = ll / LPB;               // field group: 0-2 velocity, 3-5 vorticity
+    const int l = ll % LPB;
+    const long nlines = (long)ny * nz;
+    const long line = (long)blockIdx.x * LPB + l;
+    const bool valid = line < nlines;
+    const long lc = valid ? line : 0;
+    const T k1 = freq<T>((int)(lc % ny), ny);
+    const T k2 = freq<T>((int)(lc / ny), nz);
+    const C2<T>* vb = V + lc * N;
+    C2<T> x[P::E];
+    if (g < 3) {
+        for_in<P>(tid, [&](int s, int pos) { x[s] = vb[g * F + pos]; });
+    } else if (g == 3) {  // i k1 V2 - i k2 V1
+        for_in<P>(tid, [&](int s, int pos) { x[s] = csub<T>(imul<T>(k1, vb[2 * F + pos]), imul<T>(k2, vb[F + pos])); });
+    } else if (g == 4) {  // i k2 V0 - i k0 V2
+        for_in<P>(tid, [&](int s, int pos) {
+            const T k0 = freq<T>(pos, N);
+            x[s] = csub<T>(imul<T>(k2, vb[pos]), imul<T>(k0, vb[2 * F + pos]));
+        });
+    } else {  // i k0 V1 - i k1 V0
+        for_in<P>(tid, [&](int s, int pos) {
+            const T k0 = freq<T>(pos, N);
+            x[s] = csub<T>(imul<T>(k0, vb[F + pos]), imul<T>(k1, vb[pos]));
+        });
+    }
+    auto idx = [&](int pos) { return ll * LS + pos + (pos >> 4); };
+    line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
+    if (!valid) return;
+    C2<T>* wb = W + g * F + lc * N;
+    for_out<P>(tid, [&](int s, int pos) { wb[pos] = x[s]; });
+}
+
+// C: z-pass inverse of W[0..6), u x omega, z-pass forward into W[0..3).
+template <typename T, class P, int LPB>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ W,
+                                                            const C2<T>* __restrict__ tw, int nx,
+                                                            int ny, long F, T scale) {
+    static_assert(P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1),
+                  "fused z kernel needs a symmetric plan");
+    constexpr int CS = LPB + 1;
+    constexpr int E = P::E;
+    __shared__ T sre[P::N * CS];
+    __shared__ T sim[P::N * CS];
+    const int c = threadIdx.x % LPB;
+    const int tid = threadIdx.x / LPB;
+    const int tiles = (nx + LPB - 1) / LPB;
+    const int j = blockIdx.x / tiles;
+    const int i = (blockIdx.x % tiles) * LPB + c;
+    const bool valid = i < nx && j < ny;
+    const long es = (long)nx * ny;
+    C2<T>* base = W + (valid ? (i + (long)j * nx) : 0);
+    auto idx = [&](int pos) { return pos * CS + c; };
+    C2<T> u[3][E], o[3][E], x[E];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        for_in<P>(tid, [&](int s, int pos) { x[s] = base[d * F + pos * es]; });
+        if (d > 0) __syncthreads();
+        line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+        for (int s = 0; s < E; s++) u[d][s] = rmul<T>(scale, x[s]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        for_in<P>(tid, [&](int s, int pos) { x[s] = base[(3 + d) * F + pos * es]; });
+        __syncthreads();
+        line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+        for (int s = 0; s < E; s++) {
+            const C2<T> w = rmul<T>(scale, x[s]);
+            // ret0 = u1 w2 - u2 w1, ret1 = u2 w0 - u0 w2, ret2 = u0 w1 - u1 w0 (src/Vector.cpp:79-84)
+            if (d == 0) {
+                o[1][s] = cmul<T>(u[2][s], w);
+                o[2][s] = mk<T>(T(0), T(0));
+                o[2][s] = csub<T>(o[2][s], cmul<T>(u[1][s], w));
+            } else if (d == 1) {
+                o[0][s] = csub<T>(mk<T>(T(0), T(0)), cmul<T>(u[2][s], w));
+                o[2][s] = cadd<T>(o[2][s], cmul<T>(u[0][s], w));
+            } else {
+                o[0][s] = cadd<T>(o[0][s], cmul<T>(u[1][s], w));
+                o[1][s] = csub<T>(o[1][s], cmul<T>(u[0][s], w));
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int s = 0; s < E; s++) x[s] = o[d][s];
+        __syncthreads();
+        line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
+        if (valid) for_out<P>(tid, [&](int s, int pos) { base[d * F + pos * es] = x[s]; });
+    }
+}
+
+// E: x-pass forward of the products (-> N1, Nyquist zeroed) and the
+// Fourier-space update of exec/ns.cpp:171-187; writes V1 <- V, N0 <- N1 and
+// per-block sums of |div V| (exec/ns.cpp:209-210).
+template <typename T, class P, int LPB>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(const C2<T>* __restrict__ X,
+                                                              C2<T>* __restrict__ V,
+                                                              C2<T>* __restrict__ N0,
+                                                              const C2<T>* __restrict__ tw, int ny,
+                                                              int nz, long F, StepCoef<T> cf,
+                                                              double* __restrict__ divpart) {
+    constexpr int N = P::N;
+    constexpr int LS = LdsLine<N>::LS;
+    constexpr int E = P::E;
+    __shared__ T sre[LPB * LS];
+    __shared__ T sim[LPB * LS];
+    const int tid = threadIdx.x % P::TPL;
+    const int l = threadIdx.x / P::TPL;
+    const long nlines = (long)ny * nz;
+    const long line = (long)blockIdx.x * LPB + l;
+    const bool valid = line < nlines;
+    const long lc = valid ? line : 0;
+    const int jj = (int)(lc % ny), kk = (int)(lc / ny);
+    const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
+    const bool zline = (jj == ny / 2) || (kk == nz / 2);
+    auto idx = [&](int pos) { return l * LS + pos + (pos >> 4); };
+    C2<T> n1[3][E], x[E];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const C2<T>* xb = X + d * F + lc * N;
+        for_in<P>(tid, [&](int s, int pos) { x[s] = xb[pos]; });
+        if (d > 0) __syncthreads();
+        line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+        for (int s = 0; s < E; s++) n1[d][s] = x[s];
+    }
+    double dsum = 0;
+    if (valid) {
+        const long off = lc * N;
+        for_out<P>(tid, [&](int s, int pos) {
+            const T k0 = freq<T>(pos, N);
+            const bool z = zline || pos == N / 2;
+            C2<T> nn[3], v1[3], f[3], L[3], S[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                nn[d] = z ? mk<T>(T(0), T(0)) : n1[d][s];
+                v1[d] = V[d * F + off + pos];
+                const C2<T> n0 = N0[d * F + off + pos];
+                // F = K/2*(3.*cross1 - cross0)
+                f[d] = rmul<T>(cf.Kh, csub<T>(rmul<T>(T(3), nn[d]), n0));
+            }
+            // lap1 = nu*K/2*V1.laplacian() = c1 * (-(k.k) V1)
+            const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                L[d] = rmul<T>(cf.c1, rmul<T>(q, v1[d]));
+                S[d] = cadd<T>(cadd<T>(v1[d], L[d]), f[d]);  // (V1 + lap1) + F
+            }
+            // G = div(S); G.laplacian_inverse() (a=0, b=1)
+            C2<T> G = mk<T>(T(0), T(0));
+            G = cadd<T>(G, imul<T>(k0, S[0]));
+            G = cadd<T>(G, imul<T>(k1, S[1]));
+            G = cadd<T>(G, imul<T>(k2, S[2]));
+            T sum = 0;
+            sum += k0 * k0;
+            sum += k1 * k1;
+            sum += k2 * k2;
+            if (sum != 0) {
+                const T den = T(0) - T(1) * sum;
+                G.x /= den;
+                G.y /= den;
+            }
+            const T kk3[3] = {k0, k1, k2};
+            C2<T> Vn[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1
+                C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
+                C2<T> vv = cadd<T>(vh, L[d]);
+                // V[d].laplacian_inverse(1, -nu*K/2): divide by 1 + (nu K/2)|k|^2 (FP64 factor)
+                if (sum != 0) {
+                    const double den = 1.0 - (-cf.c1d) * (double)sum;
+                    vv = mk<T>(T((double)vv.x / den), T((double)vv.y / den));
+                }
+                Vn[d] = vv;
+            }
+            C2<T> dv = mk<T>(T(0), T(0));
+            dv = cadd<T>(dv, imul<T>(k0, Vn[0]));
+            dv = cadd<T>(dv, imul<T>(k1, Vn[1]));
+            dv = cadd<T>(dv, imul<T>(k2, Vn[2]));
+            dsum += hypot((double)dv.x, (double)dv.y);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                V[d * F + off + pos] = Vn[d];
+                N0[d * F + off + pos] = nn[d];
+            }
+        });
+    }
+    // block sum of |div V| (FP64), deterministic; reuses the FFT's LDS
+    static_assert(LPB * LS * sizeof(T) >= P::TPL * LPB * sizeof(double), "LDS too small for reduction");
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(sre);
+    red[threadIdx.x] = dsum;
+    __syncthreads();
+    for (int n = blockDim.x; n > 1;) {
+        const int h = (n + 1) / 2;
+        if ((int)threadIdx.x < n - h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+        n = h;
+    }
+    if (threadIdx.x == 0) divpart[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------- conversion kernels
+__global__ void k_z_to_c(const double2* __restrict__ a, float2* __restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+__global__ void k_c_to_z(const float2* __restrict__ a, double2* __restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+__global__ void k_abs_sum_partial(const double* __restrict__ part, int np, double* __restrict__ out) {
+    __shared__ double s[256];
+    double acc = 0;
+    for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+static unsigned egrid(long n) {
+    long g = (n + 255) / 256;
+    if (g > 262144) g = 262144;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" {
+int sh_gradient_z(const void*, void*, void*, void*, int, int, int, void*);
+int sh_laplacian_inverse_z(void*, double, double, int, int, int, void*);
+int sh_divergence_z(const void*, const void*, const void*, void*, int, int, int, void*);
+int sh_curl_z(const void*, const void*, const void*, void*, void*, void*, int, int, int, void*);
+int sh_vlaplacian_z(const void*, const void*, const void*, void*, void*, void*, int, int, int, void*);
+int sh_cross_z(const void*, const void*, void*, void*, int, int, int, void*);
+int sh_add_z(void*, const void*, const void*, long, void*);
+int sh_sub_z(void*, const void*, const void*, long, void*);
+int sh_scale_z(void*, double, const void*, long, void*);
+int sh_average_z(const void*, long, double*, void*);
+}
+
+struct sh_ns_s {
+    int nx, ny, nz;
+    long N;  // elements per field
+    double nu, K;
+    int prec;    // SH_F64 / SH_F32
+    int fused;
+    int nonlinear;
+    hipStream_t st;
+    int device;
+    // FP64 op-by-op state (also used for the start-up in every mode)
+    double2 *V0, *V1, *om0, *om1, *cr0, *cr1;  // 3 fields each
+    double2 *Vn, *omn, *Fv, *lap1, *S, *Gg, *Vh, *work;  // work: 6 fields
+    double2 *G, *div, *Pi0;
+    // fused state (precision prec): V1f, N0f (3 fields each), Wf (6 fields)
+    void *V1f, *N0f, *Wf;
+    double* divpart;
+    int divblocks;
+    bool have_div;
+    double div_avg;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+int alloc_fields(sh_ns_s* s, void** p, int nfields, size_t elem) {
+    int rc = pool_alloc(p, elem * (size_t)s->N * nfields);
+    if (rc) return rc;
+    s->owned.push_back(*p);
+    return hipMemsetAsync(*p, 0, elem * (size_t)s->N * nfields, s->st) == hipSuccess ? SH_OK : SH_EHIP;
+}
+
+#define F3(p, d) ((p) + (long)(d) * s->N)
+
+int vlap(sh_ns_s* s, const double2* v, double2* o) {
+    return sh_vlaplacian_z(F3(v, 0), F3(v, 1), F3(v, 2), F3(o, 0), F3(o, 1), F3(o, 2), s->nx, s->ny,
+                           s->nz, s->st);
+}
+int curl(sh_ns_s* s, const double2* v, double2* o) {
+    return sh_curl_z(F3(v, 0), F3(v, 1), F3(v, 2), F3(o, 0), F3(o, 1), F3(o, 2), s->nx, s->ny, s->nz,
+                     s->st);
+}
+int cross(sh_ns_s* s, const double2* a, const double2* b, double2* o) {
+    if (!s->nonlinear) {
+        return hipMemsetAsync(o, 0, sizeof(double2) * 3 * s->N, s->st) == hipSuccess ? SH_OK : SH_EHIP;
+    }
+    return sh_cross_z(a, b, o, s->work, s->nx, s->ny, s->nz, s->st);
+}
+
+#define RC(x)                \
+    do {                     \
+        int _rc = (x);       \
+        if (_rc) return _rc; \
+    } while (0)
+
+// exec/ns.cpp:132-153
+int startup64(sh_ns_s* s) {
+    const long V = 3 * s->N;
+    RC(cross(s, s->V0, s->om0, s->cr1));
+    RC(vlap(s, s->V0, s->lap1));
+    RC(sh_scale_z(s->lap1, s->nu, s->lap1, V, s->st));
+    RC(sh_add_z(s->S, s->cr1, s->lap1, V, s->st));
+    RC(sh_divergence_z(F3(s->S, 0), F3(s->S, 1), F3(s->S, 2), s->Pi0, s->nx, s->ny, s->nz, s->st));
+    RC(sh_laplacian_inverse_z(s->Pi0, 0.0, 1.0, s->nx, s->ny, s->nz, s->st));
+    RC(sh_gradient_z(s->Pi0, F3(s->Gg, 0), F3(s->Gg, 1), F3(s->Gg, 2), s->nx, s->ny, s->nz, s->st));
+    RC(sh_sub_z(s->S, s->S, s->Gg, V, s->st));
+    RC(sh_scale_z(s->S, s->K, s->S, V, s->st));
+    RC(sh_add_z(s->V1, s->V0, s->S, V, s->st));
+    RC(curl(s, s->V1, s->om1));
+    RC(cross(s, s->V0, s->om0, s->cr0));
+    return SH_OK;
+}
+
+// exec/ns.cpp:167-217
+int step64(sh_ns_s* s) {
+    const long V = 3 * s->N;
+    const double K = s->K, nu = s->nu;
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    RC(cross(s, s->V1, s->om1, s->cr1));
+    RC(sh_scale_z(s->Fv, 3.0, s->cr1, V, s->st));
+    RC(sh_sub_z(s->Fv, s->Fv, s->cr0, V, s->st));
+    RC(sh_scale_z(s->Fv, K / 2, s->Fv, V, s->st));
+    RC(vlap(s, s->V1, s->lap1));
+    RC(sh_scale_z(s->lap1, nu * K / 2, s->lap1, V, s->st));
+    RC(sh_add_z(s->S, s->V1, s->lap1, V, s->st));
+    RC(sh_add_z(s->S, s->S, s->Fv, V, s->st));
+    RC(sh_divergence_z(F3(s->S, 0), F3(s->S, 1), F3(s->S, 2), s->G, nx, ny, nz, s->st));
+    RC(sh_laplacian_inverse_z(s->G, 0.0, 1.0, nx, ny, nz, s->st));
+    RC(sh_gradient_z(s->G, F3(s->Gg, 0), F3(s->Gg, 1), F3(s->Gg, 2), nx, ny, nz, s->st));
+    RC(sh_add_z(s->Vh, s->V1, s->Fv, V, s->st));
+    RC(sh_sub_z(s->Vh, s->Vh, s->Gg, V, s->st));
+    RC(sh_add_z(s->Vn, s->Vh, s->lap1, V, s->st));
+    for (int d = 0; d < 3; d++) RC(sh_laplacian_inverse_z(F3(s->Vn, d), 1.0, -nu * K / 2, nx, ny, nz, s->st));
+    RC(curl(s, s->Vn, s->omn));
+    RC(sh_divergence_z(F3(s->Vn, 0), F3(s->Vn, 1), F3(s->Vn, 2), s->div, nx, ny, nz, s->st));
+    s->have_div = true;
+    // histories: V0 = V1; V1 = V; omega0 = omega1; omega1 = omega; cross0 = cross1
+    SH_HIP(hipMemcpyAsync(s->V0, s->V1, sizeof(double2) * V, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(s->V1, s->Vn, sizeof(double2) * V, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(s->om0, s->om1, sizeof(double2) * V, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(s->om1, s->omn, sizeof(double2) * V, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(s->cr0, s->cr1, sizeof(double2) * V, hipMemcpyDeviceToDevice, s->st));
+    return SH_OK;
+}
+
+// ---------------------------------------------------------------- fused launchers
+template <typename T, class P>
+int launch_inv_x(sh_ns_s* s) {
+    constexpr int LPB = (512 / (6 * P::TPL)) >= 8 ? 8 : (512 / (6 * P::TPL)) >= 4 ? 4 : (512 / (6 * P::TPL)) >= 2 ? 2 : 1;
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
+    if (!tw) return SH_ENOMEM;
+    const long nlines = (long)s->ny * s->nz;
+    dim3 grid((unsigned)((nlines + LPB - 1) / LPB)), block(P::TPL * LPB * 6);
+    ProfScope ps("ns_inv_x", s->st);
+    hipLaunchKernelGGL((k_ns_inv_x<T, P, LPB>), grid, block, 0, s->st, (const C2<T>*)s->V1f,
+                       (C2<T>*)s->Wf, tw, s->ny, s->nz, s->N);
+    SH_LAUNCH_CHECK("k_ns_inv_x");
+    return SH_OK;
+}
+
+template <typename T, class P, int LPB>
+int launch_cross_z(sh_ns_s* s) {
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
+    if (!tw) return SH_ENOMEM;
+    const int tiles = (s->nx + LPB - 1) / LPB;
+    dim3 grid((unsigned)(tiles * s->ny)), block(P::TPL * LPB);
+    const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
+    ProfScope ps("ns_cross_z", s->st);
+    hipLaunchKernelGGL((k_ns_cross_z<T, P, LPB>), grid, block, 0, s->st, (C2<T>*)s->Wf, tw, s->nx,
+                       s->ny, s->N, scale);
+    SH_LAUNCH_CHECK("k_ns_cross_z");
+    return SH_OK;
+}
+
+template <typename T, class P>
+int launch_fourier_x(sh_ns_s* s) {
+    constexpr int LPB = 256 / P::TPL >= 1 ? 256 / P::TPL : 1;
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
+    if (!tw) return SH_ENOMEM;
+    const long nlines = (long)s->ny * s->nz;
+    const unsigned nb = (unsigned)((nlines + LPB - 1) / LPB);
+    if ((int)nb > s->divblocks) {
+        set_error("ns: divergence partial buffer too small");
+        return SH_EINVAL;
+    }
+    StepCoef<T> cf;
+    cf.Kh = T(s->K / 2);
+    cf.c1 = T(s->nu * s->K / 2);
+    cf.c1d = s->nu * s->K / 2;
+    cf.inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
+    ProfScope ps("ns_fourier_x", s->st);
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB>), dim3(nb), dim3(P::TPL * LPB), 0, s->st,
+                       (const C2<T>*)s->Wf, (C2<T>*)s->V1f, (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N,
+                       cf, s->divpart);
+    SH_LAUNCH_CHECK("k_ns_fourier_x");
+    return SH_OK;
+}
+
+template <typename T>
+int dispatch_inv_x(sh_ns_s* s) {
+    switch (s->nx) {
+#define SH_CASE(n_, P_, C_) \
+    case n_:                \
+        return launch_inv_x<T, P_>(s);
+        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    return SH_EUNSUPPORTED;
+}
+template <typename T>
+int dispatch_cross_z(sh_ns_s* s) {
+    switch (s->nz) {
+// FP64 holds 56 live complex values per thread here: half the column tile
+// keeps the block at 256 threads so the full VGPR file is available.
+#define SH_CASE(n_, P_, C_) \
+    case n_:                \
+        return launch_cross_z<T, P_, (sizeof(T) == 8 && C_ > 4) ? C_ / 2 : C_>(s);
+        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    return SH_EUNSUPPORTED;
+}
+template <typename T>
+int dispatch_fourier_x(sh_ns_s* s) {
+    switch (s->nx) {
+#define SH_CASE(n_, P_, C_) \
+    case n_:                \
+        return launch_fourier_x<T, P_>(s);
+        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    return SH_EUNSUPPORTED;
+}
+
+template <typename T>
+int step_fused(sh_ns_s* s) {
+    if (!s->nonlinear) {
+        set_error("ns: the fused schedule always includes the nonlinear term");
+        return SH_EINVAL;
+    }
+    const long sxy = (long)s->nx * s->ny;
+    Epi<T> plain{T(1), 0, 1, -1, -1};
+    RC(dispatch_inv_x<T>(s));
+    {
+        ProfScope ps("ns_y_inv", s->st);
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6, s->N, +1, plain, s->st));
+    }
+    RC(dispatch_cross_z<T>(s));
+    {
+        ProfScope ps("ns_y_fwd", s->st);
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3, s->N, -1, plain, s->st));
+    }
+    RC(dispatch_fourier_x<T>(s));
+    s->have_div = true;
+    return SH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, int precision,
+                 int fused, void* stream) {
+    if (!handle || nx <= 0 || ny <= 0 || nz <= 0 || (precision != SH_F64 && precision != SH_F32)) {
+        set_error("sh_ns_create: invalid argument");
+        return SH_EINVAL;
+    }
+    if (!fft_size_supported(nx) || !fft_size_supported(ny) || !fft_size_supported(nz)) {
+        set_error("sh_ns_create: unsupported grid size");
+        return SH_EUNSUPPORTED;
+    }
+    if (precision == SH_F32 && !fused) {
+        set_error("sh_ns_create: the FP32 solver uses the fused schedule");
+        return SH_EINVAL;
+    }
+    RC(ensure_init());
+    sh_ns_s* s = new sh_ns_s();
+    s->nx = nx;
+    s->ny = ny;
+    s->nz = nz;
+    s->N = (long)nx * ny * nz;
+    s->nu = nu;
+    s->K = K;
+    s->prec = precision;
+    s->fused = fused;
+    s->nonlinear = 1;
+    s->st = as_stream(stream);
+    s->device = current_device();
+    s->have_div = false;
+    s->div_avg = 0;
+    const size_t z = sizeof(double2);
+    int rc = 0;
+    // FP64 state used by the start-up (and by the op-by-op loop)
+    void* p;
+#define AL(field, nf)                          \
+    if (!rc) {                                 \
+        rc = alloc_fields(s, &p, nf, z);       \
+        s->field = (double2*)p;                \
+    }
+    AL(V0, 3) AL(V1, 3) AL(om0, 3) AL(om1, 3) AL(cr0, 3) AL(cr1, 3) AL(S, 3) AL(Gg, 3) AL(lap1, 3)
+    AL(work, 6) AL(Pi0, 1)
+    if (!fused) {
+        AL(Vn, 3) AL(omn, 3) AL(Fv, 3) AL(Vh, 3) AL(G, 1) AL(div, 1)
+    }
+#undef AL
+    if (!rc && fused) {
+        const size_t e = precision == SH_F64 ? sizeof(double2) : sizeof(float2);
+        rc = alloc_fields(s, &s->V1f, 3, e);
+        if (!rc) rc = alloc_fields(s, &s->N0f, 3, e);
+        if (!rc) rc = alloc_fields(s, &s->Wf, 6, e);
+        s->divblocks = (int)((long)ny * nz + 1);  // >= blocks of the fused x kernel (>= 1 line per block)
+        if (!rc) rc = pool_alloc((void**)&s->divpart, sizeof(double) * (s->divblocks + 1));
+        if (!rc) s->owned.push_back(s->divpart);
+    }
+    if (rc) {
+        for (void* q : s->owned) pool_free(q);
+        delete s;
+        return rc;
+    }
+    *handle = s;
+    return SH_OK;
+}
+
+int sh_ns_destroy(sh_ns_t s) {
+    if (!s) return SH_OK;
+    (void)hipStreamSynchronize(s->st);
+    for (void* q : s->owned) pool_free(q);
+    delete s;
+    return SH_OK;
+}
+
+int sh_ns_set_nonlinear(sh_ns_t s, int on) {
+    if (!s) return SH_EINVAL;
+    s->nonlinear = on != 0;
+    return SH_OK;
+}
+
+int sh_ns_set_initial(sh_ns_t s, const void* V0) {
+    if (!s || !V0) return SH_EINVAL;
+    SH_HIP(hipMemcpyAsync(s->V0, V0, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
+    return curl(s, s->V0, s->om0);
+}
+
+int sh_ns_startup(sh_ns_t s) {
+    if (!s) return SH_EINVAL;
+    RC(startup64(s));
+    if (s->fused) {
+        const long n3 = 3 * s->N;
+        if (s->prec == SH_F64) {
+            SH_HIP(hipMemcpyAsync(s->V1f, s->V1, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
+            SH_HIP(hipMemcpyAsync(s->N0f, s->cr0, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
+        } else {
+            hipLaunchKernelGGL(k_z_to_c, dim3(egrid(n3)), dim3(256), 0, s->st, s->V1, (float2*)s->V1f, n3);
+            hipLaunchKernelGGL(k_z_to_c, dim3(egrid(n3)), dim3(256), 0, s->st, s->cr0, (float2*)s->N0f, n3);
+            SH_LAUNCH_CHECK("k_z_to_c");
+        }
+    }
+    return SH_OK;
+}
+
+int sh_ns_steps(sh_ns_t s, int nsteps) {
+    if (!s || nsteps < 0) return SH_EINVAL;
+    for (int t = 0; t < nsteps; t++) {
+        if (!s->fused)
+            RC(step64(s));
+        else if (s->prec == SH_F64)
+            RC(step_fused<double>(s));
+        else
+            RC(step_fused<float>(s));
+    }
+    return SH_OK;
+}
+
+int sh_ns_get_velocity(sh_ns_t s, void* out) {
+    if (!s || !out) return SH_EINVAL;
+    const long n3 = 3 * s->N;
+    if (!s->fused) {
+        SH_HIP(hipMemcpyAsync(out, s->V1, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
+    } else if (s->prec == SH_F64) {
+        SH_HIP(hipMemcpyAsync(out, s->V1f, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
+    } else {
+        hipLaunchKernelGGL(k_c_to_z, dim3(egrid(n3)), dim3(256), 0, s->st, (const float2*)s->V1f, (double2*)out, n3);
+        SH_LAUNCH_CHECK("k_c_to_z");
+    }
+    return SH_OK;
+}
+
+int sh_ns_get_vorticity(sh_ns_t s, void* out) {
+    if (!s || !out) return SH_EINVAL;
+    if (!s->fused) {
+        SH_HIP(hipMemcpyAsync(out, s->om1, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
+        return SH_OK;
+    }
+    // omega1 = curl(V1) (exec/ns.cpp:206, 216)
+    RC(sh_ns_get_velocity(s, s->work));
+    return curl(s, s->work, (double2*)out);
+}
+
+int sh_ns_div_average(sh_ns_t s, double* result) {
+    if (!s || !result) return SH_EINVAL;
+    if (!s->have_div) {
+        *result = 0;
+        return SH_OK;
+    }
+    if (!s->fused) return sh_average_z(s->div, s->N, result, s->st);
+    const long nlines = (long)s->ny * s->nz;
+    // the fused x kernel wrote one partial per block; blocks = ceil(nlines / LPB) <= divblocks
+    int nb = 0;
+    switch (s->nx) {
+#define SH_CASE(n_, P_, C_)                                    \
+    case n_: {                                                 \
+        constexpr int LPB = 256 / P_::TPL >= 1 ? 256 / P_::TPL : 1; \
+        nb = (int)((nlines + LPB - 1) / LPB);                  \
+        break;                                                 \
+    }
+        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    hipLaunchKernelGGL(k_abs_sum_partial, dim3(1), dim3(256), 0, s->st, s->divpart, nb, s->divpart + s->divblocks);
+    SH_LAUNCH_CHECK("k_abs_sum_partial");
+    double v = 0;
+    SH_HIP(hipMemcpyAsync(&v, s->divpart + s->divblocks, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    SH_HIP(hipStreamSynchronize(s->st));
+    *result = v / (double)s->N;
+    return SH_OK;
+}
+
+int sh_ns_state_bytes(sh_ns_t s, size_t* bytes) {
+    if (!s || !bytes) return SH_EINVAL;
+    // every owned block was allocated through the pool with its exact size request
+    size_t fields = 0;
+    fields += 3 * 6 + 9 + 6 + 1;  // FP64 start-up state
+    if (!s->fused) fields += 3 * 4 + 2;
+    size_t b = fields * sizeof(double2) * (size_t)s->N;
+    if (s->fused) b += 12 * (s->prec == SH_F64 ? sizeof(double2) : sizeof(float2)) * (size_t)s->N;
+    *bytes = b;
+    return SH_OK;
+}
+
+}  // extern "C"
