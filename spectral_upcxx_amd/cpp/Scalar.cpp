@@ -1,0 +1,255 @@
+// Scalar.cpp -- Scalar on device memory, every operation a call into the
+// gfx950 C ABI (include/spectral_hip.h).  Reference: src/Scalar.cpp.
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+
+#include "DataObjects.H"
+#include "spectral_hip.h"
+
+// The reference prints the message and exits with status 0
+// (src/Scalar.cpp:9-13); this build exits with status 1 so that failures
+// are visible to the caller.
+#define ERROR(message)                                                                       \
+    do {                                                                                     \
+        std::cout << "ERROR in " << __func__ << " (" << __FILE__ << " " << __LINE__ << ", r" \
+                  << upcxx::rank_me() << "): " << message << std::endl;                      \
+        std::exit(1);                                                                        \
+    } while (0)
+
+#define CHECK(call)                                                                     \
+    do {                                                                                \
+        int _rc = (call);                                                               \
+        if (_rc != 0) ERROR(#call << " failed (" << _rc << "): " << sh_last_error()); \
+    } while (0)
+
+int Scalar::m_nscalars = 0;
+
+double Scalar::average() const {
+    double r = 0;
+    CHECK(sh_average_z(deviceData(), (long)m_dSize, &r, nullptr));
+    return r;
+}
+
+Scalar::Scalar() : m_Nz(0), m_Ny(0), m_N(0), m_dSize(0), m_dims{0, 0, 0}, m_pdims{0, 0, 0},
+                   m_dev(nullptr), m_where(0), m_fourier(false), m_initialized(false) {}
+
+Scalar::Scalar(std::array<INT, DIM> N, bool physical) : Scalar() { init(N, physical); }
+
+Scalar::~Scalar() {
+    if (m_initialized) {
+        sh_free(m_dev);
+        m_dev = nullptr;
+        m_nscalars--;
+        m_initialized = false;
+    }
+}
+
+Scalar::Scalar(const Scalar& scalar) : Scalar() {
+    init(scalar.getDims(), scalar.isPhysical());
+    CHECK(sh_memcpy_d2d(m_dev, scalar.deviceData(), m_dSize * sizeof(cdouble), nullptr));
+    m_where = 0;
+}
+
+void Scalar::operator=(const Scalar& scalar) {
+    if (!m_initialized) init(scalar.getDims(), scalar.isPhysical());
+    if (scalar.getDims() != m_dims) ERROR("assignment between different dimensions");
+    if (&scalar == this) return;
+    CHECK(sh_memcpy_d2d(m_dev, scalar.deviceData(), m_dSize * sizeof(cdouble), nullptr));
+    m_where = 0;
+}
+
+void Scalar::init(std::array<INT, DIM> N, bool physical) {
+    if (m_initialized) {
+        sh_free(m_dev);
+        m_nscalars--;
+    }
+    m_initialized = true;
+    m_nscalars++;
+    m_dims = N;
+    m_N = 1;
+    for (int i = 0; i < DIM; i++) m_N *= m_dims[i];
+    // one rank owns every slab (src/Scalar.cpp:84-90 with rank_n() = 1)
+    m_Nz = m_dims[2];
+    m_Ny = m_dims[1];
+    m_pdims = {m_dims[0], m_dims[1], m_Nz};
+    m_dSize = m_Nz * m_dims[0] * m_dims[1];
+    void* p = nullptr;
+    CHECK(sh_malloc(&p, m_dSize * sizeof(cdouble)));
+    m_dev = static_cast<cdouble*>(p);
+    CHECK(sh_memset(m_dev, 0, m_dSize * sizeof(cdouble), nullptr));
+    m_host.clear();
+    m_where = 0;
+    m_fft_x.init((int)m_dims[0], 1, m_dev);
+    m_fft_y.init((int)m_dims[1], (int)m_dims[0], m_dev);
+    m_fft_z.init((int)m_dims[2], (int)(m_dims[0] * m_Ny), m_dev);
+    if (physical)
+        setPhysical();
+    else
+        setFourier();
+}
+
+void Scalar::host_current() const {
+    if (m_host.size() != m_dSize) {
+        m_host.assign(m_dSize, cdouble(0, 0));
+        if (m_where == 1) m_where = 0;  // a fresh mirror never holds data
+    }
+    if (m_where == 0) {
+        CHECK(sh_memcpy_d2h(m_host.data(), m_dev, m_dSize * sizeof(cdouble)));
+        m_where = 2;
+    }
+}
+
+void Scalar::device_current() const {
+    if (m_where == 1) {
+        CHECK(sh_memcpy_h2d(m_dev, m_host.data(), m_dSize * sizeof(cdouble)));
+        m_where = 2;
+    }
+}
+
+const cdouble* Scalar::deviceData() const {
+    if (!m_initialized) ERROR("use of an uninitialised Scalar");
+    device_current();
+    return m_dev;
+}
+
+cdouble* Scalar::deviceDataMutable() {
+    if (!m_initialized) ERROR("use of an uninitialised Scalar");
+    device_current();
+    m_where = 0;
+    return m_dev;
+}
+
+void Scalar::set(std::array<INT, DIM> index, cdouble val) { (*this)[index] = val; }
+
+cdouble Scalar::get(std::array<INT, DIM> index) const { return (*this)[index]; }
+
+cdouble& Scalar::operator[](std::array<INT, DIM> index) {
+    INT rank, loc;
+    getIndex(index, rank, loc);
+    host_current();
+    m_where = 1;
+    return m_host[loc];
+}
+
+cdouble Scalar::operator[](std::array<INT, DIM> index) const {
+    INT rank, loc;
+    getIndex(index, rank, loc);
+    host_current();
+    return m_host[loc];
+}
+
+cdouble& Scalar::operator()(INT a_i, INT a_j, INT a_k) { return (*this)[std::array<INT, DIM>{a_i, a_j, a_k}]; }
+
+cdouble Scalar::operator()(INT a_i, INT a_j, INT a_k) const {
+    return (*this)[std::array<INT, DIM>{a_i, a_j, a_k}];
+}
+
+cdouble& Scalar::operator[](INT index) {
+    if (index >= m_dSize) ERROR("accessed out of local data");
+    host_current();
+    m_where = 1;
+    return m_host[index];
+}
+
+cdouble Scalar::operator[](INT index) const {
+    if (index >= m_dSize) ERROR("accessed out of local data");
+    host_current();
+    return m_host[index];
+}
+
+std::array<double, DIM> Scalar::freq(std::array<INT, DIM> ind) const {
+    std::array<double, DIM> k;
+    for (int i = 0; i < DIM; i++) {
+        if (ind[i] <= m_dims[i] / 2)
+            k[i] = (double)ind[i];
+        else
+            k[i] = -1. * (double)m_dims[i] + (double)ind[i];
+    }
+    return k;
+}
+
+void Scalar::transform(int dir) {
+    CHECK(sh_fft3d_z2z(deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], dir, 0, nullptr));
+}
+
+void Scalar::forward() {
+    if (isFourier()) ERROR("already in fourier space");
+    CHECK(sh_fft3d_z2z(deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], -1,
+                       SH_ZERO_NYQUIST, nullptr));
+    setFourier();
+}
+
+void Scalar::inverse() {
+    if (isPhysical()) ERROR("already in physical space");
+    CHECK(sh_fft3d_z2z(deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], +1,
+                       SH_NORMALIZE, nullptr));
+    setPhysical();
+}
+
+void Scalar::forward(Scalar& a_scalar) const {
+    a_scalar = (*this);
+    a_scalar.forward();
+}
+
+void Scalar::inverse(Scalar& a_scalar) const {
+    a_scalar = (*this);
+    a_scalar.inverse();
+}
+
+Vector Scalar::gradient() const {
+    if (!isFourier()) ERROR("must begin gradient in fourier space");
+    Vector v(m_dims, false);
+    CHECK(sh_gradient_z(deviceData(), v[0].deviceDataMutable(), v[1].deviceDataMutable(),
+                        v[2].deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], nullptr));
+    return v;
+}
+
+Scalar Scalar::laplacian() const {
+    if (!isFourier()) ERROR("must begin laplacian in fourier space");
+    Scalar lap(m_dims, false);
+    CHECK(sh_laplacian_z(deviceData(), lap.deviceDataMutable(), (int)m_dims[0], (int)m_dims[1],
+                         (int)m_dims[2], nullptr));
+    return lap;
+}
+
+void Scalar::laplacian_inverse(double a, double b) {
+    CHECK(sh_laplacian_inverse_z(deviceDataMutable(), a, b, (int)m_dims[0], (int)m_dims[1],
+                                 (int)m_dims[2], nullptr));
+}
+
+void Scalar::zeroHighModes() {
+    CHECK(sh_zero_high_z(deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], nullptr));
+}
+
+INT Scalar::localSize() const { return m_dSize; }
+std::array<INT, DIM> Scalar::getDims() const { return m_dims; }
+std::array<INT, DIM> Scalar::getPDims() const { return m_pdims; }
+std::array<INT, DIM> Scalar::getPStart() const { return {0, 0, (INT)upcxx::rank_me() * m_Nz}; }
+std::array<INT, DIM> Scalar::getPEnd() const {
+    return {m_dims[0], m_dims[1], ((INT)upcxx::rank_me() + 1) * m_Nz};
+}
+INT Scalar::size() const { return m_N; }
+
+// The reference's memory() has no return statement (src/Scalar.cpp:385-393);
+// this one returns the figure it prints.  No transpose buffer is needed here.
+double Scalar::memory() const {
+    double mem = (double)m_nscalars * (double)m_N * sizeof(cdouble) / 1e9;
+    if (upcxx::rank_me() == 0) std::cout << "memory requirement = " << mem << " GB" << std::endl;
+    return mem;
+}
+
+void Scalar::setPhysical() { m_fourier = false; }
+void Scalar::setFourier() { m_fourier = true; }
+bool Scalar::isPhysical() const { return !m_fourier; }
+bool Scalar::isFourier() const { return m_fourier; }
+
+void Scalar::getIndex(std::array<INT, DIM> index, INT& rank, INT& loc) const {
+    INT n = index[0] + m_dims[0] * index[1] + index[2] * m_dims[0] * m_dims[1];
+    if (n >= m_N) {
+        std::cout << n << std::endl;
+        ERROR("indexing out of full dimensions");
+    }
+    rank = n / m_dSize;
+    loc = n % m_dSize;
+}
