@@ -51,6 +51,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fft", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=2)
+    p.add_argument("--stable-dt", type=float, default=5e-4)
     return p.parse_args()
 
 
@@ -61,12 +62,12 @@ def prof_total(L, name):
     return ms.value, cnt.value
 
 
-def run_ns(S, n, precision, warmup, steps, profile):
+def run_ns(S, n, precision, warmup, steps, profile, dt=DT):
     import torch
     L = S.lib()
     V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
     L.sh_solenoidal_ic_z(S.ptr(V0), n, n, n, SEED, K0, 1.0, S.stream())
-    ns = S.NS((n, n, n), NU, DT, precision, fused=True)
+    ns = S.NS((n, n, n), NU, dt, precision, fused=True)
     ns.set_initial(V0)
     del V0
     ns.startup()
@@ -86,9 +87,13 @@ def run_ns(S, n, precision, warmup, steps, profile):
             ms, cnt = prof_total(L, name)
             kern[name] = (ms, cnt)
     div = ns.div_average()
+    V = ns.velocity()
+    e = ctypes.c_double()
+    L.sh_spectrum_z(S.ptr(V), n, n, n, NU, None, 0, ctypes.byref(e), None, S.stream())
+    del V
     ns.close()
     torch.cuda.empty_cache()
-    return t1 - t0, kern, div
+    return t1 - t0, kern, {"div_avg": div, "energy": e.value}
 
 
 def fft_bench(S, n, reps):
@@ -140,9 +145,20 @@ def main():
     S.lib().sh_init(0)
     n = a.n
     log("bench: %d^3 FP64 fused step, warmup %d, steps %d" % (n, a.warmup, a.steps))
-    dt, kern, div = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=True)
+    dt, kern, st = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=True)
     value = a.steps / dt
-    log("bench: %.2f steps/s (%.3f ms/step), div avg %.2e" % (value, 1e3 * dt / a.steps, div))
+    log("bench: %.2f steps/s (%.3f ms/step), final energy %r, div avg %.2e"
+        % (value, 1e3 * dt / a.steps, st["energy"], st["div_avg"]))
+    state = {"final_energy": st["energy"], "final_div_avg": st["div_avg"],
+             "finite": bool(st["energy"] == st["energy"] and abs(st["energy"]) < 1e30),
+             "note": "the reference scheme (explicit AB2 advection, no dealiasing) is unstable at "
+                     "256^3, dt=2e-3: the CPU oracle also overflows near step 65; per-step cost "
+                     "does not depend on the values"}
+    if a.stable_dt > 0:
+        dts, _, sts = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=False, dt=a.stable_dt)
+        state["stable_dt"] = {"dt": a.stable_dt, "steps_per_s": round(a.steps / dts, 3),
+                              "final_energy": sts["energy"], "final_div_avg": sts["div_avg"]}
+        log("bench: dt=%g: %.2f steps/s, final energy %r" % (a.stable_dt, a.steps / dts, sts["energy"]))
     pts = float(n) ** 3
     kernels = {}
     dom, dom_t = None, -1.0
@@ -212,6 +228,7 @@ def main():
         "fft": ffts,
         "kernels": kernels,
         "bytes": extra,
+        "state": state,
     }
     print(json.dumps(out), flush=True)
 

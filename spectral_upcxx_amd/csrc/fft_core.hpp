@@ -114,12 +114,14 @@ struct Dft {
     }
 };
 
-template <int N_, int TPL_, int R0, int R1 = 1, int R2 = 1, int R3 = 1>
+template <int N_, int TPL_, int R0, int R1 = 1, int R2 = 1, int R3 = 1, int R4 = 1>
 struct FftPlan {
     static constexpr int N = N_;
     static constexpr int TPL = TPL_;
-    static constexpr int S = (R1 == 1) ? 1 : (R2 == 1) ? 2 : (R3 == 1) ? 3 : 4;
-    static constexpr int radix(int s) { return s == 0 ? R0 : s == 1 ? R1 : s == 2 ? R2 : R3; }
+    static constexpr int S = (R1 == 1) ? 1 : (R2 == 1) ? 2 : (R3 == 1) ? 3 : (R4 == 1) ? 4 : 5;
+    static constexpr int radix(int s) {
+        return s == 0 ? R0 : s == 1 ? R1 : s == 2 ? R2 : s == 3 ? R3 : R4;
+    }
     static constexpr int ns(int s) {
         int p = 1;
         for (int i = 0; i < s; i++) p *= radix(i);
@@ -135,7 +137,7 @@ struct FftPlan {
         return e;
     }
     static constexpr int E = emax();
-    static_assert(R0 * R1 * R2 * R3 == N, "radices must multiply to N");
+    static_assert(R0 * R1 * R2 * R3 * R4 == N, "radices must multiply to N");
 };
 
 template <class P, int s, int SIGN, typename T>
@@ -221,6 +223,13 @@ __device__ __forceinline__ void line_fft(C2<T>* x, int tid, T* re, T* im, const 
         __syncthreads();
         lds_get<P, 3, T>(x, tid, re, im, idx);
         stage_bfly<P, 3, SIGN, T>(x, tid, tw);
+    }
+    if constexpr (P::S > 4) {
+        __syncthreads();
+        lds_put<P, 3, T>(x, tid, re, im, idx);
+        __syncthreads();
+        lds_get<P, 4, T>(x, tid, re, im, idx);
+        stage_bfly<P, 4, SIGN, T>(x, tid, tw);
     }
 }
 
@@ -322,6 +331,27 @@ int pass_contig(C2<T>* data, int n, long nlines, long ldist, int count, long fst
 template <typename T>
 int pass_strided(C2<T>* data, int n, int ncols, long cdist, int nouter, long odist, long es,
                  int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st);
+
+// FP64 plans for the fused z kernel, which holds 7 fields per thread at
+// once (3 velocity, 3 partial products, 1 in flight): at most 6 complex
+// values per field keeps it under 128 VGPRs, so four waves share a SIMD.
+using W64 = FftPlan<64, 16, 4, 4, 4>;
+using W128 = FftPlan<128, 32, 4, 2, 4, 4>;
+using W256 = FftPlan<256, 64, 4, 4, 4, 4>;
+using W512 = FftPlan<512, 128, 4, 4, 2, 4, 4>;
+
+#define SH_FOR_EACH_FUSED64_SIZE(X) \
+    X(8, P8, 16)                    \
+    X(12, Q12, 16)                  \
+    X(16, P16, 16)                  \
+    X(24, Q24, 16)                  \
+    X(32, Q32, 16)                  \
+    X(48, Q48, 16)                  \
+    X(64, W64, 16)                  \
+    X(96, Q96, 16)                  \
+    X(128, W128, 16)                \
+    X(256, W256, 16)                \
+    X(512, W512, 8)
 
 inline bool fft_size_supported(int n) {
     switch (n) {

@@ -108,39 +108,45 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
     const int j = blockIdx.x / tiles;
     const int i = (blockIdx.x % tiles) * LPB + c;
     const bool valid = i < nx && j < ny;
-    const long es = (long)nx * ny;
-    C2<T>* base = W + (valid ? (i + (long)j * nx) : 0);
+    // 32-bit element offsets from the uniform base W (6 fields of at most
+    // 2^31/6 elements; checked by the launcher) halve the address registers
+    const unsigned es = (unsigned)nx * (unsigned)ny;
+    const unsigned uF = (unsigned)F;
+    const unsigned cb = valid ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
     auto idx = [&](int pos) { return pos * CS + c; };
-    C2<T> u[3][E], o[3][E], x[E];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        for_in<P>(tid, [&](int s, int pos) { x[s] = base[d * F + pos * es]; });
-        if (d > 0) __syncthreads();
+    // inverse z transform of W[f] into x, scaled by 1/(Nx Ny Nz)
+    auto inv = [&](int f, C2<T>* x, bool first) {
+        for_in<P>(tid, [&](int s, int pos) { x[s] = W[cb + (unsigned)f * uF + (unsigned)pos * es]; });
+        if (!first) __syncthreads();
         line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
 #pragma unroll
-        for (int s = 0; s < E; s++) u[d][s] = rmul<T>(scale, x[s]);
+        for (int s = 0; s < E; s++) x[s] = rmul<T>(scale, x[s]);
+    };
+    // ret0 = u1 w2 - u2 w1, ret1 = u2 w0 - u0 w2, ret2 = u0 w1 - u1 w0 (src/Vector.cpp:79-84).
+    // Fields arrive in the order w0 w1 u2 w2 u0 u1 so at most 6 fields are
+    // live; each a - b is accumulated as a + (-b) or (-b) + a, which round
+    // identically to a - b.
+    C2<T> w[3][E], o[3][E], x[E];
+    inv(3, w[0], true);
+    inv(4, w[1], false);
+    inv(2, x, false);  // u2
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+        o[0][s] = csub<T>(mk<T>(T(0), T(0)), cmul<T>(x[s], w[1][s]));
+        o[1][s] = cmul<T>(x[s], w[0][s]);
     }
+    inv(5, w[2], false);
+    inv(0, x, false);  // u0
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        for_in<P>(tid, [&](int s, int pos) { x[s] = base[(3 + d) * F + pos * es]; });
-        __syncthreads();
-        line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
+    for (int s = 0; s < E; s++) {
+        o[1][s] = csub<T>(o[1][s], cmul<T>(x[s], w[2][s]));
+        o[2][s] = cmul<T>(x[s], w[1][s]);
+    }
+    inv(1, x, false);  // u1
 #pragma unroll
-        for (int s = 0; s < E; s++) {
-            const C2<T> w = rmul<T>(scale, x[s]);
-            // ret0 = u1 w2 - u2 w1, ret1 = u2 w0 - u0 w2, ret2 = u0 w1 - u1 w0 (src/Vector.cpp:79-84)
-            if (d == 0) {
-                o[1][s] = cmul<T>(u[2][s], w);
-                o[2][s] = mk<T>(T(0), T(0));
-                o[2][s] = csub<T>(o[2][s], cmul<T>(u[1][s], w));
-            } else if (d == 1) {
-                o[0][s] = csub<T>(mk<T>(T(0), T(0)), cmul<T>(u[2][s], w));
-                o[2][s] = cadd<T>(o[2][s], cmul<T>(u[0][s], w));
-            } else {
-                o[0][s] = cadd<T>(o[0][s], cmul<T>(u[1][s], w));
-                o[1][s] = csub<T>(o[1][s], cmul<T>(u[0][s], w));
-            }
-        }
+    for (int s = 0; s < E; s++) {
+        o[0][s] = cadd<T>(o[0][s], cmul<T>(x[s], w[2][s]));
+        o[2][s] = csub<T>(o[2][s], cmul<T>(x[s], w[0][s]));
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -148,7 +154,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
         for (int s = 0; s < E; s++) x[s] = o[d][s];
         __syncthreads();
         line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
-        if (valid) for_out<P>(tid, [&](int s, int pos) { base[d * F + pos * es] = x[s]; });
+        if (valid) for_out<P>(tid, [&](int s, int pos) { W[cb + (unsigned)d * uF + (unsigned)pos * es] = x[s]; });
     }
 }
 
@@ -477,17 +483,21 @@ int dispatch_inv_x(sh_ns_s* s) {
     }
     return SH_EUNSUPPORTED;
 }
+// The z kernel holds seven fields per thread; FP64 uses the small-register
+// plans (SH_FOR_EACH_FUSED64_SIZE), FP32 the common fused plans.
 template <typename T>
 int dispatch_cross_z(sh_ns_s* s) {
-    switch (s->nz) {
-// FP64 holds 56 live complex values per thread here: half the column tile
-// keeps the block at 256 threads so the full VGPR file is available.
 #define SH_CASE(n_, P_, C_) \
     case n_:                \
-        return launch_cross_z<T, P_, (sizeof(T) == 8 && C_ > 4) ? C_ / 2 : C_>(s);
-        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
-#undef SH_CASE
+        return launch_cross_z<T, P_, (C_ * P_::TPL <= kMaxBlk) ? C_ : kMaxBlk / P_::TPL>(s);
+    // FP64 needs ~200 VGPRs here: 512-thread blocks leave a 256-register budget
+    constexpr int kMaxBlk = sizeof(T) == 8 ? 512 : 1024;
+    if constexpr (sizeof(T) == 8) {
+        switch (s->nz) { SH_FOR_EACH_FUSED64_SIZE(SH_CASE) }
+    } else {
+        switch (s->nz) { SH_FOR_EACH_FUSED_SIZE(SH_CASE) }
     }
+#undef SH_CASE
     return SH_EUNSUPPORTED;
 }
 template <typename T>

@@ -15,7 +15,7 @@ BIN = os.path.join(ROOT, "spectral_upcxx_amd", "lib", "bin")
 
 
 def run(args, timeout=300):
-    print("running", " ".join(args), flush=True)
+    print("running", " ".join(str(a) for a in args), flush=True)
     r = subprocess.run([os.path.join(BIN, args[0])] + [str(a) for a in args[1:]], stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=timeout)
     return r.returncode, r.stdout
