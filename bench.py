@@ -27,12 +27,17 @@ HBM_PEAK_GBPS = 8000.0
 NU, DT, SEED, K0 = 1.0 / 800.0, 2e-3, 2024, 4.0
 # algorithmic field passes (one complex element read or written = one pass
 # unit of 16 B in FP64 / 8 B in FP32, per grid point) of each kernel class
+# HBM bytes per launch at 256^3 measured with rocprofv3 PMC (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, factors from a 1 GiB device copy); see
+# profiles/pmc_traffic_ns256.md.  Measured for the kernel variants in this tree.
+MEASURED_TRAFFIC_256 = {"ns_cross_z": 2.416e9, "ns_fourier_x": 5.638e9, "ns_inv_x": 2.416e9,
+                        "ns_y_inv": 3.221e9, "ns_y_fwd": 1.611e9}
 PASSES = {
     "ns_inv_x": 3 + 6,        # read V, write 6 x-transformed fields
     "ns_y_inv": 6 + 6,        # y pass of 6 fields
     "ns_cross_z": 6 + 3,      # z inverse of 6, cross, z forward of 3
     "ns_y_fwd": 3 + 3,        # y pass of 3 fields
-    "ns_fourier_x": 9 + 6,    # read X, V1, N0; write V1, N0
+    "ns_fourier_x": 9 + 12,   # read X, V1, N0; write V1, N0 and next step's 6 x-inverse fields
     "fft_x": 2, "fft_y": 2, "fft_z": 2,
 }
 
@@ -52,6 +57,9 @@ def parse():
     p.add_argument("--no-fft", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=2)
     p.add_argument("--stable-dt", type=float, default=5e-4)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="first run three 1 GiB device-to-device copies (the reference kernel for "
+                        "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -144,6 +152,16 @@ def main():
     torch.cuda.set_device(0)
     S.lib().sh_init(0)
     n = a.n
+    if a.calibrate_copy:
+        nb = 1 << 30
+        src = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        dst = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        for _ in range(3):
+            S.lib().sh_memcpy_d2d(S.ptr(dst), S.ptr(src), nb, S.stream())
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
+        log("bench: calibration copies done (3 x 1 GiB)")
     log("bench: %d^3 FP64 fused step, warmup %d, steps %d" % (n, a.warmup, a.steps))
     dt, kern, st = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=True)
     value = a.steps / dt
@@ -176,9 +194,10 @@ def main():
         k = kernels[dom]
         ach = k["GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBPS, 4),
+                "traffic": MEASURED_TRAFFIC_256.get(dom) if n == 256 else None,
                 "algo_bytes_per_launch": PASSES[dom] * 16.0 * pts}
-    step_bytes = sum(PASSES[k] for k in kern) * 16.0 * pts
+    step_bytes = sum(PASSES[k] for k, (ms, cnt) in kern.items() if cnt > 0) * 16.0 * pts
     extra = {"step_algo_GB_fused": round(step_bytes / 1e9, 3),
              "step_GBps_fused": round(step_bytes * value / 1e9, 1),
              "step_contract_GB": round(1056.0 * pts / 1e9, 3),
@@ -221,7 +240,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ns_fractional_step_fp64_%d^3" % n, "grid": n, "nu": NU, "dt": DT,
                    "ic": "random solenoidal, E(k)~k^4 exp(-2(k/4)^2), u_rms=1, seed %d" % SEED,
-                   "schedule": "fused (5 kernels/step)"},
+                   "schedule": "fused (4 kernels/step)"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "chosen_form_fp32_steps_per_s": fp32,

@@ -5,16 +5,18 @@
 //    operations (exec/ns.cpp:167-217), one kernel per operation, with the
 //    temporaries the reference allocates per step kept resident instead.
 //
-//  * fused (fused = 1, FP64 or FP32): five kernels per step.
-//      A  x-pass inverse of V and of omega = i k x V (curl formed on load)
+//  * fused (fused = 1, FP64 or FP32): four kernels per step.
+//      A  x-pass inverse of V and of omega = i k x V (curl formed on load);
+//         runs once after a (re)start, later steps get it from E
 //      B  y-pass inverse of the six fields
 //      C  z-pass inverse of the six fields, complex pointwise u x omega,
 //         z-pass forward of the three products -- all in registers
 //      D  y-pass forward of the products
 //      E  x-pass forward of the products + Nyquist zeroing + the whole
 //         Fourier-space update (F, L, G = lap^-1 div(V+L+F), V - grad G,
-//         CN divide) + |div V| partial sums
-//    Per step this moves 51 field passes (816 N^3 bytes in FP64) against the
+//         CN divide) + |div V| partial sums, then A's work for the next
+//         step from the new V still in registers
+//    Per step this moves 48 field passes (768 N^3 bytes in FP64) against the
 //    reference's 9 3D FFTs + ~20 array sweeps.  Every operation is mode-local
 //    except the transforms, so the fusion changes no arithmetic beyond the
 //    rounding of a different (equally exact) operation order.
@@ -114,9 +116,12 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
     const unsigned uF = (unsigned)F;
     const unsigned cb = valid ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
     auto idx = [&](int pos) { return pos * CS + c; };
-    // inverse z transform of W[f] into x, scaled by 1/(Nx Ny Nz)
-    auto inv = [&](int f, C2<T>* x, bool first) {
+    auto load = [&](int f, C2<T>* x) {
         for_in<P>(tid, [&](int s, int pos) { x[s] = W[cb + (unsigned)f * uF + (unsigned)pos * es]; });
+    };
+    // inverse z transform of x (scaled by 1/(Nx Ny Nz)); the loads of the
+    // field after it were issued beforehand so they overlap the LDS phases
+    auto inv = [&](C2<T>* x, bool first) {
         if (!first) __syncthreads();
         line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
 #pragma unroll
@@ -126,27 +131,33 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
     // Fields arrive in the order w0 w1 u2 w2 u0 u1 so at most 6 fields are
     // live; each a - b is accumulated as a + (-b) or (-b) + a, which round
     // identically to a - b.
-    C2<T> w[3][E], o[3][E], x[E];
-    inv(3, w[0], true);
-    inv(4, w[1], false);
-    inv(2, x, false);  // u2
+    C2<T> w[3][E], o[3][E], x[E], pf[E];
+    load(3, w[0]);
+    load(4, w[1]);
+    inv(w[0], true);
+    load(2, pf);
+    inv(w[1], false);
+    load(5, w[2]);
+    inv(pf, false);  // u2
 #pragma unroll
     for (int s = 0; s < E; s++) {
-        o[0][s] = csub<T>(mk<T>(T(0), T(0)), cmul<T>(x[s], w[1][s]));
-        o[1][s] = cmul<T>(x[s], w[0][s]);
+        o[0][s] = csub<T>(mk<T>(T(0), T(0)), cmul<T>(pf[s], w[1][s]));
+        o[1][s] = cmul<T>(pf[s], w[0][s]);
     }
-    inv(5, w[2], false);
-    inv(0, x, false);  // u0
+    load(0, x);
+    inv(w[2], false);
+    load(1, pf);
+    inv(x, false);  // u0
 #pragma unroll
     for (int s = 0; s < E; s++) {
         o[1][s] = csub<T>(o[1][s], cmul<T>(x[s], w[2][s]));
         o[2][s] = cmul<T>(x[s], w[1][s]);
     }
-    inv(1, x, false);  // u1
+    inv(pf, false);  // u1
 #pragma unroll
     for (int s = 0; s < E; s++) {
-        o[0][s] = cadd<T>(o[0][s], cmul<T>(x[s], w[2][s]));
-        o[2][s] = csub<T>(o[2][s], cmul<T>(x[s], w[0][s]));
+        o[0][s] = cadd<T>(o[0][s], cmul<T>(pf[s], w[2][s]));
+        o[2][s] = csub<T>(o[2][s], cmul<T>(pf[s], w[0][s]));
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -160,14 +171,21 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
 
 // E: x-pass forward of the products (-> N1, Nyquist zeroed) and the
 // Fourier-space update of exec/ns.cpp:171-187; writes V1 <- V, N0 <- N1 and
-// per-block sums of |div V| (exec/ns.cpp:209-210).
-template <typename T, class P, int LPB>
-__global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(const C2<T>* __restrict__ X,
+// per-block sums of |div V| (exec/ns.cpp:209-210).  With EMIT it also runs
+// the next step's first pass: the inverse x transform of the new V and of
+// omega = i k x V (exec/ns.cpp:206, 216) into W[0..6), which kernel A would
+// otherwise do (same slot->position map, so no reshuffle is needed).  The
+// products are read from W[0..3) before any line of W is rewritten, and a
+// block only rewrites its own lines.
+template <typename T, class P, int LPB, bool EMIT>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
                                                               const C2<T>* __restrict__ tw, int ny,
                                                               int nz, long F, StepCoef<T> cf,
                                                               double* __restrict__ divpart) {
+    static_assert(!EMIT || (P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1)),
+                  "emitting the next inverse pass needs a symmetric plan");
     constexpr int N = P::N;
     constexpr int LS = LdsLine<N>::LS;
     constexpr int E = P::E;
@@ -182,20 +200,35 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(const C2<T>* __res
     const int jj = (int)(lc % ny), kk = (int)(lc / ny);
     const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
     const bool zline = (jj == ny / 2) || (kk == nz / 2);
+    const long off = lc * N;
     auto idx = [&](int pos) { return l * LS + pos + (pos >> 4); };
-    C2<T> n1[3][E], x[E];
+    auto load = [&](int f, C2<T>* x) { for_in<P>(tid, [&](int s, int pos) { x[s] = W[f * F + off + pos]; }); };
+    // forward x transforms of the three products; the next field's loads are
+    // issued before the current transform so they overlap its LDS phases
+    C2<T> n1[3][E], x[E], pf[E];
+    load(0, x);
+    load(1, pf);
+    line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const C2<T>* xb = X + d * F + lc * N;
-        for_in<P>(tid, [&](int s, int pos) { x[s] = xb[pos]; });
-        if (d > 0) __syncthreads();
-        line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
+    for (int s = 0; s < E; s++) n1[0][s] = x[s];
 #pragma unroll
-        for (int s = 0; s < E; s++) n1[d][s] = x[s];
-    }
+    for (int s = 0; s < E; s++) x[s] = pf[s];
+    load(2, pf);
+    __syncthreads();
+    line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+    for (int s = 0; s < E; s++) n1[1][s] = x[s];
+    __syncthreads();
+    line_fft<P, -1, T>(pf, tid, sre, sim, idx, tw);
+#pragma unroll
+    for (int s = 0; s < E; s++) n1[2][s] = pf[s];
     double dsum = 0;
+    C2<T> vn[3][E];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int s = 0; s < E; s++) vn[d][s] = mk<T>(T(0), T(0));
     if (valid) {
-        const long off = lc * N;
         for_out<P>(tid, [&](int s, int pos) {
             const T k0 = freq<T>(pos, N);
             const bool z = zline || pos == N / 2;
@@ -230,7 +263,6 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(const C2<T>* __res
                 G.y /= den;
             }
             const T kk3[3] = {k0, k1, k2};
-            C2<T> Vn[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1
@@ -241,19 +273,39 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(const C2<T>* __res
                     const double den = 1.0 - (-cf.c1d) * (double)sum;
                     vv = mk<T>(T((double)vv.x / den), T((double)vv.y / den));
                 }
-                Vn[d] = vv;
+                vn[d][s] = vv;
             }
             C2<T> dv = mk<T>(T(0), T(0));
-            dv = cadd<T>(dv, imul<T>(k0, Vn[0]));
-            dv = cadd<T>(dv, imul<T>(k1, Vn[1]));
-            dv = cadd<T>(dv, imul<T>(k2, Vn[2]));
+            dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
+            dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
+            dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
             dsum += hypot((double)dv.x, (double)dv.y);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                V[d * F + off + pos] = Vn[d];
+                V[d * F + off + pos] = vn[d][s];
                 N0[d * F + off + pos] = nn[d];
             }
         });
+    }
+    if constexpr (EMIT) {
+        // next step's kernel A: inverse x of V (g = 0..2) and of i k x V (g = 3..5)
+#pragma unroll
+        for (int g = 0; g < 6; g++) {
+            for_out<P>(tid, [&](int s, int pos) {
+                const T k0 = freq<T>(pos, N);
+                if (g < 3)
+                    x[s] = vn[g][s];
+                else if (g == 3)
+                    x[s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
+                else if (g == 4)
+                    x[s] = csub<T>(imul<T>(k2, vn[0][s]), imul<T>(k0, vn[2][s]));
+                else
+                    x[s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
+            });
+            __syncthreads();
+            line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
+            if (valid) for_out<P>(tid, [&](int s, int pos) { W[g * F + off + pos] = x[s]; });
+        }
     }
     // block sum of |div V| (FP64), deterministic; reuses the FFT's LDS
     static_assert(LPB * LS * sizeof(T) >= P::TPL * LPB * sizeof(double), "LDS too small for reduction");
@@ -333,6 +385,7 @@ struct sh_ns_s {
     double* divpart;
     int divblocks;
     bool have_div;
+    bool w_primed;  // W holds the x-inverse of V1 and curl V1 for the next step
     double div_avg;
     std::vector<void*> owned;
 };
@@ -438,6 +491,10 @@ template <typename T, class P, int LPB>
 int launch_cross_z(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
+    if (6 * s->N > 0xffffffffL) {  // the kernel addresses W with 32-bit element offsets
+        set_error("ns: grid too large for the fused z kernel");
+        return SH_EINVAL;
+    }
     const int tiles = (s->nx + LPB - 1) / LPB;
     dim3 grid((unsigned)(tiles * s->ny)), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
@@ -448,7 +505,7 @@ int launch_cross_z(sh_ns_s* s) {
     return SH_OK;
 }
 
-template <typename T, class P>
+template <typename T, class P, bool EMIT>
 int launch_fourier_x(sh_ns_s* s) {
     constexpr int LPB = 256 / P::TPL >= 1 ? 256 / P::TPL : 1;
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
@@ -465,9 +522,9 @@ int launch_fourier_x(sh_ns_s* s) {
     cf.c1d = s->nu * s->K / 2;
     cf.inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB>), dim3(nb), dim3(P::TPL * LPB), 0, s->st,
-                       (const C2<T>*)s->Wf, (C2<T>*)s->V1f, (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N,
-                       cf, s->divpart);
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT>), dim3(nb), dim3(P::TPL * LPB), 0, s->st,
+                       (C2<T>*)s->Wf, (C2<T>*)s->V1f, (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N, cf,
+                       s->divpart);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     return SH_OK;
 }
@@ -500,15 +557,19 @@ int dispatch_cross_z(sh_ns_s* s) {
 #undef SH_CASE
     return SH_EUNSUPPORTED;
 }
+// kernel E holds the three products and then the new V per thread: FP64
+// uses the small-register plans as kernel C does
 template <typename T>
 int dispatch_fourier_x(sh_ns_s* s) {
-    switch (s->nx) {
 #define SH_CASE(n_, P_, C_) \
     case n_:                \
-        return launch_fourier_x<T, P_>(s);
-        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
-#undef SH_CASE
+        return launch_fourier_x<T, P_, true>(s);
+    if constexpr (sizeof(T) == 8) {
+        switch (s->nx) { SH_FOR_EACH_FUSED64_SIZE(SH_CASE) }
+    } else {
+        switch (s->nx) { SH_FOR_EACH_FUSED_SIZE(SH_CASE) }
     }
+#undef SH_CASE
     return SH_EUNSUPPORTED;
 }
 
@@ -520,7 +581,11 @@ int step_fused(sh_ns_s* s) {
     }
     const long sxy = (long)s->nx * s->ny;
     Epi<T> plain{T(1), 0, 1, -1, -1};
-    RC(dispatch_inv_x<T>(s));
+    // kernel A runs once after a (re)start; afterwards kernel E emits its output
+    if (!s->w_primed) {
+        RC(dispatch_inv_x<T>(s));
+        s->w_primed = true;
+    }
     {
         ProfScope ps("ns_y_inv", s->st);
         RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6, s->N, +1, plain, s->st));
@@ -567,6 +632,7 @@ int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, i
     s->st = as_stream(stream);
     s->device = current_device();
     s->have_div = false;
+    s->w_primed = false;
     s->div_avg = 0;
     const size_t z = sizeof(double2);
     int rc = 0;
@@ -623,6 +689,7 @@ int sh_ns_set_initial(sh_ns_t s, const void* V0) {
 
 int sh_ns_startup(sh_ns_t s) {
     if (!s) return SH_EINVAL;
+    s->w_primed = false;
     RC(startup64(s));
     if (s->fused) {
         const long n3 = 3 * s->N;
@@ -686,16 +753,18 @@ int sh_ns_div_average(sh_ns_t s, double* result) {
     const long nlines = (long)s->ny * s->nz;
     // the fused x kernel wrote one partial per block; blocks = ceil(nlines / LPB) <= divblocks
     int nb = 0;
-    switch (s->nx) {
-#define SH_CASE(n_, P_, C_)                                    \
-    case n_: {                                                 \
+#define SH_CASE(n_, P_, C_)                                         \
+    case n_: {                                                      \
         constexpr int LPB = 256 / P_::TPL >= 1 ? 256 / P_::TPL : 1; \
-        nb = (int)((nlines + LPB - 1) / LPB);                  \
-        break;                                                 \
+        nb = (int)((nlines + LPB - 1) / LPB);                       \
+        break;                                                      \
     }
-        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
+    if (s->prec == SH_F64) {
+        switch (s->nx) { SH_FOR_EACH_FUSED64_SIZE(SH_CASE) }
+    } else {
+        switch (s->nx) { SH_FOR_EACH_FUSED_SIZE(SH_CASE) }
+    }
 #undef SH_CASE
-    }
     hipLaunchKernelGGL(k_abs_sum_partial, dim3(1), dim3(256), 0, s->st, s->divpart, nb, s->divpart + s->divblocks);
     SH_LAUNCH_CHECK("k_abs_sum_partial");
     double v = 0;
