@@ -57,6 +57,7 @@ def parse():
     p.add_argument("--no-fft", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=2)
     p.add_argument("--stable-dt", type=float, default=5e-4)
+    p.add_argument("--no-tune", action="store_true", help="use the default schedules")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run three 1 GiB device-to-device copies (the reference kernel for "
                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -79,6 +80,8 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT):
     ns.set_initial(V0)
     del V0
     ns.startup()
+    if TUNE:
+        L.sh_ns_autotune(ns.h, 3)
     ns.steps(warmup)
     torch.cuda.synchronize()
     if profile:
@@ -108,6 +111,8 @@ def fft_bench(S, n, reps):
     """exec/fft.cpp:52-58: min over repeats of forward() + inverse()."""
     import torch
     L = S.lib()
+    if TUNE:
+        L.sh_autotune_fft3d(n, n, n, S.SH_F64, 3, S.stream())
     x = torch.empty((n, n, n), dtype=torch.complex128, device="cuda")
     L.sh_prng_complex_z(S.ptr(x), x.numel(), 7, S.stream())
     best = 1e30
@@ -141,8 +146,19 @@ def cpu_baseline(n, steps):
     return steps / (t1 - t0), O.num_threads()
 
 
+TUNE = True
+
+
+def tune_report(L):
+    buf = ctypes.create_string_buffer(8192)
+    L.sh_tune_report(buf, 8192)
+    return [l for l in buf.value.decode().splitlines() if l]
+
+
 def main():
+    global TUNE
     a = parse()
+    TUNE = not a.no_tune
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU slab all-to-all is not part of this "
               "build (see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -248,6 +264,7 @@ def main():
         "kernels": kernels,
         "bytes": extra,
         "state": state,
+        "schedules": tune_report(S.lib()) if TUNE else "defaults",
     }
     print(json.dumps(out), flush=True)
 

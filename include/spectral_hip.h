@@ -165,6 +165,30 @@ int sh_ns_set_nonlinear(sh_ns_t handle, int on);
 /* device bytes held by the solver state */
 int sh_ns_state_bytes(sh_ns_t handle, size_t* bytes);
 
+/* ------------------------------------------------------------------ schedule autotuner
+ * Each pass role has a list of compiled radix/tile schedules that compute the
+ * same transform; the tuner times them on the device and the dispatchers use
+ * the fastest.  No counterpart in the reference (FFTW_MEASURE planning,
+ * src/FFT1D.cpp:18-22, plays this part there).  Choices persist per process. */
+enum {
+    SH_ROLE_X = 0,          /* x lines of a 3D transform (contiguous pass) */
+    SH_ROLE_Y = 1,          /* y lines (strided pass; also the solver's y passes) */
+    SH_ROLE_Z = 2,          /* z lines (strided pass) */
+    SH_ROLE_NS_CROSS = 3,   /* the solver's fused z / cross-product kernel */
+    SH_ROLE_NS_FOURIER = 4  /* the solver's fused x / Fourier-update kernel */
+};
+/* time the x, y, z schedules for an nx x ny x nz transform.  Synchronous. */
+int sh_autotune_fft3d(int nx, int ny, int nz, int precision, int reps, void* stream);
+/* time the solver's schedules on its own grid (reps steps each) and restore
+ * its state.  Synchronous. */
+int sh_ns_autotune(sh_ns_t handle, int reps);
+int sh_tune_variant_count(int precision, int n, int role, int* count);
+/* force a schedule (0 = default) */
+int sh_tune_set_variant(int precision, int n, int role, int variant);
+int sh_tune_reset(void);
+/* one line per tuned (precision, length, role): chosen schedule and its time */
+int sh_tune_report(char* buf, size_t len);
+
 /* ------------------------------------------------------------------ kernel timing
  * When enabled, every launch of the named kernel classes is bracketed with
  * HIP events on its own stream; totals are read after a synchronize. */

@@ -14,6 +14,9 @@
 // moves 3 x (read + write) x N^3 elements -- the algorithmic minimum for
 // three axis passes.  The optional epilogue (normalisation, Nyquist zeroing)
 // rides on the last pass.
+#include <map>
+#include <vector>
+
 #include "fft_core.hpp"
 
 namespace sh {
@@ -124,8 +127,63 @@ static int launch_strided(C2<T>* data, int ncols, long cdist, int nouter, long o
 }
 
 template <typename T>
+using ContigLaunch = int (*)(C2<T>*, long, long, int, long, int, const Epi<T>&, hipStream_t);
+template <typename T>
+using StridedLaunch = int (*)(C2<T>*, int, long, int, long, long, int, long, int, const Epi<T>&,
+                              hipStream_t);
+
+template <typename F>
+struct VariantList {
+    std::vector<const char*> names;
+    std::vector<F> fns;
+};
+
+// the tunable schedules of a length (empty when the length has only its default)
+template <typename T>
+const VariantList<ContigLaunch<T>>& contig_variants(int n) {
+    static const std::map<int, VariantList<ContigLaunch<T>>> tab = [] {
+        std::map<int, VariantList<ContigLaunch<T>>> m;
+#define SH_V(n_, P_, L_)                          \
+    m[n_].names.push_back(#P_ " lines=" #L_);     \
+    m[n_].fns.push_back(&launch_contig<T, P_, L_>);
+        SH_CONTIG_VARIANTS(SH_V)
+#undef SH_V
+        return m;
+    }();
+    static const VariantList<ContigLaunch<T>> none;
+    auto it = tab.find(n);
+    return it == tab.end() ? none : it->second;
+}
+
+template <typename T>
+const VariantList<StridedLaunch<T>>& strided_variants(int n) {
+    static const std::map<int, VariantList<StridedLaunch<T>>> tab = [] {
+        std::map<int, VariantList<StridedLaunch<T>>> m;
+#define SH_V(n_, P_, C_)                          \
+    m[n_].names.push_back(#P_ " columns=" #C_);   \
+    m[n_].fns.push_back(&launch_strided<T, P_, C_>);
+        SH_STRIDED_VARIANTS(SH_V)
+#undef SH_V
+        return m;
+    }();
+    static const VariantList<StridedLaunch<T>> none;
+    auto it = tab.find(n);
+    return it == tab.end() ? none : it->second;
+}
+
+template <typename T>
+constexpr int prec_of() {
+    return sizeof(T) == 8 ? SH_F64 : SH_F32;
+}
+
+template <typename T>
 int pass_contig(C2<T>* data, int n, long nlines, long ldist, int count, long fstride, int dir,
-                const Epi<T>& ep, hipStream_t st) {
+                const Epi<T>& ep, hipStream_t st, int role) {
+    const auto& vl = contig_variants<T>(n);
+    if (!vl.fns.empty() && role >= 0) {
+        const int v = tune_choice(prec_of<T>(), n, role);
+        if (v >= 0 && v < (int)vl.fns.size()) return vl.fns[v](data, nlines, ldist, count, fstride, dir, ep, st);
+    }
     switch (n) {
 #define SH_CASE(n_, P_, L_, C_) \
     case n_:                    \
@@ -140,7 +198,13 @@ int pass_contig(C2<T>* data, int n, long nlines, long ldist, int count, long fst
 
 template <typename T>
 int pass_strided(C2<T>* data, int n, int ncols, long cdist, int nouter, long odist, long es,
-                 int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st) {
+                 int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st, int role) {
+    const auto& vl = strided_variants<T>(n);
+    if (!vl.fns.empty() && role >= 0) {
+        const int v = tune_choice(prec_of<T>(), n, role);
+        if (v >= 0 && v < (int)vl.fns.size())
+            return vl.fns[v](data, ncols, cdist, nouter, odist, es, count, fstride, dir, ep, st);
+    }
     switch (n) {
 #define SH_CASE(n_, P_, L_, C_)                                                                 \
     case n_:                                                                                    \
@@ -155,13 +219,32 @@ int pass_strided(C2<T>* data, int n, int ncols, long cdist, int nouter, long odi
 }
 
 template int pass_contig<double>(double2*, int, long, long, int, long, int, const Epi<double>&,
-                                 hipStream_t);
+                                 hipStream_t, int);
 template int pass_contig<float>(float2*, int, long, long, int, long, int, const Epi<float>&,
-                                hipStream_t);
+                                hipStream_t, int);
 template int pass_strided<double>(double2*, int, int, long, int, long, long, int, long, int,
-                                  const Epi<double>&, hipStream_t);
+                                  const Epi<double>&, hipStream_t, int);
 template int pass_strided<float>(float2*, int, int, long, int, long, long, int, long, int,
-                                 const Epi<float>&, hipStream_t);
+                                 const Epi<float>&, hipStream_t, int);
+
+// number and names of the variants of a pass role at length n
+int pass_variant_count(int precision, int n, int role) {
+    if (role == ROLE_X)
+        return precision == SH_F64 ? (int)contig_variants<double>(n).fns.size()
+                                   : (int)contig_variants<float>(n).fns.size();
+    return precision == SH_F64 ? (int)strided_variants<double>(n).fns.size()
+                               : (int)strided_variants<float>(n).fns.size();
+}
+
+const char* pass_variant_name(int precision, int n, int role, int v) {
+    if (role == ROLE_X) {
+        const auto& vl = contig_variants<double>(n);
+        return v < (int)vl.names.size() ? vl.names[v] : "default";
+    }
+    const auto& vl = strided_variants<double>(n);
+    (void)precision;
+    return v < (int)vl.names.size() ? vl.names[v] : "default";
+}
 
 template <typename T>
 static int fft1d(void* data, int n, long stride, long dist, long batch, int dir, void* stream) {
@@ -178,12 +261,12 @@ static int fft1d(void* data, int n, long stride, long dist, long batch, int dir,
     if (rc) return rc;
     Epi<T> ep{T(1), 0, 1, -1, -1};
     hipStream_t st = as_stream(stream);
-    if (stride == 1) return pass_contig<T>((C2<T>*)data, n, batch, dist, 1, 0, dir, ep, st);
+    if (stride == 1) return pass_contig<T>((C2<T>*)data, n, batch, dist, 1, 0, dir, ep, st, -1);
     if (batch > 0x7fffffffL) {
         set_error("sh_fft1d: batch too large for a strided transform");
         return SH_EINVAL;
     }
-    return pass_strided<T>((C2<T>*)data, n, (int)batch, dist, 1, 0, stride, 1, 0, dir, ep, st);
+    return pass_strided<T>((C2<T>*)data, n, (int)batch, dist, 1, 0, stride, 1, 0, dir, ep, st, -1);
 }
 
 // 3D transform of `count` fields: x pass, y pass, z pass (+ epilogue).
@@ -207,12 +290,12 @@ int fft3d(void* data, int count, long fstride, int nx, int ny, int nz, int dir, 
     Epi<T> plain{T(1), 0, 1, -1, -1};
     {
         ProfScope ps("fft_x", st);
-        rc = pass_contig<T>(f, nx, (long)ny * nz, nx, count, fstride, dir, plain, st);
+        rc = pass_contig<T>(f, nx, (long)ny * nz, nx, count, fstride, dir, plain, st, ROLE_X);
     }
     if (rc) return rc;
     {
         ProfScope ps("fft_y", st);
-        rc = pass_strided<T>(f, ny, nx, 1, nz, sxy, nx, count, fstride, dir, plain, st);
+        rc = pass_strided<T>(f, ny, nx, 1, nz, sxy, nx, count, fstride, dir, plain, st, ROLE_Y);
     }
     if (rc) return rc;
     Epi<T> last{T(1), 0, 1, nx / 2, ny / 2};
@@ -223,7 +306,7 @@ int fft3d(void* data, int count, long fstride, int nx, int ny, int nz, int dir, 
     if (flags & SH_ZERO_NYQUIST) last.flags |= 2;
     {
         ProfScope ps("fft_z", st);
-        rc = pass_strided<T>(f, nz, nx, 1, ny, nx, sxy, count, fstride, dir, last, st);
+        rc = pass_strided<T>(f, nz, nx, 1, ny, nx, sxy, count, fstride, dir, last, st, ROLE_Z);
     }
     return rc;
 }

@@ -325,12 +325,15 @@ struct Epi {
 };
 
 // batched passes (fft3d.hip); dir = -1 forward, +1 backward
+// role: ROLE_X/Y/Z (tuned variant of that role) or -1 (the default schedule)
 template <typename T>
 int pass_contig(C2<T>* data, int n, long nlines, long ldist, int count, long fstride, int dir,
-                const Epi<T>& ep, hipStream_t st);
+                const Epi<T>& ep, hipStream_t st, int role);
 template <typename T>
 int pass_strided(C2<T>* data, int n, int ncols, long cdist, int nouter, long odist, long es,
-                 int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st);
+                 int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st, int role);
+int pass_variant_count(int precision, int n, int role);
+const char* pass_variant_name(int precision, int n, int role, int v);
 
 // FP64 plans for the fused z kernel, which holds 7 fields per thread at
 // once (3 velocity, 3 partial products, 1 in flight): at most 6 complex
@@ -363,5 +366,54 @@ inline bool fft_size_supported(int n) {
             return false;
     }
 }
+
+// ---------------------------------------------------------------- tunable variants
+// Alternative radix/tile schedules for the lengths the workloads use.  Every
+// variant computes the same transform; the autotuner (tune.hip) times them on
+// the device and the dispatchers use its choice (index 0 = the default above).
+using X512 = FftPlan<512, 32, 16, 2, 16>;
+
+// strided pass: X(n, plan, columns per block)
+#define SH_STRIDED_VARIANTS(X) \
+    X(64, P64, 16)             \
+    X(64, P64, 32)             \
+    X(64, W64, 16)             \
+    X(96, P96, 16)             \
+    X(96, P96, 32)             \
+    X(96, Q96, 16)             \
+    X(256, P256, 16)           \
+    X(256, P256, 8)            \
+    X(256, Q256, 16)           \
+    X(256, Q256, 8)            \
+    X(256, W256, 16)           \
+    X(512, P512, 8)            \
+    X(512, P512, 4)            \
+    X(512, X512, 8)            \
+    X(512, X512, 16)           \
+    X(512, W512, 8)
+
+// contiguous pass: X(n, plan, lines per block)
+#define SH_CONTIG_VARIANTS(X) \
+    X(64, P64, 32)            \
+    X(64, P64, 16)            \
+    X(64, W64, 16)            \
+    X(96, P96, 16)            \
+    X(96, Q96, 8)             \
+    X(256, P256, 16)          \
+    X(256, P256, 8)           \
+    X(256, Q256, 8)           \
+    X(256, W256, 4)           \
+    X(512, P512, 4)           \
+    X(512, P512, 2)           \
+    X(512, X512, 8)           \
+    X(512, X512, 4)           \
+    X(512, W512, 2)
+
+// pass roles, so y and z lines (different strides) are tuned separately
+enum { ROLE_X = 0, ROLE_Y = 1, ROLE_Z = 2, ROLE_CROSS = 3, ROLE_FOURIER = 4, NROLES = 5 };
+
+// chosen variant for (precision, length, role); 0 when untuned
+int tune_choice(int precision, int n, int role);
+void tune_set(int precision, int n, int role, int variant, double ms);
 
 }  // namespace sh

@@ -24,6 +24,7 @@
 // State (Fourier space): V1, N0 (= cross0), and six work fields W.
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "fft_core.hpp"
@@ -384,6 +385,7 @@ struct sh_ns_s {
     void *V1f, *N0f, *Wf;
     double* divpart;
     int divblocks;
+    int div_nb;  // partials written by the last kernel E
     bool have_div;
     bool w_primed;  // W holds the x-inverse of V1 and curl V1 for the next step
     double div_avg;
@@ -505,9 +507,8 @@ int launch_cross_z(sh_ns_s* s) {
     return SH_OK;
 }
 
-template <typename T, class P, bool EMIT>
+template <typename T, class P, int LPB>
 int launch_fourier_x(sh_ns_s* s) {
-    constexpr int LPB = 256 / P::TPL >= 1 ? 256 / P::TPL : 1;
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
     const long nlines = (long)s->ny * s->nz;
@@ -522,11 +523,91 @@ int launch_fourier_x(sh_ns_s* s) {
     cf.c1d = s->nu * s->K / 2;
     cf.inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT>), dim3(nb), dim3(P::TPL * LPB), 0, s->st,
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, true>), dim3(nb), dim3(P::TPL * LPB), 0, s->st,
                        (C2<T>*)s->Wf, (C2<T>*)s->V1f, (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N, cf,
                        s->divpart);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
+    s->div_nb = (int)nb;
     return SH_OK;
+}
+
+// ---------------------------------------------------------------- tunable fused schedules
+// X(n, plan, columns or lines per block); the first entry of a length is its
+// default.  Kernel C holds up to seven fields per thread and kernel E three
+// products plus the new V, so FP64 defaults to the 4-values-per-field plans.
+#define SH_CROSS_F64_VARIANTS(X)                                                                  \
+    X(8, P8, 16) X(12, Q12, 16) X(16, P16, 16) X(24, Q24, 16) X(32, Q32, 16) X(48, Q48, 16)       \
+    X(64, W64, 16) X(64, W64, 8) X(96, Q96, 16) X(96, Q96, 8) X(128, W128, 16) X(256, W256, 8)    \
+    X(256, W256, 4) X(256, Q256, 8) X(256, Q256, 4) X(512, W512, 4) X(512, W512, 2) X(512, P512, 4)
+#define SH_CROSS_F32_VARIANTS(X)                                                                  \
+    X(8, P8, 16) X(12, Q12, 16) X(16, P16, 16) X(24, Q24, 16) X(32, Q32, 16) X(48, Q48, 16)       \
+    X(64, P64, 16) X(64, W64, 16) X(96, Q96, 16) X(96, Q96, 8) X(128, Q128, 16) X(256, Q256, 16)  \
+    X(256, Q256, 8) X(256, W256, 16) X(256, W256, 8) X(512, P512, 8) X(512, P512, 4)              \
+    X(512, W512, 4)
+#define SH_FOURIER_F64_VARIANTS(X)                                                                \
+    X(8, P8, 256) X(12, Q12, 64) X(16, P16, 64) X(24, Q24, 32) X(32, Q32, 32) X(48, Q48, 21)      \
+    X(64, W64, 16) X(64, P64, 32) X(96, Q96, 10) X(128, W128, 8) X(256, W256, 4) X(256, Q256, 8)  \
+    X(256, W256, 2) X(512, W512, 2) X(512, P512, 4)
+#define SH_FOURIER_F32_VARIANTS(X)                                                                \
+    X(8, P8, 256) X(12, Q12, 64) X(16, P16, 64) X(24, Q24, 32) X(32, Q32, 32) X(48, Q48, 21)      \
+    X(64, P64, 32) X(64, W64, 16) X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4) \
+    X(256, Q256, 4) X(512, P512, 4) X(512, W512, 2)
+
+using NsLaunch = int (*)(sh_ns_s*);
+struct NsVariants {
+    std::vector<const char*> names;
+    std::vector<NsLaunch> fns;
+};
+
+template <typename T>
+const NsVariants& cross_variants(int n) {
+    static const std::map<int, NsVariants> tab = [] {
+        std::map<int, NsVariants> m;
+#define SH_V(n_, P_, C_)                           \
+    m[n_].names.push_back(#P_ " columns=" #C_);    \
+    m[n_].fns.push_back(&launch_cross_z<T, P_, C_>);
+        if constexpr (sizeof(T) == 8) {
+            SH_CROSS_F64_VARIANTS(SH_V)
+        } else {
+            SH_CROSS_F32_VARIANTS(SH_V)
+        }
+#undef SH_V
+        return m;
+    }();
+    static const NsVariants none;
+    auto it = tab.find(n);
+    return it == tab.end() ? none : it->second;
+}
+
+template <typename T>
+const NsVariants& fourier_variants(int n) {
+    static const std::map<int, NsVariants> tab = [] {
+        std::map<int, NsVariants> m;
+#define SH_V(n_, P_, L_)                         \
+    m[n_].names.push_back(#P_ " lines=" #L_);    \
+    m[n_].fns.push_back(&launch_fourier_x<T, P_, L_>);
+        if constexpr (sizeof(T) == 8) {
+            SH_FOURIER_F64_VARIANTS(SH_V)
+        } else {
+            SH_FOURIER_F32_VARIANTS(SH_V)
+        }
+#undef SH_V
+        return m;
+    }();
+    static const NsVariants none;
+    auto it = tab.find(n);
+    return it == tab.end() ? none : it->second;
+}
+
+template <typename T>
+int run_variant(const NsVariants& vl, sh_ns_s* s, int n, int role) {
+    if (vl.fns.empty()) {
+        set_error("ns: unsupported grid size");
+        return SH_EUNSUPPORTED;
+    }
+    int v = tune_choice(sizeof(T) == 8 ? SH_F64 : SH_F32, n, role);
+    if (v < 0 || v >= (int)vl.fns.size()) v = 0;
+    return vl.fns[v](s);
 }
 
 template <typename T>
@@ -540,37 +621,14 @@ int dispatch_inv_x(sh_ns_s* s) {
     }
     return SH_EUNSUPPORTED;
 }
-// The z kernel holds seven fields per thread; FP64 uses the small-register
-// plans (SH_FOR_EACH_FUSED64_SIZE), FP32 the common fused plans.
 template <typename T>
 int dispatch_cross_z(sh_ns_s* s) {
-#define SH_CASE(n_, P_, C_) \
-    case n_:                \
-        return launch_cross_z<T, P_, (C_ * P_::TPL <= kMaxBlk) ? C_ : kMaxBlk / P_::TPL>(s);
-    // FP64 needs ~200 VGPRs here: 512-thread blocks leave a 256-register budget
-    constexpr int kMaxBlk = sizeof(T) == 8 ? 512 : 1024;
-    if constexpr (sizeof(T) == 8) {
-        switch (s->nz) { SH_FOR_EACH_FUSED64_SIZE(SH_CASE) }
-    } else {
-        switch (s->nz) { SH_FOR_EACH_FUSED_SIZE(SH_CASE) }
-    }
-#undef SH_CASE
-    return SH_EUNSUPPORTED;
+    return run_variant<T>(cross_variants<T>(s->nz), s, s->nz, ROLE_CROSS);
 }
-// kernel E holds the three products and then the new V per thread: FP64
-// uses the small-register plans as kernel C does
+
 template <typename T>
 int dispatch_fourier_x(sh_ns_s* s) {
-#define SH_CASE(n_, P_, C_) \
-    case n_:                \
-        return launch_fourier_x<T, P_, true>(s);
-    if constexpr (sizeof(T) == 8) {
-        switch (s->nx) { SH_FOR_EACH_FUSED64_SIZE(SH_CASE) }
-    } else {
-        switch (s->nx) { SH_FOR_EACH_FUSED_SIZE(SH_CASE) }
-    }
-#undef SH_CASE
-    return SH_EUNSUPPORTED;
+    return run_variant<T>(fourier_variants<T>(s->nx), s, s->nx, ROLE_FOURIER);
 }
 
 template <typename T>
@@ -588,21 +646,105 @@ int step_fused(sh_ns_s* s) {
     }
     {
         ProfScope ps("ns_y_inv", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6, s->N, +1, plain, s->st));
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6, s->N, +1, plain, s->st, ROLE_Y));
     }
     RC(dispatch_cross_z<T>(s));
     {
         ProfScope ps("ns_y_fwd", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3, s->N, -1, plain, s->st));
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3, s->N, -1, plain, s->st, ROLE_Y));
     }
     RC(dispatch_fourier_x<T>(s));
     s->have_div = true;
     return SH_OK;
 }
 
+// time `reps` steps (after one warm step) of the solver as it is now configured
+int time_steps(sh_ns_s* s, int reps, hipEvent_t e0, hipEvent_t e1, double* ms) {
+    auto step = [&]() { return s->prec == SH_F64 ? step_fused<double>(s) : step_fused<float>(s); };
+    RC(step());
+    SH_HIP(hipEventRecord(e0, s->st));
+    for (int r = 0; r < reps; r++) RC(step());
+    SH_HIP(hipEventRecord(e1, s->st));
+    SH_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    SH_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    return SH_OK;
+}
+
 }  // namespace
 
+namespace sh {
+int ns_variant_count(int precision, int n, int role) {
+    if (role == ROLE_CROSS)
+        return precision == SH_F64 ? (int)cross_variants<double>(n).fns.size()
+                                   : (int)cross_variants<float>(n).fns.size();
+    return precision == SH_F64 ? (int)fourier_variants<double>(n).fns.size()
+                               : (int)fourier_variants<float>(n).fns.size();
+}
+
+const char* ns_variant_name(int precision, int n, int role, int v) {
+    const NsVariants& vl = role == ROLE_CROSS
+                               ? (precision == SH_F64 ? cross_variants<double>(n) : cross_variants<float>(n))
+                               : (precision == SH_F64 ? fourier_variants<double>(n) : fourier_variants<float>(n));
+    return v >= 0 && v < (int)vl.names.size() ? vl.names[v] : "default";
+}
+}  // namespace sh
+
 extern "C" {
+
+// Times every schedule of the solver's tunable kernels (y pass, fused z,
+// fused Fourier/x) on this solver's grid, keeps the fastest of each, and
+// restores the solver state afterwards.  Synchronous.
+int sh_ns_autotune(sh_ns_t s, int reps) {
+    if (!s || reps <= 0) return SH_EINVAL;
+    if (!s->fused) {
+        set_error("sh_ns_autotune: the op-by-op schedule has nothing to tune");
+        return SH_EINVAL;
+    }
+    const size_t e = s->prec == SH_F64 ? sizeof(double2) : sizeof(float2);
+    const size_t fb = e * (size_t)s->N;
+    void* save = nullptr;
+    RC(pool_alloc(&save, 12 * fb));
+    char* sv = (char*)save;
+    SH_HIP(hipMemcpyAsync(sv, s->V1f, 3 * fb, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(sv + 3 * fb, s->N0f, 3 * fb, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(sv + 6 * fb, s->Wf, 6 * fb, hipMemcpyDeviceToDevice, s->st));
+    const bool primed = s->w_primed;
+    hipEvent_t e0, e1;
+    SH_HIP(hipEventCreate(&e0));
+    SH_HIP(hipEventCreate(&e1));
+    int rc = SH_OK;
+    const int roles[3] = {ROLE_Y, ROLE_CROSS, ROLE_FOURIER};
+    const int lens[3] = {s->ny, s->nz, s->nx};
+    for (int k = 0; k < 3 && !rc; k++) {
+        const int role = roles[k], n = lens[k];
+        const int nv = role == ROLE_Y ? pass_variant_count(s->prec, n, role) : ns_variant_count(s->prec, n, role);
+        if (nv <= 1) continue;
+        int best = 0;
+        double best_ms = INFINITY;
+        for (int v = 0; v < nv && !rc; v++) {
+            tune_set(s->prec, n, role, v, 0);
+            double ms = 0;
+            rc = time_steps(s, reps, e0, e1, &ms);
+            if (!rc && ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        if (!rc) tune_set(s->prec, n, role, best, best_ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SH_HIP(hipMemcpyAsync(s->V1f, sv, 3 * fb, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(s->N0f, sv + 3 * fb, 3 * fb, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipMemcpyAsync(s->Wf, sv + 6 * fb, 6 * fb, hipMemcpyDeviceToDevice, s->st));
+    SH_HIP(hipStreamSynchronize(s->st));
+    pool_free(save);
+    s->w_primed = primed;
+    s->have_div = false;
+    return rc;
+}
 
 int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, int precision,
                  int fused, void* stream) {
@@ -633,6 +775,7 @@ int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, i
     s->device = current_device();
     s->have_div = false;
     s->w_primed = false;
+    s->div_nb = 0;
     s->div_avg = 0;
     const size_t z = sizeof(double2);
     int rc = 0;
@@ -750,22 +893,9 @@ int sh_ns_div_average(sh_ns_t s, double* result) {
         return SH_OK;
     }
     if (!s->fused) return sh_average_z(s->div, s->N, result, s->st);
-    const long nlines = (long)s->ny * s->nz;
-    // the fused x kernel wrote one partial per block; blocks = ceil(nlines / LPB) <= divblocks
-    int nb = 0;
-#define SH_CASE(n_, P_, C_)                                         \
-    case n_: {                                                      \
-        constexpr int LPB = 256 / P_::TPL >= 1 ? 256 / P_::TPL : 1; \
-        nb = (int)((nlines + LPB - 1) / LPB);                       \
-        break;                                                      \
-    }
-    if (s->prec == SH_F64) {
-        switch (s->nx) { SH_FOR_EACH_FUSED64_SIZE(SH_CASE) }
-    } else {
-        switch (s->nx) { SH_FOR_EACH_FUSED_SIZE(SH_CASE) }
-    }
-#undef SH_CASE
-    hipLaunchKernelGGL(k_abs_sum_partial, dim3(1), dim3(256), 0, s->st, s->divpart, nb, s->divpart + s->divblocks);
+    // the last kernel E wrote one partial per block (div_nb <= divblocks)
+    hipLaunchKernelGGL(k_abs_sum_partial, dim3(1), dim3(256), 0, s->st, s->divpart, s->div_nb,
+                       s->divpart + s->divblocks);
     SH_LAUNCH_CHECK("k_abs_sum_partial");
     double v = 0;
     SH_HIP(hipMemcpyAsync(&v, s->divpart + s->divblocks, sizeof(double), hipMemcpyDeviceToHost, s->st));

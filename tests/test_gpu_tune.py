@@ -1,0 +1,182 @@
+"""Every tunable schedule computes the same transform: each variant of each pass
+role is checked against the oracle (FFT bar 1e-12, step bar 1e-10), and the
+autotuners leave results (and the solver state) unchanged."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+ROLES_FFT = (0, 1, 2)  # x, y, z
+ROLE_CROSS, ROLE_FOURIER = 3, 4
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _count(L, prec, n, role):
+    c = ctypes.c_int()
+    L.sh_tune_variant_count(prec, n, role, ctypes.byref(c))
+    return c.value
+
+
+@pytest.mark.parametrize("n", [64, 96, 256])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fft_variants_vs_oracle(hip, n, prec):
+    L = hip.lib()
+    L.sh_tune_reset()
+    dims = (n, n, n)
+    x = O.prng_complex(500 + n, dims)
+    ref = O.transform3d(x, -1)
+    tol = 1e-12 if prec == 64 else 3e-6
+    try:
+        for role in ROLES_FFT:
+            for v in range(_count(L, prec, n, role)):
+                L.sh_tune_reset()
+                L.sh_tune_set_variant(prec, n, role, v)
+                t = _dev(x if prec == 64 else x.astype(np.complex64))
+                hip.fft3d(t, -1, 0)
+                err = rel_l2(_host(t).astype(np.complex128), ref)
+                print("fp%d n=%d role=%d variant=%d err=%.2e" % (prec, n, role, v, err), flush=True)
+                assert err < tol, (role, v, err)
+    finally:
+        L.sh_tune_reset()
+
+
+def test_fft_variants_512_round_trip(hip):
+    """512^3: each schedule against the default one, plus round trip."""
+    import torch
+    L = hip.lib()
+    L.sh_tune_reset()
+    n = 512
+    x = torch.empty((n, n, n), dtype=torch.complex128, device="cuda")
+    L.sh_prng_complex_z(hip.ptr(x), x.numel(), 3, hip.stream())
+    ref = x.clone()
+    hip.fft3d(ref, -1, 0)
+    try:
+        for role in ROLES_FFT:
+            for v in range(1, _count(L, 64, n, role)):
+                L.sh_tune_reset()
+                L.sh_tune_set_variant(64, n, role, v)
+                y = x.clone()
+                hip.fft3d(y, -1, 0)
+                err = float(torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref))
+                print("n=512 role=%d variant=%d err=%.2e" % (role, v, err), flush=True)
+                assert err < 1e-13, (role, v, err)
+                hip.fft3d(y, 1, hip.SH_NORMALIZE)
+                assert float(torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)) < 1e-13
+                del y
+    finally:
+        L.sh_tune_reset()
+        del x, ref
+        torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [64, 96])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ns_variants_vs_oracle(hip, n, prec):
+    import torch
+    L = hip.lib()
+    dims = (n, n, n)
+    V0 = O.solenoidal(dims, 31, 3.0, 1.0)
+    ref = O.NS(dims, 0.01, 1e-3)
+    ref.set_initial(V0)
+    ref.startup()
+    ref.steps(8)
+    Vr = ref.velocity()
+    tol = 1e-10 if prec == 64 else 1e-5
+    try:
+        for role in (1, ROLE_CROSS, ROLE_FOURIER):
+            for v in range(_count(L, prec, n, role)):
+                L.sh_tune_reset()
+                L.sh_tune_set_variant(prec, n, role, v)
+                ns = hip.NS(dims, 0.01, 1e-3, prec, fused=True)
+                ns.set_initial(_dev(V0))
+                ns.startup()
+                ns.steps(8)
+                err = rel_l2(_host(ns.velocity()), Vr)
+                ns.close()
+                print("fp%d n=%d role=%d variant=%d err=%.2e" % (prec, n, role, v, err), flush=True)
+                assert err < tol, (role, v, err)
+    finally:
+        L.sh_tune_reset()
+
+
+def test_ns_variants_256_agree(hip):
+    """At 256^3 each schedule of the fused kernels reproduces the default's 3 steps."""
+    import torch
+    L = hip.lib()
+    L.sh_tune_reset()
+    n = 256
+    V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    L.sh_solenoidal_ic_z(hip.ptr(V0), n, n, n, 9, 4.0, 1.0, hip.stream())
+
+    def run():
+        ns = hip.NS((n, n, n), 1 / 800, 5e-4, 64, fused=True)
+        ns.set_initial(V0)
+        ns.startup()
+        ns.steps(3)
+        V = ns.velocity()
+        ns.close()
+        return V
+
+    ref = run()
+    try:
+        for role in (1, ROLE_CROSS, ROLE_FOURIER):
+            for v in range(1, _count(L, 64, n, role)):
+                L.sh_tune_reset()
+                L.sh_tune_set_variant(64, n, role, v)
+                V = run()
+                err = float(torch.linalg.vector_norm(V - ref) / torch.linalg.vector_norm(ref))
+                print("n=256 role=%d variant=%d err=%.2e" % (role, v, err), flush=True)
+                assert err < 1e-13, (role, v, err)
+                del V
+    finally:
+        L.sh_tune_reset()
+        del V0, ref
+        torch.cuda.empty_cache()
+
+
+def test_autotuners_keep_results(hip):
+    import torch
+    L = hip.lib()
+    L.sh_tune_reset()
+    n = 64
+    dims = (n, n, n)
+    L.sh_autotune_fft3d(n, n, n, 64, 2, hip.stream())
+    x = O.prng_complex(77, dims)
+    t = _dev(x)
+    hip.fft3d(t, -1, 0)
+    assert rel_l2(_host(t), O.transform3d(x, -1)) < 1e-12
+    V0 = O.solenoidal(dims, 32, 3.0, 1.0)
+    ns = hip.NS(dims, 0.01, 1e-3, 64, fused=True)
+    ns.set_initial(_dev(V0))
+    ns.startup()
+    ns.steps(2)
+    before = _host(ns.velocity())
+    L.sh_ns_autotune(ns.h, 2)
+    assert np.array_equal(_host(ns.velocity()), before)
+    ns.steps(4)
+    ref = O.NS(dims, 0.01, 1e-3)
+    ref.set_initial(V0)
+    ref.startup()
+    ref.steps(6)
+    assert rel_l2(_host(ns.velocity()), ref.velocity()) < 1e-10
+    buf = ctypes.create_string_buffer(4096)
+    L.sh_tune_report(buf, 4096)
+    report = buf.value.decode()
+    print(report)
+    assert "ns_cross_z" in report and "n=64" in report
+    ns.close()
+    L.sh_tune_reset()
