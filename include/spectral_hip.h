@@ -165,6 +165,35 @@ int sh_ns_set_nonlinear(sh_ns_t handle, int on);
 /* device bytes held by the solver state */
 int sh_ns_state_bytes(sh_ns_t handle, size_t* bytes);
 
+/* ------------------------------------------------------------------ ensembles of equal boxes
+ * No counterpart in the reference (one domain per run there).  nboxes boxes
+ * of n^3 points share every launch of the fused step; box b has viscosity
+ * nu[b] (host array), all share the time step K.  Boxes of different sizes
+ * are separate handles created on one stream and captured into one graph. */
+int sh_ns_create_ensemble(sh_ns_t* handle, int n, int nboxes, const double* nu, double K,
+                          int precision, void* stream);
+int sh_ns_box_count(sh_ns_t handle, int* count);
+/* V0 of box `box` (3 contiguous complex128 Fourier fields), then that box's
+ * start-up (exec/ns.cpp:132-153) */
+int sh_ns_set_initial_box(sh_ns_t handle, int box, const void* V0);
+int sh_ns_get_velocity_box(sh_ns_t handle, int box, void* out_v3);
+int sh_ns_div_average_box(sh_ns_t handle, int box, double* result);
+/* Box-filtered training snapshot of every box: physical velocity averaged
+ * over ratio^3 cells.  out: float[nboxes][9][(n/ratio)^3] holding ubar_x,
+ * ubar_y, ubar_z, tau_xx, tau_yy, tau_zz, tau_xy, tau_xz, tau_yz with
+ * tau_ij = <u_i u_j> - ubar_i ubar_j.  The first call allocates scratch. */
+int sh_ns_snapshot(sh_ns_t handle, int ratio, void* out);
+
+/* ------------------------------------------------------------------ streams and graphs
+ * Default-priority non-blocking streams; stream capture into a hipGraph
+ * (launch everything enqueued between begin and end as one graph). */
+int sh_stream_create(void** stream);
+int sh_stream_destroy(void* stream);
+int sh_graph_begin(void* stream);
+int sh_graph_end(void* stream, void** graph);
+int sh_graph_launch(void* graph, void* stream);
+int sh_graph_destroy(void* graph);
+
 /* ------------------------------------------------------------------ schedule autotuner
  * Each pass role has a list of compiled radix/tile schedules that compute the
  * same transform; the tuner times them on the device and the dispatchers use

@@ -23,6 +23,7 @@ _CTYPES = {
     "size_t": ctypes.c_size_t,
     "uint64_t": ctypes.c_uint64,
     "double*": ctypes.POINTER(ctypes.c_double),
+    "const double*": ctypes.POINTER(ctypes.c_double),
     "long*": ctypes.POINTER(ctypes.c_long),
     "int*": ctypes.POINTER(ctypes.c_int),
     "size_t*": ctypes.POINTER(ctypes.c_size_t),

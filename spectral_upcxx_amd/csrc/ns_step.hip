@@ -50,12 +50,19 @@ struct LdsLine {
     static constexpr int LS = N + N / 16 + 1;
 };
 
+// Layout of an ensemble of B equal boxes (B = 1 for a single solver): field f
+// of box b starts at (f*B + b)*Nx*Ny*Nz, so F below is the field stride B*N
+// and box b is blockIdx.y, offset b*N.  The y passes then cover all boxes of a
+// field group in one strided launch.
+
 // A: inverse x-pass of V (3 fields) and omega = i k x V (3 fields) into W.
 template <typename T, class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB * 6) void k_ns_inv_x(const C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ W,
                                                               const C2<T>* __restrict__ tw,
-                                                              int ny, int nz, long F) {
+                                                              int ny, int nz, long F, long boff) {
+    V += blockIdx.y * boff;
+    W += blockIdx.y * boff;
     constexpr int N = P::N;
     constexpr int LS = LdsLine<N>::LS;
     __shared__ T sre[6 * LPB * LS];
@@ -98,9 +105,10 @@ __global__ __launch_bounds__(P::TPL* LPB * 6) void k_ns_inv_x(const C2<T>* __res
 template <typename T, class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ W,
                                                             const C2<T>* __restrict__ tw, int nx,
-                                                            int ny, long F, T scale) {
+                                                            int ny, long F, T scale, long boff) {
     static_assert(P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1),
                   "fused z kernel needs a symmetric plan");
+    W += blockIdx.y * boff;
     constexpr int CS = LPB + 1;
     constexpr int E = P::E;
     __shared__ T sre[P::N * CS];
@@ -183,10 +191,18 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
                                                               const C2<T>* __restrict__ tw, int ny,
-                                                              int nz, long F, StepCoef<T> cf,
-                                                              double* __restrict__ divpart) {
+                                                              int nz, long F,
+                                                              const StepCoef<T>* __restrict__ cfs,
+                                                              double* __restrict__ divpart,
+                                                              long boff, int divstride) {
     static_assert(!EMIT || (P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1)),
                   "emitting the next inverse pass needs a symmetric plan");
+    // per-box coefficients (each box of an ensemble has its own viscosity)
+    const StepCoef<T> cf = cfs[blockIdx.y];
+    W += blockIdx.y * boff;
+    V += blockIdx.y * boff;
+    N0 += blockIdx.y * boff;
+    divpart += (long)blockIdx.y * divstride;
     constexpr int N = P::N;
     constexpr int LS = LdsLine<N>::LS;
     constexpr int E = P::E;
@@ -345,6 +361,46 @@ __global__ void k_abs_sum_partial(const double* __restrict__ part, int np, doubl
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// Box filter of ratio r on physical velocity u (3 fields, real parts): each
+// coarse cell gets ubar_i = <u_i> and the subgrid stress
+// tau_ij = <u_i u_j> - ubar_i ubar_j over its r^3 fine cells.  Output: 9
+// float fields (ubar x y z, tau xx yy zz xy xz yz) of nc coarse cells.
+// Box b is blockIdx.y: its field d is at u + (d*nboxes + b)*N (field stride
+// fs = nboxes*N) and its output at out + b*9*nc.
+template <typename T>
+__global__ void k_box_filter(const C2<T>* __restrict__ u, long N, long fs, int nx, int ny, int r,
+                             int cx, int cy, long nc, float* __restrict__ out) {
+    u += blockIdx.y * N;
+    out += blockIdx.y * 9 * nc;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nc; c += (long)gridDim.x * blockDim.x) {
+        const int I = (int)(c % cx);
+        const long rr = c / cx;
+        const int J = (int)(rr % cy);
+        const int K = (int)(rr / cy);
+        double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int dk = 0; dk < r; dk++)
+            for (int dj = 0; dj < r; dj++)
+                for (int di = 0; di < r; di++) {
+                    const long n = (long)(I * r + di) + (long)nx * ((long)(J * r + dj) + (long)ny * (K * r + dk));
+                    const double a = (double)u[n].x, b = (double)u[fs + n].x, d = (double)u[2 * fs + n].x;
+                    s[0] += a; s[1] += b; s[2] += d;
+                    s[3] += a * a; s[4] += b * b; s[5] += d * d;
+                    s[6] += a * b; s[7] += a * d; s[8] += b * d;
+                }
+        const double inv = 1.0 / ((double)r * r * r);
+        const double ua = s[0] * inv, ub = s[1] * inv, ud = s[2] * inv;
+        out[c] = (float)ua;
+        out[nc + c] = (float)ub;
+        out[2 * nc + c] = (float)ud;
+        out[3 * nc + c] = (float)(s[3] * inv - ua * ua);
+        out[4 * nc + c] = (float)(s[4] * inv - ub * ub);
+        out[5 * nc + c] = (float)(s[5] * inv - ud * ud);
+        out[6 * nc + c] = (float)(s[6] * inv - ua * ub);
+        out[7 * nc + c] = (float)(s[7] * inv - ua * ud);
+        out[8 * nc + c] = (float)(s[8] * inv - ub * ud);
+    }
+}
+
 static unsigned egrid(long n) {
     long g = (n + 255) / 256;
     if (g > 262144) g = 262144;
@@ -381,9 +437,14 @@ struct sh_ns_s {
     double2 *V0, *V1, *om0, *om1, *cr0, *cr1;  // 3 fields each
     double2 *Vn, *omn, *Fv, *lap1, *S, *Gg, *Vh, *work;  // work: 6 fields
     double2 *G, *div, *Pi0;
-    // fused state (precision prec): V1f, N0f (3 fields each), Wf (6 fields)
+    // fused state (precision prec): V1f, N0f (3 fields each), Wf (6 fields),
+    // for nboxes equal boxes laid out field-major: field f of box b at (f*nboxes + b)*N
     void *V1f, *N0f, *Wf;
-    double* divpart;
+    int nboxes;
+    std::vector<double> nus;  // viscosity of each box
+    void* coefs;              // StepCoef<T>[nboxes] on the device
+    void* snap;               // 3 physical fields for snapshots (allocated on first use)
+    double* divpart;          // [nboxes][divblocks + 1]
     int divblocks;
     int div_nb;  // partials written by the last kernel E
     bool have_div;
@@ -481,10 +542,10 @@ int launch_inv_x(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
     const long nlines = (long)s->ny * s->nz;
-    dim3 grid((unsigned)((nlines + LPB - 1) / LPB)), block(P::TPL * LPB * 6);
+    dim3 grid((unsigned)((nlines + LPB - 1) / LPB), (unsigned)s->nboxes), block(P::TPL * LPB * 6);
     ProfScope ps("ns_inv_x", s->st);
     hipLaunchKernelGGL((k_ns_inv_x<T, P, LPB>), grid, block, 0, s->st, (const C2<T>*)s->V1f,
-                       (C2<T>*)s->Wf, tw, s->ny, s->nz, s->N);
+                       (C2<T>*)s->Wf, tw, s->ny, s->nz, s->N * s->nboxes, s->N);
     SH_LAUNCH_CHECK("k_ns_inv_x");
     return SH_OK;
 }
@@ -493,16 +554,16 @@ template <typename T, class P, int LPB>
 int launch_cross_z(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
-    if (6 * s->N > 0xffffffffL) {  // the kernel addresses W with 32-bit element offsets
+    if (6 * s->N * s->nboxes > 0xffffffffL) {  // the kernel addresses W with 32-bit element offsets
         set_error("ns: grid too large for the fused z kernel");
         return SH_EINVAL;
     }
     const int tiles = (s->nx + LPB - 1) / LPB;
-    dim3 grid((unsigned)(tiles * s->ny)), block(P::TPL * LPB);
+    dim3 grid((unsigned)(tiles * s->ny), (unsigned)s->nboxes), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
     ProfScope ps("ns_cross_z", s->st);
     hipLaunchKernelGGL((k_ns_cross_z<T, P, LPB>), grid, block, 0, s->st, (C2<T>*)s->Wf, tw, s->nx,
-                       s->ny, s->N, scale);
+                       s->ny, s->N * s->nboxes, scale, s->N);
     SH_LAUNCH_CHECK("k_ns_cross_z");
     return SH_OK;
 }
@@ -517,15 +578,11 @@ int launch_fourier_x(sh_ns_s* s) {
         set_error("ns: divergence partial buffer too small");
         return SH_EINVAL;
     }
-    StepCoef<T> cf;
-    cf.Kh = T(s->K / 2);
-    cf.c1 = T(s->nu * s->K / 2);
-    cf.c1d = s->nu * s->K / 2;
-    cf.inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, true>), dim3(nb), dim3(P::TPL * LPB), 0, s->st,
-                       (C2<T>*)s->Wf, (C2<T>*)s->V1f, (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N, cf,
-                       s->divpart);
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, true>), dim3(nb, (unsigned)s->nboxes),
+                       dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
+                       (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N * s->nboxes,
+                       (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
     return SH_OK;
@@ -646,12 +703,14 @@ int step_fused(sh_ns_s* s) {
     }
     {
         ProfScope ps("ns_y_inv", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6, s->N, +1, plain, s->st, ROLE_Y));
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6 * s->nboxes, s->N, +1,
+                           plain, s->st, ROLE_Y));
     }
     RC(dispatch_cross_z<T>(s));
     {
         ProfScope ps("ns_y_fwd", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3, s->N, -1, plain, s->st, ROLE_Y));
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3 * s->nboxes, s->N, -1,
+                           plain, s->st, ROLE_Y));
     }
     RC(dispatch_fourier_x<T>(s));
     s->have_div = true;
@@ -703,7 +762,7 @@ int sh_ns_autotune(sh_ns_t s, int reps) {
         return SH_EINVAL;
     }
     const size_t e = s->prec == SH_F64 ? sizeof(double2) : sizeof(float2);
-    const size_t fb = e * (size_t)s->N;
+    const size_t fb = e * (size_t)s->N * s->nboxes;  // one field of every box
     void* save = nullptr;
     RC(pool_alloc(&save, 12 * fb));
     char* sv = (char*)save;
@@ -746,12 +805,61 @@ int sh_ns_autotune(sh_ns_t s, int reps) {
     return rc;
 }
 
-int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, int precision,
-                 int fused, void* stream) {
-    if (!handle || nx <= 0 || ny <= 0 || nz <= 0 || (precision != SH_F64 && precision != SH_F32)) {
+}  // extern "C"
+
+namespace {
+
+// per-box step coefficients on the device (exec/ns.cpp:171, 174, 186)
+template <typename T>
+int upload_coefs_t(sh_ns_s* s) {
+    std::vector<StepCoef<T>> h(s->nboxes);
+    for (int b = 0; b < s->nboxes; b++) {
+        const double nu = s->nus[b];
+        h[b].Kh = T(s->K / 2);
+        h[b].c1 = T(nu * s->K / 2);
+        h[b].c1d = nu * s->K / 2;
+        h[b].inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
+    }
+    SH_HIP(hipMemcpyAsync(s->coefs, h.data(), sizeof(StepCoef<T>) * s->nboxes, hipMemcpyHostToDevice, s->st));
+    SH_HIP(hipStreamSynchronize(s->st));
+    return SH_OK;
+}
+
+int upload_coefs(sh_ns_s* s) { return s->prec == SH_F64 ? upload_coefs_t<double>(s) : upload_coefs_t<float>(s); }
+
+// copy the FP64 start-up result (V1, cross0) into box b of the fused state
+int install_box(sh_ns_s* s, int b) {
+    const long N = s->N;
+    for (int d = 0; d < 3; d++) {
+        const long off = ((long)d * s->nboxes + b) * N;
+        if (s->prec == SH_F64) {
+            SH_HIP(hipMemcpyAsync((double2*)s->V1f + off, s->V1 + d * N, sizeof(double2) * N, hipMemcpyDeviceToDevice, s->st));
+            SH_HIP(hipMemcpyAsync((double2*)s->N0f + off, s->cr0 + d * N, sizeof(double2) * N, hipMemcpyDeviceToDevice, s->st));
+        } else {
+            hipLaunchKernelGGL(k_z_to_c, dim3(egrid(N)), dim3(256), 0, s->st, s->V1 + d * N, (float2*)s->V1f + off, N);
+            hipLaunchKernelGGL(k_z_to_c, dim3(egrid(N)), dim3(256), 0, s->st, s->cr0 + d * N, (float2*)s->N0f + off, N);
+            SH_LAUNCH_CHECK("k_z_to_c");
+        }
+    }
+    return SH_OK;
+}
+
+int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double* nus, double K,
+              int precision, int fused, void* stream) {
+    if (!handle || nx <= 0 || ny <= 0 || nz <= 0 || nboxes <= 0 || !nus ||
+        (precision != SH_F64 && precision != SH_F32)) {
         set_error("sh_ns_create: invalid argument");
         return SH_EINVAL;
     }
+    if (nboxes > 1 && !fused) {
+        set_error("sh_ns_create_ensemble: ensembles use the fused schedule");
+        return SH_EINVAL;
+    }
+    if (nboxes > 65535) {
+        set_error("sh_ns_create_ensemble: too many boxes");
+        return SH_EINVAL;
+    }
+    const double nu = nus[0];
     if (!fft_size_supported(nx) || !fft_size_supported(ny) || !fft_size_supported(nz)) {
         set_error("sh_ns_create: unsupported grid size");
         return SH_EUNSUPPORTED;
@@ -777,6 +885,10 @@ int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, i
     s->w_primed = false;
     s->div_nb = 0;
     s->div_avg = 0;
+    s->nboxes = nboxes;
+    s->nus.assign(nus, nus + nboxes);
+    s->coefs = nullptr;
+    s->snap = nullptr;
     const size_t z = sizeof(double2);
     int rc = 0;
     // FP64 state used by the start-up (and by the op-by-op loop)
@@ -794,12 +906,15 @@ int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, i
 #undef AL
     if (!rc && fused) {
         const size_t e = precision == SH_F64 ? sizeof(double2) : sizeof(float2);
-        rc = alloc_fields(s, &s->V1f, 3, e);
-        if (!rc) rc = alloc_fields(s, &s->N0f, 3, e);
-        if (!rc) rc = alloc_fields(s, &s->Wf, 6, e);
+        rc = alloc_fields(s, &s->V1f, 3 * nboxes, e);
+        if (!rc) rc = alloc_fields(s, &s->N0f, 3 * nboxes, e);
+        if (!rc) rc = alloc_fields(s, &s->Wf, 6 * nboxes, e);
         s->divblocks = (int)((long)ny * nz + 1);  // >= blocks of the fused x kernel (>= 1 line per block)
-        if (!rc) rc = pool_alloc((void**)&s->divpart, sizeof(double) * (s->divblocks + 1));
+        if (!rc) rc = pool_alloc((void**)&s->divpart, sizeof(double) * ((size_t)(s->divblocks + 1) * nboxes + 1));
         if (!rc) s->owned.push_back(s->divpart);
+        if (!rc) rc = pool_alloc(&s->coefs, sizeof(StepCoef<double>) * nboxes);
+        if (!rc) s->owned.push_back(s->coefs);
+        if (!rc) rc = upload_coefs(s);
     }
     if (rc) {
         for (void* q : s->owned) pool_free(q);
@@ -807,6 +922,115 @@ int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, i
         return rc;
     }
     *handle = s;
+    return SH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, int precision,
+                 int fused, void* stream) {
+    return ns_create(handle, nx, ny, nz, 1, &nu, K, precision, fused, stream);
+}
+
+int sh_ns_create_ensemble(sh_ns_t* handle, int n, int nboxes, const double* nu, double K,
+                          int precision, void* stream) {
+    return ns_create(handle, n, n, n, nboxes, nu, K, precision, 1, stream);
+}
+
+int sh_ns_box_count(sh_ns_t s, int* count) {
+    if (!s || !count) return SH_EINVAL;
+    *count = s->nboxes;
+    return SH_OK;
+}
+
+// V0 of box b (Fourier, 3 contiguous complex128 fields) followed by that box's
+// start-up (exec/ns.cpp:132-153) with its own viscosity
+int sh_ns_set_initial_box(sh_ns_t s, int box, const void* V0) {
+    if (!s || !V0 || box < 0 || box >= s->nboxes) return SH_EINVAL;
+    SH_HIP(hipMemcpyAsync(s->V0, V0, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
+    RC(curl(s, s->V0, s->om0));
+    const double nu0 = s->nu;
+    s->nu = s->nus[box];
+    int rc = startup64(s);
+    s->nu = nu0;
+    if (rc) return rc;
+    s->w_primed = false;
+    return install_box(s, box);
+}
+
+int sh_ns_get_velocity_box(sh_ns_t s, int box, void* out) {
+    if (!s || !out || box < 0 || box >= s->nboxes || !s->fused) return SH_EINVAL;
+    const long N = s->N;
+    for (int d = 0; d < 3; d++) {
+        const long off = ((long)d * s->nboxes + box) * N;
+        if (s->prec == SH_F64) {
+            SH_HIP(hipMemcpyAsync((double2*)out + d * N, (double2*)s->V1f + off, sizeof(double2) * N,
+                                  hipMemcpyDeviceToDevice, s->st));
+        } else {
+            hipLaunchKernelGGL(k_c_to_z, dim3(egrid(N)), dim3(256), 0, s->st, (const float2*)s->V1f + off,
+                               (double2*)out + d * N, N);
+            SH_LAUNCH_CHECK("k_c_to_z");
+        }
+    }
+    return SH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+int snapshot_t(sh_ns_s* s, int r, float* out) {
+    const long N = s->N;
+    const int cx = s->nx / r, cy = s->ny / r, cz = s->nz / r;
+    const long nc = (long)cx * cy * cz;
+    // all boxes at once: one copy, one 3-pass inverse transform, one filter launch
+    C2<T>* snap = (C2<T>*)s->snap;
+    const long nf = 3L * s->nboxes;
+    SH_HIP(hipMemcpyAsync(snap, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
+    RC(fft3d<T>(snap, (int)nf, N, s->nx, s->ny, s->nz, +1, SH_NORMALIZE, s->st));
+    unsigned gx = egrid(nc);
+    hipLaunchKernelGGL(k_box_filter<T>, dim3(gx, (unsigned)s->nboxes), dim3(256), 0, s->st, snap, N,
+                       N * s->nboxes, s->nx, s->ny, r, cx, cy, nc, out);
+    SH_LAUNCH_CHECK("k_box_filter");
+    return SH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// Box-filtered snapshot of every box (training data): physical velocity by
+// an inverse transform, then ratio^3 averages.  out: float[nboxes][9][nc],
+// nc = (nx/ratio)(ny/ratio)(nz/ratio); fields ubar_x ubar_y ubar_z and
+// tau_xx tau_yy tau_zz tau_xy tau_xz tau_yz.  The first call allocates a
+// scratch buffer, so make it before any graph capture.
+int sh_ns_snapshot(sh_ns_t s, int ratio, void* out) {
+    if (!s || !out || !s->fused || ratio <= 0 || s->nx % ratio || s->ny % ratio || s->nz % ratio) {
+        set_error("sh_ns_snapshot: invalid argument (fused solver, ratio dividing the grid)");
+        return SH_EINVAL;
+    }
+    if (!s->snap) {
+        const size_t e = s->prec == SH_F64 ? sizeof(double2) : sizeof(float2);
+        RC(pool_alloc(&s->snap, 3 * e * (size_t)s->N * s->nboxes));
+        s->owned.push_back(s->snap);
+    }
+    return s->prec == SH_F64 ? snapshot_t<double>(s, ratio, (float*)out) : snapshot_t<float>(s, ratio, (float*)out);
+}
+
+int sh_ns_div_average_box(sh_ns_t s, int box, double* result) {
+    if (!s || !result || box < 0 || box >= s->nboxes || !s->fused) return SH_EINVAL;
+    if (!s->have_div) {
+        *result = 0;
+        return SH_OK;
+    }
+    double* part = s->divpart + (long)box * (s->divblocks + 1);
+    hipLaunchKernelGGL(k_abs_sum_partial, dim3(1), dim3(256), 0, s->st, part, s->div_nb, part + s->divblocks);
+    SH_LAUNCH_CHECK("k_abs_sum_partial");
+    double v = 0;
+    SH_HIP(hipMemcpyAsync(&v, part + s->divblocks, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    SH_HIP(hipStreamSynchronize(s->st));
+    *result = v / (double)s->N;
     return SH_OK;
 }
 
@@ -832,19 +1056,13 @@ int sh_ns_set_initial(sh_ns_t s, const void* V0) {
 
 int sh_ns_startup(sh_ns_t s) {
     if (!s) return SH_EINVAL;
+    if (s->nboxes > 1) {
+        set_error("sh_ns_startup: an ensemble starts each box in sh_ns_set_initial_box");
+        return SH_EINVAL;
+    }
     s->w_primed = false;
     RC(startup64(s));
-    if (s->fused) {
-        const long n3 = 3 * s->N;
-        if (s->prec == SH_F64) {
-            SH_HIP(hipMemcpyAsync(s->V1f, s->V1, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
-            SH_HIP(hipMemcpyAsync(s->N0f, s->cr0, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
-        } else {
-            hipLaunchKernelGGL(k_z_to_c, dim3(egrid(n3)), dim3(256), 0, s->st, s->V1, (float2*)s->V1f, n3);
-            hipLaunchKernelGGL(k_z_to_c, dim3(egrid(n3)), dim3(256), 0, s->st, s->cr0, (float2*)s->N0f, n3);
-            SH_LAUNCH_CHECK("k_z_to_c");
-        }
-    }
+    if (s->fused) return install_box(s, 0);
     return SH_OK;
 }
 

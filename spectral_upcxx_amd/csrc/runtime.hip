@@ -324,6 +324,53 @@ int sh_prof_query(const char* name, double* total_ms, long* launches) {
     return SH_OK;
 }
 
+int sh_stream_create(void** stream) {
+    if (!stream) return SH_EINVAL;
+    hipStream_t s = nullptr;
+    SH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+    *stream = s;
+    return SH_OK;
+}
+
+int sh_stream_destroy(void* stream) {
+    if (!stream) return SH_OK;
+    SH_HIP(hipStreamDestroy(as_stream(stream)));
+    return SH_OK;
+}
+
+int sh_graph_begin(void* stream) {
+    if (!stream) {
+        set_error("sh_graph_begin: the default stream cannot be captured; use sh_stream_create");
+        return SH_EINVAL;
+    }
+    SH_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+    return SH_OK;
+}
+
+int sh_graph_end(void* stream, void** graph) {
+    if (!stream || !graph) return SH_EINVAL;
+    hipGraph_t g = nullptr;
+    SH_HIP(hipStreamEndCapture(as_stream(stream), &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph = ge;
+    return SH_OK;
+}
+
+int sh_graph_launch(void* graph, void* stream) {
+    if (!graph) return SH_EINVAL;
+    SH_HIP(hipGraphLaunch((hipGraphExec_t)graph, as_stream(stream)));
+    return SH_OK;
+}
+
+int sh_graph_destroy(void* graph) {
+    if (!graph) return SH_OK;
+    SH_HIP(hipGraphExecDestroy((hipGraphExec_t)graph));
+    return SH_OK;
+}
+
 int sh_prof_names(char* buf, size_t len) {
     if (!buf || len == 0) return SH_EINVAL;
     std::lock_guard<std::mutex> lk(g_mu);

@@ -1,0 +1,273 @@
+"""Harnesses for the solver's forms, driving the C ABI (device buffers via torch).
+
+  python -m spectral_upcxx_amd.forms fp32 --n 512 --steps 100 --dt 2.5e-4
+
+fp32: the FP32 solver (complex64 transforms and operators; FP64 twiddles, CN
+factors and reductions) against the FP64 plain method from the same seed.
+Reports the relative L2 velocity error, the shell-spectrum error and the
+dissipation-rate error, and both solvers' steps/s.
+"""
+import argparse
+import ctypes
+import json
+import sys
+import time
+
+from . import lib, ptr, stream, NS, SH_F32, SH_F64
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def solenoidal(n, seed, k0=4.0, urms=1.0):
+    torch = _torch()
+    V = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    lib().sh_solenoidal_ic_z(ptr(V), n, n, n, int(seed), float(k0), float(urms), stream())
+    return V
+
+
+def spectrum(V, nu, nbins):
+    """Shell spectrum E(k), total energy and dissipation 2 nu sum k^2 E (FP64)."""
+    nz, ny, nx = V.shape[-3:]
+    Ek = (ctypes.c_double * nbins)()
+    e = ctypes.c_double()
+    d = ctypes.c_double()
+    lib().sh_spectrum_z(ptr(V), nx, ny, nz, float(nu), Ek, nbins, ctypes.byref(e), ctypes.byref(d), stream())
+    return list(Ek), e.value, d.value
+
+
+def run_solver(V0, nu, dt, steps, precision, tune=True):
+    torch = _torch()
+    n = V0.shape[-1]
+    ns = NS((n, n, n), nu, dt, precision, fused=True)
+    ns.set_initial(V0)
+    ns.startup()
+    if tune:
+        lib().sh_ns_autotune(ns.h, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ns.steps(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    V = ns.velocity()
+    ns.close()
+    lib().sh_pool_trim()
+    return V, steps / (t1 - t0)
+
+
+def fp32_error(n=512, steps=100, dt=2.5e-4, nu=1.0 / 800, seed=2024, k0=4.0):
+    torch = _torch()
+    V0 = solenoidal(n, seed, k0)
+    V64, sps64 = run_solver(V0, nu, dt, steps, SH_F64)
+    V32, sps32 = run_solver(V0, nu, dt, steps, SH_F32)
+    del V0
+    nb = n // 2
+    E64, e64, d64 = spectrum(V64, nu, nb)
+    E32, e32, d32 = spectrum(V32, nu, nb)
+    rel_v = float(torch.linalg.vector_norm(V32 - V64) / torch.linalg.vector_norm(V64))
+    shells = [k for k in range(1, nb) if E64[k] > 1e-12 * max(E64)]
+    rel_ek = [abs(E32[k] - E64[k]) / E64[k] for k in shells]
+    k16 = [abs(E32[k] - E64[k]) / E64[k] for k in shells if k <= 16]
+    del V64, V32
+    torch.cuda.empty_cache()
+    return {
+        "form": "fp32", "grid": n, "steps": steps, "dt": dt, "nu": nu, "seed": seed, "k0": k0,
+        "velocity_rel_l2": rel_v,
+        "spectrum_max_rel_err": max(rel_ek), "spectrum_max_rel_err_k_le_16": max(k16),
+        "spectrum_shells_compared": len(shells),
+        "energy_fp64": e64, "energy_fp32": e32, "energy_rel_err": abs(e32 - e64) / e64,
+        "dissipation_fp64": d64, "dissipation_fp32": d32, "dissipation_rel_err": abs(d32 - d64) / d64,
+        "steps_per_s_fp64": sps64, "steps_per_s_fp32": sps32,
+    }
+
+
+def _u01(seed, counter):
+    """splitmix64 draw (the PRNG of the ICs), U[0,1)."""
+    m = (1 << 64) - 1
+    z = (seed + (counter + 1) * 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return (z >> 11) * 2.0 ** -53
+
+
+def ensemble_params(groups, seed):
+    """Per-box (seed, k0 ~ U[2,6], nu ~ U[1/1000, 1/300]) for groups [(n, nboxes)]."""
+    out = []
+    c = 0
+    for n, nb in groups:
+        boxes = []
+        for b in range(nb):
+            k0 = 2.0 + 4.0 * _u01(seed, 2 * c)
+            nu = 1.0 / 1000 + (1.0 / 300 - 1.0 / 1000) * _u01(seed, 2 * c + 1)
+            boxes.append({"seed": seed * 1000 + c, "k0": k0, "nu": nu})
+            c += 1
+        out.append((n, boxes))
+    return out
+
+
+class Ensemble:
+    """Groups of equal boxes (one sh_ns ensemble handle per size) on one stream."""
+
+    def __init__(self, groups, dt, precision=SH_F32, seed=4242, tune=True):
+        torch = _torch()
+        self.L = lib()
+        self.stream = ctypes.c_void_p()
+        self.L.sh_stream_create(ctypes.byref(self.stream))
+        self.params = ensemble_params(groups, seed)
+        self.handles = []
+        self.dt = dt
+        self.precision = precision
+        for n, boxes in self.params:
+            h = ctypes.c_void_p()
+            nus = (ctypes.c_double * len(boxes))(*[b["nu"] for b in boxes])
+            self.L.sh_ns_create_ensemble(ctypes.byref(h), n, len(boxes), nus, float(dt), int(precision), self.stream)
+            V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+            for i, b in enumerate(boxes):
+                self.L.sh_solenoidal_ic_z(ptr(V0), n, n, n, b["seed"], b["k0"], 1.0, self.stream)
+                self.L.sh_ns_set_initial_box(h, i, ptr(V0))
+            self.L.sh_stream_synchronize(self.stream)
+            del V0
+            if tune:
+                self.L.sh_ns_autotune(h, 2)
+            self.handles.append(h)
+
+    def steps(self, k):
+        for h in self.handles:
+            self.L.sh_ns_steps(h, int(k))
+
+    def snapshot_shapes(self, ratio):
+        return [(len(boxes), 9, (n // ratio) ** 3) for n, boxes in self.params]
+
+    def snapshot(self, bufs, ratio):
+        for h, b in zip(self.handles, bufs):
+            self.L.sh_ns_snapshot(h, int(ratio), ptr(b))
+
+    def capture(self, nsteps, every, ratio, snaps):
+        """One graph: nsteps steps of every group with a snapshot after each `every`."""
+        self.L.sh_graph_begin(self.stream)
+        try:
+            for k in range(nsteps // every):
+                self.steps(every)
+                self.snapshot([s[k] for s in snaps], ratio)
+        except Exception:
+            g = ctypes.c_void_p()
+            self.L.sh_graph_end(self.stream, ctypes.byref(g))
+            raise
+        g = ctypes.c_void_p()
+        self.L.sh_graph_end(self.stream, ctypes.byref(g))
+        return g
+
+    def velocity(self, group, box):
+        torch = _torch()
+        n = self.params[group][0]
+        V = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+        self.L.sh_ns_get_velocity_box(self.handles[group], box, ptr(V))
+        self.L.sh_stream_synchronize(self.stream)
+        return V
+
+    def close(self):
+        for h in self.handles:
+            self.L.sh_ns_destroy(h)
+        self.handles = []
+        if self.stream:
+            self.L.sh_stream_destroy(self.stream)
+            self.stream = ctypes.c_void_p()
+
+
+def write_snapshots(path, ens, snaps, every, ratio):
+    """Training-data file: a JSON header line, then float32 arrays in the
+    order of the header's `arrays` list (C order)."""
+    import numpy as np
+    header = {"format": "spectral-snapshots-v1", "ratio": ratio, "every": every, "dt": ens.dt,
+              "fields": ["ubar_x", "ubar_y", "ubar_z", "tau_xx", "tau_yy", "tau_zz", "tau_xy", "tau_xz", "tau_yz"],
+              "arrays": []}
+    for (n, boxes), s in zip(ens.params, snaps):
+        header["arrays"].append({"n": n, "coarse": n // ratio, "boxes": boxes,
+                                 "shape": [int(s.shape[0]), len(boxes), 9] + [n // ratio] * 3})
+    with open(path, "wb") as f:
+        f.write((json.dumps(header) + "\n").encode())
+        for s in snaps:
+            f.write(np.ascontiguousarray(s.cpu().numpy(), dtype=np.float32).tobytes())
+
+
+def read_snapshots(path):
+    import numpy as np
+    with open(path, "rb") as f:
+        header = json.loads(f.readline().decode())
+        arrays = []
+        for a in header["arrays"]:
+            cnt = int(np.prod(a["shape"]))
+            arrays.append(np.frombuffer(f.read(4 * cnt), dtype=np.float32).reshape(a["shape"]))
+    return header, arrays
+
+
+def ensemble_run(groups=((64, 32), (96, 16)), steps=500, every=10, ratio=4, dt=1e-3,
+                 precision=SH_F32, seed=4242, out=None):
+    torch = _torch()
+    ens = Ensemble(groups, dt, precision, seed)
+    nsnap = steps // every
+    snaps = [torch.empty((nsnap,) + shp, dtype=torch.float32, device="cuda") for shp in ens.snapshot_shapes(ratio)]
+    ens.snapshot([s[0] for s in snaps], ratio)  # allocates the snapshot scratch before capture
+    t0 = time.perf_counter()
+    g = ens.capture(steps, every, ratio, snaps)
+    t1 = time.perf_counter()
+    lib().sh_graph_launch(g, ens.stream)
+    lib().sh_stream_synchronize(ens.stream)
+    t2 = time.perf_counter()
+    lib().sh_graph_destroy(g)
+    boxes = sum(nb for _, nb in groups)
+    energies = []
+    for gi, (n, bx) in enumerate(ens.params):
+        for b in (0, len(bx) - 1):
+            V = ens.velocity(gi, b)
+            _, e, _ = spectrum(V, bx[b]["nu"], 2)
+            energies.append({"n": n, "box": b, "energy": e})
+    res = {"form": "ensemble", "groups": [list(x) for x in groups], "boxes": boxes, "steps": steps,
+           "snapshot_every": every, "ratio": ratio, "dt": dt, "precision": precision,
+           "capture_s": t1 - t0, "graph_run_s": t2 - t1,
+           "box_steps_per_s": boxes * steps / (t2 - t1),
+           "grid_points_steps_per_s": sum(n ** 3 * nb for n, nb in groups) * steps / (t2 - t1),
+           "sample_energies": energies}
+    if out:
+        write_snapshots(out, ens, snaps, every, ratio)
+        res["snapshots_file"] = out
+    ens.close()
+    del snaps
+    torch.cuda.empty_cache()
+    return res
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser()
+    sub = p.add_subparsers(dest="form", required=True)
+    f = sub.add_parser("fp32")
+    f.add_argument("--n", type=int, default=512)
+    f.add_argument("--steps", type=int, default=100)
+    f.add_argument("--dt", type=float, default=2.5e-4)
+    f.add_argument("--nu", type=float, default=1.0 / 800)
+    f.add_argument("--seed", type=int, default=2024)
+    e = sub.add_parser("ensemble")
+    e.add_argument("--steps", type=int, default=500)
+    e.add_argument("--every", type=int, default=10)
+    e.add_argument("--ratio", type=int, default=4)
+    e.add_argument("--dt", type=float, default=1e-3)
+    e.add_argument("--seed", type=int, default=4242)
+    e.add_argument("--out", default=None)
+    e.add_argument("--fp64", action="store_true")
+    a = p.parse_args(argv)
+    torch = _torch()
+    torch.cuda.set_device(0)
+    lib().sh_init(0)
+    if a.form == "fp32":
+        print(json.dumps(fp32_error(a.n, a.steps, a.dt, a.nu, a.seed)), flush=True)
+    elif a.form == "ensemble":
+        print(json.dumps(ensemble_run(steps=a.steps, every=a.every, ratio=a.ratio, dt=a.dt,
+                                      precision=SH_F64 if a.fp64 else SH_F32, seed=a.seed,
+                                      out=a.out)), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -97,3 +97,17 @@ def test_oracle_library_exports():
     L = O.lib()
     for name in ("or_transform3d", "or_cross", "or_ns_steps", "or_ic_solenoidal"):
         assert hasattr(L, name)
+
+
+def test_ensemble_parameters_are_seeded_and_in_range():
+    """Config-4 box parameters: k0 ~ U[2,6], nu ~ U[1/1000, 1/300], reproducible."""
+    from spectral_upcxx_amd import forms
+    p1 = forms.ensemble_params(((64, 32), (96, 16)), 4242)
+    p2 = forms.ensemble_params(((64, 32), (96, 16)), 4242)
+    assert p1 == p2
+    boxes = [b for _, bx in p1 for b in bx]
+    assert len(boxes) == 48 and len({b["seed"] for b in boxes}) == 48
+    assert all(2 <= b["k0"] <= 6 and 1 / 1000 <= b["nu"] <= 1 / 300 for b in boxes)
+    # the draws use the IC generator's splitmix64 stream
+    from oracle import pyoracle as O
+    assert forms._u01(4242, 5) == O.prng_uniform(4242, 5, 1)[0]
