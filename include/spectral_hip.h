@@ -183,6 +183,14 @@ int sh_ns_div_average_box(sh_ns_t handle, int box, double* result);
  * ubar_y, ubar_z, tau_xx, tau_yy, tau_zz, tau_xy, tau_xz, tau_yz with
  * tau_ij = <u_i u_j> - ubar_i ubar_j.  The first call allocates scratch. */
 int sh_ns_snapshot(sh_ns_t handle, int ratio, void* out);
+/* Body force added to the nonlinear term: each step uses cross1 + f in place
+ * of cross1 (exec/ns.cpp:170-171), so f is Adams-Bashforth extrapolated with
+ * it; the start-up adds f to both cross products it forms.  f: 3 Fourier
+ * fields in the solver's precision (complex128, or complex64 for SH_F32),
+ * laid out like the state (field d of box b at (d*nboxes + b)*N).  Its
+ * contents are read at every step, so the caller may rewrite them between
+ * steps.  NULL removes the force.  No counterpart in the reference. */
+int sh_ns_set_forcing(sh_ns_t handle, const void* f3);
 
 /* ------------------------------------------------------------------ streams and graphs
  * Default-priority non-blocking streams; stream capture into a hipGraph
@@ -217,6 +225,63 @@ int sh_tune_set_variant(int precision, int n, int role, int variant);
 int sh_tune_reset(void);
 /* one line per tuned (precision, length, role): chosen schedule and its time */
 int sh_tune_report(char* buf, size_t len);
+
+/* ------------------------------------------------------------------ learned subgrid-stress closure
+ * No counterpart in the reference.  An MLP 9 -> 64 -> 64 -> 6 (tanh hidden
+ * layers) maps the filtered velocity-gradient tensor g_{3i+j} = d ubar_i/d x_j
+ * to the stress components tau_xx, tau_yy, tau_zz, tau_xy, tau_xz, tau_yz.
+ * Weights are FP32 master copies rounded to BF16 for the MFMA products,
+ * which accumulate in FP32.  Inputs are multiplied by in_scale before the
+ * network and outputs by out_scale after it. */
+typedef struct sh_mlp_s* sh_mlp_t;
+/* number of floats in the packed parameter vector: W1[16][64] (rows 9..15
+ * zero), b1[64], W2[64][64], b2[64], W3[64][16] (columns 6..15 zero), b3[16] */
+int sh_mlp_param_count(int* count);
+/* Glorot-uniform weights from the splitmix64 stream of `seed`, zero biases,
+ * unit scales */
+int sh_mlp_create(sh_mlp_t* handle, uint64_t seed, void* stream);
+int sh_mlp_destroy(sh_mlp_t handle);
+int sh_mlp_set_params(sh_mlp_t handle, const float* host_params);
+int sh_mlp_get_params(sh_mlp_t handle, float* host_params);
+int sh_mlp_set_scaling(sh_mlp_t handle, const float* in_scale9, const float* out_scale6);
+/* Y[f*ldy + s] = out_scale_f * net(in_scale * X[:, s])_f for s < n;
+ * X: float device memory [9][ld], Y: float [6][ldy] */
+int sh_mlp_forward(sh_mlp_t handle, const void* X, long ld, long n, void* Y, long ldy);
+/* Gradient of the loss mean_{s<n, f<6} (net(X_s)_f - T[f*ld+s]/out_scale_f)^2
+ * w.r.t. the packed parameters (padding entries zero) into grad_out (float
+ * device memory, may be NULL); *loss (may be NULL; synchronous then) */
+int sh_mlp_gradient(sh_mlp_t handle, const void* X, const void* T, long ld, long n, void* grad_out,
+                    double* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient */
+int sh_mlp_train_step(sh_mlp_t handle, const void* X, const void* T, long ld, long n, float lr,
+                      double* loss);
+/* D = A B for 16x16 float matrices (row-major) through one BF16 MFMA, to check
+ * the operand layout the kernels assume */
+int sh_mfma_probe(const void* A, const void* B, void* D, void* stream);
+
+/* Coarse-grid LES: the FP32 fused solver with f = -div tau_model as its body
+ * force, tau from the network at every step (gradient tensor by spectral
+ * derivatives, transforms in FP32).  model = NULL is the no-model solve.
+ * clip = 1 zeroes tau where it would feed energy to the resolved scales
+ * (tau_ij S_ij > 0). */
+typedef struct sh_les_s* sh_les_t;
+int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, int clip, void* stream);
+int sh_les_destroy(sh_les_t handle);
+/* initial condition from a physical filtered velocity (float [3][n^3]):
+ * transform, zero the Nyquist modes, project onto divergence-free fields,
+ * then the start-up step (exec/ns.cpp:132-153) with the closure force */
+int sh_les_set_initial_physical(sh_les_t handle, const void* ubar3);
+int sh_les_steps(sh_les_t handle, int nsteps);
+/* velocity in Fourier space, 3 contiguous complex128 fields */
+int sh_les_get_velocity(sh_les_t handle, void* out_v3);
+/* the model stress of the current velocity, float [6][n^3] physical */
+int sh_les_stress(sh_les_t handle, void* out_tau6);
+/* the body force -div tau of the current velocity, 3 Fourier complex64 fields */
+int sh_les_force(sh_les_t handle, void* out_f3);
+/* network inputs of a physical filtered velocity ubar (float [3][n^3]):
+ * X (float [9][ldx]) = d ubar_i / d x_j by spectral derivatives on the n^3
+ * grid (Nyquist modes dropped), the same features the LES feeds the model */
+int sh_les_features(int n, const void* ubar3, void* X, long ldx, void* stream);
 
 /* ------------------------------------------------------------------ kernel timing
  * When enabled, every launch of the named kernel classes is bracketed with

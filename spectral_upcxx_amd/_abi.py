@@ -32,6 +32,13 @@ _CTYPES = {
     "const char*": ctypes.c_char_p,
     "sh_ns_t": ctypes.c_void_p,
     "sh_ns_t*": ctypes.POINTER(ctypes.c_void_p),
+    "sh_mlp_t": ctypes.c_void_p,
+    "sh_mlp_t*": ctypes.POINTER(ctypes.c_void_p),
+    "sh_les_t": ctypes.c_void_p,
+    "sh_les_t*": ctypes.POINTER(ctypes.c_void_p),
+    "float": ctypes.c_float,
+    "float*": ctypes.POINTER(ctypes.c_float),
+    "const float*": ctypes.POINTER(ctypes.c_float),
 }
 
 

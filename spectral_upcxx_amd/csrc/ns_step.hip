@@ -190,6 +190,7 @@ template <typename T, class P, int LPB, bool EMIT>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
+                                                              const C2<T>* __restrict__ frc,
                                                               const C2<T>* __restrict__ tw, int ny,
                                                               int nz, long F,
                                                               const StepCoef<T>* __restrict__ cfs,
@@ -202,6 +203,8 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     W += blockIdx.y * boff;
     V += blockIdx.y * boff;
     N0 += blockIdx.y * boff;
+    const bool forced = frc != nullptr;  // body force added to the nonlinear term
+    if (forced) frc += blockIdx.y * boff;
     divpart += (long)blockIdx.y * divstride;
     constexpr int N = P::N;
     constexpr int LS = LdsLine<N>::LS;
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             C2<T> nn[3], v1[3], f[3], L[3], S[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                nn[d] = z ? mk<T>(T(0), T(0)) : n1[d][s];
+                nn[d] = z ? mk<T>(T(0), T(0)) : forced ? cadd<T>(n1[d][s], frc[d * F + off + pos]) : n1[d][s];
                 v1[d] = V[d * F + off + pos];
                 const C2<T> n0 = N0[d * F + off + pos];
                 // F = K/2*(3.*cross1 - cross0)
@@ -450,6 +453,8 @@ struct sh_ns_s {
     bool have_div;
     bool w_primed;  // W holds the x-inverse of V1 and curl V1 for the next step
     double div_avg;
+    const void* forcing;  // body force (3 fields per box, precision prec) or null
+    int cur_box;          // box whose start-up is running
     std::vector<void*> owned;
 };
 
@@ -485,10 +490,29 @@ int cross(sh_ns_s* s, const double2* a, const double2* b, double2* o) {
         if (_rc) return _rc; \
     } while (0)
 
+// cr += the body force of the current box (an FP32 force is widened through work)
+int add_forcing64(sh_ns_s* s, double2* cr) {
+    if (!s->forcing) return SH_OK;
+    const long N = s->N;
+    for (int d = 0; d < 3; d++) {
+        const long off = ((long)d * s->nboxes + s->cur_box) * N;
+        const void* f = (const double2*)s->forcing + off;
+        if (s->prec == SH_F32) {
+            hipLaunchKernelGGL(k_c_to_z, dim3(egrid(N)), dim3(256), 0, s->st, (const float2*)s->forcing + off,
+                               F3(s->work, d), N);
+            SH_LAUNCH_CHECK("k_c_to_z");
+            f = F3(s->work, d);
+        }
+        RC(sh_add_z(F3(cr, d), F3(cr, d), f, N, s->st));
+    }
+    return SH_OK;
+}
+
 // exec/ns.cpp:132-153
 int startup64(sh_ns_s* s) {
     const long V = 3 * s->N;
     RC(cross(s, s->V0, s->om0, s->cr1));
+    RC(add_forcing64(s, s->cr1));
     RC(vlap(s, s->V0, s->lap1));
     RC(sh_scale_z(s->lap1, s->nu, s->lap1, V, s->st));
     RC(sh_add_z(s->S, s->cr1, s->lap1, V, s->st));
@@ -500,6 +524,7 @@ int startup64(sh_ns_s* s) {
     RC(sh_add_z(s->V1, s->V0, s->S, V, s->st));
     RC(curl(s, s->V1, s->om1));
     RC(cross(s, s->V0, s->om0, s->cr0));
+    RC(add_forcing64(s, s->cr0));
     return SH_OK;
 }
 
@@ -509,6 +534,7 @@ int step64(sh_ns_s* s) {
     const double K = s->K, nu = s->nu;
     const int nx = s->nx, ny = s->ny, nz = s->nz;
     RC(cross(s, s->V1, s->om1, s->cr1));
+    RC(add_forcing64(s, s->cr1));
     RC(sh_scale_z(s->Fv, 3.0, s->cr1, V, s->st));
     RC(sh_sub_z(s->Fv, s->Fv, s->cr0, V, s->st));
     RC(sh_scale_z(s->Fv, K / 2, s->Fv, V, s->st));
@@ -581,7 +607,7 @@ int launch_fourier_x(sh_ns_s* s) {
     ProfScope ps("ns_fourier_x", s->st);
     hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, true>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
-                       (C2<T>*)s->N0f, tw, s->ny, s->nz, s->N * s->nboxes,
+                       (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
@@ -889,6 +915,8 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
     s->nus.assign(nus, nus + nboxes);
     s->coefs = nullptr;
     s->snap = nullptr;
+    s->forcing = nullptr;
+    s->cur_box = 0;
     const size_t z = sizeof(double2);
     int rc = 0;
     // FP64 state used by the start-up (and by the op-by-op loop)
@@ -953,8 +981,10 @@ int sh_ns_set_initial_box(sh_ns_t s, int box, const void* V0) {
     RC(curl(s, s->V0, s->om0));
     const double nu0 = s->nu;
     s->nu = s->nus[box];
+    s->cur_box = box;
     int rc = startup64(s);
     s->nu = nu0;
+    s->cur_box = 0;
     if (rc) return rc;
     s->w_primed = false;
     return install_box(s, box);
@@ -1039,6 +1069,12 @@ int sh_ns_destroy(sh_ns_t s) {
     (void)hipStreamSynchronize(s->st);
     for (void* q : s->owned) pool_free(q);
     delete s;
+    return SH_OK;
+}
+
+int sh_ns_set_forcing(sh_ns_t s, const void* f3) {
+    if (!s) return SH_EINVAL;
+    s->forcing = f3;
     return SH_OK;
 }
 
@@ -1135,3 +1171,8 @@ int sh_ns_state_bytes(sh_ns_t s, size_t* bytes) {
 }
 
 }  // extern "C"
+
+namespace sh {
+// the fused solver's current velocity (3 Fourier fields per box, precision prec)
+void* ns_fused_velocity(sh_ns_t s) { return s && s->fused ? s->V1f : nullptr; }
+}  // namespace sh

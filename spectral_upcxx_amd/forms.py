@@ -240,6 +240,223 @@ def ensemble_run(groups=((64, 32), (96, 16)), steps=500, every=10, ratio=4, dt=1
     return res
 
 
+def physical_to_fourier(u):
+    """[3, n, n, n] float32 physical field -> complex128 Fourier fields (the
+    solver's unnormalised forward transform)."""
+    torch = _torch()
+    n = u.shape[-1]
+    V = u.to(torch.complex128).contiguous()
+    lib().sh_fft3d_many_z2z(ptr(V), 3, n ** 3, n, n, n, -1, 0, stream())
+    return V
+
+
+def _spectrum_rel(Ea, Eb, kmax):
+    return [abs(Ea[k] - Eb[k]) / Eb[k] for k in range(1, kmax + 1)]
+
+
+def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=2.0, snap_dt=0.25,
+              train_seeds=8, test_seeds=4, seed0=7100, k0=4.0, iters=3000, batch=65536, lr=2e-3,
+              clip=True, eval_dt=0.5, kcut=16, net_seed=5, log=None):
+    """Learned-closure form: FP64 DNS at n_dns^3 box-filtered by `ratio`
+    gives training pairs (filtered velocity gradients -> stresses); the
+    network is trained on the GPU; the FP32 LES on the coarse grid with and
+    without the model is compared with the filtered DNS spectrum for k <=
+    kcut on held-out seeds."""
+    import math
+    torch = _torch()
+    from . import closure
+    log = log or (lambda *a: print(*a, file=sys.stderr, flush=True))
+    L = lib()
+    n = n_dns // ratio
+    nc = n ** 3
+    every = int(round(les_dt / dt_dns))
+    assert abs(every * dt_dns - les_dt) < 1e-12, "the LES step must be a multiple of the DNS step"
+    snap_steps = int(round(snap_dt / dt_dns))
+    eval_steps = int(round(eval_dt / dt_dns))
+    total = int(round(t_end / dt_dns))
+    assert snap_steps % every == 0 and eval_steps % every == 0 and total % eval_steps == 0
+    # ---- DNS: one solver reused for every seed
+    dns = NS((n_dns,) * 3, nu, dt_dns, SH_F64, fused=True)
+    snap = torch.empty((1, 9, nc), dtype=torch.float32, device="cuda")
+    train, test = [], []
+    dns_time = 0.0
+    dns_steps = 0
+    tuned = False
+    for si in range(train_seeds + test_seeds):
+        seed = seed0 + si
+        is_train = si < train_seeds
+        V0 = solenoidal(n_dns, seed, k0)
+        dns.set_initial(V0)
+        del V0
+        dns.startup()
+        if not tuned:
+            L.sh_ns_autotune(dns.h, 2)
+            tuned = True
+        L.sh_ns_snapshot(dns.h, ratio, ptr(snap))
+        snaps = {0: snap[0].clone()}
+        chunk = snap_steps if is_train else eval_steps
+        done = 0
+        while done < total:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dns.steps(chunk)
+            torch.cuda.synchronize()
+            dns_time += time.perf_counter() - t0
+            dns_steps += chunk
+            done += chunk
+            L.sh_ns_snapshot(dns.h, ratio, ptr(snap))
+            snaps[done] = snap[0].clone()
+        finite = all(bool(torch.isfinite(s).all()) for s in snaps.values())
+        (train if is_train else test).append({"seed": seed, "snaps": snaps, "finite": finite})
+        log("dns seed %d (%s): %d steps, finite=%s, %.1f steps/s" % (
+            seed, "train" if is_train else "test", total, finite, dns_steps / dns_time))
+    dns.close()
+    L.sh_pool_trim()
+    dns_sps = dns_steps / dns_time
+    # ---- training pairs: every training snapshot after t = 0
+    keys = [k for k in sorted(train[0]["snaps"]) if k > 0]
+    M = len(train) * len(keys) * nc
+    X = torch.empty((9, M), dtype=torch.float32, device="cuda")
+    T = torch.empty((6, M), dtype=torch.float32, device="cuda")
+    off = 0
+    for tr in train:
+        for k in keys:
+            s = tr["snaps"][k]
+            ub = s[:3].reshape(3, n, n, n).contiguous()
+            L.sh_les_features(n, ptr(ub), closure._offset(X, off), M, stream())
+            T[:, off:off + nc] = s[3:]
+            off += nc
+        tr["snaps"] = {0: tr["snaps"][0]}
+    in_scale = (1.0 / X.std(dim=1)).cpu().numpy()
+    out_scale = T.std(dim=1).cpu().numpy()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1234)
+    perm = torch.randperm(M, device="cuda", generator=g)
+    X = X[:, perm].contiguous()
+    T = T[:, perm].contiguous()
+    del perm
+    model = closure.MLP(seed=net_seed)
+    model.set_scaling(in_scale, out_scale)
+    nbatch = M // batch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for it in range(iters):
+        lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / iters)))
+        want = it % 500 == 0 or it == iters - 1
+        l = model.train_step(X, T, lr_t, n=batch, start=(it % nbatch) * batch, want_loss=want)
+        if want:
+            losses.append((it, l))
+            log("train it %d loss %.4f" % (it, l))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del X, T
+    torch.cuda.empty_cache()
+    # ---- a-priori check on held-out snapshots: correlation of the stresses
+    corr = []
+    for te in test:
+        s = te["snaps"][total]
+        ub = s[:3].reshape(3, n, n, n).contiguous()
+        Xt = closure.features(ub)
+        Y = model.forward(Xt)
+        for c in range(6):
+            a = Y[c].double() - Y[c].double().mean()
+            b = s[3 + c].double() - s[3 + c].double().mean()
+            corr.append(float((a * b).sum() / (a.norm() * b.norm())))
+    # ---- a-posteriori: LES with and without the model against the filtered DNS
+    nb = n // 2
+    kcut = min(kcut, nb - 1)
+    evals = []
+    for te in test:
+        ub0 = te["snaps"][0][:3].reshape(3, n, n, n).contiguous()
+        runs = {}
+        for name, mdl in (("model", model), ("no_model", None)):
+            les = closure.LES(n, nu, les_dt, mdl, clip=clip)
+            les.set_initial(ub0)
+            # after the start-up step the LES is at DNS step `every`
+            cur = every
+            Es = {}
+            for target in range(eval_steps, total + 1, eval_steps):
+                les.steps((target - cur) // every)
+                cur = target
+                Vl = les.velocity()
+                Es[target] = spectrum(Vl, nu, nb)
+            les.close()
+            runs[name] = Es
+        res = {"seed": te["seed"], "times": []}
+        for target in range(eval_steps, total + 1, eval_steps):
+            uf = te["snaps"][target][:3].reshape(3, n, n, n).contiguous()
+            Ef, ef, df = spectrum(physical_to_fourier(uf), nu, nb)
+            row = {"t": target * dt_dns, "energy_fdns": ef}
+            for name in runs:
+                E, e, d = runs[name][target]
+                rel = _spectrum_rel(E, Ef, kcut)
+                finite = all(math.isfinite(x) for x in E[:kcut + 1])
+                row[name] = {"max_rel_err_k_le_%d" % kcut: max(rel) if finite else float("inf"),
+                             "mean_rel_err_k_le_%d" % kcut: sum(rel) / len(rel) if finite else float("inf"),
+                             "energy": e, "rel_err_by_k": [round(x, 5) for x in rel]}
+            res["times"].append(row)
+        evals.append(res)
+        log("les seed %d: " % te["seed"] + ", ".join(
+            "t=%.2f model %.3f no-model %.3f" % (r["t"], r["model"]["max_rel_err_k_le_%d" % kcut],
+                                                 r["no_model"]["max_rel_err_k_le_%d" % kcut])
+            for r in res["times"]))
+    # ---- speed: LES steps/s with each step's launches replayed from one graph
+    speed = {}
+    ub0 = test[0]["snaps"][0][:3].reshape(3, n, n, n).contiguous()
+    st = ctypes.c_void_p()
+    L.sh_stream_create(ctypes.byref(st))
+    for name, mdl in (("model", model), ("no_model", None)):
+        m2 = None
+        if mdl is not None:
+            m2 = closure.MLP(seed=net_seed, strm=st)
+            m2.set_params(model.params())
+            m2.set_scaling(in_scale, out_scale)
+        les = closure.LES(n, nu, les_dt, m2, clip=clip, strm=st)
+        torch.cuda.synchronize()
+        les.set_initial(ub0)
+        les.steps(2)
+        L.sh_stream_synchronize(st)
+        nrep, per = 5, 100
+        L.sh_graph_begin(st)
+        les.steps(per)
+        gr = ctypes.c_void_p()
+        L.sh_graph_end(st, ctypes.byref(gr))
+        L.sh_graph_launch(gr, st)
+        L.sh_stream_synchronize(st)
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            L.sh_graph_launch(gr, st)
+        L.sh_stream_synchronize(st)
+        speed[name] = nrep * per / (time.perf_counter() - t0)
+        L.sh_graph_destroy(gr)
+        les.close()
+        if m2:
+            m2.close()
+    L.sh_stream_destroy(st)
+    model_params = model.params()
+    model.close()
+    last = [r["times"][-1] for r in evals]
+    key = "max_rel_err_k_le_%d" % kcut
+    return {
+        "form": "les", "dns_grid": n_dns, "les_grid": n, "ratio": ratio, "nu": nu, "dt_dns": dt_dns,
+        "dt_les": les_dt, "t_end": t_end, "k0": k0, "train_seeds": [t["seed"] for t in train],
+        "test_seeds": [t["seed"] for t in test], "dns_all_finite": all(t["finite"] for t in train + test),
+        "training_samples": M, "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
+        "loss_history": losses, "clip": clip,
+        "a_priori_corr_by_component": [round(c, 4) for c in corr[:6]],
+        "a_priori_corr_mean": sum(corr) / len(corr),
+        "evals": evals,
+        "final_%s_model_mean" % key: sum(r["model"][key] for r in last) / len(last),
+        "final_%s_no_model_mean" % key: sum(r["no_model"][key] for r in last) / len(last),
+        "dns_fp64_steps_per_s": dns_sps, "les_model_steps_per_s": speed["model"],
+        "les_no_model_steps_per_s": speed["no_model"],
+        "les_vs_dns_steps_ratio": speed["model"] / dns_sps,
+        "les_vs_dns_sim_time_ratio": speed["model"] * les_dt / (dns_sps * dt_dns),
+        "param_checksum": float(abs(model_params).sum()),
+    }
+
+
 def main(argv=None):
     p = argparse.ArgumentParser()
     sub = p.add_subparsers(dest="form", required=True)
@@ -257,6 +474,21 @@ def main(argv=None):
     e.add_argument("--seed", type=int, default=4242)
     e.add_argument("--out", default=None)
     e.add_argument("--fp64", action="store_true")
+    s = sub.add_parser("les")
+    s.add_argument("--n-dns", type=int, default=256)
+    s.add_argument("--ratio", type=int, default=4)
+    s.add_argument("--nu", type=float, default=1.0 / 800)
+    s.add_argument("--dt-dns", type=float, default=5e-4)
+    s.add_argument("--dt-les", type=float, default=2e-3)
+    s.add_argument("--t-end", type=float, default=2.0)
+    s.add_argument("--snap-dt", type=float, default=0.25)
+    s.add_argument("--eval-dt", type=float, default=0.5)
+    s.add_argument("--train-seeds", type=int, default=8)
+    s.add_argument("--test-seeds", type=int, default=4)
+    s.add_argument("--iters", type=int, default=3000)
+    s.add_argument("--batch", type=int, default=65536)
+    s.add_argument("--lr", type=float, default=2e-3)
+    s.add_argument("--no-clip", action="store_true")
     a = p.parse_args(argv)
     torch = _torch()
     torch.cuda.set_device(0)
@@ -267,6 +499,10 @@ def main(argv=None):
         print(json.dumps(ensemble_run(steps=a.steps, every=a.every, ratio=a.ratio, dt=a.dt,
                                       precision=SH_F64 if a.fp64 else SH_F32, seed=a.seed,
                                       out=a.out)), flush=True)
+    elif a.form == "les":
+        print(json.dumps(les_study(a.n_dns, a.ratio, a.nu, a.dt_dns, a.dt_les, a.t_end, a.snap_dt,
+                                   a.train_seeds, a.test_seeds, iters=a.iters, batch=a.batch, lr=a.lr,
+                                   clip=not a.no_clip, eval_dt=a.eval_dt)), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,354 @@
+"""The learned closure on the GPU against the CPU oracle (oracle/closure.py):
+the MFMA operand layout, network forward and gradient in bf16 mode, Adam,
+training progress, the LES feature and force pipelines, and the solver's body
+force hook (Kolmogorov flow is an exact steady state of the discrete
+scheme; fused and op-by-op schedules agree under a random force)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import closure as C
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    P = C.init_params(seed).astype(np.float64)
+    P[C.OFF_B1:C.OFF_W2] = rng.normal(size=64) * 0.1
+    P[C.OFF_B2:C.OFF_W3] = rng.normal(size=64) * 0.1
+    P[C.OFF_B3:C.OFF_B3 + 6] = rng.normal(size=6) * 0.1
+    return P.astype(np.float32)
+
+
+def test_mfma_operand_layout(hip):
+    import torch
+    rng = np.random.default_rng(0)
+    # small multiples of 1/4: bf16-exact, and every product sum is exact in FP32
+    A = rng.integers(-8, 9, size=(16, 16)).astype(np.float32) / 4
+    B = rng.integers(-8, 9, size=(16, 16)).astype(np.float32) / 4
+    D = torch.empty((16, 16), dtype=torch.float32, device="cuda")
+    Ad, Bd = _dev(A), _dev(B)  # held until the launch has run
+    hip.lib().sh_mfma_probe(hip.ptr(Ad), hip.ptr(Bd), hip.ptr(D), hip.stream())
+    assert np.array_equal(_host(D), A @ B)
+    # general values: the products of the bf16-rounded operands
+    A = rng.normal(size=(16, 16)).astype(np.float32)
+    B = rng.normal(size=(16, 16)).astype(np.float32)
+    Ad, Bd = _dev(A), _dev(B)
+    hip.lib().sh_mfma_probe(hip.ptr(Ad), hip.ptr(Bd), hip.ptr(D), hip.stream())
+    want = C.bf16(A).astype(np.float64) @ C.bf16(B).astype(np.float64)
+    assert np.abs(_host(D) - want).max() < 1e-5
+
+
+def _mlp(hip, P, sc_in=None, sc_out=None):
+    from spectral_upcxx_amd import closure
+    m = closure.MLP(seed=1)
+    m.set_params(P)
+    if sc_in is not None:
+        m.set_scaling(sc_in, sc_out)
+    return m
+
+
+@pytest.mark.parametrize("n", [5, 16, 1000, 4099])
+def test_forward_matches_oracle(hip, n):
+    P = _params(3)
+    rng = np.random.default_rng(n)
+    X = rng.normal(size=(n, 9)).astype(np.float32)
+    sc_in = np.linspace(0.5, 2.0, 9).astype(np.float32)
+    sc_out = np.linspace(0.1, 0.6, 6).astype(np.float32)
+    m = _mlp(hip, P, sc_in, sc_out)
+    Y = _host(m.forward(_dev(X.T.copy())))
+    want = C.forward(P, X, sc_in, sc_out, bf=True)
+    assert Y.shape == (6, n)
+    assert rel_l2(Y.T, want) < 1e-3
+    # and within bf16 distance of the exact network
+    assert rel_l2(Y.T, C.forward(P, X, sc_in, sc_out, bf=False)) < 3e-2
+    m.close()
+
+
+@pytest.mark.parametrize("n", [7, 3000, 16 * 1024 * 2 + 3])
+def test_gradient_matches_oracle(hip, n):
+    """Backward kernels (BF16 MFMA, FP32 accumulate, deterministic partial
+    sums) against the oracle's bf16-mode gradient, which finite differences
+    pin in exact mode."""
+    P = _params(5)
+    rng = np.random.default_rng(100 + n)
+    X = rng.normal(size=(n, 9)).astype(np.float32)
+    T = (rng.normal(size=(n, 6)) * 0.2).astype(np.float32)
+    sc_in = np.linspace(0.7, 1.3, 9).astype(np.float32)
+    sc_out = np.linspace(0.2, 0.5, 6).astype(np.float32)
+    m = _mlp(hip, P, sc_in, sc_out)
+    loss, g = m.gradient(_dev(X.T.copy()), _dev(T.T.copy()))
+    g = _host(g)
+    lo, go = C.loss_grad(P, X, T, sc_in, sc_out, bf=True)
+    assert abs(loss - lo) / lo < 1e-5
+    assert rel_l2(g, go) < 2e-3
+    for a, b in ((C.OFF_B1, C.OFF_W2), (C.OFF_W2, C.OFF_B2), (C.OFF_B3, C.OFF_B3 + 6)):
+        assert rel_l2(g[a:b], go[a:b]) < 5e-3, (a, b)
+    # deterministic: the same call gives the same bits
+    _, g2 = m.gradient(_dev(X.T.copy()), _dev(T.T.copy()))
+    assert np.array_equal(g, _host(g2))
+    m.close()
+
+
+def test_adam_steps_match_oracle(hip):
+    P = _params(8)
+    rng = np.random.default_rng(8)
+    n = 2048
+    X = rng.normal(size=(n, 9)).astype(np.float32)
+    T = (rng.normal(size=(n, 6)) * 0.2).astype(np.float32)
+    m = _mlp(hip, P)
+    Xd, Td = _dev(X.T.copy()), _dev(T.T.copy())
+    Pw = P.astype(np.float64)
+    mo = np.zeros(C.NPARAM)
+    vo = np.zeros(C.NPARAM)
+    for t in range(1, 4):
+        _, g = m.gradient(Xd, Td)
+        g = _host(g).astype(np.float64)
+        m.train_step(Xd, Td, 1e-3)
+        Pw, mo, vo = C.adam(Pw, mo, vo, g, t, 1e-3)
+        got = m.params()
+        assert np.abs(got - Pw).max() < 1e-6, t
+        Pw = got.astype(np.float64)  # continue from the device's rounding
+    m.close()
+
+
+def test_training_reduces_loss(hip):
+    """Teacher: the gradient model tau_ij = g_ik g_jk on random gradients."""
+    rng = np.random.default_rng(21)
+    n = 65536
+    G = rng.normal(size=(n, 3, 3)).astype(np.float32) * 0.5
+    tau = np.einsum("nik,njk->nij", G, G)
+    T = np.stack([tau[:, i, j] for i, j in C.PAIRS], 1).astype(np.float32)
+    X = G.reshape(n, 9)
+    from spectral_upcxx_amd import closure
+    m = closure.MLP(seed=2)
+    m.set_scaling(np.full(9, 2.0, np.float32), T.std(0).astype(np.float32))
+    Xd, Td = _dev(X.T.copy()), _dev(T.T.copy())
+    l0 = m.gradient(Xd, Td)[0]
+    for it in range(400):
+        m.train_step(Xd, Td, 3e-3, n=8192, start=(it % 8) * 8192)
+    l1 = m.gradient(Xd, Td)[0]
+    assert l1 < 0.1 * l0, (l0, l1)
+    m.close()
+
+
+def _physical_field(n, seed, k0=3.0):
+    V = O.solenoidal((n, n, n), seed, k0, 1.0)
+    return np.stack([O.inverse(c).real for c in V])
+
+
+def test_les_features_match_oracle(hip):
+    from spectral_upcxx_amd import closure
+    n = 32
+    u = _physical_field(n, 31)
+    X = _host(closure.features(_dev(u.astype(np.float32))))
+    want = C.gradient_features(u.astype(np.float32).astype(np.float64))
+    assert rel_l2(X.T, want) < 1e-5
+
+
+def _les_with_model(hip, n, P, sc_in, sc_out, clip, nu=0.01, K=1e-3):
+    from spectral_upcxx_amd import closure
+    m = _mlp(hip, P, sc_in, sc_out)
+    les = closure.LES(n, nu, K, m, clip=clip)
+    return m, les
+
+
+@pytest.mark.parametrize("clip", [False, True])
+def test_les_stress_and_force_match_oracle(hip, clip):
+    n = 32
+    P = _params(12)
+    sc_in = np.full(9, 0.3, np.float32)
+    sc_out = np.full(6, 0.05, np.float32)
+    m, les = _les_with_model(hip, n, P, sc_in, sc_out, clip)
+    u = _physical_field(n, 41)
+    les.set_initial(_dev(u.astype(np.float32)))
+    V = _host(les.velocity())
+    u1 = np.stack([O.inverse(c).real for c in V])
+    g = C.gradient_features(u1)
+    tau = C.forward(P, g, sc_in, sc_out, bf=True)
+    if clip:
+        tau = C.clip(tau, g)
+        assert (np.abs(tau).sum(1) == 0).mean() > 0.05  # some points are clipped
+    got = _host(les.stress())
+    keep = np.ones(n ** 3, dtype=bool)
+    if clip:
+        # the clip decision may differ where tau:S rounds across zero
+        zg = np.abs(got.T).sum(1) == 0
+        zo = np.abs(tau).sum(1) == 0
+        assert (zg != zo).mean() < 1e-3
+        keep = zg == zo
+    assert rel_l2(got.T[keep], tau[keep]) < 2e-3
+    # the force pipeline (packed transforms, Hermitian unpacking, -div)
+    # applied to the device's own stress field
+    f = _host(les.force())
+    fo = C.sgs_force(got.astype(np.float64).reshape(6, n, n, n))
+    assert rel_l2(f, fo) < 1e-5
+    les.close()
+    m.close()
+
+
+def test_les_zero_model_equals_no_model(hip):
+    """A network that outputs zero adds exactly nothing to the solver."""
+    from spectral_upcxx_amd import closure
+    n = 32
+    u = _dev(_physical_field(n, 51).astype(np.float32))
+    m = closure.MLP(seed=1)
+    m.set_params(np.zeros(closure.param_count(), np.float32))
+    a = closure.LES(n, 0.01, 1e-3, m, clip=False)
+    b = closure.LES(n, 0.01, 1e-3, None)
+    a.set_initial(u)
+    b.set_initial(u)
+    a.steps(20)
+    b.steps(20)
+    assert np.array_equal(_host(a.velocity()), _host(b.velocity()))
+    # the initial field is projected: divergence-free, Nyquist modes zero
+    V = _host(b.velocity())
+    assert np.abs(O.divergence(V)).max() < 1e-5 * np.abs(V).max() * n
+    h = n // 2
+    assert not V[:, h].any() and not V[:, :, h].any() and not V[:, :, :, h].any()
+    a.close()
+    b.close()
+    m.close()
+
+
+def test_les_graph_replay_matches_eager(hip):
+    from spectral_upcxx_amd import closure
+    L = hip.lib()
+    n = 32
+    st = ctypes.c_void_p()
+    L.sh_stream_create(ctypes.byref(st))
+    P = _params(14)
+    u = _dev(_physical_field(n, 61).astype(np.float32))
+    ms = [closure.MLP(seed=1, strm=st) for _ in range(2)]
+    for m in ms:
+        m.set_params(P)
+        m.set_scaling(np.full(9, 0.3, np.float32), np.full(6, 0.02, np.float32))
+    g_les = closure.LES(n, 0.01, 1e-3, ms[0], clip=True, strm=st)
+    e_les = closure.LES(n, 0.01, 1e-3, ms[1], clip=True, strm=st)
+    L.sh_stream_synchronize(st)
+    g_les.set_initial(u)
+    e_les.set_initial(u)
+    g_les.steps(1)
+    e_les.steps(1)
+    L.sh_stream_synchronize(st)
+    L.sh_graph_begin(st)
+    g_les.steps(4)
+    g = ctypes.c_void_p()
+    L.sh_graph_end(st, ctypes.byref(g))
+    for rep in range(3):
+        L.sh_graph_launch(g, st)
+        L.sh_stream_synchronize(st)
+        e_les.steps(4)
+        L.sh_stream_synchronize(st)
+        a = g_les.velocity()
+        b = e_les.velocity()
+        L.sh_stream_synchronize(st)
+        assert np.array_equal(_host(a), _host(b)), rep
+    L.sh_graph_destroy(g)
+    g_les.close()
+    e_les.close()
+    for m in ms:
+        m.close()
+    L.sh_stream_destroy(st)
+
+
+def _kolmogorov(n, A=0.7, k=2):
+    """u = (A sin(k y), 0, 0) in Fourier space (complex128, 3 fields)."""
+    x = 2 * np.pi * np.arange(n) / n
+    y = x[None, :, None] * np.ones((n, n, n))
+    u = np.stack([A * np.sin(k * y), 0 * y, 0 * y])
+    return np.stack([np.fft.fftn(c) for c in u])
+
+
+@pytest.mark.parametrize("mode", ["fused64", "op64", "fused32"])
+def test_forcing_kolmogorov_steady_state(hip, mode):
+    """f = nu k^2 u balances viscosity, u x omega is a pure gradient: the
+    discrete scheme (Euler start-up, AB2 + CN) keeps u to rounding."""
+    import torch
+    n, nu, k = 32, 0.05, 2
+    V0 = _kolmogorov(n, k=k)
+    prec = 32 if mode == "fused32" else 64
+    ns = hip.NS((n, n, n), nu, 1e-2, prec, fused=mode != "op64")
+    f = nu * k * k * V0
+    fd = _dev(f, torch.complex64 if prec == 32 else torch.complex128)
+    hip.lib().sh_ns_set_forcing(ns.h, hip.ptr(fd))
+    ns.set_initial(_dev(V0))
+    ns.startup()
+    ns.steps(50)
+    V = _host(ns.velocity())
+    assert rel_l2(V, V0) < (1e-6 if prec == 32 else 1e-12)
+    # without the force the flow decays
+    hip.lib().sh_ns_set_forcing(ns.h, None)
+    ns.steps(50)
+    assert rel_l2(_host(ns.velocity()), V0) > 1e-2
+    ns.close()
+
+
+def test_forcing_fused_matches_op_by_op(hip):
+    import torch
+    n, nu, K = 32, 0.01, 1e-3
+    V0 = O.solenoidal((n, n, n), 71, 3.0, 1.0)
+    f = 0.5 * O.solenoidal((n, n, n), 72, 2.0, 1.0)
+    fd = _dev(f)
+    out = []
+    for fused in (True, False):
+        ns = hip.NS((n, n, n), nu, K, 64, fused=fused)
+        hip.lib().sh_ns_set_forcing(ns.h, hip.ptr(fd))
+        ns.set_initial(_dev(V0))
+        ns.startup()
+        ns.steps(10)
+        out.append(_host(ns.velocity()))
+        ns.close()
+    assert rel_l2(out[0], out[1]) < 1e-12
+    # the force changed the trajectory
+    ref = O.NS((n, n, n), nu, K)
+    ref.set_initial(V0)
+    ref.startup()
+    ref.steps(10)
+    assert rel_l2(out[0], ref.velocity()) > 1e-4
+    del torch
+
+
+def test_forcing_ensemble_boxes(hip):
+    """Per-box forces in the ensemble layout: each box stays on its own
+    Kolmogorov steady state."""
+    import torch
+    n = 32
+    nus = [0.05, 0.02]
+    amps = [0.7, 0.3]
+    L = hip.lib()
+    h = ctypes.c_void_p()
+    L.sh_ns_create_ensemble(ctypes.byref(h), n, 2, (ctypes.c_double * 2)(*nus), 1e-2, 64, None)
+    F = torch.empty((3, 2, n, n, n), dtype=torch.complex128, device="cuda")
+    V0s = []
+    for b in range(2):
+        V0 = _kolmogorov(n, A=amps[b], k=2)
+        V0s.append(V0)
+        F[:, b] = _dev(nus[b] * 4 * V0)
+    L.sh_ns_set_forcing(h, hip.ptr(F))
+    for b in range(2):
+        L.sh_ns_set_initial_box(h, b, hip.ptr(_dev(V0s[b])))
+    L.sh_ns_steps(h, 30)
+    for b in range(2):
+        V = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+        L.sh_ns_get_velocity_box(h, b, hip.ptr(V))
+        assert rel_l2(_host(V), V0s[b]) < 1e-12, b
+    L.sh_ns_destroy(h)
