@@ -191,6 +191,11 @@ int sh_ns_snapshot(sh_ns_t handle, int ratio, void* out);
  * contents are read at every step, so the caller may rewrite them between
  * steps.  NULL removes the force.  No counterpart in the reference. */
 int sh_ns_set_forcing(sh_ns_t handle, const void* f3);
+/* on = 1: drop the nonlinear term (with the body force) at every mode with
+ * some |k_i| > N_i/3 (2/3-rule dealiasing).  Off by default, as in the
+ * reference, which only zeroes the Nyquist modes.  No counterpart in the
+ * reference; used by the coarse-grid LES.  Synchronous. */
+int sh_ns_set_dealias(sh_ns_t handle, int on);
 
 /* ------------------------------------------------------------------ streams and graphs
  * Default-priority non-blocking streams; stream capture into a hipGraph
@@ -267,11 +272,17 @@ int sh_mfma_probe(const void* A, const void* B, void* D, void* stream);
 typedef struct sh_les_s* sh_les_t;
 int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, int clip, void* stream);
 int sh_les_destroy(sh_les_t handle);
+/* 2/3-rule dealiasing of the coarse solver (sh_ns_set_dealias); set it
+ * before the initial condition.  Synchronous. */
+int sh_les_set_dealias(sh_les_t handle, int on);
 /* initial condition from a physical filtered velocity (float [3][n^3]):
  * transform, zero the Nyquist modes, project onto divergence-free fields,
  * then the start-up step (exec/ns.cpp:132-153) with the closure force */
 int sh_les_set_initial_physical(sh_les_t handle, const void* ubar3);
 int sh_les_steps(sh_les_t handle, int nsteps);
+/* tune the solver's and the closure transforms' schedules on this grid
+ * (after the initial condition is set; the state is kept).  Synchronous. */
+int sh_les_autotune(sh_les_t handle, int reps);
 /* velocity in Fourier space, 3 contiguous complex128 fields */
 int sh_les_get_velocity(sh_les_t handle, void* out_v3);
 /* the model stress of the current velocity, float [6][n^3] physical */

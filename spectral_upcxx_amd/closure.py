@@ -100,12 +100,14 @@ class LES:
     """FP32 coarse-grid solver with the network's stress as body force
     (model=None: the no-model solve)."""
 
-    def __init__(self, n, nu, K, model=None, clip=True, strm=None):
+    def __init__(self, n, nu, K, model=None, clip=True, strm=None, dealias=False):
         self.n = n
         self.h = ctypes.c_void_p()
         self.stream = stream() if strm is None else strm
         lib().sh_les_create(ctypes.byref(self.h), int(n), float(nu), float(K), model.h if model else None,
                             int(bool(clip)), self.stream)
+        if dealias:
+            lib().sh_les_set_dealias(self.h, 1)
 
     def close(self):
         if self.h:
@@ -117,6 +119,9 @@ class LES:
 
     def steps(self, k):
         lib().sh_les_steps(self.h, int(k))
+
+    def autotune(self, reps=2):
+        lib().sh_les_autotune(self.h, int(reps))
 
     def velocity(self):
         import torch

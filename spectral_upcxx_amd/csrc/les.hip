@@ -14,7 +14,7 @@
 // is the transform of real fields and the packing is exact.
 #include <cmath>
 
-#include "sh_common.hpp"
+#include "fft_core.hpp"
 
 namespace sh {
 
@@ -24,29 +24,55 @@ int mlp_infer_packed(sh_mlp_t h, const float2* X, long ld, long n, float2* Y, lo
 int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float* Y, long ldy, int clip,
                             hipStream_t st);
 void* ns_fused_velocity(sh_ns_t s);
+int ns_step_phase(sh_ns_t s, int phase);
 
 // wavenumber of index i on an n-point axis, with the Nyquist index as zero
 __device__ __forceinline__ float kd(int i, int n) { return (2 * i == n) ? 0.f : (float)(i <= n / 2 ? i : i - n); }
 
 __device__ __forceinline__ float2 ik(float k, float2 v) { return make_float2(-k * v.y, k * v.x); }
 
-// Z_m = g_{2m} + i g_{2m+1} with g_{3a+b} = i k_b V_a
-__global__ void k_les_grad_pack(const float2* __restrict__ V, long fs, float2* __restrict__ Z, long zs, int n) {
-    const long N = (long)n * n * n;
-    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((long)n * n));
-        const float kk[3] = {kd(i, n), kd(j, n), kd(k, n)};
-        float2 g[10];
-        for (int a = 0; a < 3; a++) {
-            const float2 v = V[a * fs + p];
-            for (int b = 0; b < 3; b++) g[3 * a + b] = ik(kk[b], v);
+// Step 1 (Z_m = g_{2m} + i g_{2m+1}, g_{3a+b} = i k_b V_a) fused with the x pass of step 2: block (x, m) forms packed field m
+// of LPB x-lines on load (the derivative factors of line (j, k) and of each
+// position) and transforms it back along x in registers and LDS.
+template <class P, int LPB>
+__global__ __launch_bounds__(P::TPL* LPB) void k_les_grad_x(const float2* __restrict__ V, float2* __restrict__ Z,
+                                                            const float2* __restrict__ tw, int n) {
+    constexpr int N = P::N;
+    constexpr int LS = N + N / 16 + 1;
+    __shared__ float sre[LPB * LS];
+    __shared__ float sim[LPB * LS];
+    const int tid = threadIdx.x % P::TPL;
+    const int l = threadIdx.x / P::TPL;
+    const long nlines = (long)n * n;
+    const long N3 = nlines * N;
+    const long line = (long)blockIdx.x * LPB + l;
+    const bool valid = line < nlines;
+    const long lc = valid ? line : 0;
+    const int m = blockIdx.y;
+    // components 2m and 2m+1 of g_{3a+b} = i k_b V_a (the second is absent for m = 4)
+    const int a0 = (2 * m) / 3, b0 = (2 * m) % 3, a1 = (2 * m + 1) / 3, b1 = (2 * m + 1) % 3;
+    const bool two = m < 4;
+    const float ky = kd((int)(lc % n), n), kz = kd((int)(lc / n), n);
+    const float2* v0 = V + a0 * N3 + lc * N;
+    const float2* v1 = V + (two ? a1 : a0) * N3 + lc * N;
+    float2 x[P::E];
+    for_in<P>(tid, [&](int s, int pos) {
+        const float kx = kd(pos, N);
+        const float kb0 = b0 == 0 ? kx : b0 == 1 ? ky : kz;
+        const float2 g0 = ik(kb0, v0[pos]);
+        float2 z = g0;
+        if (two) {
+            const float kb1 = b1 == 0 ? kx : b1 == 1 ? ky : kz;
+            const float2 g1 = ik(kb1, v1[pos]);
+            z = make_float2(g0.x - g1.y, g0.y + g1.x);
         }
-        g[9] = make_float2(0.f, 0.f);
-        for (int m = 0; m < 5; m++) {
-            const float2 a = g[2 * m], b = g[2 * m + 1];
-            Z[m * zs + p] = make_float2(a.x - b.y, a.y + b.x);
-        }
-    }
+        x[s] = z;
+    });
+    auto idx = [&](int pos) { return l * LS + pos + (pos >> 4); };
+    line_fft<P, 1, float>(x, tid, sre, sim, idx, tw);
+    if (!valid) return;
+    float2* zb = Z + m * N3 + lc * N;
+    for_out<P>(tid, [&](int s, int pos) { zb[pos] = x[s]; });
 }
 
 // f_a = -sum_b i k_b tau_ab from the packed stress transforms P_c:
@@ -138,16 +164,40 @@ static unsigned lgrid(long n) {
         if (_rc) return _rc; \
     } while (0)
 
-// physical packed gradient fields Z (5 complex64 fields) of V (3 Fourier fields)
-int les_gradients(const float2* V, float2* Z, int n, hipStream_t st) {
-    const long N = (long)n * n * n;
-    hipLaunchKernelGGL(k_les_grad_pack, dim3(lgrid(N)), dim3(256), 0, st, V, N, Z, N, n);
-    SH_LAUNCH_CHECK("k_les_grad_pack");
-    {
-        ProfScope ps("les_fft_inv", st);
-        RC(fft3d<float>(Z, 5, N, n, n, n, +1, SH_NORMALIZE, st));
-    }
+template <class P, int LPB>
+int launch_grad_x(const float2* V, float2* Z, int n, hipStream_t st) {
+    const float2* tw = (const float2*)twiddles(P::N, SH_F32);
+    if (!tw) return SH_ENOMEM;
+    const long nlines = (long)n * n;
+    hipLaunchKernelGGL((k_les_grad_x<P, LPB>), dim3((unsigned)((nlines + LPB - 1) / LPB), 5), dim3(P::TPL * LPB), 0,
+                       st, V, Z, tw, n);
+    SH_LAUNCH_CHECK("k_les_grad_x");
     return SH_OK;
+}
+
+// physical packed gradient fields Z (5 complex64 fields) of V (3 Fourier
+// fields, n^3 each): the fused derivative + x pass, then the y and z passes
+// (the z pass scales by 1/n^3)
+int les_gradients(const float2* V, float2* Z, int n, hipStream_t st) {
+    const long sxy = (long)n * n, N = sxy * n;
+    ProfScope ps("les_fft_inv", st);
+    int rc = SH_EUNSUPPORTED;
+    switch (n) {
+#define SH_CASE(n_, P_, L_, S_)                   \
+    case n_:                                      \
+        rc = launch_grad_x<P_, L_>(V, Z, n, st); \
+        break;
+        SH_FOR_EACH_FFT_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    if (rc) {
+        if (rc == SH_EUNSUPPORTED) set_error("les: unsupported grid size");
+        return rc;
+    }
+    Epi<float> plain{1.f, 0, 1, -1, -1};
+    RC(pass_strided<float>(Z, n, n, 1, n, sxy, n, 5, N, +1, plain, st, ROLE_Y));
+    Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
+    return pass_strided<float>(Z, n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z);
 }
 
 }  // namespace sh
@@ -163,27 +213,34 @@ struct sh_les_s {
     sh_ns_t ns;
     float2 *F, *Z, *T, *Vc;  // force (3), packed gradients (5), packed stresses (3), FP32 velocity (3)
     double2* V64;            // 3 fields for the initial condition
+    // the closure runs on `side` while the solver's kernels A-D run on st;
+    // kernel E waits for ev_force, the next closure for ev_state
+    hipStream_t side;
+    hipEvent_t ev_state, ev_force;
 };
 
 namespace {
 
-int sgs_force(sh_les_s* h, const float2* V) {
+int sgs_force(sh_les_s* h, const float2* V, hipStream_t st) {
     const long N = h->N;
-    RC(les_gradients(V, h->Z, h->n, h->st));
+    RC(les_gradients(V, h->Z, h->n, st));
     {
-        ProfScope ps("les_mlp", h->st);
-        RC(mlp_infer_packed(h->model, h->Z, N, N, h->T, N, h->clip, h->st));
+        ProfScope ps("les_mlp", st);
+        RC(mlp_infer_packed(h->model, h->Z, N, N, h->T, N, h->clip, st));
     }
     {
-        ProfScope ps("les_fft_fwd", h->st);
-        RC(fft3d<float>(h->T, 3, N, h->n, h->n, h->n, -1, 0, h->st));
+        ProfScope ps("les_fft_fwd", st);
+        RC(fft3d<float>(h->T, 3, N, h->n, h->n, h->n, -1, 0, st));
     }
-    hipLaunchKernelGGL(k_les_div_unpack, dim3(lgrid(N)), dim3(256), 0, h->st, h->T, N, h->F, N, h->n);
+    hipLaunchKernelGGL(k_les_div_unpack, dim3(lgrid(N)), dim3(256), 0, st, h->T, N, h->F, N, h->n);
     SH_LAUNCH_CHECK("k_les_div_unpack");
     return SH_OK;
 }
 
 void free_les(sh_les_s* h) {
+    if (h->ev_state) (void)hipEventDestroy(h->ev_state);
+    if (h->ev_force) (void)hipEventDestroy(h->ev_force);
+    if (h->side) (void)hipStreamDestroy(h->side);
     if (h->ns) sh_ns_destroy(h->ns);
     float2* bufs[4] = {h->F, h->Z, h->T, h->Vc};
     for (float2* b : bufs)
@@ -221,7 +278,12 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
     if (!rc) rc = pool_alloc(&p, sizeof(double2) * h->N * 3);
     h->V64 = (double2*)p;
     if (!rc && model) rc = sh_ns_set_forcing(h->ns, h->F);
+    // default-priority side stream and ordering events (no timing)
+    if (!rc && hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) rc = SH_EHIP;
+    if (!rc && hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
+    if (!rc && hipEventCreateWithFlags(&h->ev_force, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
     if (rc) {
+        if (rc == SH_EHIP) set_error("sh_les_create: stream or event creation failed");
         free_les(h);
         return rc;
     }
@@ -232,6 +294,7 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
 int sh_les_destroy(sh_les_t h) {
     if (!h) return SH_OK;
     (void)hipStreamSynchronize(h->st);
+    (void)hipStreamSynchronize(h->side);
     free_les(h);
     return SH_OK;
 }
@@ -248,7 +311,7 @@ int sh_les_set_initial_physical(sh_les_t h, const void* ubar3) {
     if (h->model) {
         hipLaunchKernelGGL(k_z_to_c2, dim3(lgrid(3 * N)), dim3(256), 0, h->st, h->V64, h->Vc, 3 * N);
         SH_LAUNCH_CHECK("k_z_to_c2");
-        RC(sgs_force(h, h->Vc));
+        RC(sgs_force(h, h->Vc, h->st));
     }
     return sh_ns_startup(h->ns);
 }
@@ -257,10 +320,33 @@ int sh_les_steps(sh_les_t h, int nsteps) {
     if (!h || nsteps < 0) return SH_EINVAL;
     const float2* V = (const float2*)ns_fused_velocity(h->ns);
     for (int t = 0; t < nsteps; t++) {
-        if (h->model) RC(sgs_force(h, V));
-        RC(sh_ns_steps(h->ns, 1));
+        if (!h->model) {
+            RC(sh_ns_steps(h->ns, 1));
+            continue;
+        }
+        // the closure of V^n on the side stream, concurrent with kernels A-D,
+        // which read only the work fields; kernel E reads the force and then
+        // overwrites V, after which the next closure may start
+        SH_HIP(hipEventRecord(h->ev_state, h->st));
+        SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
+        RC(sgs_force(h, V, h->side));
+        SH_HIP(hipEventRecord(h->ev_force, h->side));
+        RC(ns_step_phase(h->ns, 0));
+        SH_HIP(hipStreamWaitEvent(h->st, h->ev_force, 0));
+        RC(ns_step_phase(h->ns, 1));
     }
     return SH_OK;
+}
+
+int sh_les_set_dealias(sh_les_t h, int on) {
+    if (!h) return SH_EINVAL;
+    return sh_ns_set_dealias(h->ns, on);
+}
+
+int sh_les_autotune(sh_les_t h, int reps) {
+    if (!h || reps <= 0) return SH_EINVAL;
+    RC(sh_autotune_fft3d(h->n, h->n, h->n, SH_F32, reps, h->st));
+    return sh_ns_autotune(h->ns, reps);
 }
 
 int sh_les_get_velocity(sh_les_t h, void* out_v3) {
@@ -282,7 +368,7 @@ int sh_les_force(sh_les_t h, void* out_f3) {
         set_error("sh_les_force: invalid argument (an LES with a model)");
         return SH_EINVAL;
     }
-    RC(sgs_force(h, (const float2*)ns_fused_velocity(h->ns)));
+    RC(sgs_force(h, (const float2*)ns_fused_velocity(h->ns), h->st));
     SH_HIP(hipMemcpyAsync(out_f3, h->F, sizeof(float2) * 3 * h->N, hipMemcpyDeviceToDevice, h->st));
     return SH_OK;
 }

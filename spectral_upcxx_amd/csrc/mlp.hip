@@ -26,30 +26,27 @@ namespace sh {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kIn = 9, kOut = 6, kH = 64;
 constexpr int OFF_W1 = 0, OFF_B1 = 16 * 64, OFF_W2 = OFF_B1 + 64, OFF_B2 = OFF_W2 + 64 * 64,
               OFF_W3 = OFF_B2 + 64, OFF_B3 = OFF_W3 + 64 * 16, NPARAM = OFF_B3 + 16;
 constexpr int kGradWaves = 1024;  // partial-gradient slots
 
-// round-to-nearest-even FP32 -> BF16 (finite inputs)
-__device__ __forceinline__ short f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
-
 __device__ __forceinline__ f32x4 mfma(s16x4 a, s16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// two FP32 -> BF16 conversions in one v_cvt_pk_bf16_f32 (round to nearest even)
+__device__ __forceinline__ s16x2 cvt2(float a, float b) {
+    return __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+
 __device__ __forceinline__ s16x4 pack4(float a, float b, float c, float d) {
-    s16x4 r;
-    r[0] = f2bf(a);
-    r[1] = f2bf(b);
-    r[2] = f2bf(c);
-    r[3] = f2bf(d);
-    return r;
+    const s16x2 lo = cvt2(a, b), hi = cvt2(c, d);
+    return s16x4{lo[0], lo[1], hi[0], hi[1]};
 }
 
 __device__ __forceinline__ s16x4 to_bf(f32x4 v) { return pack4(v[0], v[1], v[2], v[3]); }
@@ -90,17 +87,32 @@ __device__ void load_bwd(const float* __restrict__ P, BwdW& w, int lane) {
     }
 }
 
-// biases of features 16 mt + 4q + j as a D-layout accumulator
-__device__ __forceinline__ f32x4 bias4(const float* __restrict__ b, int mt, int lane) {
+// biases of features 16 mt + 4q + j as D-layout accumulators, held in registers
+struct FwdB {
+    f32x4 b1[4], b2[4], b3;
+};
+
+__device__ void load_bias(const float* __restrict__ P, FwdB& b, int lane) {
     const int q = lane / 16;
-    f32x4 r;
-    for (int j = 0; j < 4; j++) r[j] = b[16 * mt + 4 * q + j];
-    return r;
+    for (int mt = 0; mt < 4; mt++)
+        for (int j = 0; j < 4; j++) {
+            b.b1[mt][j] = P[OFF_B1 + 16 * mt + 4 * q + j];
+            b.b2[mt][j] = P[OFF_B2 + 16 * mt + 4 * q + j];
+        }
+    for (int j = 0; j < 4; j++) b.b3[j] = P[OFF_B3 + 4 * q + j];
+}
+
+// tanh x = sign(x) (1 - e) / (1 + e), e = exp(-2|x|) = 2^(-2 log2(e) |x|), on
+// the hardware exp2 and reciprocal (about 1e-7 absolute error, far below the
+// BF16 rounding that follows)
+__device__ __forceinline__ float tanh_fast(float x) {
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(x));
+    return copysignf((1.f - e) * __builtin_amdgcn_rcpf(1.f + e), x);
 }
 
 __device__ __forceinline__ f32x4 tanh4(f32x4 v) {
     f32x4 r;
-    for (int j = 0; j < 4; j++) r[j] = tanhf(v[j]);
+    for (int j = 0; j < 4; j++) r[j] = tanh_fast(v[j]);
     return r;
 }
 
@@ -115,17 +127,25 @@ __device__ __forceinline__ long fidx(int f, long s, long ld) {
         return (long)f * ld + s;
 }
 
+// per-lane input scales of features 4q + j (zero for the padding features)
+__device__ __forceinline__ f32x4 load_scale_in(const float* __restrict__ sc, int lane) {
+    const int q = lane / 16;
+    f32x4 r;
+    for (int j = 0; j < 4; j++) r[j] = 4 * q + j < kIn ? sc[4 * q + j] : 0.f;
+    return r;
+}
+
 // B operand X^T for samples s0..s0+15: lane holds features 4q+j of sample
 // s0 + r, scaled; features >= 9 and samples >= n are zero
 template <bool PACKED>
-__device__ __forceinline__ s16x4 load_x(const float* __restrict__ X, long ld, long s0, long n,
-                                        const float* __restrict__ sc, int lane) {
+__device__ __forceinline__ s16x4 load_x(const float* __restrict__ X, long ld, long s0, long n, f32x4 sc,
+                                        int lane) {
     const long s = s0 + lane % 16;
     const int q = lane / 16;
     float xf[4];
     for (int j = 0; j < 4; j++) {
         const int f = 4 * q + j;
-        xf[j] = (f < kIn && s < n) ? X[fidx<PACKED>(f, s, ld)] * sc[f] : 0.f;
+        xf[j] = (f < kIn && s < n) ? X[fidx<PACKED>(f, s, ld)] * sc[j] : 0.f;
     }
     return pack4(xf[0], xf[1], xf[2], xf[3]);
 }
@@ -135,21 +155,19 @@ struct Fwd {
     s16x4 h1b[4], h2b[4];
 };
 
-__device__ __forceinline__ void forward(const FwdW& w, const float* __restrict__ P, s16x4 xb, Fwd& o,
-                                        int lane) {
+__device__ __forceinline__ void forward(const FwdW& w, const FwdB& b, s16x4 xb, Fwd& o) {
     for (int mt = 0; mt < 4; mt++) {
-        f32x4 acc = bias4(P + OFF_B1, mt, lane);
-        acc = mfma(w.w1t[mt], xb, acc);
+        const f32x4 acc = mfma(w.w1t[mt], xb, b.b1[mt]);
         o.h1[mt] = tanh4(acc);
         o.h1b[mt] = to_bf(o.h1[mt]);
     }
     for (int mt = 0; mt < 4; mt++) {
-        f32x4 acc = bias4(P + OFF_B2, mt, lane);
+        f32x4 acc = b.b2[mt];
         for (int kb = 0; kb < 4; kb++) acc = mfma(w.w2t[mt][kb], o.h1b[kb], acc);
         o.h2[mt] = tanh4(acc);
         o.h2b[mt] = to_bf(o.h2[mt]);
     }
-    f32x4 acc = bias4(P + OFF_B3, 0, lane);
+    f32x4 acc = b.b3;
     for (int kb = 0; kb < 4; kb++) acc = mfma(w.w3t[kb], o.h2b[kb], acc);
     o.y = acc;
 }
@@ -165,19 +183,19 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, 
     const int r = lane % 16, q = lane / 16;
     const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) / 64;
     const long nwaves = (long)gridDim.x * blockDim.x / 64;
-    const float* sc_out = sc + kIn;
     FwdW w;
+    FwdB bias;
     load_fwd(P, w, lane);
+    load_bias(P, bias, lane);
+    const f32x4 sc_in = load_scale_in(sc, lane);
+    f32x4 sc_out;
+    for (int j = 0; j < 4; j++) sc_out[j] = 4 * q + j < kOut ? sc[kIn + 4 * q + j] : 0.f;
     for (long s0 = wave * 16; s0 < n; s0 += nwaves * 16) {
-        const s16x4 xb = load_x<PACKED_IN>(X, ld, s0, n, sc, lane);
+        const s16x4 xb = load_x<PACKED_IN>(X, ld, s0, n, sc_in, lane);
         Fwd o;
-        forward(w, P, xb, o, lane);
+        forward(w, bias, xb, o);
         const long s = s0 + r;
-        f32x4 y;
-        for (int j = 0; j < 4; j++) {
-            const int f = 4 * q + j;
-            y[j] = f < kOut ? o.y[j] * sc_out[f] : 0.f;
-        }
+        f32x4 y = o.y * sc_out;
         if (clip) {
             // every lane gathers its sample's six stresses (lanes q = 0, 1 hold them)
             float t[6];
@@ -231,11 +249,15 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, co
     const int r = lane % 16, q = lane / 16;
     const long wave = blockIdx.x;
     const long nwaves = gridDim.x;
-    const float* sc_out = sc + kIn;
     FwdW w;
     BwdW wb;
+    FwdB bias;
     load_fwd(P, w, lane);
     load_bwd(P, wb, lane);
+    load_bias(P, bias, lane);
+    const f32x4 sc_in = load_scale_in(sc, lane);
+    f32x4 inv_out;  // targets are divided by the output scale
+    for (int j = 0; j < 4; j++) inv_out[j] = 4 * q + j < kOut ? 1.f / sc[kIn + 4 * q + j] : 0.f;
     f32x4 gw1[4], gw2[4][4], gw3[4], gb1[4], gb2[4], gb3;
     for (int a = 0; a < 4; a++) {
         gw1[a] = gw3[a] = gb1[a] = gb2[a] = f32x4{0, 0, 0, 0};
@@ -244,9 +266,9 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, co
     gb3 = f32x4{0, 0, 0, 0};
     float sq = 0;
     for (long s0 = wave * 16; s0 < n; s0 += nwaves * 16) {
-        const s16x4 xb = load_x<false>(X, ld, s0, n, sc, lane);
+        const s16x4 xb = load_x<false>(X, ld, s0, n, sc_in, lane);
         Fwd o;
-        forward(w, P, xb, o, lane);
+        forward(w, bias, xb, o);
         // dY = 2 (Y - T) / count on valid outputs and samples
         const long s = s0 + r;
         f32x4 dy;
@@ -254,7 +276,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, co
             const int f = 4 * q + j;
             float d = 0;
             if (f < kOut && s < n) {
-                const float e = o.y[j] - Tg[f * ld + s] / sc_out[f];
+                const float e = o.y[j] - Tg[f * ld + s] * inv_out[j];
                 sq += e * e;
                 d = 2.f * e * inv_count;
             }
@@ -365,12 +387,9 @@ __global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ m, float* 
 __global__ void k_mfma_probe(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ D) {
     const int lane = threadIdx.x;
     const int r = lane % 16, q = lane / 16;
-    s16x4 a, b;
-    for (int j = 0; j < 4; j++) {
-        a[j] = f2bf(A[r * 16 + 4 * q + j]);
-        b[j] = f2bf(B[(4 * q + j) * 16 + r]);
-    }
-    const f32x4 d = mfma(a, b, f32x4{0, 0, 0, 0});
+    const float* a = A + r * 16 + 4 * q;
+    const float* b = B + 4 * q * 16 + r;
+    const f32x4 d = mfma(pack4(a[0], a[1], a[2], a[3]), pack4(b[0], b[16], b[32], b[48]), f32x4{0, 0, 0, 0});
     for (int j = 0; j < 4; j++) D[(4 * q + j) * 16 + r] = d[j];
 }
 

@@ -43,6 +43,7 @@ struct StepCoef {
     T c1;          // nu*K/2                (exec/ns.cpp:174)
     double c1d;    // nu*K/2 in FP64 for the CN factor (exec/ns.cpp:186)
     T inv_n3;      // 1/(Nx Ny Nz)          (src/Scalar.cpp:223-227)
+    int dealias;   // 1: products with some |k_i| > N_i/3 are dropped (2/3 rule)
 };
 
 template <int N>
@@ -251,7 +252,8 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     if (valid) {
         for_out<P>(tid, [&](int s, int pos) {
             const T k0 = freq<T>(pos, N);
-            const bool z = zline || pos == N / 2;
+            const bool z = zline || pos == N / 2 ||
+                           (cf.dealias && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
             C2<T> nn[3], v1[3], f[3], L[3], S[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -455,6 +457,7 @@ struct sh_ns_s {
     double div_avg;
     const void* forcing;  // body force (3 fields per box, precision prec) or null
     int cur_box;          // box whose start-up is running
+    int dealias;          // 2/3-rule truncation of the nonlinear term (0: none, as the reference)
     std::vector<void*> owned;
 };
 
@@ -490,6 +493,24 @@ int cross(sh_ns_s* s, const double2* a, const double2* b, double2* o) {
         if (_rc) return _rc; \
     } while (0)
 
+// 2/3-rule truncation of 3 fields (the fused kernel E applies the same mask)
+__global__ void k_dealias3(double2* __restrict__ f, long fs, int nx, int ny, int nz) {
+    const long N = (long)nx * ny * nz;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((long)nx * ny));
+        if (3 * fabs(freq<double>(i, nx)) > nx || 3 * fabs(freq<double>(j, ny)) > ny ||
+            3 * fabs(freq<double>(k, nz)) > nz)
+            for (int d = 0; d < 3; d++) f[d * fs + p] = make_double2(0.0, 0.0);
+    }
+}
+
+int dealias64(sh_ns_s* s, double2* cr) {
+    if (!s->dealias) return SH_OK;
+    hipLaunchKernelGGL(k_dealias3, dim3(egrid(s->N)), dim3(256), 0, s->st, cr, s->N, s->nx, s->ny, s->nz);
+    SH_LAUNCH_CHECK("k_dealias3");
+    return SH_OK;
+}
+
 // cr += the body force of the current box (an FP32 force is widened through work)
 int add_forcing64(sh_ns_s* s, double2* cr) {
     if (!s->forcing) return SH_OK;
@@ -513,6 +534,7 @@ int startup64(sh_ns_s* s) {
     const long V = 3 * s->N;
     RC(cross(s, s->V0, s->om0, s->cr1));
     RC(add_forcing64(s, s->cr1));
+    RC(dealias64(s, s->cr1));
     RC(vlap(s, s->V0, s->lap1));
     RC(sh_scale_z(s->lap1, s->nu, s->lap1, V, s->st));
     RC(sh_add_z(s->S, s->cr1, s->lap1, V, s->st));
@@ -525,6 +547,7 @@ int startup64(sh_ns_s* s) {
     RC(curl(s, s->V1, s->om1));
     RC(cross(s, s->V0, s->om0, s->cr0));
     RC(add_forcing64(s, s->cr0));
+    RC(dealias64(s, s->cr0));
     return SH_OK;
 }
 
@@ -535,6 +558,7 @@ int step64(sh_ns_s* s) {
     const int nx = s->nx, ny = s->ny, nz = s->nz;
     RC(cross(s, s->V1, s->om1, s->cr1));
     RC(add_forcing64(s, s->cr1));
+    RC(dealias64(s, s->cr1));
     RC(sh_scale_z(s->Fv, 3.0, s->cr1, V, s->st));
     RC(sh_sub_z(s->Fv, s->Fv, s->cr0, V, s->st));
     RC(sh_scale_z(s->Fv, K / 2, s->Fv, V, s->st));
@@ -624,16 +648,19 @@ int launch_fourier_x(sh_ns_s* s) {
     X(256, W256, 4) X(256, Q256, 8) X(256, Q256, 4) X(512, W512, 4) X(512, W512, 2) X(512, P512, 4)
 #define SH_CROSS_F32_VARIANTS(X)                                                                  \
     X(8, P8, 16) X(12, Q12, 16) X(16, P16, 16) X(24, Q24, 16) X(32, Q32, 16) X(48, Q48, 16)       \
-    X(64, P64, 16) X(64, W64, 16) X(96, Q96, 16) X(96, Q96, 8) X(128, Q128, 16) X(256, Q256, 16)  \
+    X(64, P64, 16) X(64, W64, 16) X(64, P64, 8) X(64, W64, 8)                                     \
+    X(96, Q96, 16) X(96, Q96, 8) X(128, Q128, 16) X(256, Q256, 16)                                \
     X(256, Q256, 8) X(256, W256, 16) X(256, W256, 8) X(512, P512, 8) X(512, P512, 4)              \
     X(512, W512, 4)
 #define SH_FOURIER_F64_VARIANTS(X)                                                                \
     X(8, P8, 256) X(12, Q12, 64) X(16, P16, 64) X(24, Q24, 32) X(32, Q32, 32) X(48, Q48, 21)      \
-    X(64, W64, 16) X(64, P64, 32) X(96, Q96, 10) X(128, W128, 8) X(256, W256, 4) X(256, Q256, 8)  \
+    X(64, W64, 16) X(64, P64, 32) X(64, W64, 8) X(64, P64, 8)                                     \
+    X(96, Q96, 10) X(128, W128, 8) X(256, W256, 4) X(256, Q256, 8)                                \
     X(256, W256, 2) X(512, W512, 2) X(512, P512, 4)
 #define SH_FOURIER_F32_VARIANTS(X)                                                                \
     X(8, P8, 256) X(12, Q12, 64) X(16, P16, 64) X(24, Q24, 32) X(32, Q32, 32) X(48, Q48, 21)      \
-    X(64, P64, 32) X(64, W64, 16) X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4) \
+    X(64, P64, 32) X(64, W64, 16) X(64, P64, 16) X(64, P64, 8) X(64, W64, 8) X(64, W64, 4)       \
+    X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4)                               \
     X(256, Q256, 4) X(512, P512, 4) X(512, W512, 2)
 
 using NsLaunch = int (*)(sh_ns_s*);
@@ -714,8 +741,10 @@ int dispatch_fourier_x(sh_ns_s* s) {
     return run_variant<T>(fourier_variants<T>(s->nx), s, s->nx, ROLE_FOURIER);
 }
 
+// kernels A (after a restart), B, C, D of a step: everything before the
+// Fourier update, which is the only kernel that reads the body force
 template <typename T>
-int step_fused(sh_ns_s* s) {
+int step_fused_front(sh_ns_s* s) {
     if (!s->nonlinear) {
         set_error("ns: the fused schedule always includes the nonlinear term");
         return SH_EINVAL;
@@ -738,6 +767,12 @@ int step_fused(sh_ns_s* s) {
         RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3 * s->nboxes, s->N, -1,
                            plain, s->st, ROLE_Y));
     }
+    return SH_OK;
+}
+
+template <typename T>
+int step_fused(sh_ns_s* s) {
+    RC(step_fused_front<T>(s));
     RC(dispatch_fourier_x<T>(s));
     s->have_div = true;
     return SH_OK;
@@ -845,6 +880,7 @@ int upload_coefs_t(sh_ns_s* s) {
         h[b].c1 = T(nu * s->K / 2);
         h[b].c1d = nu * s->K / 2;
         h[b].inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
+        h[b].dealias = s->dealias;
     }
     SH_HIP(hipMemcpyAsync(s->coefs, h.data(), sizeof(StepCoef<T>) * s->nboxes, hipMemcpyHostToDevice, s->st));
     SH_HIP(hipStreamSynchronize(s->st));
@@ -917,6 +953,7 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
     s->snap = nullptr;
     s->forcing = nullptr;
     s->cur_box = 0;
+    s->dealias = 0;
     const size_t z = sizeof(double2);
     int rc = 0;
     // FP64 state used by the start-up (and by the op-by-op loop)
@@ -1078,6 +1115,12 @@ int sh_ns_set_forcing(sh_ns_t s, const void* f3) {
     return SH_OK;
 }
 
+int sh_ns_set_dealias(sh_ns_t s, int on) {
+    if (!s) return SH_EINVAL;
+    s->dealias = on != 0;
+    return s->fused ? upload_coefs(s) : SH_OK;
+}
+
 int sh_ns_set_nonlinear(sh_ns_t s, int on) {
     if (!s) return SH_EINVAL;
     s->nonlinear = on != 0;
@@ -1175,4 +1218,17 @@ int sh_ns_state_bytes(sh_ns_t s, size_t* bytes) {
 namespace sh {
 // the fused solver's current velocity (3 Fourier fields per box, precision prec)
 void* ns_fused_velocity(sh_ns_t s) { return s && s->fused ? s->V1f : nullptr; }
+}  // namespace sh
+
+namespace sh {
+// one fused step in two parts: phase 0 = kernels A-D, phase 1 = kernel E
+// (the only one that reads the body force), so a caller can compute the
+// force concurrently with phase 0
+int ns_step_phase(sh_ns_t s, int phase) {
+    if (!s || !s->fused) return SH_EINVAL;
+    if (phase == 0) return s->prec == SH_F64 ? step_fused_front<double>(s) : step_fused_front<float>(s);
+    int rc = s->prec == SH_F64 ? dispatch_fourier_x<double>(s) : dispatch_fourier_x<float>(s);
+    if (!rc) s->have_div = true;
+    return rc;
+}
 }  // namespace sh

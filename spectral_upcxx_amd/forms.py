@@ -13,7 +13,7 @@ import json
 import sys
 import time
 
-from . import lib, ptr, stream, NS, SH_F32, SH_F64
+from . import lib, ptr, stream, NS, SH_F32, SH_F64, SH_NORMALIZE
 
 
 def _torch():
@@ -256,7 +256,8 @@ def _spectrum_rel(Ea, Eb, kmax):
 
 def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=2.0, snap_dt=0.25,
               train_seeds=8, test_seeds=4, seed0=7100, k0=4.0, iters=3000, batch=65536, lr=2e-3,
-              clip=True, eval_dt=0.5, kcut=16, net_seed=5, log=None):
+              variants=((True, False), (False, False), (True, True), (False, True)), eval_dt=0.5, kcut=16,
+              net_seed=5, gain_candidates=(0.5, 1.0, 1.5, 2.0, 3.0, 4.0), log=None):
     """Learned-closure form: FP64 DNS at n_dns^3 box-filtered by `ratio`
     gives training pairs (filtered velocity gradients -> stresses); the
     network is trained on the GPU; the FP32 LES on the coarse grid with and
@@ -326,7 +327,6 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
             L.sh_les_features(n, ptr(ub), closure._offset(X, off), M, stream())
             T[:, off:off + nc] = s[3:]
             off += nc
-        tr["snaps"] = {0: tr["snaps"][0]}
     in_scale = (1.0 / X.std(dim=1)).cpu().numpy()
     out_scale = T.std(dim=1).cpu().numpy()
     g = torch.Generator(device="cuda")
@@ -366,41 +366,80 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     # ---- a-posteriori: LES with and without the model against the filtered DNS
     nb = n // 2
     kcut = min(kcut, nb - 1)
+    targets = list(range(eval_steps, total + 1, eval_steps))
+    key = "max_rel_err_k_le_%d" % kcut
+
+    def fdns_spectra(snaps):
+        out = {}
+        for t in targets:
+            uf = snaps[t][:3].reshape(3, n, n, n).contiguous()
+            out[t] = spectrum(physical_to_fourier(uf), nu, nb)
+        return out
+
+    def trajectory(ub0, mdl, clip, dealias):
+        les = closure.LES(n, nu, les_dt, mdl, clip=clip, dealias=dealias)
+        les.set_initial(ub0)
+        cur = every  # after the start-up step the LES is at DNS step `every`
+        Es = {}
+        for t in targets:
+            les.steps((t - cur) // every)
+            cur = t
+            Es[t] = spectrum(les.velocity(), nu, nb)
+        les.close()
+        return Es
+
+    def compare(Es, Fs):
+        rows = []
+        for t in targets:
+            E, e, d = Es[t]
+            Ef, ef, df = Fs[t]
+            finite = all(math.isfinite(x) for x in E[:kcut + 1]) and math.isfinite(e)
+            rel = _spectrum_rel(E, Ef, kcut) if finite else [float("inf")] * kcut
+            rows.append({"t": t * dt_dns, key: max(rel), "mean_rel_err_k_le_%d" % kcut: sum(rel) / kcut,
+                         "energy": e, "energy_fdns": ef, "dissipation": d,
+                         "ratio_by_k": [round(E[k] / Ef[k], 4) if finite else None for k in range(1, kcut + 1)]})
+        return rows
+
+    def score(rows):
+        return sum(r["mean_rel_err_k_le_%d" % kcut] for r in rows) / len(rows)
+
+    # schedules for the coarse grid (kept per process)
+    les = closure.LES(n, nu, les_dt, model, clip=True)
+    les.set_initial(train[0]["snaps"][0][:3].reshape(3, n, n, n).contiguous())
+    les.autotune(2)
+    les.close()
+    # a-posteriori choices -- backscatter clipping, 2/3 dealiasing of the
+    # coarse solver, the gain of the network's output -- made on training
+    # seeds only; the held-out seeds are evaluated once with the choice
+    calib = []
+    for tr in train[:2]:
+        calib.append((tr["snaps"][0][:3].reshape(3, n, n, n).contiguous(), fdns_spectra(tr["snaps"])))
+    scores = {}
+    for clip_v, dealias_v in variants:
+        for gain in gain_candidates:
+            model.set_scaling(in_scale, out_scale * gain)
+            s = sum(score(compare(trajectory(ub, model, clip_v, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
+            scores[(clip_v, dealias_v, gain)] = s
+            log("calibration clip=%d dealias=%d gain %.2f: mean rel err %.4f" % (clip_v, dealias_v, gain, s))
+    for dealias_v in sorted({d for _, d in variants}):
+        s = sum(score(compare(trajectory(ub, None, False, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
+        scores[("no_model", dealias_v)] = s
+        log("calibration no-model dealias=%d: mean rel err %.4f" % (dealias_v, s))
+    best = min((k for k in scores if k[0] != "no_model"), key=scores.get)
+    clip, dealias, best_gain = best
+    model.set_scaling(in_scale, out_scale * best_gain)
+    for tr in train:
+        tr["snaps"] = {0: tr["snaps"][0]}
     evals = []
     for te in test:
         ub0 = te["snaps"][0][:3].reshape(3, n, n, n).contiguous()
-        runs = {}
-        for name, mdl in (("model", model), ("no_model", None)):
-            les = closure.LES(n, nu, les_dt, mdl, clip=clip)
-            les.set_initial(ub0)
-            # after the start-up step the LES is at DNS step `every`
-            cur = every
-            Es = {}
-            for target in range(eval_steps, total + 1, eval_steps):
-                les.steps((target - cur) // every)
-                cur = target
-                Vl = les.velocity()
-                Es[target] = spectrum(Vl, nu, nb)
-            les.close()
-            runs[name] = Es
-        res = {"seed": te["seed"], "times": []}
-        for target in range(eval_steps, total + 1, eval_steps):
-            uf = te["snaps"][target][:3].reshape(3, n, n, n).contiguous()
-            Ef, ef, df = spectrum(physical_to_fourier(uf), nu, nb)
-            row = {"t": target * dt_dns, "energy_fdns": ef}
-            for name in runs:
-                E, e, d = runs[name][target]
-                rel = _spectrum_rel(E, Ef, kcut)
-                finite = all(math.isfinite(x) for x in E[:kcut + 1])
-                row[name] = {"max_rel_err_k_le_%d" % kcut: max(rel) if finite else float("inf"),
-                             "mean_rel_err_k_le_%d" % kcut: sum(rel) / len(rel) if finite else float("inf"),
-                             "energy": e, "rel_err_by_k": [round(x, 5) for x in rel]}
-            res["times"].append(row)
+        Fs = fdns_spectra(te["snaps"])
+        res = {"seed": te["seed"], "model": compare(trajectory(ub0, model, clip, dealias), Fs),
+               "no_model": compare(trajectory(ub0, None, False, dealias), Fs),
+               "no_model_plain": compare(trajectory(ub0, None, False, False), Fs)}
         evals.append(res)
         log("les seed %d: " % te["seed"] + ", ".join(
-            "t=%.2f model %.3f no-model %.3f" % (r["t"], r["model"]["max_rel_err_k_le_%d" % kcut],
-                                                 r["no_model"]["max_rel_err_k_le_%d" % kcut])
-            for r in res["times"]))
+            "t=%.2f model %.3f no-model %.3f" % (a["t"], a[key], b[key]) for a, b in zip(res["model"], res["no_model"])))
     # ---- speed: LES steps/s with each step's launches replayed from one graph
     speed = {}
     ub0 = test[0]["snaps"][0][:3].reshape(3, n, n, n).contiguous()
@@ -411,8 +450,8 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         if mdl is not None:
             m2 = closure.MLP(seed=net_seed, strm=st)
             m2.set_params(model.params())
-            m2.set_scaling(in_scale, out_scale)
-        les = closure.LES(n, nu, les_dt, m2, clip=clip, strm=st)
+            m2.set_scaling(in_scale, out_scale * best_gain)
+        les = closure.LES(n, nu, les_dt, m2, clip=clip, strm=st, dealias=dealias)
         torch.cuda.synchronize()
         les.set_initial(ub0)
         les.steps(2)
@@ -436,19 +475,21 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     L.sh_stream_destroy(st)
     model_params = model.params()
     model.close()
-    last = [r["times"][-1] for r in evals]
-    key = "max_rel_err_k_le_%d" % kcut
     return {
         "form": "les", "dns_grid": n_dns, "les_grid": n, "ratio": ratio, "nu": nu, "dt_dns": dt_dns,
         "dt_les": les_dt, "t_end": t_end, "k0": k0, "train_seeds": [t["seed"] for t in train],
         "test_seeds": [t["seed"] for t in test], "dns_all_finite": all(t["finite"] for t in train + test),
         "training_samples": M, "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
-        "loss_history": losses, "clip": clip,
+        "loss_history": losses, "clip": clip, "dealias": dealias,
         "a_priori_corr_by_component": [round(c, 4) for c in corr[:6]],
         "a_priori_corr_mean": sum(corr) / len(corr),
+        "calibration_scores": {str(k): v for k, v in scores.items()}, "best_gain": best_gain,
         "evals": evals,
-        "final_%s_model_mean" % key: sum(r["model"][key] for r in last) / len(last),
-        "final_%s_no_model_mean" % key: sum(r["no_model"][key] for r in last) / len(last),
+        "final_%s_model_mean" % key: sum(r["model"][-1][key] for r in evals) / len(evals),
+        "final_%s_no_model_mean" % key: sum(r["no_model"][-1][key] for r in evals) / len(evals),
+        "mean_rel_err_model": sum(score(r["model"]) for r in evals) / len(evals),
+        "mean_rel_err_no_model": sum(score(r["no_model"]) for r in evals) / len(evals),
+        "mean_rel_err_no_model_plain": sum(score(r["no_model_plain"]) for r in evals) / len(evals),
         "dns_fp64_steps_per_s": dns_sps, "les_model_steps_per_s": speed["model"],
         "les_no_model_steps_per_s": speed["no_model"],
         "les_vs_dns_steps_ratio": speed["model"] / dns_sps,
@@ -457,9 +498,58 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     }
 
 
+def les_speed(n=64, steps=200, reps=5, clip=True, seed=7100, nu=1.0 / 800, dt=2e-3):
+    """LES step rate with and without the closure (untrained network with a
+    small output scale; the cost does not depend on the weights), each
+    `steps` steps replayed from one hipGraph."""
+    torch = _torch()
+    from . import closure
+    L = lib()
+    V = solenoidal(n, seed)
+    L.sh_fft3d_many_z2z(ptr(V), 3, n ** 3, n, n, n, 1, SH_NORMALIZE, stream())
+    ub = V.real.float().contiguous()
+    del V
+    st = ctypes.c_void_p()
+    L.sh_stream_create(ctypes.byref(st))
+    out = {"form": "les_speed", "grid": n, "steps": steps, "reps": reps, "clip": clip}
+    for name in ("model", "no_model"):
+        m = None
+        if name == "model":
+            m = closure.MLP(seed=3, strm=st)
+            m.set_scaling([1.0] * 9, [1e-3] * 6)
+        les = closure.LES(n, nu, dt, m, clip=clip, strm=st)
+        torch.cuda.synchronize()
+        les.set_initial(ub)
+        if name == "model":
+            les.autotune(2)
+        les.steps(2)
+        L.sh_stream_synchronize(st)
+        L.sh_graph_begin(st)
+        les.steps(steps)
+        g = ctypes.c_void_p()
+        L.sh_graph_end(st, ctypes.byref(g))
+        L.sh_graph_launch(g, st)
+        L.sh_stream_synchronize(st)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L.sh_graph_launch(g, st)
+        L.sh_stream_synchronize(st)
+        out[name + "_steps_per_s"] = reps * steps / (time.perf_counter() - t0)
+        L.sh_graph_destroy(g)
+        les.close()
+        if m:
+            m.close()
+    L.sh_stream_destroy(st)
+    return out
+
+
 def main(argv=None):
     p = argparse.ArgumentParser()
     sub = p.add_subparsers(dest="form", required=True)
+    q = sub.add_parser("les-speed")
+    q.add_argument("--n", type=int, default=64)
+    q.add_argument("--steps", type=int, default=200)
+    q.add_argument("--reps", type=int, default=5)
     f = sub.add_parser("fp32")
     f.add_argument("--n", type=int, default=512)
     f.add_argument("--steps", type=int, default=100)
@@ -488,7 +578,8 @@ def main(argv=None):
     s.add_argument("--iters", type=int, default=3000)
     s.add_argument("--batch", type=int, default=65536)
     s.add_argument("--lr", type=float, default=2e-3)
-    s.add_argument("--no-clip", action="store_true")
+    s.add_argument("--variants", default="10,00,11,01",
+                   help="comma-separated (clip, dealias) bit pairs tried on the training seeds")
     a = p.parse_args(argv)
     torch = _torch()
     torch.cuda.set_device(0)
@@ -499,10 +590,13 @@ def main(argv=None):
         print(json.dumps(ensemble_run(steps=a.steps, every=a.every, ratio=a.ratio, dt=a.dt,
                                       precision=SH_F64 if a.fp64 else SH_F32, seed=a.seed,
                                       out=a.out)), flush=True)
+    elif a.form == "les-speed":
+        print(json.dumps(les_speed(a.n, a.steps, a.reps)), flush=True)
     elif a.form == "les":
         print(json.dumps(les_study(a.n_dns, a.ratio, a.nu, a.dt_dns, a.dt_les, a.t_end, a.snap_dt,
                                    a.train_seeds, a.test_seeds, iters=a.iters, batch=a.batch, lr=a.lr,
-                                   clip=not a.no_clip, eval_dt=a.eval_dt)), flush=True)
+                                   variants=[(v[0] == "1", v[1] == "1") for v in a.variants.split(",")],
+                                   eval_dt=a.eval_dt)), flush=True)
 
 
 if __name__ == "__main__":
