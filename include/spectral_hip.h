@@ -183,6 +183,10 @@ int sh_ns_div_average_box(sh_ns_t handle, int box, double* result);
  * ubar_y, ubar_z, tau_xx, tau_yy, tau_zz, tau_xy, tau_xz, tau_yz with
  * tau_ij = <u_i u_j> - ubar_i ubar_j.  The first call allocates scratch. */
 int sh_ns_snapshot(sh_ns_t handle, int ratio, void* out);
+/* Box-filtered nonlinear term of every box (target of the force-matching
+ * closure loss): ratio^3 averages of the physical u x omega.
+ * out: float[nboxes][3][(n/ratio)^3].  Shares the snapshot scratch. */
+int sh_ns_snapshot_force(sh_ns_t handle, int ratio, void* out);
 /* Body force added to the nonlinear term: each step uses cross1 + f in place
  * of cross1 (exec/ns.cpp:170-171), so f is Adams-Bashforth extrapolated with
  * it; the start-up adds f to both cross products it forms.  f: 3 Fourier
@@ -260,6 +264,20 @@ int sh_mlp_gradient(sh_mlp_t handle, const void* X, const void* T, long ld, long
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient */
 int sh_mlp_train_step(sh_mlp_t handle, const void* X, const void* T, long ld, long n, float lr,
                       double* loss);
+/* Force-matching loss on one n^3 coarse field: with tau = net(X) (X: float
+ * [9][n^3] features), f = -div tau as the LES applies it, and the target
+ * force ft (3 Fourier complex64 fields), r = M P (f - ft) with P the
+ * divergence-free projection and M zero on the Nyquist planes, at k = 0 and,
+ * with dealias, where some |k_i| > n/3; loss = weight * mean_x |r(x)|^2
+ * (weight > 0, e.g. the inverse of the target's own loss for a relative
+ * loss).  Its parameter gradient goes back through the divergence by its
+ * adjoint (dL/dtau_ij = weight (2/n^3)(d_j r_i + d_i r_j), halved on the
+ * diagonal) and through the network's backward kernel. */
+int sh_mlp_force_gradient(sh_mlp_t handle, int n, const void* X, const void* ft, int dealias,
+                          double weight, void* grad_out, double* loss);
+/* one Adam step on that gradient */
+int sh_mlp_force_train_step(sh_mlp_t handle, int n, const void* X, const void* ft, int dealias,
+                            double weight, float lr, double* loss);
 /* D = A B for 16x16 float matrices (row-major) through one BF16 MFMA, to check
  * the operand layout the kernels assume */
 int sh_mfma_probe(const void* A, const void* B, void* D, void* stream);

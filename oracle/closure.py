@@ -105,12 +105,11 @@ def forward(P, X, sc_in=None, sc_out=None, bf=True):
     return y * sc_out
 
 
-def loss_grad(P, X, T, sc_in=None, sc_out=None, bf=True):
-    """Loss mean((net(X) - T / out_scale)^2) and its gradient (packed)."""
+def _net(P, X, sc_in, bf):
+    """Forward pass keeping what the backward pass needs; y is the raw
+    network output (before the output scale)."""
     W1, b1, W2, b2, W3, b3 = unpack(P)
     sc_in = np.ones(NIN) if sc_in is None else np.asarray(sc_in, dtype=np.float64)
-    sc_out = np.ones(NOUT) if sc_out is None else np.asarray(sc_out, dtype=np.float64)
-    n = X.shape[0]
     x = _r(np.asarray(X, dtype=np.float64) * sc_in, bf)
     w1, w2, w3 = _r(W1, bf), _r(W2, bf), _r(W3, bf)
     h1 = np.tanh(x @ w1 + b1)
@@ -118,16 +117,71 @@ def loss_grad(P, X, T, sc_in=None, sc_out=None, bf=True):
     h2 = np.tanh(h1r @ w2 + b2)
     h2r = _r(h2, bf)
     y = h2r @ w3 + b3
-    e = y - np.asarray(T, dtype=np.float64) / sc_out
-    loss = float(np.mean(e * e))
-    dy = 2.0 * e / (n * NOUT)
+    return {"x": x, "w2": w2, "w3": w3, "h1": h1, "h1r": h1r, "h2": h2, "h2r": h2r, "y": y}
+
+
+def _backward(c, dy, bf):
+    """Packed parameter gradient for dL/dy (raw outputs) dy [n, 6]."""
     dyr = _r(dy, bf)
-    dz2 = (dyr @ w3.T) * (1 - h2 * h2)
+    dz2 = (dyr @ c["w3"].T) * (1 - c["h2"] * c["h2"])
     dz2r = _r(dz2, bf)
-    dz1 = (dz2r @ w2.T) * (1 - h1 * h1)
+    dz1 = (dz2r @ c["w2"].T) * (1 - c["h1"] * c["h1"])
     dz1r = _r(dz1, bf)
-    g = pack(x.T @ dz1r, dz1.sum(0), h1r.T @ dz2r, dz2.sum(0), h2r.T @ dyr, dy.sum(0))
-    return loss, g
+    return pack(c["x"].T @ dz1r, dz1.sum(0), c["h1r"].T @ dz2r, dz2.sum(0), c["h2r"].T @ dyr, dy.sum(0))
+
+
+def loss_grad(P, X, T, sc_in=None, sc_out=None, bf=True):
+    """Loss mean((net(X) - T / out_scale)^2) and its gradient (packed)."""
+    sc_out = np.ones(NOUT) if sc_out is None else np.asarray(sc_out, dtype=np.float64)
+    n = X.shape[0]
+    c = _net(P, X, sc_in, bf)
+    e = c["y"] - np.asarray(T, dtype=np.float64) / sc_out
+    loss = float(np.mean(e * e))
+    return loss, _backward(c, 2.0 * e / (n * NOUT), bf)
+
+
+def _mask(n, dealias):
+    """Modes the force residual keeps: not on a Nyquist plane, not k = 0 and,
+    with dealias, every |k_i| <= n/3."""
+    k = np.fft.fftfreq(n, 1.0 / n)
+    keep = np.ones((n, n, n), dtype=bool)
+    h = n // 2
+    keep[h, :, :] = keep[:, h, :] = keep[:, :, h] = False
+    keep[0, 0, 0] = False
+    if dealias:
+        big = 3 * np.abs(k) > n
+        keep &= ~big[:, None, None] & ~big[None, :, None] & ~big[None, None, :]
+    return keep
+
+
+def project(v):
+    """Divergence-free part of Fourier fields v [3, n, n, n] (Nyquist
+    wavenumbers as zero; k = 0 kept)."""
+    n = v.shape[-1]
+    k = _k(n)
+    ks = (k[None, None, :], k[None, :, None], k[:, None, None])
+    k2 = ks[0] ** 2 + ks[1] ** 2 + ks[2] ** 2
+    k2 = np.where(k2 == 0, 1.0, k2)
+    kv = sum(ks[a] * v[a] for a in range(3)) / k2
+    return np.stack([v[a] - ks[a] * kv for a in range(3)])
+
+
+def force_loss_grad(P, X, ft, sc_in=None, sc_out=None, dealias=True, bf=True):
+    """Force-matching loss mean_x |r(x)|^2, r = M P (-div tau - ft), tau =
+    out_scale * net(X) on an n^3 grid (X: [n^3, 9], ft: Fourier [3, n, n, n]),
+    and its packed parameter gradient through the adjoint of the divergence."""
+    sc_out = np.ones(NOUT) if sc_out is None else np.asarray(sc_out, dtype=np.float64)
+    n = ft.shape[-1]
+    N = n ** 3
+    c = _net(P, X, sc_in, bf)
+    tau = c["y"] * sc_out
+    f = sgs_force(tau.T.reshape(6, n, n, n))
+    r_hat = project(f - ft) * _mask(n, dealias)
+    loss = float(np.sum(np.abs(r_hat) ** 2) / N ** 2)
+    g = gradient_features(np.stack([np.fft.ifftn(x).real for x in r_hat]))  # g[3a+b] = d r_a / d x_b
+    dtau = (2.0 / N) * np.stack([g[:, 0], g[:, 4], g[:, 8], g[:, 1] + g[:, 3], g[:, 2] + g[:, 6],
+                                 g[:, 5] + g[:, 7]], 1)
+    return loss, _backward(c, dtau * sc_out, bf)
 
 
 def adam(P, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):

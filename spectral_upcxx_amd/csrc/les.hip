@@ -25,15 +25,23 @@ int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float*
                             hipStream_t st);
 void* ns_fused_velocity(sh_ns_t s);
 int ns_step_phase(sh_ns_t s, int phase);
+int mlp_infer_planar_packed(sh_mlp_t h, const float* X, long ld, long n, float2* Y, long ldy, hipStream_t st);
+int mlp_grad_dy(sh_mlp_t h, const float* X, const float* Dy, long ld, long n, hipStream_t st);
+int mlp_adam(sh_mlp_t h, float lr, hipStream_t st);
+void* mlp_aux(sh_mlp_t h, size_t bytes);
+float* mlp_grad_buffer(sh_mlp_t h);
+const float* mlp_scales(sh_mlp_t h);
+hipStream_t mlp_stream(sh_mlp_t h);
 
 // wavenumber of index i on an n-point axis, with the Nyquist index as zero
 __device__ __forceinline__ float kd(int i, int n) { return (2 * i == n) ? 0.f : (float)(i <= n / 2 ? i : i - n); }
 
 __device__ __forceinline__ float2 ik(float k, float2 v) { return make_float2(-k * v.y, k * v.x); }
 
-// Step 1 (Z_m = g_{2m} + i g_{2m+1}, g_{3a+b} = i k_b V_a) fused with the x pass of step 2: block (x, m) forms packed field m
-// of LPB x-lines on load (the derivative factors of line (j, k) and of each
-// position) and transforms it back along x in registers and LDS.
+// Step 1 (Z_m = g_{2m} + i g_{2m+1}, g_{3a+b} = i k_b V_a) fused with the x
+// pass of step 2: block (x, m) forms packed field m of LPB x-lines on load
+// (the derivative factors of line (j, k) and of each position) and
+// transforms it back along x in registers and LDS.
 template <class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB) void k_les_grad_x(const float2* __restrict__ V, float2* __restrict__ Z,
                                                             const float2* __restrict__ tw, int n) {
@@ -100,6 +108,74 @@ __global__ void k_les_div_unpack(const float2* __restrict__ Pt, long ts, float2*
             }
             F[a * fs + p] = nyq ? make_float2(0.f, 0.f) : make_float2(-s.x, -s.y);
         }
+    }
+}
+
+// ---------------------------------------------------------------- force-matching loss
+// Residual of the closure force against a target, r = M P (f - ft) in place
+// of f (P: divergence-free projection, M: zero on the Nyquist planes and, with
+// dealias, where some |k_i| > n/3), with per-block sums of |r|^2 (FP64).
+constexpr int kResidBlocks = 512;
+
+__global__ __launch_bounds__(256) void k_force_resid(float2* __restrict__ f, const float2* __restrict__ ft, long fs,
+                                                     int n, int dealias, double* __restrict__ part) {
+    const long N = (long)n * n * n;
+    double acc = 0;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((long)n * n));
+        const float kk[3] = {kd(i, n), kd(j, n), kd(k, n)};
+        bool zero = 2 * i == n || 2 * j == n || 2 * k == n || (i == 0 && j == 0 && k == 0);
+        if (dealias) zero = zero || 3 * fabsf(kk[0]) > n || 3 * fabsf(kk[1]) > n || 3 * fabsf(kk[2]) > n;
+        float2 d[3];
+        for (int a = 0; a < 3; a++) {
+            const float2 x = f[a * fs + p], y = ft[a * fs + p];
+            d[a] = make_float2(x.x - y.x, x.y - y.y);
+        }
+        if (zero) {
+            for (int a = 0; a < 3; a++) d[a] = make_float2(0.f, 0.f);
+        } else {
+            const float k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+            const float dr = (kk[0] * d[0].x + kk[1] * d[1].x + kk[2] * d[2].x) / k2;
+            const float di = (kk[0] * d[0].y + kk[1] * d[1].y + kk[2] * d[2].y) / k2;
+            for (int a = 0; a < 3; a++) d[a] = make_float2(d[a].x - kk[a] * dr, d[a].y - kk[a] * di);
+        }
+        for (int a = 0; a < 3; a++) {
+            f[a * fs + p] = d[a];
+            acc += (double)d[a].x * d[a].x + (double)d[a].y * d[a].y;
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ part, int np, double scale, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < np; i++) s += part[i];
+        *out = s * scale;
+    }
+}
+
+// dL/dy (6 planar fields) from the packed physical gradients of the residual,
+// g_{3a+b} = d r_a / d x_b: the adjoint of f = -div tau applied to 2 r / N,
+// times the output scale
+__global__ void k_force_dy(const float2* __restrict__ Z, long zs, float* __restrict__ D, long ld, long n, float c,
+                           const float* __restrict__ sc_out) {
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+        float g[10];
+        for (int m = 0; m < 5; m++) {
+            const float2 z = Z[m * zs + p];
+            g[2 * m] = z.x;
+            g[2 * m + 1] = z.y;
+        }
+        const float d[6] = {g[0], g[4], g[8], g[1] + g[3], g[2] + g[6], g[5] + g[7]};
+        for (int o = 0; o < 6; o++) D[o * ld + p] = c * sc_out[o] * d[o];
     }
 }
 
@@ -399,6 +475,86 @@ int sh_les_features(int n, const void* ubar3, void* X, long ldx, void* stream) {
     if (z) pool_free(z);
     if (rc) set_error("sh_les_features: failed");
     return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// the force-matching loss and its parameter gradient (into the handle's
+// gradient buffer); the loss itself is left in part[kResidBlocks]
+int force_loss_grad(sh_mlp_t m, int n, const float* X, const float2* ft, int dealias, double weight,
+                    double** loss_dev) {
+    const long N = (long)n * n * n;
+    const size_t bytes = sizeof(float2) * N * (3 + 3 + 5) + sizeof(float) * N * 6 + sizeof(double) * (kResidBlocks + 1);
+    char* base = (char*)mlp_aux(m, bytes);
+    if (!base) return SH_ENOMEM;
+    float2* T = (float2*)base;
+    float2* R = T + 3 * N;
+    float2* Z = R + 3 * N;
+    float* D = (float*)(Z + 5 * N);
+    double* part = (double*)(base + sizeof(float2) * N * 11 + sizeof(float) * N * 6);
+    hipStream_t st = mlp_stream(m);
+    // f = -div tau(X), as the LES forms it
+    RC(mlp_infer_planar_packed(m, X, N, N, T, N, st));
+    RC(fft3d<float>(T, 3, N, n, n, n, -1, 0, st));
+    hipLaunchKernelGGL(k_les_div_unpack, dim3(lgrid(N)), dim3(256), 0, st, T, N, R, N, n);
+    SH_LAUNCH_CHECK("k_les_div_unpack");
+    // r = M P (f - ft) and |r|^2
+    hipLaunchKernelGGL(k_force_resid, dim3(kResidBlocks), dim3(256), 0, st, R, ft, N, n, dealias, part);
+    SH_LAUNCH_CHECK("k_force_resid");
+    const double Nd = (double)N;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, part, kResidBlocks, weight / (Nd * Nd),
+                       part + kResidBlocks);
+    SH_LAUNCH_CHECK("k_sum_partials");
+    // dL/dtau by the adjoint of the divergence, then back through the network
+    RC(les_gradients(R, Z, n, st));
+    hipLaunchKernelGGL(k_force_dy, dim3(lgrid(N)), dim3(256), 0, st, Z, N, D, N, N, (float)(2.0 * weight / Nd),
+                       mlp_scales(m) + 9);
+    SH_LAUNCH_CHECK("k_force_dy");
+    RC(mlp_grad_dy(m, X, D, N, N, st));
+    *loss_dev = part + kResidBlocks;
+    return SH_OK;
+}
+
+int read_loss(sh_mlp_t m, const double* dev, double* loss) {
+    if (!loss) return SH_OK;
+    SH_HIP(hipMemcpyAsync(loss, dev, sizeof(double), hipMemcpyDeviceToHost, mlp_stream(m)));
+    SH_HIP(hipStreamSynchronize(mlp_stream(m)));
+    return SH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sh_mlp_force_gradient(sh_mlp_t h, int n, const void* X, const void* ft, int dealias, double weight,
+                          void* grad_out, double* loss) {
+    if (!h || n <= 0 || !X || !ft || !(weight > 0)) {
+        set_error("sh_mlp_force_gradient: invalid argument");
+        return SH_EINVAL;
+    }
+    double* dev = nullptr;
+    RC(force_loss_grad(h, n, (const float*)X, (const float2*)ft, dealias, weight, &dev));
+    if (grad_out) {
+        int np = 0;
+        RC(sh_mlp_param_count(&np));
+        SH_HIP(hipMemcpyAsync(grad_out, mlp_grad_buffer(h), sizeof(float) * np, hipMemcpyDeviceToDevice,
+                              mlp_stream(h)));
+    }
+    return read_loss(h, dev, loss);
+}
+
+int sh_mlp_force_train_step(sh_mlp_t h, int n, const void* X, const void* ft, int dealias, double weight,
+                            float lr, double* loss) {
+    if (!h || n <= 0 || !X || !ft || !(weight > 0)) {
+        set_error("sh_mlp_force_train_step: invalid argument");
+        return SH_EINVAL;
+    }
+    double* dev = nullptr;
+    RC(force_loss_grad(h, n, (const float*)X, (const float2*)ft, dealias, weight, &dev));
+    RC(mlp_adam(h, lr, mlp_stream(h)));
+    return read_loss(h, dev, loss);
 }
 
 }  // extern "C"

@@ -240,6 +240,10 @@ __device__ __forceinline__ s16x4 transpose(s16x4 d, short* t, int lane) {
 // block) accumulates its tiles' gradient and squared error into
 // part[wave][NPARAM + 1].  Targets are divided by the output scale (the
 // network predicts stress / out_scale).
+// DY_IN: the loss gradient with respect to the network outputs is given
+// (Tg holds dL/dy, 6 planar fields ld apart) instead of being formed from
+// targets; the squared-error slot is then left zero.
+template <bool DY_IN>
 __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X,
                                                  const float* __restrict__ Tg, long ld, long n,
                                                  const float* __restrict__ sc, float inv_count,
@@ -276,9 +280,13 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, co
             const int f = 4 * q + j;
             float d = 0;
             if (f < kOut && s < n) {
-                const float e = o.y[j] - Tg[f * ld + s] * inv_out[j];
-                sq += e * e;
-                d = 2.f * e * inv_count;
+                if constexpr (DY_IN) {
+                    d = Tg[f * ld + s];
+                } else {
+                    const float e = o.y[j] - Tg[f * ld + s] * inv_out[j];
+                    sq += e * e;
+                    d = 2.f * e * inv_count;
+                }
             }
             dy[j] = d;
         }
@@ -413,6 +421,8 @@ struct sh_mlp_s {
     float* scale;  // [9] input scale, [6] output scale
     long t;        // Adam step count
     hipStream_t st;
+    void* aux;          // scratch of the force-matching loss (grown on demand)
+    size_t aux_bytes;
 };
 
 namespace sh {
@@ -431,6 +441,50 @@ int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float*
     SH_LAUNCH_CHECK("k_mlp_infer");
     return SH_OK;
 }
+// planar features in, packed stresses out (the force-matching loss)
+int mlp_infer_planar_packed(sh_mlp_t h, const float* X, long ld, long n, float2* Y, long ldy, hipStream_t st) {
+    hipLaunchKernelGGL((k_mlp_infer<false, true>), dim3(infer_blocks(n)), dim3(256), 0, st, h->P, X, ld, n,
+                       h->scale, (float*)Y, ldy, 0);
+    SH_LAUNCH_CHECK("k_mlp_infer");
+    return SH_OK;
+}
+// parameter gradient for a given dL/dy (6 planar fields ld apart) into h->grad
+int mlp_grad_dy(sh_mlp_t h, const float* X, const float* Dy, long ld, long n, hipStream_t st) {
+    const long tiles = (n + 15) / 16;
+    const int nw = (int)(tiles < kGradWaves ? tiles : kGradWaves);
+    hipLaunchKernelGGL(k_mlp_grad<true>, dim3(nw), dim3(64), 0, st, h->P, X, Dy, ld, n, h->scale, 0.f, h->part);
+    SH_LAUNCH_CHECK("k_mlp_grad");
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 256) / 256), dim3(256), 0, st, h->part, nw, h->grad);
+    SH_LAUNCH_CHECK("k_mlp_reduce");
+    return SH_OK;
+}
+// one Adam step on h->grad
+int mlp_adam(sh_mlp_t h, float lr, hipStream_t st) {
+    h->t++;
+    const double b1 = 0.9, b2 = 0.999;
+    const float c1 = (float)(1.0 - std::pow(b1, (double)h->t)), c2 = (float)(1.0 - std::pow(b2, (double)h->t));
+    hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, h->P, h->m, h->v, h->grad, lr,
+                       (float)b1, (float)b2, 1e-8f, c1, c2);
+    SH_LAUNCH_CHECK("k_mlp_adam");
+    return SH_OK;
+}
+// scratch of at least `bytes` held by the handle (previous contents dropped)
+void* mlp_aux(sh_mlp_t h, size_t bytes) {
+    if (h->aux_bytes < bytes) {
+        if (h->aux) {
+            if (hipStreamSynchronize(h->st) != hipSuccess) return nullptr;
+            pool_free(h->aux);
+            h->aux = nullptr;
+            h->aux_bytes = 0;
+        }
+        if (pool_alloc(&h->aux, bytes)) return nullptr;
+        h->aux_bytes = bytes;
+    }
+    return h->aux;
+}
+float* mlp_grad_buffer(sh_mlp_t h) { return h->grad; }
+const float* mlp_scales(sh_mlp_t h) { return h->scale; }
+hipStream_t mlp_stream(sh_mlp_t h) { return h->st; }
 }  // namespace sh
 
 extern "C" {
@@ -497,6 +551,7 @@ int sh_mlp_destroy(sh_mlp_t h) {
     pool_free(h->grad);
     pool_free(h->part);
     pool_free(h->scale);
+    if (h->aux) pool_free(h->aux);
     delete h;
     return SH_OK;
 }
@@ -542,8 +597,8 @@ int sh_mlp_gradient(sh_mlp_t h, const void* X, const void* T, long ld, long n, v
     }
     const long tiles = (n + 15) / 16;
     const int nw = (int)(tiles < kGradWaves ? tiles : kGradWaves);
-    hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(64), 0, h->st, h->P, (const float*)X, (const float*)T, ld, n,
-                       h->scale, (float)(1.0 / ((double)n * kOut)), h->part);
+    hipLaunchKernelGGL(k_mlp_grad<false>, dim3(nw), dim3(64), 0, h->st, h->P, (const float*)X, (const float*)T, ld,
+                       n, h->scale, (float)(1.0 / ((double)n * kOut)), h->part);
     SH_LAUNCH_CHECK("k_mlp_grad");
     hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 256) / 256), dim3(256), 0, h->st, h->part, nw, h->grad);
     SH_LAUNCH_CHECK("k_mlp_reduce");
@@ -561,13 +616,7 @@ int sh_mlp_gradient(sh_mlp_t h, const void* X, const void* T, long ld, long n, v
 int sh_mlp_train_step(sh_mlp_t h, const void* X, const void* T, long ld, long n, float lr, double* loss) {
     int rc = sh_mlp_gradient(h, X, T, ld, n, nullptr, loss);
     if (rc) return rc;
-    h->t++;
-    const double b1 = 0.9, b2 = 0.999;
-    const float c1 = (float)(1.0 - std::pow(b1, (double)h->t)), c2 = (float)(1.0 - std::pow(b2, (double)h->t));
-    hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, h->st, h->P, h->m, h->v, h->grad,
-                       lr, (float)b1, (float)b2, 1e-8f, c1, c2);
-    SH_LAUNCH_CHECK("k_mlp_adam");
-    return SH_OK;
+    return mlp_adam(h, lr, h->st);
 }
 
 int sh_mfma_probe(const void* A, const void* B, void* D, void* stream) {

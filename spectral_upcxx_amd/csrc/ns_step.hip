@@ -406,6 +406,52 @@ __global__ void k_box_filter(const C2<T>* __restrict__ u, long N, long fs, int n
     }
 }
 
+// ratio-r box averages of the physical u x w (3 fields of each box; u and w
+// field-major with field stride fs, box = blockIdx.y): out[box][3][nc]
+template <typename T>
+__global__ void k_box_cross(const C2<T>* __restrict__ u, const C2<T>* __restrict__ w, long N, long fs, int nx,
+                            int ny, int r, int cx, int cy, long nc, float* __restrict__ out) {
+    u += blockIdx.y * N;
+    w += blockIdx.y * N;
+    out += blockIdx.y * 3 * nc;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nc; c += (long)gridDim.x * blockDim.x) {
+        const int I = (int)(c % cx);
+        const long rr = c / cx;
+        const int J = (int)(rr % cy);
+        const int K = (int)(rr / cy);
+        double s[3] = {0, 0, 0};
+        for (int dk = 0; dk < r; dk++)
+            for (int dj = 0; dj < r; dj++)
+                for (int di = 0; di < r; di++) {
+                    const long n = (long)(I * r + di) + (long)nx * ((long)(J * r + dj) + (long)ny * (K * r + dk));
+                    const double u0 = u[n].x, u1 = u[fs + n].x, u2 = u[2 * fs + n].x;
+                    const double w0 = w[n].x, w1 = w[fs + n].x, w2 = w[2 * fs + n].x;
+                    s[0] += u1 * w2 - u2 * w1;
+                    s[1] += u2 * w0 - u0 * w2;
+                    s[2] += u0 * w1 - u1 * w0;
+                }
+        const double inv = 1.0 / ((double)r * r * r);
+        for (int a = 0; a < 3; a++) out[a * nc + c] = (float)(s[a] * inv);
+    }
+}
+
+// w = i k x v for the field-major layout (box = blockIdx.y)
+template <typename T>
+__global__ void k_curl_boxes(const C2<T>* __restrict__ v, C2<T>* __restrict__ w, long N, long fs, int nx, int ny,
+                             int nz) {
+    v += blockIdx.y * N;
+    w += blockIdx.y * N;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(p % nx);
+        const long rr = p / nx;
+        const T k0 = freq<T>(i, nx), k1 = freq<T>((int)(rr % ny), ny), k2 = freq<T>((int)(rr / ny), nz);
+        const C2<T> a = v[p], b = v[fs + p], c = v[2 * fs + p];
+        w[p] = csub<T>(imul<T>(k1, c), imul<T>(k2, b));
+        w[fs + p] = csub<T>(imul<T>(k2, a), imul<T>(k0, c));
+        w[2 * fs + p] = csub<T>(imul<T>(k0, b), imul<T>(k1, a));
+    }
+}
+
 static unsigned egrid(long n) {
     long g = (n + 255) / 256;
     if (g > 262144) g = 262144;
@@ -1063,6 +1109,34 @@ int snapshot_t(sh_ns_s* s, int r, float* out) {
     SH_LAUNCH_CHECK("k_box_filter");
     return SH_OK;
 }
+
+// snapshot scratch: 6 fields per box (velocity and vorticity), first use only
+int ensure_snap(sh_ns_s* s) {
+    if (s->snap) return SH_OK;
+    const size_t e = s->prec == SH_F64 ? sizeof(double2) : sizeof(float2);
+    RC(pool_alloc(&s->snap, 6 * e * (size_t)s->N * s->nboxes));
+    s->owned.push_back(s->snap);
+    return SH_OK;
+}
+
+template <typename T>
+int snapshot_force_t(sh_ns_s* s, int r, float* out) {
+    const long N = s->N;
+    const int cx = s->nx / r, cy = s->ny / r, cz = s->nz / r;
+    const long nc = (long)cx * cy * cz;
+    const long nf = 3L * s->nboxes;
+    C2<T>* u = (C2<T>*)s->snap;
+    C2<T>* w = u + nf * N;
+    SH_HIP(hipMemcpyAsync(u, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
+    hipLaunchKernelGGL(k_curl_boxes<T>, dim3(egrid(N), (unsigned)s->nboxes), dim3(256), 0, s->st, u, w, N,
+                       N * s->nboxes, s->nx, s->ny, s->nz);
+    SH_LAUNCH_CHECK("k_curl_boxes");
+    RC(fft3d<T>(u, (int)(2 * nf), N, s->nx, s->ny, s->nz, +1, SH_NORMALIZE, s->st));
+    hipLaunchKernelGGL(k_box_cross<T>, dim3(egrid(nc), (unsigned)s->nboxes), dim3(256), 0, s->st, u, w, N,
+                       N * s->nboxes, s->nx, s->ny, r, cx, cy, nc, out);
+    SH_LAUNCH_CHECK("k_box_cross");
+    return SH_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1077,12 +1151,20 @@ int sh_ns_snapshot(sh_ns_t s, int ratio, void* out) {
         set_error("sh_ns_snapshot: invalid argument (fused solver, ratio dividing the grid)");
         return SH_EINVAL;
     }
-    if (!s->snap) {
-        const size_t e = s->prec == SH_F64 ? sizeof(double2) : sizeof(float2);
-        RC(pool_alloc(&s->snap, 3 * e * (size_t)s->N * s->nboxes));
-        s->owned.push_back(s->snap);
-    }
+    RC(ensure_snap(s));
     return s->prec == SH_F64 ? snapshot_t<double>(s, ratio, (float*)out) : snapshot_t<float>(s, ratio, (float*)out);
+}
+
+// Box-filtered nonlinear term of every box: the ratio^3 averages of the
+// physical u x omega.  out: float[nboxes][3][nc].  Shares the snapshot scratch.
+int sh_ns_snapshot_force(sh_ns_t s, int ratio, void* out) {
+    if (!s || !out || !s->fused || ratio <= 0 || s->nx % ratio || s->ny % ratio || s->nz % ratio) {
+        set_error("sh_ns_snapshot_force: invalid argument (fused solver, ratio dividing the grid)");
+        return SH_EINVAL;
+    }
+    RC(ensure_snap(s));
+    return s->prec == SH_F64 ? snapshot_force_t<double>(s, ratio, (float*)out)
+                             : snapshot_force_t<float>(s, ratio, (float*)out);
 }
 
 int sh_ns_div_average_box(sh_ns_t s, int box, double* result) {

@@ -250,6 +250,47 @@ def physical_to_fourier(u):
     return V
 
 
+def project_fourier(V):
+    """In place: zero the Nyquist modes of Fourier fields V (complex128
+    [3, n, n, n]) and remove their gradient part (V - grad lap^-1 div V)."""
+    torch = _torch()
+    L = lib()
+    n = V.shape[-1]
+    for i in range(3):
+        L.sh_zero_high_z(ptr(V[i]), n, n, n, stream())
+    d = torch.empty((n, n, n), dtype=torch.complex128, device=V.device)
+    L.sh_divergence_z(ptr(V[0]), ptr(V[1]), ptr(V[2]), ptr(d), n, n, n, stream())
+    L.sh_laplacian_inverse_z(ptr(d), 0.0, 1.0, n, n, n, stream())
+    G = torch.empty_like(V)
+    L.sh_gradient_z(ptr(d), ptr(G[0]), ptr(G[1]), ptr(G[2]), n, n, n, stream())
+    L.sh_sub_z(ptr(V), ptr(V), ptr(G), 3 * n ** 3, stream())
+    return V
+
+
+def closure_target(ubar, A):
+    """Training pair of the force-matching loss from one filtered DNS
+    snapshot: the network inputs X ([9, n^3] float32) of the projected
+    filtered velocity and the force the coarse solver lacks,
+    ft = F(box(u x omega)) - N(ubar) (complex64 [3, n, n, n]), N being the
+    coarse solver's own nonlinear term of the projected ubar."""
+    torch = _torch()
+    from . import closure
+    L = lib()
+    n = ubar.shape[-1]
+    V = project_fourier(physical_to_fourier(ubar))
+    up = V.clone()
+    L.sh_fft3d_many_z2z(ptr(up), 3, n ** 3, n, n, n, 1, SH_NORMALIZE, stream())
+    X = closure.features(up.real.float().contiguous())
+    W = torch.empty_like(V)
+    L.sh_curl_z(ptr(V[0]), ptr(V[1]), ptr(V[2]), ptr(W[0]), ptr(W[1]), ptr(W[2]), n, n, n, stream())
+    Nl = torch.empty_like(V)
+    work = torch.empty((6, n, n, n), dtype=torch.complex128, device=V.device)
+    L.sh_cross_z(ptr(V), ptr(W), ptr(Nl), ptr(work), n, n, n, stream())
+    Ah = physical_to_fourier(A)
+    L.sh_sub_z(ptr(Ah), ptr(Ah), ptr(Nl), 3 * n ** 3, stream())
+    return X, Ah.to(torch.complex64).contiguous()
+
+
 def _spectrum_rel(Ea, Eb, kmax):
     return [abs(Ea[k] - Eb[k]) / Eb[k] for k in range(1, kmax + 1)]
 
@@ -257,7 +298,8 @@ def _spectrum_rel(Ea, Eb, kmax):
 def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=2.0, snap_dt=0.25,
               train_seeds=8, test_seeds=4, seed0=7100, k0=4.0, iters=3000, batch=65536, lr=2e-3,
               variants=((True, False), (False, False), (True, True), (False, True)), eval_dt=0.5, kcut=16,
-              net_seed=5, gain_candidates=(0.5, 1.0, 1.5, 2.0, 3.0, 4.0), log=None):
+              net_seed=5, gain_candidates=(0.5, 1.0, 1.5, 2.0, 3.0, 4.0), force_iters=2000, force_lr=1e-3,
+              log=None):
     """Learned-closure form: FP64 DNS at n_dns^3 box-filtered by `ratio`
     gives training pairs (filtered velocity gradients -> stresses); the
     network is trained on the GPU; the FP32 LES on the coarse grid with and
@@ -279,6 +321,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     # ---- DNS: one solver reused for every seed
     dns = NS((n_dns,) * 3, nu, dt_dns, SH_F64, fused=True)
     snap = torch.empty((1, 9, nc), dtype=torch.float32, device="cuda")
+    fsnap = torch.empty((1, 3, nc), dtype=torch.float32, device="cuda")
     train, test = [], []
     dns_time = 0.0
     dns_steps = 0
@@ -295,6 +338,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
             tuned = True
         L.sh_ns_snapshot(dns.h, ratio, ptr(snap))
         snaps = {0: snap[0].clone()}
+        forces = {}
         chunk = snap_steps if is_train else eval_steps
         done = 0
         while done < total:
@@ -307,8 +351,10 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
             done += chunk
             L.sh_ns_snapshot(dns.h, ratio, ptr(snap))
             snaps[done] = snap[0].clone()
+            L.sh_ns_snapshot_force(dns.h, ratio, ptr(fsnap))
+            forces[done] = fsnap[0].clone()
         finite = all(bool(torch.isfinite(s).all()) for s in snaps.values())
-        (train if is_train else test).append({"seed": seed, "snaps": snaps, "finite": finite})
+        (train if is_train else test).append({"seed": seed, "snaps": snaps, "forces": forces, "finite": finite})
         log("dns seed %d (%s): %d steps, finite=%s, %.1f steps/s" % (
             seed, "train" if is_train else "test", total, finite, dns_steps / dns_time))
     dns.close()
@@ -351,6 +397,68 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del X, T
+    torch.cuda.empty_cache()
+    # ---- force matching: fine-tune so that -div tau, as the dealiased coarse
+    # solver applies it, supplies the force that solver lacks (gradients go
+    # back through the spectral divergence into the network's backward kernel)
+    force_log = {}
+    if force_iters > 0:
+        import random
+
+        def pairs(runs, ks):
+            out = []
+            for r in runs:
+                for k in ks:
+                    ub = r["snaps"][k][:3].reshape(3, n, n, n).contiguous()
+                    out.append(closure_target(ub, r["forces"][k].reshape(3, n, n, n).contiguous()))
+            return out
+
+        fit = pairs(train, keys)
+        held = pairs(test, [k for k in sorted(test[0]["forces"])])
+        # each field's loss is taken relative to its target's own (the loss of
+        # a network that outputs zero), so early and late fields weigh alike
+        zero = closure.MLP(seed=1)
+        zero.set_params([0.0] * closure.param_count())
+        zero.set_scaling(in_scale, out_scale)
+        b = ctypes.c_double()
+        for pp in (fit, held):
+            for i, (Xs, fts) in enumerate(pp):
+                L.sh_mlp_force_gradient(zero.h, n, ptr(Xs), ptr(fts), 1, 1.0, None, ctypes.byref(b))
+                pp[i] = (Xs, fts, 1.0 / b.value)
+
+        def rel_loss(pp):
+            a = ctypes.c_double()
+            tot = 0.0
+            for Xs, fts, w in pp:
+                L.sh_mlp_force_gradient(model.h, n, ptr(Xs), ptr(fts), 1, w, None, ctypes.byref(a))
+                tot += a.value
+            return tot / len(pp)
+
+        force_log["held_out_rel_loss_before"] = rel_loss(held)
+        order = list(range(len(fit)))
+        rng = random.Random(7)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(force_iters):
+            if it % len(order) == 0:
+                rng.shuffle(order)
+            Xs, fts, w = fit[order[it % len(order)]]
+            lr_t = force_lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / force_iters)))
+            want = it % 250 == 0 or it == force_iters - 1
+            lv = ctypes.c_double()
+            L.sh_mlp_force_train_step(model.h, n, ptr(Xs), ptr(fts), 1, w, lr_t,
+                                      ctypes.byref(lv) if want else None)
+            if want:
+                log("force it %d loss %.4e" % (it, lv.value))
+        torch.cuda.synchronize()
+        force_log["train_s"] = time.perf_counter() - t0
+        force_log["held_out_rel_loss_after"] = rel_loss(held)
+        log("force matching: held-out relative loss %.4f -> %.4f" % (
+            force_log["held_out_rel_loss_before"], force_log["held_out_rel_loss_after"]))
+        zero.close()
+        del fit, held
+    for r in train + test:
+        r["forces"] = {}
     torch.cuda.empty_cache()
     # ---- a-priori check on held-out snapshots: correlation of the stresses
     corr = []
@@ -480,7 +588,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         "dt_les": les_dt, "t_end": t_end, "k0": k0, "train_seeds": [t["seed"] for t in train],
         "test_seeds": [t["seed"] for t in test], "dns_all_finite": all(t["finite"] for t in train + test),
         "training_samples": M, "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
-        "loss_history": losses, "clip": clip, "dealias": dealias,
+        "loss_history": losses, "force_matching": force_log, "clip": clip, "dealias": dealias,
         "a_priori_corr_by_component": [round(c, 4) for c in corr[:6]],
         "a_priori_corr_mean": sum(corr) / len(corr),
         "calibration_scores": {str(k): v for k, v in scores.items()}, "best_gain": best_gain,
@@ -578,6 +686,8 @@ def main(argv=None):
     s.add_argument("--iters", type=int, default=3000)
     s.add_argument("--batch", type=int, default=65536)
     s.add_argument("--lr", type=float, default=2e-3)
+    s.add_argument("--force-iters", type=int, default=2000)
+    s.add_argument("--force-lr", type=float, default=1e-3)
     s.add_argument("--variants", default="10,00,11,01",
                    help="comma-separated (clip, dealias) bit pairs tried on the training seeds")
     a = p.parse_args(argv)
@@ -595,6 +705,7 @@ def main(argv=None):
     elif a.form == "les":
         print(json.dumps(les_study(a.n_dns, a.ratio, a.nu, a.dt_dns, a.dt_les, a.t_end, a.snap_dt,
                                    a.train_seeds, a.test_seeds, iters=a.iters, batch=a.batch, lr=a.lr,
+                                   force_iters=a.force_iters, force_lr=a.force_lr,
                                    variants=[(v[0] == "1", v[1] == "1") for v in a.variants.split(",")],
                                    eval_dt=a.eval_dt)), flush=True)
 

@@ -164,6 +164,55 @@ def test_clip_zeroes_backscatter_only():
     assert np.array_equal(C.clip(iso, g2), iso)
 
 
+def _force_case(n, seed):
+    rng = np.random.default_rng(seed)
+    P = C.init_params(seed).astype(np.float64)
+    P[C.OFF_B3:C.OFF_B3 + 6] = rng.normal(size=6) * 0.1
+    X = rng.normal(size=(n ** 3, 9))
+    ft = np.stack([np.fft.fftn(rng.normal(size=(n, n, n))) for _ in range(3)]) * 0.05
+    return P, X, ft
+
+
+@pytest.mark.parametrize("dealias", [False, True])
+def test_force_loss_backward_matches_finite_differences(dealias):
+    """The adjoint of f = -div tau (symmetric stress, off-diagonals counted in
+    both force components) and the projection/mask against central
+    differences of the force-matching loss, exact mode."""
+    n = 8
+    P, X, ft = _force_case(n, 13)
+    sc_in = np.linspace(0.6, 1.2, 9)
+    sc_out = np.linspace(0.05, 0.1, 6)
+    _, g = C.force_loss_grad(P, X, ft, sc_in, sc_out, dealias=dealias, bf=False)
+    rng = np.random.default_rng(1)
+    idx = list(range(C.OFF_B3, C.OFF_B3 + 6)) + [C.OFF_B1 + i for i in rng.integers(0, 64, 6)]
+    for lo, cols, vr, vc in ((C.OFF_W1, 64, 9, 64), (C.OFF_W2, 64, 64, 64), (C.OFF_W3, 16, 64, 6)):
+        idx += [lo + rng.integers(vr) * cols + rng.integers(vc) for _ in range(8)]
+    h = 1e-5
+    for i in idx:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        fd = (C.force_loss_grad(Pp, X, ft, sc_in, sc_out, dealias, bf=False)[0]
+              - C.force_loss_grad(Pm, X, ft, sc_in, sc_out, dealias, bf=False)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-9 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_force_loss_zero_at_own_force():
+    """With the target equal to the network's own force the loss and the
+    gradient vanish; a pure gradient target is invisible to the loss."""
+    n = 8
+    P, X, _ = _force_case(n, 17)
+    tau = C.forward(P, X, bf=False)
+    f = C.sgs_force(tau.T.reshape(6, n, n, n))
+    loss, g = C.force_loss_grad(P, X, f, dealias=False, bf=False)
+    assert loss < 1e-28 and np.abs(g).max() < 1e-12
+    k = C._k(n)
+    phi = np.fft.fftn(np.random.default_rng(2).normal(size=(n, n, n)))
+    grad_phi = np.stack([1j * k[None, None, :] * phi, 1j * k[None, :, None] * phi, 1j * k[:, None, None] * phi])
+    loss2, _ = C.force_loss_grad(P, X, f + grad_phi, dealias=False, bf=False)
+    assert loss2 < 1e-20
+
+
 def test_box_filter_direct_sums():
     rng = np.random.default_rng(4)
     u = rng.normal(size=(3, 8, 8, 8))

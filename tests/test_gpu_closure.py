@@ -150,6 +150,89 @@ def test_training_reduces_loss(hip):
     m.close()
 
 
+@pytest.mark.parametrize("dealias", [0, 1])
+def test_force_gradient_matches_oracle(hip, dealias):
+    """Force-matching loss: network forward, packed transform, -div, the
+    projected residual, the adjoint divergence and the backward kernel
+    against the oracle in bf16 mode (which finite differences pin exactly)."""
+    import torch
+    from spectral_upcxx_amd import closure
+    n = 16
+    P = _params(21)
+    rng = np.random.default_rng(22)
+    X = rng.normal(size=(n ** 3, 9)).astype(np.float32)
+    ft = np.stack([np.fft.fftn(rng.normal(size=(n, n, n))) for _ in range(3)]) * 0.05
+    sc_in = np.linspace(0.6, 1.2, 9).astype(np.float32)
+    sc_out = np.linspace(0.05, 0.1, 6).astype(np.float32)
+    m = _mlp(hip, P, sc_in, sc_out)
+    Xd = _dev(X.T.copy())
+    ftd = _dev(ft.astype(np.complex64))
+    g = torch.empty(closure.param_count(), dtype=torch.float32, device="cuda")
+    loss = ctypes.c_double()
+    hip.lib().sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), dealias, 1.0, hip.ptr(g), ctypes.byref(loss))
+    lo, go = C.force_loss_grad(P, X, ft.astype(np.complex64).astype(np.complex128), sc_in, sc_out,
+                               dealias=bool(dealias), bf=True)
+    assert abs(loss.value - lo) / lo < 1e-4
+    assert rel_l2(_host(g), go) < 5e-3
+    m.close()
+
+
+def test_force_training_reduces_loss(hip):
+    """Targets from a teacher network's force: Adam on the force loss
+    (gradients through the spectral divergence) brings the loss down."""
+    import torch
+    from spectral_upcxx_amd import closure
+    n = 32
+    u = _physical_field(n, 91)
+    X = C.gradient_features(u)
+    teacher = _params(31)
+    tau = C.forward(teacher, X, None, np.full(6, 0.02), bf=False)
+    ft = C.sgs_force(tau.T.reshape(6, n, n, n))
+    m = closure.MLP(seed=4)
+    m.set_scaling(1.0 / X.std(0), np.full(6, 0.02, np.float32))
+    Xd = _dev(X.T.astype(np.float32).copy())
+    ftd = _dev(ft.astype(np.complex64))
+    L = hip.lib()
+    l0 = ctypes.c_double()
+    L.sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, None, ctypes.byref(l0))
+    for _ in range(300):
+        L.sh_mlp_force_train_step(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, 3e-3, None)
+    l1 = ctypes.c_double()
+    L.sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, None, ctypes.byref(l1))
+    assert l1.value < 0.2 * l0.value, (l0.value, l1.value)
+    m.close()
+    del torch
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_snapshot_force_box_filter(hip, prec):
+    """Box-filtered nonlinear term of an ensemble against numpy: ratio^3
+    averages of u x omega from the oracle's inverse transforms."""
+    import torch
+    n, r = 32, 4
+    L = hip.lib()
+    ics = [O.solenoidal((n, n, n), 95 + b, 3.0, 1.0) for b in range(2)]
+    h = ctypes.c_void_p()
+    L.sh_ns_create_ensemble(ctypes.byref(h), n, 2, (ctypes.c_double * 2)(0.01, 0.005), 1e-3, prec, None)
+    for b, V0 in enumerate(ics):
+        L.sh_ns_set_initial_box(h, b, hip.ptr(_dev(V0)))
+    L.sh_ns_steps(h, 2)
+    out = torch.empty((2, 3, (n // r) ** 3), dtype=torch.float32, device="cuda")
+    L.sh_ns_snapshot_force(h, r, hip.ptr(out))
+    got = _host(out)
+    for b in range(2):
+        V = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+        L.sh_ns_get_velocity_box(h, b, hip.ptr(V))
+        V = _host(V)
+        u = np.stack([O.inverse(c).real for c in V])
+        w = np.stack([O.inverse(c).real for c in O.curl(V)])
+        cr = np.cross(u, w, axis=0)
+        c = n // r
+        want = cr.reshape(3, c, r, c, r, c, r).mean(axis=(2, 4, 6)).reshape(3, -1)
+        assert rel_l2(got[b], want) < (1e-6 if prec == 64 else 2e-5), b
+    L.sh_ns_destroy(h)
+
+
 def _physical_field(n, seed, k0=3.0):
     V = O.solenoidal((n, n, n), seed, k0, 1.0)
     return np.stack([O.inverse(c).real for c in V])
