@@ -53,6 +53,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--fp32-steps", type=int, default=100)
+    p.add_argument("--les-steps", type=int, default=200, help="0 skips the LES step-rate line")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fft", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=2)
@@ -224,6 +225,16 @@ def main():
         dt32, _, _ = run_ns(S, n, S.SH_F32, a.warmup, a.fp32_steps, profile=False)
         fp32 = round(a.fp32_steps / dt32, 2)
         log("bench: FP32 %.2f steps/s" % fp32)
+    les = None
+    if a.les_steps > 0:
+        log("bench: 64^3 LES step rate (closure on / off), %d steps per graph" % a.les_steps)
+        from spectral_upcxx_amd import forms
+        r = forms.les_speed(n=64, steps=a.les_steps, reps=5)
+        les = {"grid": 64, "model_steps_per_s": round(r["model_steps_per_s"], 1),
+               "no_model_steps_per_s": round(r["no_model_steps_per_s"], 1),
+               "model_vs_fp64_dns_steps": round(r["model_steps_per_s"] / value, 2)}
+        log("bench: LES %.0f steps/s with the closure (%.1fx the FP64 step rate)"
+            % (les["model_steps_per_s"], les["model_vs_fp64_dns_steps"]))
     ffts = {}
     if not a.no_fft:
         for nf, reps in ((256, 10), (512, 4)):
@@ -260,6 +271,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "chosen_form_fp32_steps_per_s": fp32,
+        "les_64": les,
         "fft": ffts,
         "kernels": kernels,
         "bytes": extra,

@@ -268,16 +268,19 @@ int sh_mlp_train_step(sh_mlp_t handle, const void* X, const void* T, long ld, lo
  * [9][n^3] features), f = -div tau as the LES applies it, and the target
  * force ft (3 Fourier complex64 fields), r = M P (f - ft) with P the
  * divergence-free projection and M zero on the Nyquist planes, at k = 0 and,
- * with dealias, where some |k_i| > n/3; loss = weight * mean_x |r(x)|^2
- * (weight > 0, e.g. the inverse of the target's own loss for a relative
- * loss).  Its parameter gradient goes back through the divergence by its
- * adjoint (dL/dtau_ij = weight (2/n^3)(d_j r_i + d_i r_j), halved on the
- * diagonal) and through the network's backward kernel. */
+ * with dealias, where some |k_i| > n/3; loss = weight * mean over modes of
+ * w(k) |r(k)|^2 in Parseval's normalisation, w = 1 / (1 + |k|^2 / k_weight^2)
+ * (k_weight = 0: w = 1, the mean of |r(x)|^2 over the grid); weight > 0,
+ * e.g. the inverse of the target's own loss for a relative loss.  The
+ * parameter gradient goes back through the divergence by its adjoint
+ * (dL/dtau_ij = weight (2/n^3)(d_j s_i + d_i s_j) with s the inverse
+ * transform of w r, halved on the diagonal) and through the network's
+ * backward kernel. */
 int sh_mlp_force_gradient(sh_mlp_t handle, int n, const void* X, const void* ft, int dealias,
-                          double weight, void* grad_out, double* loss);
+                          double weight, double k_weight, void* grad_out, double* loss);
 /* one Adam step on that gradient */
 int sh_mlp_force_train_step(sh_mlp_t handle, int n, const void* X, const void* ft, int dealias,
-                            double weight, float lr, double* loss);
+                            double weight, double k_weight, float lr, double* loss);
 /* D = A B for 16x16 float matrices (row-major) through one BF16 MFMA, to check
  * the operand layout the kernels assume */
 int sh_mfma_probe(const void* A, const void* B, void* D, void* stream);

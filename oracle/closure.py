@@ -166,10 +166,12 @@ def project(v):
     return np.stack([v[a] - ks[a] * kv for a in range(3)])
 
 
-def force_loss_grad(P, X, ft, sc_in=None, sc_out=None, dealias=True, bf=True):
-    """Force-matching loss mean_x |r(x)|^2, r = M P (-div tau - ft), tau =
-    out_scale * net(X) on an n^3 grid (X: [n^3, 9], ft: Fourier [3, n, n, n]),
-    and its packed parameter gradient through the adjoint of the divergence."""
+def force_loss_grad(P, X, ft, sc_in=None, sc_out=None, dealias=True, bf=True, weight=1.0, kw=0.0):
+    """Force-matching loss weight * sum_k w(k) |r(k)|^2 / N^2 (N = n^3; with
+    w = 1 the mean of |r(x)|^2 over the grid), r = M P (-div tau - ft),
+    tau = out_scale * net(X) on an n^3 grid (X: [n^3, 9], ft: Fourier
+    [3, n, n, n]), w = 1/(1 + |k|^2/kw^2) (kw = 0: w = 1), and its packed
+    parameter gradient through the adjoint of the divergence."""
     sc_out = np.ones(NOUT) if sc_out is None else np.asarray(sc_out, dtype=np.float64)
     n = ft.shape[-1]
     N = n ** 3
@@ -177,8 +179,13 @@ def force_loss_grad(P, X, ft, sc_in=None, sc_out=None, dealias=True, bf=True):
     tau = c["y"] * sc_out
     f = sgs_force(tau.T.reshape(6, n, n, n))
     r_hat = project(f - ft) * _mask(n, dealias)
-    loss = float(np.sum(np.abs(r_hat) ** 2) / N ** 2)
-    g = gradient_features(np.stack([np.fft.ifftn(x).real for x in r_hat]))  # g[3a+b] = d r_a / d x_b
+    w = np.ones((n, n, n))
+    if kw > 0:
+        k = _k(n)
+        w = 1.0 / (1.0 + (k[None, None, :] ** 2 + k[None, :, None] ** 2 + k[:, None, None] ** 2) / kw ** 2)
+    loss = float(weight * np.sum(w * np.abs(r_hat) ** 2) / N ** 2)
+    s = w * r_hat * weight
+    g = gradient_features(np.stack([np.fft.ifftn(x).real for x in s]))  # g[3a+b] = d s_a / d x_b
     dtau = (2.0 / N) * np.stack([g[:, 0], g[:, 4], g[:, 8], g[:, 1] + g[:, 3], g[:, 2] + g[:, 6],
                                  g[:, 5] + g[:, 7]], 1)
     return loss, _backward(c, dtau * sc_out, bf)

@@ -117,8 +117,10 @@ __global__ void k_les_div_unpack(const float2* __restrict__ Pt, long ts, float2*
 // dealias, where some |k_i| > n/3), with per-block sums of |r|^2 (FP64).
 constexpr int kResidBlocks = 512;
 
+// With kw2_inv = 1/k_w^2 > 0 each mode is weighted by w(k) = 1/(1 + |k|^2/k_w^2):
+// the sums hold w|r|^2 and f is left holding w r, the seed of the adjoint.
 __global__ __launch_bounds__(256) void k_force_resid(float2* __restrict__ f, const float2* __restrict__ ft, long fs,
-                                                     int n, int dealias, double* __restrict__ part) {
+                                                     int n, int dealias, float kw2_inv, double* __restrict__ part) {
     const long N = (long)n * n * n;
     double acc = 0;
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
@@ -131,6 +133,7 @@ __global__ __launch_bounds__(256) void k_force_resid(float2* __restrict__ f, con
             const float2 x = f[a * fs + p], y = ft[a * fs + p];
             d[a] = make_float2(x.x - y.x, x.y - y.y);
         }
+        float wk = 1.f;
         if (zero) {
             for (int a = 0; a < 3; a++) d[a] = make_float2(0.f, 0.f);
         } else {
@@ -138,10 +141,11 @@ __global__ __launch_bounds__(256) void k_force_resid(float2* __restrict__ f, con
             const float dr = (kk[0] * d[0].x + kk[1] * d[1].x + kk[2] * d[2].x) / k2;
             const float di = (kk[0] * d[0].y + kk[1] * d[1].y + kk[2] * d[2].y) / k2;
             for (int a = 0; a < 3; a++) d[a] = make_float2(d[a].x - kk[a] * dr, d[a].y - kk[a] * di);
+            wk = 1.f / (1.f + k2 * kw2_inv);
         }
         for (int a = 0; a < 3; a++) {
-            f[a * fs + p] = d[a];
-            acc += (double)d[a].x * d[a].x + (double)d[a].y * d[a].y;
+            f[a * fs + p] = make_float2(wk * d[a].x, wk * d[a].y);
+            acc += (double)wk * ((double)d[a].x * d[a].x + (double)d[a].y * d[a].y);
         }
     }
     __shared__ double red[256];
@@ -483,7 +487,7 @@ namespace {
 
 // the force-matching loss and its parameter gradient (into the handle's
 // gradient buffer); the loss itself is left in part[kResidBlocks]
-int force_loss_grad(sh_mlp_t m, int n, const float* X, const float2* ft, int dealias, double weight,
+int force_loss_grad(sh_mlp_t m, int n, const float* X, const float2* ft, int dealias, double weight, double kw,
                     double** loss_dev) {
     const long N = (long)n * n * n;
     const size_t bytes = sizeof(float2) * N * (3 + 3 + 5) + sizeof(float) * N * 6 + sizeof(double) * (kResidBlocks + 1);
@@ -501,7 +505,8 @@ int force_loss_grad(sh_mlp_t m, int n, const float* X, const float2* ft, int dea
     hipLaunchKernelGGL(k_les_div_unpack, dim3(lgrid(N)), dim3(256), 0, st, T, N, R, N, n);
     SH_LAUNCH_CHECK("k_les_div_unpack");
     // r = M P (f - ft) and |r|^2
-    hipLaunchKernelGGL(k_force_resid, dim3(kResidBlocks), dim3(256), 0, st, R, ft, N, n, dealias, part);
+    const float kw2_inv = kw > 0 ? (float)(1.0 / (kw * kw)) : 0.f;
+    hipLaunchKernelGGL(k_force_resid, dim3(kResidBlocks), dim3(256), 0, st, R, ft, N, n, dealias, kw2_inv, part);
     SH_LAUNCH_CHECK("k_force_resid");
     const double Nd = (double)N;
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, part, kResidBlocks, weight / (Nd * Nd),
@@ -529,13 +534,13 @@ int read_loss(sh_mlp_t m, const double* dev, double* loss) {
 extern "C" {
 
 int sh_mlp_force_gradient(sh_mlp_t h, int n, const void* X, const void* ft, int dealias, double weight,
-                          void* grad_out, double* loss) {
-    if (!h || n <= 0 || !X || !ft || !(weight > 0)) {
+                          double k_weight, void* grad_out, double* loss) {
+    if (!h || n <= 0 || !X || !ft || !(weight > 0) || k_weight < 0) {
         set_error("sh_mlp_force_gradient: invalid argument");
         return SH_EINVAL;
     }
     double* dev = nullptr;
-    RC(force_loss_grad(h, n, (const float*)X, (const float2*)ft, dealias, weight, &dev));
+    RC(force_loss_grad(h, n, (const float*)X, (const float2*)ft, dealias, weight, k_weight, &dev));
     if (grad_out) {
         int np = 0;
         RC(sh_mlp_param_count(&np));
@@ -546,13 +551,13 @@ int sh_mlp_force_gradient(sh_mlp_t h, int n, const void* X, const void* ft, int 
 }
 
 int sh_mlp_force_train_step(sh_mlp_t h, int n, const void* X, const void* ft, int dealias, double weight,
-                            float lr, double* loss) {
-    if (!h || n <= 0 || !X || !ft || !(weight > 0)) {
+                            double k_weight, float lr, double* loss) {
+    if (!h || n <= 0 || !X || !ft || !(weight > 0) || k_weight < 0) {
         set_error("sh_mlp_force_train_step: invalid argument");
         return SH_EINVAL;
     }
     double* dev = nullptr;
-    RC(force_loss_grad(h, n, (const float*)X, (const float2*)ft, dealias, weight, &dev));
+    RC(force_loss_grad(h, n, (const float*)X, (const float2*)ft, dealias, weight, k_weight, &dev));
     RC(mlp_adam(h, lr, mlp_stream(h)));
     return read_loss(h, dev, loss);
 }

@@ -172,6 +172,34 @@ __device__ __forceinline__ void forward(const FwdW& w, const FwdB& b, s16x4 xb, 
     o.y = acc;
 }
 
+// two tiles through the network, interleaved layer by layer (independent
+// chains for the scheduler)
+__device__ __forceinline__ void forward2(const FwdW& w, const FwdB& b, s16x4 xa, s16x4 xb, f32x4& ya, f32x4& yb) {
+    s16x4 ha[4], hb[4];
+    for (int mt = 0; mt < 4; mt++) {
+        const f32x4 a = mfma(w.w1t[mt], xa, b.b1[mt]);
+        const f32x4 c = mfma(w.w1t[mt], xb, b.b1[mt]);
+        ha[mt] = to_bf(tanh4(a));
+        hb[mt] = to_bf(tanh4(c));
+    }
+    s16x4 ga[4], gb[4];
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 a = b.b2[mt], c = b.b2[mt];
+        for (int kb = 0; kb < 4; kb++) {
+            a = mfma(w.w2t[mt][kb], ha[kb], a);
+            c = mfma(w.w2t[mt][kb], hb[kb], c);
+        }
+        ga[mt] = to_bf(tanh4(a));
+        gb[mt] = to_bf(tanh4(c));
+    }
+    ya = b.b3;
+    yb = b.b3;
+    for (int kb = 0; kb < 4; kb++) {
+        ya = mfma(w.w3t[kb], ga[kb], ya);
+        yb = mfma(w.w3t[kb], gb[kb], yb);
+    }
+}
+
 // Inference over samples [0, n): Y = out_scale * net(in_scale * X).  With
 // clip, a sample's stress is zeroed where its deviatoric part has
 // tau_ij S_ij > 0 (S the strain rate of the gradient inputs g_{3i+j}).
@@ -190,34 +218,39 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, 
     const f32x4 sc_in = load_scale_in(sc, lane);
     f32x4 sc_out;
     for (int j = 0; j < 4; j++) sc_out[j] = 4 * q + j < kOut ? sc[kIn + 4 * q + j] : 0.f;
-    for (long s0 = wave * 16; s0 < n; s0 += nwaves * 16) {
-        const s16x4 xb = load_x<PACKED_IN>(X, ld, s0, n, sc_in, lane);
-        Fwd o;
-        forward(w, bias, xb, o);
-        const long s = s0 + r;
-        f32x4 y = o.y * sc_out;
-        if (clip) {
-            // every lane gathers its sample's six stresses (lanes q = 0, 1 hold them)
-            float t[6];
-            for (int j = 0; j < 4; j++) t[j] = __shfl(y[j], r, 64);
-            t[4] = __shfl(y[0], r + 16, 64);
-            t[5] = __shfl(y[1], r + 16, 64);
-            float g[9];
-            for (int f = 0; f < 9; f++) g[f] = s < n ? X[fidx<PACKED_IN>(f, s, ld)] : 0.f;
-            const float tr = (t[0] + t[1] + t[2]) * (1.f / 3.f);
-            const float sxy = 0.5f * (g[1] + g[3]), sxz = 0.5f * (g[2] + g[6]), syz = 0.5f * (g[5] + g[7]);
-            const float ts = (t[0] - tr) * g[0] + (t[1] - tr) * g[4] + (t[2] - tr) * g[8] +
-                             2.f * (t[3] * sxy + t[4] * sxz + t[5] * syz);
-            if (ts > 0.f) y = f32x4{0, 0, 0, 0};
-        }
-        if (s < n && q < 2) {
-            if constexpr (PACKED_OUT) {
-                float2* Y2 = reinterpret_cast<float2*>(Y);
-                Y2[(2 * q) * ldy + s] = make_float2(y[0], y[1]);
-                if (q == 0) Y2[ldy + s] = make_float2(y[2], y[3]);
-            } else {
-                for (int j = 0; j < 4; j++)
-                    if (4 * q + j < kOut) Y[(4 * q + j) * ldy + s] = y[j];
+    // two tiles per trip: their independent MFMA -> tanh -> convert chains
+    // interleave, which hides the latency one tile alone leaves exposed
+    for (long s00 = wave * 32; s00 < n; s00 += nwaves * 32) {
+        const s16x4 xa = load_x<PACKED_IN>(X, ld, s00, n, sc_in, lane);
+        const s16x4 xb = load_x<PACKED_IN>(X, ld, s00 + 16, n, sc_in, lane);
+        f32x4 ya, yb;
+        forward2(w, bias, xa, xb, ya, yb);
+        for (int half = 0; half < 2; half++) {
+            const long s = s00 + 16 * half + r;
+            f32x4 y = (half ? yb : ya) * sc_out;
+            if (clip) {
+                // every lane gathers its sample's six stresses (lanes q = 0, 1 hold them)
+                float t[6];
+                for (int j = 0; j < 4; j++) t[j] = __shfl(y[j], r, 64);
+                t[4] = __shfl(y[0], r + 16, 64);
+                t[5] = __shfl(y[1], r + 16, 64);
+                float g[9];
+                for (int f = 0; f < 9; f++) g[f] = s < n ? X[fidx<PACKED_IN>(f, s, ld)] : 0.f;
+                const float tr = (t[0] + t[1] + t[2]) * (1.f / 3.f);
+                const float sxy = 0.5f * (g[1] + g[3]), sxz = 0.5f * (g[2] + g[6]), syz = 0.5f * (g[5] + g[7]);
+                const float ts = (t[0] - tr) * g[0] + (t[1] - tr) * g[4] + (t[2] - tr) * g[8] +
+                                 2.f * (t[3] * sxy + t[4] * sxz + t[5] * syz);
+                if (ts > 0.f) y = f32x4{0, 0, 0, 0};
+            }
+            if (s < n && q < 2) {
+                if constexpr (PACKED_OUT) {
+                    float2* Y2 = reinterpret_cast<float2*>(Y);
+                    Y2[(2 * q) * ldy + s] = make_float2(y[0], y[1]);
+                    if (q == 0) Y2[ldy + s] = make_float2(y[2], y[3]);
+                } else {
+                    for (int j = 0; j < 4; j++)
+                        if (4 * q + j < kOut) Y[(4 * q + j) * ldy + s] = y[j];
+                }
             }
         }
     }

@@ -297,8 +297,9 @@ def _spectrum_rel(Ea, Eb, kmax):
 
 def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=2.0, snap_dt=0.25,
               train_seeds=8, test_seeds=4, seed0=7100, k0=4.0, iters=3000, batch=65536, lr=2e-3,
-              variants=((True, False), (False, False), (True, True), (False, True)), eval_dt=0.5, kcut=16,
-              net_seed=5, gain_candidates=(0.5, 1.0, 1.5, 2.0, 3.0, 4.0), force_iters=2000, force_lr=1e-3,
+              variants=((True, True), (False, True)), eval_dt=0.5, kcut=16, net_seed=5,
+              gain_candidates=(0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0), force_iters=2000, force_lr=1e-3,
+              force_variants=(("force", False, 0.0), ("force_kw8", False, 8.0), ("force_rel", True, 0.0)),
               log=None):
     """Learned-closure form: FP64 DNS at n_dns^3 box-filtered by `ratio`
     gives training pairs (filtered velocity gradients -> stresses); the
@@ -402,6 +403,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     # solver applies it, supplies the force that solver lacks (gradients go
     # back through the spectral divergence into the network's backward kernel)
     force_log = {}
+    candidates = [("a_priori", model.params())]
     if force_iters > 0:
         import random
 
@@ -415,47 +417,55 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
 
         fit = pairs(train, keys)
         held = pairs(test, [k for k in sorted(test[0]["forces"])])
-        # each field's loss is taken relative to its target's own (the loss of
-        # a network that outputs zero), so early and late fields weigh alike
+        # each field's own target loss (that of a network that outputs zero):
+        # the relative variant divides by it, and the held-out check reports it
         zero = closure.MLP(seed=1)
         zero.set_params([0.0] * closure.param_count())
         zero.set_scaling(in_scale, out_scale)
         b = ctypes.c_double()
         for pp in (fit, held):
             for i, (Xs, fts) in enumerate(pp):
-                L.sh_mlp_force_gradient(zero.h, n, ptr(Xs), ptr(fts), 1, 1.0, None, ctypes.byref(b))
+                L.sh_mlp_force_gradient(zero.h, n, ptr(Xs), ptr(fts), 1, 1.0, 0.0, None, ctypes.byref(b))
                 pp[i] = (Xs, fts, 1.0 / b.value)
+        zero.close()
 
-        def rel_loss(pp):
+        def rel_loss(m, pp):
             a = ctypes.c_double()
             tot = 0.0
             for Xs, fts, w in pp:
-                L.sh_mlp_force_gradient(model.h, n, ptr(Xs), ptr(fts), 1, w, None, ctypes.byref(a))
+                L.sh_mlp_force_gradient(m.h, n, ptr(Xs), ptr(fts), 1, w, 0.0, None, ctypes.byref(a))
                 tot += a.value
             return tot / len(pp)
 
-        force_log["held_out_rel_loss_before"] = rel_loss(held)
-        order = list(range(len(fit)))
-        rng = random.Random(7)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for it in range(force_iters):
-            if it % len(order) == 0:
-                rng.shuffle(order)
-            Xs, fts, w = fit[order[it % len(order)]]
-            lr_t = force_lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / force_iters)))
-            want = it % 250 == 0 or it == force_iters - 1
-            lv = ctypes.c_double()
-            L.sh_mlp_force_train_step(model.h, n, ptr(Xs), ptr(fts), 1, w, lr_t,
-                                      ctypes.byref(lv) if want else None)
-            if want:
-                log("force it %d loss %.4e" % (it, lv.value))
-        torch.cuda.synchronize()
-        force_log["train_s"] = time.perf_counter() - t0
-        force_log["held_out_rel_loss_after"] = rel_loss(held)
-        log("force matching: held-out relative loss %.4f -> %.4f" % (
-            force_log["held_out_rel_loss_before"], force_log["held_out_rel_loss_after"]))
-        zero.close()
+        force_log["held_out_rel_loss_a_priori"] = rel_loss(model, held)
+        base = model.params()
+        for name, relative, kw in force_variants:
+            # a fresh optimiser state from the a-priori weights for each variant
+            m = closure.MLP(seed=net_seed)
+            m.set_params(base)
+            m.set_scaling(in_scale, out_scale)
+            order = list(range(len(fit)))
+            rng = random.Random(7)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for it in range(force_iters):
+                if it % len(order) == 0:
+                    rng.shuffle(order)
+                Xs, fts, w = fit[order[it % len(order)]]
+                lr_t = force_lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / force_iters)))
+                want = it % 500 == 0 or it == force_iters - 1
+                lv = ctypes.c_double()
+                L.sh_mlp_force_train_step(m.h, n, ptr(Xs), ptr(fts), 1, w if relative else 1.0, kw, lr_t,
+                                          ctypes.byref(lv) if want else None)
+                if want:
+                    log("force %s it %d loss %.4e" % (name, it, lv.value))
+            torch.cuda.synchronize()
+            force_log[name] = {"relative": relative, "k_weight": kw, "train_s": time.perf_counter() - t0,
+                               "held_out_rel_loss": rel_loss(m, held)}
+            log("force matching %s: held-out relative loss %.4f (a priori %.4f)" % (
+                name, force_log[name]["held_out_rel_loss"], force_log["held_out_rel_loss_a_priori"]))
+            candidates.append((name, m.params()))
+            m.close()
         del fit, held
     for r in train + test:
         r["forces"] = {}
@@ -523,18 +533,25 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     for tr in train[:2]:
         calib.append((tr["snaps"][0][:3].reshape(3, n, n, n).contiguous(), fdns_spectra(tr["snaps"])))
     scores = {}
-    for clip_v, dealias_v in variants:
-        for gain in gain_candidates:
-            model.set_scaling(in_scale, out_scale * gain)
-            s = sum(score(compare(trajectory(ub, model, clip_v, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
-            scores[(clip_v, dealias_v, gain)] = s
-            log("calibration clip=%d dealias=%d gain %.2f: mean rel err %.4f" % (clip_v, dealias_v, gain, s))
+    params = dict(candidates)
+    for cname, cparams in candidates:
+        model.set_params(cparams)
+        for clip_v, dealias_v in variants:
+            for gain in gain_candidates:
+                model.set_scaling(in_scale, out_scale * gain)
+                s = sum(score(compare(trajectory(ub, model, clip_v, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
+                scores[(cname, clip_v, dealias_v, gain)] = s
+            best_c = min((k for k in scores if k[0] == cname), key=scores.get)
+            log("calibration %s clip=%d dealias=%d: best gain %.2f mean rel err %.4f" % (
+                cname, clip_v, dealias_v, best_c[3], scores[best_c]))
     for dealias_v in sorted({d for _, d in variants}):
         s = sum(score(compare(trajectory(ub, None, False, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
         scores[("no_model", dealias_v)] = s
         log("calibration no-model dealias=%d: mean rel err %.4f" % (dealias_v, s))
     best = min((k for k in scores if k[0] != "no_model"), key=scores.get)
-    clip, dealias, best_gain = best
+    chosen, clip, dealias, best_gain = best
+    log("chosen: %s clip=%d dealias=%d gain %.2f" % best)
+    model.set_params(params[chosen])
     model.set_scaling(in_scale, out_scale * best_gain)
     for tr in train:
         tr["snaps"] = {0: tr["snaps"][0]}
@@ -588,7 +605,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         "dt_les": les_dt, "t_end": t_end, "k0": k0, "train_seeds": [t["seed"] for t in train],
         "test_seeds": [t["seed"] for t in test], "dns_all_finite": all(t["finite"] for t in train + test),
         "training_samples": M, "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
-        "loss_history": losses, "force_matching": force_log, "clip": clip, "dealias": dealias,
+        "loss_history": losses, "force_matching": force_log, "network": chosen, "clip": clip, "dealias": dealias,
         "a_priori_corr_by_component": [round(c, 4) for c in corr[:6]],
         "a_priori_corr_mean": sum(corr) / len(corr),
         "calibration_scores": {str(k): v for k, v in scores.items()}, "best_gain": best_gain,
@@ -688,7 +705,7 @@ def main(argv=None):
     s.add_argument("--lr", type=float, default=2e-3)
     s.add_argument("--force-iters", type=int, default=2000)
     s.add_argument("--force-lr", type=float, default=1e-3)
-    s.add_argument("--variants", default="10,00,11,01",
+    s.add_argument("--variants", default="11,01",
                    help="comma-separated (clip, dealias) bit pairs tried on the training seeds")
     a = p.parse_args(argv)
     torch = _torch()

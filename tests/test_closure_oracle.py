@@ -173,16 +173,20 @@ def _force_case(n, seed):
     return P, X, ft
 
 
-@pytest.mark.parametrize("dealias", [False, True])
-def test_force_loss_backward_matches_finite_differences(dealias):
+@pytest.mark.parametrize("dealias,weight,kw", [(False, 1.0, 0.0), (True, 1.0, 0.0), (True, 3.0, 2.5)])
+def test_force_loss_backward_matches_finite_differences(dealias, weight, kw):
     """The adjoint of f = -div tau (symmetric stress, off-diagonals counted in
-    both force components) and the projection/mask against central
-    differences of the force-matching loss, exact mode."""
+    both force components), the projection/mask and the spectral weight
+    against central differences of the force-matching loss, exact mode."""
     n = 8
     P, X, ft = _force_case(n, 13)
     sc_in = np.linspace(0.6, 1.2, 9)
     sc_out = np.linspace(0.05, 0.1, 6)
-    _, g = C.force_loss_grad(P, X, ft, sc_in, sc_out, dealias=dealias, bf=False)
+
+    def lossg(Pv):
+        return C.force_loss_grad(Pv, X, ft, sc_in, sc_out, dealias=dealias, bf=False, weight=weight, kw=kw)
+
+    _, g = lossg(P)
     rng = np.random.default_rng(1)
     idx = list(range(C.OFF_B3, C.OFF_B3 + 6)) + [C.OFF_B1 + i for i in rng.integers(0, 64, 6)]
     for lo, cols, vr, vc in ((C.OFF_W1, 64, 9, 64), (C.OFF_W2, 64, 64, 64), (C.OFF_W3, 16, 64, 6)):
@@ -192,8 +196,7 @@ def test_force_loss_backward_matches_finite_differences(dealias):
         Pp, Pm = P.copy(), P.copy()
         Pp[i] += h
         Pm[i] -= h
-        fd = (C.force_loss_grad(Pp, X, ft, sc_in, sc_out, dealias, bf=False)[0]
-              - C.force_loss_grad(Pm, X, ft, sc_in, sc_out, dealias, bf=False)[0]) / (2 * h)
+        fd = (lossg(Pp)[0] - lossg(Pm)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-9 + 1e-5 * abs(fd), (i, fd, g[i])
 
 

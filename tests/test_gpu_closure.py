@@ -150,8 +150,8 @@ def test_training_reduces_loss(hip):
     m.close()
 
 
-@pytest.mark.parametrize("dealias", [0, 1])
-def test_force_gradient_matches_oracle(hip, dealias):
+@pytest.mark.parametrize("dealias,weight,kw", [(0, 1.0, 0.0), (1, 1.0, 0.0), (1, 2.0, 4.0)])
+def test_force_gradient_matches_oracle(hip, dealias, weight, kw):
     """Force-matching loss: network forward, packed transform, -div, the
     projected residual, the adjoint divergence and the backward kernel
     against the oracle in bf16 mode (which finite differences pin exactly)."""
@@ -169,9 +169,10 @@ def test_force_gradient_matches_oracle(hip, dealias):
     ftd = _dev(ft.astype(np.complex64))
     g = torch.empty(closure.param_count(), dtype=torch.float32, device="cuda")
     loss = ctypes.c_double()
-    hip.lib().sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), dealias, 1.0, hip.ptr(g), ctypes.byref(loss))
+    hip.lib().sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), dealias, weight, kw, hip.ptr(g),
+                                    ctypes.byref(loss))
     lo, go = C.force_loss_grad(P, X, ft.astype(np.complex64).astype(np.complex128), sc_in, sc_out,
-                               dealias=bool(dealias), bf=True)
+                               dealias=bool(dealias), bf=True, weight=weight, kw=kw)
     assert abs(loss.value - lo) / lo < 1e-4
     assert rel_l2(_host(g), go) < 5e-3
     m.close()
@@ -194,11 +195,11 @@ def test_force_training_reduces_loss(hip):
     ftd = _dev(ft.astype(np.complex64))
     L = hip.lib()
     l0 = ctypes.c_double()
-    L.sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, None, ctypes.byref(l0))
+    L.sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, 0.0, None, ctypes.byref(l0))
     for _ in range(300):
-        L.sh_mlp_force_train_step(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, 3e-3, None)
+        L.sh_mlp_force_train_step(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, 0.0, 3e-3, None)
     l1 = ctypes.c_double()
-    L.sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, None, ctypes.byref(l1))
+    L.sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), 1, 1.0, 0.0, None, ctypes.byref(l1))
     assert l1.value < 0.2 * l0.value, (l0.value, l1.value)
     m.close()
     del torch
