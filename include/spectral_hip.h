@@ -195,11 +195,12 @@ int sh_ns_snapshot_force(sh_ns_t handle, int ratio, void* out);
  * contents are read at every step, so the caller may rewrite them between
  * steps.  NULL removes the force.  No counterpart in the reference. */
 int sh_ns_set_forcing(sh_ns_t handle, const void* f3);
-/* on = 1: drop the nonlinear term (with the body force) at every mode with
- * some |k_i| > N_i/3 (2/3-rule dealiasing).  Off by default, as in the
- * reference, which only zeroes the Nyquist modes.  No counterpart in the
- * reference; used by the coarse-grid LES.  Synchronous. */
-int sh_ns_set_dealias(sh_ns_t handle, int on);
+/* Dealiasing of the nonlinear term (with the body force): mode 1 drops every
+ * mode with some |k_i| > N_i/3 (2/3 rule); mode 2 scales each mode by
+ * prod_i exp(-36 (|k_i| / (N_i/2))^36) (exponential filter); mode 0, the
+ * default, is the reference's scheme, which only zeroes the Nyquist modes.
+ * No counterpart in the reference; used by the coarse-grid LES.  Synchronous. */
+int sh_ns_set_dealias(sh_ns_t handle, int mode);
 
 /* ------------------------------------------------------------------ streams and graphs
  * Default-priority non-blocking streams; stream capture into a hipGraph
@@ -267,8 +268,9 @@ int sh_mlp_train_step(sh_mlp_t handle, const void* X, const void* T, long ld, lo
 /* Force-matching loss on one n^3 coarse field: with tau = net(X) (X: float
  * [9][n^3] features), f = -div tau as the LES applies it, and the target
  * force ft (3 Fourier complex64 fields), r = M P (f - ft) with P the
- * divergence-free projection and M zero on the Nyquist planes, at k = 0 and,
- * with dealias, where some |k_i| > n/3; loss = weight * mean over modes of
+ * divergence-free projection and M zero on the Nyquist planes and at k = 0,
+ * and times the coarse solver's dealiasing factor of mode `dealias`
+ * (sh_ns_set_dealias); loss = weight * mean over modes of
  * w(k) |r(k)|^2 in Parseval's normalisation, w = 1 / (1 + |k|^2 / k_weight^2)
  * (k_weight = 0: w = 1, the mean of |r(x)|^2 over the grid); weight > 0,
  * e.g. the inverse of the target's own loss for a relative loss.  The
@@ -293,9 +295,9 @@ int sh_mfma_probe(const void* A, const void* B, void* D, void* stream);
 typedef struct sh_les_s* sh_les_t;
 int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, int clip, void* stream);
 int sh_les_destroy(sh_les_t handle);
-/* 2/3-rule dealiasing of the coarse solver (sh_ns_set_dealias); set it
- * before the initial condition.  Synchronous. */
-int sh_les_set_dealias(sh_les_t handle, int on);
+/* dealiasing mode of the coarse solver (sh_ns_set_dealias); set it before
+ * the initial condition.  Synchronous. */
+int sh_les_set_dealias(sh_les_t handle, int mode);
 /* initial condition from a physical filtered velocity (float [3][n^3]):
  * transform, zero the Nyquist modes, project onto divergence-free fields,
  * then the start-up step (exec/ns.cpp:132-153) with the closure force */

@@ -142,16 +142,23 @@ def loss_grad(P, X, T, sc_in=None, sc_out=None, bf=True):
 
 def _mask(n, dealias):
     """Modes the force residual keeps: not on a Nyquist plane, not k = 0 and,
-    with dealias, every |k_i| <= n/3."""
+    with dealias mode 1, every |k_i| <= n/3."""
     k = np.fft.fftfreq(n, 1.0 / n)
     keep = np.ones((n, n, n), dtype=bool)
     h = n // 2
     keep[h, :, :] = keep[:, h, :] = keep[:, :, h] = False
     keep[0, 0, 0] = False
-    if dealias:
+    if dealias == 1:
         big = 3 * np.abs(k) > n
         keep &= ~big[:, None, None] & ~big[None, :, None] & ~big[None, None, :]
     return keep
+
+
+def expfilter(n):
+    """The coarse solver's exponential dealiasing factor (mode 2),
+    prod_i exp(-36 (|k_i| / (n/2))^36), on the (z, y, x) mode grid."""
+    s = np.exp(-36.0 * (np.abs(_k(n)) / (0.5 * n)) ** 36)
+    return s[:, None, None] * s[None, :, None] * s[None, None, :]
 
 
 def project(v):
@@ -183,6 +190,8 @@ def force_loss_grad(P, X, ft, sc_in=None, sc_out=None, dealias=True, bf=True, we
     if kw > 0:
         k = _k(n)
         w = 1.0 / (1.0 + (k[None, None, :] ** 2 + k[None, :, None] ** 2 + k[:, None, None] ** 2) / kw ** 2)
+    if dealias == 2:
+        w = w * expfilter(n) ** 2
     loss = float(weight * np.sum(w * np.abs(r_hat) ** 2) / N ** 2)
     s = w * r_hat * weight
     g = gradient_features(np.stack([np.fft.ifftn(x).real for x in s]))  # g[3a+b] = d s_a / d x_b

@@ -100,14 +100,15 @@ class LES:
     """FP32 coarse-grid solver with the network's stress as body force
     (model=None: the no-model solve)."""
 
-    def __init__(self, n, nu, K, model=None, clip=True, strm=None, dealias=False):
+    def __init__(self, n, nu, K, model=None, clip=True, strm=None, dealias=0):
+        """dealias: 0 none (the reference scheme), 1 the 2/3 rule, 2 the exponential filter"""
         self.n = n
         self.h = ctypes.c_void_p()
         self.stream = stream() if strm is None else strm
         lib().sh_les_create(ctypes.byref(self.h), int(n), float(nu), float(K), model.h if model else None,
                             int(bool(clip)), self.stream)
         if dealias:
-            lib().sh_les_set_dealias(self.h, 1)
+            lib().sh_les_set_dealias(self.h, int(dealias))
 
     def close(self):
         if self.h:

@@ -38,6 +38,14 @@ __device__ __forceinline__ float kd(int i, int n) { return (2 * i == n) ? 0.f : 
 
 __device__ __forceinline__ float2 ik(float k, float2 v) { return make_float2(-k * v.y, k * v.x); }
 
+// exp(-36 (|k| / (n/2))^36), the solver's exponential dealiasing factor of one axis
+__device__ __forceinline__ float expfilter_f(float k, int n) {
+    const double r = fabs((double)k) / (0.5 * n);
+    const double r4 = (r * r) * (r * r);
+    const double r8 = r4 * r4, r16 = r8 * r8;
+    return (float)exp(-36.0 * (r16 * r16) * r4);  // r^32 r^4 = r^36
+}
+
 // Step 1 (Z_m = g_{2m} + i g_{2m+1}, g_{3a+b} = i k_b V_a) fused with the x
 // pass of step 2: block (x, m) forms packed field m of LPB x-lines on load
 // (the derivative factors of line (j, k) and of each position) and
@@ -127,7 +135,9 @@ __global__ __launch_bounds__(256) void k_force_resid(float2* __restrict__ f, con
         const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((long)n * n));
         const float kk[3] = {kd(i, n), kd(j, n), kd(k, n)};
         bool zero = 2 * i == n || 2 * j == n || 2 * k == n || (i == 0 && j == 0 && k == 0);
-        if (dealias) zero = zero || 3 * fabsf(kk[0]) > n || 3 * fabsf(kk[1]) > n || 3 * fabsf(kk[2]) > n;
+        if (dealias == 1) zero = zero || 3 * fabsf(kk[0]) > n || 3 * fabsf(kk[1]) > n || 3 * fabsf(kk[2]) > n;
+        // mode 2: the residual is the solver's filtered force, so its square carries sigma^2
+        const float sig = dealias == 2 ? expfilter_f(kk[0], n) * expfilter_f(kk[1], n) * expfilter_f(kk[2], n) : 1.f;
         float2 d[3];
         for (int a = 0; a < 3; a++) {
             const float2 x = f[a * fs + p], y = ft[a * fs + p];
@@ -141,7 +151,7 @@ __global__ __launch_bounds__(256) void k_force_resid(float2* __restrict__ f, con
             const float dr = (kk[0] * d[0].x + kk[1] * d[1].x + kk[2] * d[2].x) / k2;
             const float di = (kk[0] * d[0].y + kk[1] * d[1].y + kk[2] * d[2].y) / k2;
             for (int a = 0; a < 3; a++) d[a] = make_float2(d[a].x - kk[a] * dr, d[a].y - kk[a] * di);
-            wk = 1.f / (1.f + k2 * kw2_inv);
+            wk = sig * sig / (1.f + k2 * kw2_inv);
         }
         for (int a = 0; a < 3; a++) {
             f[a * fs + p] = make_float2(wk * d[a].x, wk * d[a].y);
@@ -418,9 +428,9 @@ int sh_les_steps(sh_les_t h, int nsteps) {
     return SH_OK;
 }
 
-int sh_les_set_dealias(sh_les_t h, int on) {
+int sh_les_set_dealias(sh_les_t h, int mode) {
     if (!h) return SH_EINVAL;
-    return sh_ns_set_dealias(h->ns, on);
+    return sh_ns_set_dealias(h->ns, mode);
 }
 
 int sh_les_autotune(sh_les_t h, int reps) {

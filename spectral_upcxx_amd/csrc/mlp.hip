@@ -39,14 +39,17 @@ __device__ __forceinline__ f32x4 mfma(s16x4 a, s16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// two FP32 -> BF16 conversions in one v_cvt_pk_bf16_f32 (round to nearest even)
-__device__ __forceinline__ s16x2 cvt2(float a, float b) {
-    return __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{a, b}, bf16x2));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// two FP32 -> BF16 conversions in one v_cvt_pk_bf16_f32 (round to nearest even),
+// as the 32-bit register that holds them
+__device__ __forceinline__ unsigned cvt2(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 
+// the 4 x 16-bit MFMA operand is two such registers as they stand
 __device__ __forceinline__ s16x4 pack4(float a, float b, float c, float d) {
-    const s16x2 lo = cvt2(a, b), hi = cvt2(c, d);
-    return s16x4{lo[0], lo[1], hi[0], hi[1]};
+    return __builtin_bit_cast(s16x4, u32x2{cvt2(a, b), cvt2(c, d)});
 }
 
 __device__ __forceinline__ s16x4 to_bf(f32x4 v) { return pack4(v[0], v[1], v[2], v[3]); }
@@ -139,14 +142,16 @@ __device__ __forceinline__ f32x4 load_scale_in(const float* __restrict__ sc, int
 // s0 + r, scaled; features >= 9 and samples >= n are zero
 template <bool PACKED>
 __device__ __forceinline__ s16x4 load_x(const float* __restrict__ X, long ld, long s0, long n, f32x4 sc,
-                                        int lane) {
+                                        int lane, f32x4* raw = nullptr) {
     const long s = s0 + lane % 16;
     const int q = lane / 16;
-    float xf[4];
+    f32x4 xf;
     for (int j = 0; j < 4; j++) {
         const int f = 4 * q + j;
-        xf[j] = (f < kIn && s < n) ? X[fidx<PACKED>(f, s, ld)] * sc[j] : 0.f;
+        xf[j] = (f < kIn && s < n) ? X[fidx<PACKED>(f, s, ld)] : 0.f;
     }
+    if (raw) *raw = xf;  // unscaled inputs, for the clip test
+    xf *= sc;
     return pack4(xf[0], xf[1], xf[2], xf[3]);
 }
 
@@ -221,21 +226,24 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, 
     // two tiles per trip: their independent MFMA -> tanh -> convert chains
     // interleave, which hides the latency one tile alone leaves exposed
     for (long s00 = wave * 32; s00 < n; s00 += nwaves * 32) {
-        const s16x4 xa = load_x<PACKED_IN>(X, ld, s00, n, sc_in, lane);
-        const s16x4 xb = load_x<PACKED_IN>(X, ld, s00 + 16, n, sc_in, lane);
+        f32x4 ra, rb;
+        const s16x4 xa = load_x<PACKED_IN>(X, ld, s00, n, sc_in, lane, &ra);
+        const s16x4 xb = load_x<PACKED_IN>(X, ld, s00 + 16, n, sc_in, lane, &rb);
         f32x4 ya, yb;
         forward2(w, bias, xa, xb, ya, yb);
         for (int half = 0; half < 2; half++) {
             const long s = s00 + 16 * half + r;
             f32x4 y = (half ? yb : ya) * sc_out;
             if (clip) {
-                // every lane gathers its sample's six stresses (lanes q = 0, 1 hold them)
+                // every lane gathers its sample's six stresses (lanes q = 0, 1
+                // hold them) and nine gradients (lanes q = 0, 1, 2)
                 float t[6];
                 for (int j = 0; j < 4; j++) t[j] = __shfl(y[j], r, 64);
                 t[4] = __shfl(y[0], r + 16, 64);
                 t[5] = __shfl(y[1], r + 16, 64);
+                const f32x4 raw = half ? rb : ra;
                 float g[9];
-                for (int f = 0; f < 9; f++) g[f] = s < n ? X[fidx<PACKED_IN>(f, s, ld)] : 0.f;
+                for (int f = 0; f < 9; f++) g[f] = __shfl(raw[f & 3], r + 16 * (f >> 2), 64);
                 const float tr = (t[0] + t[1] + t[2]) * (1.f / 3.f);
                 const float sxy = 0.5f * (g[1] + g[3]), sxz = 0.5f * (g[2] + g[6]), syz = 0.5f * (g[5] + g[7]);
                 const float ts = (t[0] - tr) * g[0] + (t[1] - tr) * g[4] + (t[2] - tr) * g[8] +
@@ -436,7 +444,7 @@ __global__ void k_mfma_probe(const float* __restrict__ A, const float* __restric
 
 static unsigned infer_blocks(long n) {
     const long tiles = (n + 15) / 16;
-    long b = (tiles + 4 * 8 - 1) / (4 * 8);  // about 8 tiles per wave, 4 waves per block
+    long b = (tiles + 4 * 4 - 1) / (4 * 4);  // about 4 tiles per wave, 4 waves per block
     if (b > 1024) b = 1024;
     return (unsigned)(b < 1 ? 1 : b);
 }

@@ -43,8 +43,18 @@ struct StepCoef {
     T c1;          // nu*K/2                (exec/ns.cpp:174)
     double c1d;    // nu*K/2 in FP64 for the CN factor (exec/ns.cpp:186)
     T inv_n3;      // 1/(Nx Ny Nz)          (src/Scalar.cpp:223-227)
-    int dealias;   // 1: products with some |k_i| > N_i/3 are dropped (2/3 rule)
+    int dealias;   // 1: products with some |k_i| > N_i/3 are dropped (2/3 rule);
+                   // 2: products scaled by prod_i exp(-36 (|k_i| / (N_i/2))^36)
 };
+
+// the smooth dealiasing factor of one axis, exp(-36 (|k| / (n/2))^36)
+template <typename T>
+__host__ __device__ __forceinline__ T expfilter(T k, int n) {
+    const double r = fabs((double)k) / (0.5 * n);
+    const double r4 = (r * r) * (r * r);
+    const double r8 = r4 * r4, r16 = r8 * r8;
+    return T(exp(-36.0 * (r16 * r16) * r4));  // r^32 r^4 = r^36
+}
 
 template <int N>
 struct LdsLine {
@@ -253,11 +263,13 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
         for_out<P>(tid, [&](int s, int pos) {
             const T k0 = freq<T>(pos, N);
             const bool z = zline || pos == N / 2 ||
-                           (cf.dealias && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
+                           (cf.dealias == 1 && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
+            const T sig = cf.dealias == 2 ? expfilter<T>(k0, N) * expfilter<T>(k1, ny) * expfilter<T>(k2, nz) : T(1);
             C2<T> nn[3], v1[3], f[3], L[3], S[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 nn[d] = z ? mk<T>(T(0), T(0)) : forced ? cadd<T>(n1[d][s], frc[d * F + off + pos]) : n1[d][s];
+                if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
                 v1[d] = V[d * F + off + pos];
                 const C2<T> n0 = N0[d * F + off + pos];
                 // F = K/2*(3.*cross1 - cross0)
@@ -539,20 +551,27 @@ int cross(sh_ns_s* s, const double2* a, const double2* b, double2* o) {
         if (_rc) return _rc; \
     } while (0)
 
-// 2/3-rule truncation of 3 fields (the fused kernel E applies the same mask)
-__global__ void k_dealias3(double2* __restrict__ f, long fs, int nx, int ny, int nz) {
+// dealiasing of 3 fields as the fused kernel E does it: mode 1 the 2/3-rule
+// truncation, mode 2 the exponential filter
+__global__ void k_dealias3(double2* __restrict__ f, long fs, int nx, int ny, int nz, int mode) {
     const long N = (long)nx * ny * nz;
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
         const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((long)nx * ny));
-        if (3 * fabs(freq<double>(i, nx)) > nx || 3 * fabs(freq<double>(j, ny)) > ny ||
-            3 * fabs(freq<double>(k, nz)) > nz)
-            for (int d = 0; d < 3; d++) f[d * fs + p] = make_double2(0.0, 0.0);
+        const double k0 = freq<double>(i, nx), k1 = freq<double>(j, ny), k2 = freq<double>(k, nz);
+        if (mode == 1) {
+            if (3 * fabs(k0) > nx || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz)
+                for (int d = 0; d < 3; d++) f[d * fs + p] = make_double2(0.0, 0.0);
+        } else {
+            const double s = expfilter<double>(k0, nx) * expfilter<double>(k1, ny) * expfilter<double>(k2, nz);
+            for (int d = 0; d < 3; d++) f[d * fs + p] = make_double2(s * f[d * fs + p].x, s * f[d * fs + p].y);
+        }
     }
 }
 
 int dealias64(sh_ns_s* s, double2* cr) {
     if (!s->dealias) return SH_OK;
-    hipLaunchKernelGGL(k_dealias3, dim3(egrid(s->N)), dim3(256), 0, s->st, cr, s->N, s->nx, s->ny, s->nz);
+    hipLaunchKernelGGL(k_dealias3, dim3(egrid(s->N)), dim3(256), 0, s->st, cr, s->N, s->nx, s->ny, s->nz,
+                       s->dealias);
     SH_LAUNCH_CHECK("k_dealias3");
     return SH_OK;
 }
@@ -1197,9 +1216,12 @@ int sh_ns_set_forcing(sh_ns_t s, const void* f3) {
     return SH_OK;
 }
 
-int sh_ns_set_dealias(sh_ns_t s, int on) {
-    if (!s) return SH_EINVAL;
-    s->dealias = on != 0;
+int sh_ns_set_dealias(sh_ns_t s, int mode) {
+    if (!s || mode < 0 || mode > 2) {
+        set_error("sh_ns_set_dealias: mode must be 0, 1 or 2");
+        return SH_EINVAL;
+    }
+    s->dealias = mode;
     return s->fused ? upload_coefs(s) : SH_OK;
 }
 

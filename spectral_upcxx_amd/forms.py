@@ -297,9 +297,9 @@ def _spectrum_rel(Ea, Eb, kmax):
 
 def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=2.0, snap_dt=0.25,
               train_seeds=8, test_seeds=4, seed0=7100, k0=4.0, iters=3000, batch=65536, lr=2e-3,
-              variants=((True, True), (False, True)), eval_dt=0.5, kcut=16, net_seed=5,
+              variants=((True, 1), (False, 1), (True, 2), (False, 2)), eval_dt=0.5, kcut=16, net_seed=5,
               gain_candidates=(0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0), force_iters=2000, force_lr=1e-3,
-              force_variants=(("force", False, 0.0), ("force_kw8", False, 8.0), ("force_rel", True, 0.0)),
+              force_variants=(("force", False, 0.0, 1), ("force_exp", False, 0.0, 2)),
               log=None):
     """Learned-closure form: FP64 DNS at n_dns^3 box-filtered by `ratio`
     gives training pairs (filtered velocity gradients -> stresses); the
@@ -439,7 +439,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
 
         force_log["held_out_rel_loss_a_priori"] = rel_loss(model, held)
         base = model.params()
-        for name, relative, kw in force_variants:
+        for name, relative, kw, mode in force_variants:
             # a fresh optimiser state from the a-priori weights for each variant
             m = closure.MLP(seed=net_seed)
             m.set_params(base)
@@ -455,12 +455,13 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
                 lr_t = force_lr * (0.05 + 0.95 * 0.5 * (1 + math.cos(math.pi * it / force_iters)))
                 want = it % 500 == 0 or it == force_iters - 1
                 lv = ctypes.c_double()
-                L.sh_mlp_force_train_step(m.h, n, ptr(Xs), ptr(fts), 1, w if relative else 1.0, kw, lr_t,
+                L.sh_mlp_force_train_step(m.h, n, ptr(Xs), ptr(fts), mode, w if relative else 1.0, kw, lr_t,
                                           ctypes.byref(lv) if want else None)
                 if want:
                     log("force %s it %d loss %.4e" % (name, it, lv.value))
             torch.cuda.synchronize()
-            force_log[name] = {"relative": relative, "k_weight": kw, "train_s": time.perf_counter() - t0,
+            force_log[name] = {"relative": relative, "k_weight": kw, "dealias": mode,
+                               "train_s": time.perf_counter() - t0,
                                "held_out_rel_loss": rel_loss(m, held)}
             log("force matching %s: held-out relative loss %.4f (a priori %.4f)" % (
                 name, force_log[name]["held_out_rel_loss"], force_log["held_out_rel_loss_a_priori"]))
@@ -526,8 +527,9 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     les.set_initial(train[0]["snaps"][0][:3].reshape(3, n, n, n).contiguous())
     les.autotune(2)
     les.close()
-    # a-posteriori choices -- backscatter clipping, 2/3 dealiasing of the
-    # coarse solver, the gain of the network's output -- made on training
+    # a-posteriori choices -- the network (a-priori or force-matched),
+    # backscatter clipping, the coarse solver's dealiasing mode, the gain of
+    # the network's output -- made on training
     # seeds only; the held-out seeds are evaluated once with the choice
     calib = []
     for tr in train[:2]:
@@ -541,7 +543,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
                 model.set_scaling(in_scale, out_scale * gain)
                 s = sum(score(compare(trajectory(ub, model, clip_v, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
                 scores[(cname, clip_v, dealias_v, gain)] = s
-            best_c = min((k for k in scores if k[0] == cname), key=scores.get)
+            best_c = min((k for k in scores if k[:3] == (cname, clip_v, dealias_v)), key=scores.get)
             log("calibration %s clip=%d dealias=%d: best gain %.2f mean rel err %.4f" % (
                 cname, clip_v, dealias_v, best_c[3], scores[best_c]))
     for dealias_v in sorted({d for _, d in variants}):
@@ -705,8 +707,8 @@ def main(argv=None):
     s.add_argument("--lr", type=float, default=2e-3)
     s.add_argument("--force-iters", type=int, default=2000)
     s.add_argument("--force-lr", type=float, default=1e-3)
-    s.add_argument("--variants", default="11,01",
-                   help="comma-separated (clip, dealias) bit pairs tried on the training seeds")
+    s.add_argument("--variants", default="11,01,12,02",
+                   help="comma-separated (clip bit, dealias mode) pairs tried on the training seeds")
     a = p.parse_args(argv)
     torch = _torch()
     torch.cuda.set_device(0)
@@ -723,7 +725,7 @@ def main(argv=None):
         print(json.dumps(les_study(a.n_dns, a.ratio, a.nu, a.dt_dns, a.dt_les, a.t_end, a.snap_dt,
                                    a.train_seeds, a.test_seeds, iters=a.iters, batch=a.batch, lr=a.lr,
                                    force_iters=a.force_iters, force_lr=a.force_lr,
-                                   variants=[(v[0] == "1", v[1] == "1") for v in a.variants.split(",")],
+                                   variants=[(v[0] == "1", int(v[1])) for v in a.variants.split(",")],
                                    eval_dt=a.eval_dt)), flush=True)
 
 

@@ -173,7 +173,7 @@ def _force_case(n, seed):
     return P, X, ft
 
 
-@pytest.mark.parametrize("dealias,weight,kw", [(False, 1.0, 0.0), (True, 1.0, 0.0), (True, 3.0, 2.5)])
+@pytest.mark.parametrize("dealias,weight,kw", [(0, 1.0, 0.0), (1, 1.0, 0.0), (1, 3.0, 2.5), (2, 1.0, 0.0)])
 def test_force_loss_backward_matches_finite_differences(dealias, weight, kw):
     """The adjoint of f = -div tau (symmetric stress, off-diagonals counted in
     both force components), the projection/mask and the spectral weight
@@ -198,6 +198,17 @@ def test_force_loss_backward_matches_finite_differences(dealias, weight, kw):
         Pm[i] -= h
         fd = (lossg(Pp)[0] - lossg(Pm)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-9 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_expfilter_closed_form():
+    """exp(-36 (|k|/(n/2))^36): 1 at k = 0, e^-36 at |k| = n/2 on one axis,
+    and a product over axes."""
+    n = 16
+    s = C.expfilter(n)
+    assert s[0, 0, 0] == 1.0
+    assert np.isclose(s[0, 0, 7], np.exp(-36 * (7 / 8) ** 36))
+    assert np.isclose(s[3, 5, 0], np.exp(-36 * (3 / 8) ** 36) * np.exp(-36 * (5 / 8) ** 36))
+    assert s.max() <= 1.0 and s.min() > 0
 
 
 def test_force_loss_zero_at_own_force():

@@ -150,7 +150,7 @@ def test_training_reduces_loss(hip):
     m.close()
 
 
-@pytest.mark.parametrize("dealias,weight,kw", [(0, 1.0, 0.0), (1, 1.0, 0.0), (1, 2.0, 4.0)])
+@pytest.mark.parametrize("dealias,weight,kw", [(0, 1.0, 0.0), (1, 1.0, 0.0), (1, 2.0, 4.0), (2, 1.0, 0.0)])
 def test_force_gradient_matches_oracle(hip, dealias, weight, kw):
     """Force-matching loss: network forward, packed transform, -div, the
     projected residual, the adjoint divergence and the backward kernel
@@ -172,7 +172,7 @@ def test_force_gradient_matches_oracle(hip, dealias, weight, kw):
     hip.lib().sh_mlp_force_gradient(m.h, n, hip.ptr(Xd), hip.ptr(ftd), dealias, weight, kw, hip.ptr(g),
                                     ctypes.byref(loss))
     lo, go = C.force_loss_grad(P, X, ft.astype(np.complex64).astype(np.complex128), sc_in, sc_out,
-                               dealias=bool(dealias), bf=True, weight=weight, kw=kw)
+                               dealias=dealias, bf=True, weight=weight, kw=kw)
     assert abs(loss.value - lo) / lo < 1e-4
     assert rel_l2(_host(g), go) < 5e-3
     m.close()
@@ -434,6 +434,53 @@ def test_dealias_fused_matches_op_by_op(hip, n):
     hi = (3 * k[:, None, None] > n) | (3 * k[None, :, None] > n) | (3 * k[None, None, :] > n)
     V0hi = np.abs(V0[:, hi]).max()
     assert np.abs(out[0][:, hi]).max() <= V0hi * 1.0000001
+
+
+def test_expfilter_factor_closed_form(hip):
+    """From rest with a divergence-free force f and the exponential filter,
+    the start-up step (forward Euler) gives V1 = K sigma(k) f exactly,
+    sigma = prod_i exp(-36 (|k_i|/(n/2))^36)."""
+    n, K = 32, 1e-2
+    f = O.solenoidal((n, n, n), 87, 12.0, 1.0)
+    ns = hip.NS((n, n, n), 0.0, K, 64, fused=False)
+    fd = _dev(f)
+    hip.lib().sh_ns_set_forcing(ns.h, hip.ptr(fd))
+    hip.lib().sh_ns_set_dealias(ns.h, 2)
+    ns.set_initial(_dev(np.zeros_like(f)))
+    ns.startup()
+    V1 = _host(ns.velocity())
+    ns.close()
+    sig = C.expfilter(n)
+    want = K * sig[None] * f
+    assert np.abs(V1 - want).max() <= 1e-12 * np.abs(want).max()
+    # the filter acts: near the cutoff V1 is well below K f
+    assert np.abs(want).max() > 0 and sig.min() < 1e-3
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_expfilter_dealias_fused_matches_op_by_op(hip, prec):
+    """Exponential-filter dealiasing (mode 2): the fused kernel's factor and
+    the op-by-op kernel's agree (FP64), the FP32 fused solver follows, and the
+    filter changes the trajectory."""
+    n, nu, K = 32, 0.005, 1e-3
+    V0 = O.solenoidal((n, n, n), 83, 5.0, 1.0)
+    out = []
+    for fused, p, mode in ((False, 64, 2), (True, prec, 2), (True, 64, 0)):
+        ns = hip.NS((n, n, n), nu, K, p, fused=fused)
+        hip.lib().sh_ns_set_dealias(ns.h, mode)
+        ns.set_initial(_dev(V0))
+        ns.startup()
+        ns.steps(10)
+        out.append(_host(ns.velocity()))
+        ns.close()
+    assert rel_l2(out[1], out[0]) < (1e-12 if prec == 64 else 1e-5)
+    assert rel_l2(out[0], out[2]) > 1e-9
+    with pytest.raises(RuntimeError):
+        ns = hip.NS((n, n, n), nu, K, 64, fused=True)
+        try:
+            hip.lib().sh_ns_set_dealias(ns.h, 3)
+        finally:
+            ns.close()
 
 
 def test_forcing_ensemble_boxes(hip):
