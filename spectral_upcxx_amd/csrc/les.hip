@@ -25,6 +25,7 @@ int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float*
                             hipStream_t st);
 void* ns_fused_velocity(sh_ns_t s);
 int ns_step_phase(sh_ns_t s, int phase);
+int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx);
 int mlp_infer_planar_packed(sh_mlp_t h, const float* X, long ld, long n, float2* Y, long ldy, hipStream_t st);
 int mlp_grad_dy(sh_mlp_t h, const float* X, const float* Dy, long ld, long n, hipStream_t st);
 int mlp_adam(sh_mlp_t h, float lr, hipStream_t st);
@@ -327,6 +328,24 @@ int sgs_force(sh_les_s* h, const float2* V, hipStream_t st) {
     return SH_OK;
 }
 
+// one LES step: the closure of V^n on the side stream, concurrent with the
+// solver's kernels A-D (which read V and the work fields and write only the
+// latter); kernel E reads the force and then overwrites V, after which the
+// next closure may start
+int les_step(sh_les_s* h) {
+    if (!h->model) return sh_ns_steps(h->ns, 1);
+    const float2* V = (const float2*)ns_fused_velocity(h->ns);
+    SH_HIP(hipEventRecord(h->ev_state, h->st));
+    SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
+    RC(sgs_force(h, V, h->side));
+    SH_HIP(hipEventRecord(h->ev_force, h->side));
+    RC(ns_step_phase(h->ns, 0));
+    SH_HIP(hipStreamWaitEvent(h->st, h->ev_force, 0));
+    return ns_step_phase(h->ns, 1);
+}
+
+int les_step_cb(void* p) { return les_step((sh_les_s*)p); }
+
 void free_les(sh_les_s* h) {
     if (h->ev_state) (void)hipEventDestroy(h->ev_state);
     if (h->ev_force) (void)hipEventDestroy(h->ev_force);
@@ -408,23 +427,7 @@ int sh_les_set_initial_physical(sh_les_t h, const void* ubar3) {
 
 int sh_les_steps(sh_les_t h, int nsteps) {
     if (!h || nsteps < 0) return SH_EINVAL;
-    const float2* V = (const float2*)ns_fused_velocity(h->ns);
-    for (int t = 0; t < nsteps; t++) {
-        if (!h->model) {
-            RC(sh_ns_steps(h->ns, 1));
-            continue;
-        }
-        // the closure of V^n on the side stream, concurrent with kernels A-D,
-        // which read only the work fields; kernel E reads the force and then
-        // overwrites V, after which the next closure may start
-        SH_HIP(hipEventRecord(h->ev_state, h->st));
-        SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
-        RC(sgs_force(h, V, h->side));
-        SH_HIP(hipEventRecord(h->ev_force, h->side));
-        RC(ns_step_phase(h->ns, 0));
-        SH_HIP(hipStreamWaitEvent(h->st, h->ev_force, 0));
-        RC(ns_step_phase(h->ns, 1));
-    }
+    for (int t = 0; t < nsteps; t++) RC(les_step(h));
     return SH_OK;
 }
 
@@ -436,7 +439,9 @@ int sh_les_set_dealias(sh_les_t h, int mode) {
 int sh_les_autotune(sh_les_t h, int reps) {
     if (!h || reps <= 0) return SH_EINVAL;
     RC(sh_autotune_fft3d(h->n, h->n, h->n, SH_F32, reps, h->st));
-    return sh_ns_autotune(h->ns, reps);
+    // the solver's schedules timed on the whole LES step (closure overlapped
+    // on the side stream), which is what they run in
+    return ns_autotune_with(h->ns, reps, &les_step_cb, h);
 }
 
 int sh_les_get_velocity(sh_les_t h, void* out_v3) {

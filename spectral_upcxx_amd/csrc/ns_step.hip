@@ -683,7 +683,9 @@ int launch_cross_z(sh_ns_s* s) {
     return SH_OK;
 }
 
-template <typename T, class P, int LPB>
+// EMIT = false leaves kernel A to the next step (w_primed false), which moves
+// that work off this kernel into the next step's first phase
+template <typename T, class P, int LPB, bool EMIT = true>
 int launch_fourier_x(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
@@ -694,12 +696,13 @@ int launch_fourier_x(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, true>), dim3(nb, (unsigned)s->nboxes),
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
                        (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
+    s->w_primed = EMIT;
     return SH_OK;
 }
 
@@ -727,6 +730,12 @@ int launch_fourier_x(sh_ns_s* s) {
     X(64, P64, 32) X(64, W64, 16) X(64, P64, 16) X(64, P64, 8) X(64, W64, 8) X(64, W64, 4)       \
     X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4)                               \
     X(256, Q256, 4) X(512, P512, 4) X(512, W512, 2)
+
+// kernel E without the next step's x-inverse (kernel A runs each step): for
+// small grids, where A can overlap other work and E alone is latency-bound
+#define SH_FOURIER_F64_NOEMIT_VARIANTS(X) X(32, Q32, 32) X(64, W64, 16) X(64, W64, 8) X(96, Q96, 10)
+#define SH_FOURIER_F32_NOEMIT_VARIANTS(X) \
+    X(32, Q32, 32) X(64, P64, 32) X(64, W64, 16) X(64, P64, 8) X(64, W64, 4) X(96, Q96, 10)
 
 using NsLaunch = int (*)(sh_ns_s*);
 struct NsVariants {
@@ -761,12 +770,18 @@ const NsVariants& fourier_variants(int n) {
 #define SH_V(n_, P_, L_)                         \
     m[n_].names.push_back(#P_ " lines=" #L_);    \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_>);
+#define SH_VN(n_, P_, L_)                                  \
+    m[n_].names.push_back(#P_ " lines=" #L_ " no-emit");   \
+    m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, false>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
+            SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
+            SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
         }
 #undef SH_V
+#undef SH_VN
         return m;
     }();
     static const NsVariants none;
@@ -844,8 +859,14 @@ int step_fused(sh_ns_s* s) {
 }
 
 // time `reps` steps (after one warm step) of the solver as it is now configured
-int time_steps(sh_ns_s* s, int reps, hipEvent_t e0, hipEvent_t e1, double* ms) {
-    auto step = [&]() { return s->prec == SH_F64 ? step_fused<double>(s) : step_fused<float>(s); };
+// step_cb: the caller's step (e.g. one that adds work on other streams and
+// joins them back into s->st); null: the solver's own fused step
+int time_steps(sh_ns_s* s, int reps, hipEvent_t e0, hipEvent_t e1, double* ms, int (*step_cb)(void*),
+               void* ctx) {
+    auto step = [&]() {
+        if (step_cb) return step_cb(ctx);
+        return s->prec == SH_F64 ? step_fused<double>(s) : step_fused<float>(s);
+    };
     RC(step());
     SH_HIP(hipEventRecord(e0, s->st));
     for (int r = 0; r < reps; r++) RC(step());
@@ -860,6 +881,8 @@ int time_steps(sh_ns_s* s, int reps, hipEvent_t e0, hipEvent_t e1, double* ms) {
 }  // namespace
 
 namespace sh {
+int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx);
+
 int ns_variant_count(int precision, int n, int role) {
     if (role == ROLE_CROSS)
         return precision == SH_F64 ? (int)cross_variants<double>(n).fns.size()
@@ -881,7 +904,14 @@ extern "C" {
 // Times every schedule of the solver's tunable kernels (y pass, fused z,
 // fused Fourier/x) on this solver's grid, keeps the fastest of each, and
 // restores the solver state afterwards.  Synchronous.
-int sh_ns_autotune(sh_ns_t s, int reps) {
+int sh_ns_autotune(sh_ns_t s, int reps) { return ns_autotune_with(s, reps, nullptr, nullptr); }
+
+}  // extern "C"
+
+namespace sh {
+// the solver's schedule tuner timing `step_cb(ctx)` steps (null: its own step);
+// the solver state is saved and restored around the timing
+int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     if (!s || reps <= 0) return SH_EINVAL;
     if (!s->fused) {
         set_error("sh_ns_autotune: the op-by-op schedule has nothing to tune");
@@ -911,7 +941,7 @@ int sh_ns_autotune(sh_ns_t s, int reps) {
         for (int v = 0; v < nv && !rc; v++) {
             tune_set(s->prec, n, role, v, 0);
             double ms = 0;
-            rc = time_steps(s, reps, e0, e1, &ms);
+            rc = time_steps(s, reps, e0, e1, &ms, step_cb, ctx);
             if (!rc && ms < best_ms) {
                 best_ms = ms;
                 best = v;
@@ -930,8 +960,7 @@ int sh_ns_autotune(sh_ns_t s, int reps) {
     s->have_div = false;
     return rc;
 }
-
-}  // extern "C"
+}  // namespace sh
 
 namespace {
 
