@@ -7,7 +7,8 @@
 //
 //  * fused (fused = 1, FP64 or FP32): four kernels per step.
 //      A  x-pass inverse of V and of omega = i k x V (curl formed on load);
-//         runs once after a (re)start, later steps get it from E
+//         runs once after a (re)start, later steps get it from E (every
+//         step with a Fourier-update schedule that does not emit it)
 //      B  y-pass inverse of the six fields
 //      C  z-pass inverse of the six fields, complex pointwise u x omega,
 //         z-pass forward of the three products -- all in registers
