@@ -488,6 +488,11 @@ void or_ns_destroy(or_ns* s) {
 
 void or_ns_set_nonlinear(or_ns* s, int on) { s->nonlinear = on; }
 
+/* 0: exec/ns.cpp (Crank-Nicolson viscous term), 1: exec/ns2.cpp (backward
+ * Euler viscous term, exec/ns2.cpp:171-189); the start-up is the same
+ * (exec/ns2.cpp:133-137 = exec/ns.cpp:132-153) */
+void or_ns_set_scheme(or_ns* s, int scheme) { s->scheme = scheme; }
+
 #define FX(p) (p)
 #define FY(p) ((p) + s->F)
 #define FZ(p) ((p) + 2 * s->F)
@@ -549,9 +554,9 @@ int or_ns_steps(or_ns* s, int nsteps) {
         v_scale(s->F3, 3.0, s->cross1, V);
         v_sub(s->F3, s->F3, s->cross0, V);
         v_scale(s->F3, K / 2, s->F3, V);
-        /* lap1 = nu*K/2*V1.laplacian() */
+        /* lap1 = nu*K/2*V1.laplacian()  (exec/ns2.cpp:178: nu*K*V1.laplacian()) */
         ns_vlap(s, s->V1, s->lap1);
-        v_scale(s->lap1, nu * K / 2, s->lap1, V);
+        v_scale(s->lap1, s->scheme ? nu * K : nu * K / 2, s->lap1, V);
         /* G = (V1 + lap1 + F).divergence(); G.laplacian_inverse() */
         v_add(s->S, s->V1, s->lap1, V);
         v_add(s->S, s->S, s->F3, V);
@@ -561,10 +566,17 @@ int or_ns_steps(or_ns* s, int nsteps) {
         or_gradient(s->G, FX(s->Gg), FY(s->Gg), FZ(s->Gg), nx, ny, nz);
         v_add(s->Vh, s->V1, s->F3, V);
         v_sub(s->Vh, s->Vh, s->Gg, V);
-        /* V = Vhalf + lap1; V[d].laplacian_inverse(1, -nu*K/2) */
-        v_add(s->V, s->Vh, s->lap1, V);
-        for (int d = 0; d < 3; d++)
-            or_laplacian_inverse(s->V + d * s->F, 1.0, -nu * K / 2, nx, ny, nz);
+        if (s->scheme == 0) {
+            /* V = Vhalf + lap1; V[d].laplacian_inverse(1, -nu*K/2) */
+            v_add(s->V, s->Vh, s->lap1, V);
+            for (int d = 0; d < 3; d++)
+                or_laplacian_inverse(s->V + d * s->F, 1.0, -nu * K / 2, nx, ny, nz);
+        } else {
+            /* exec/ns2.cpp:185-189: V = Vhalf; V[d].laplacian_inverse(1, -nu*K) */
+            v_copy(s->V, s->Vh, V);
+            for (int d = 0; d < 3; d++)
+                or_laplacian_inverse(s->V + d * s->F, 1.0, -nu * K, nx, ny, nz);
+        }
         /* omega = V.curl(); div = V.divergence(); div.average() */
         or_curl(FX(s->V), FY(s->V), FZ(s->V), FX(s->omega), FY(s->omega), FZ(s->omega),
                 nx, ny, nz);
