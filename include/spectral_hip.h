@@ -140,6 +140,36 @@ int sh_solenoidal_ic_z(void* v3, int nx, int ny, int nz, uint64_t seed, double k
  * pointers are non-null.  Synchronous. */
 int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek, int nbins,
                   double* energy, double* dissipation, void* stream);
+/* enstrophy 0.5<|curl u|^2> and divergence norm sqrt(<|div u|^2>) of a Fourier
+ * velocity, curl and divergence as src/Vector.cpp:92-138 (either output may
+ * be NULL; FP64 block sums added in a fixed order).  Synchronous. */
+int sh_vorticity_stats_z(const void* v3, int nx, int ny, int nz, double* enstrophy, double* divnorm,
+                         void* stream);
+/* Gaussian-vortex initial condition of the reference drivers: omega0 =
+ * (0, 0, w_z), w_z = 50 sum_v exp(-(x-x_v)^2/w) exp(-(y-y_v)^2/w), w = pi/8,
+ * on x_i = 2 pi i / N, forward()-ed (Nyquist zeroed); psi = -w_z/(kx^2+ky^2)
+ * where (kx, ky) != 0; V0 = (-d_y psi, d_x psi, 0).  set 0: three vortices
+ * (exec/ns.cpp:84-131); set 1: two vortices (exec/ns2.cpp:84-130).  V0 and
+ * omega0 are 3 contiguous Fourier complex128 fields each. */
+int sh_vortex_ic_z(void* V0, void* omega0, int nx, int ny, int nz, int set, void* stream);
+
+/* ------------------------------------------------------------------ marker particles (src/Particle.cpp) */
+/* Particle::Particle (src/Particle.cpp:5-10) for particles first ..
+ * first+np-1: positions loc[np][3] in [0, 2 pi); the reference draws rand(),
+ * here coordinate d of particle p is 2 pi U(seed, 3p + d) (splitmix64,
+ * 53-bit).  Device memory. */
+int sh_particles_seed(void* loc, long np, uint64_t seed, long first, void* stream);
+/* Particle::move(velocity, K) (src/Particle.cpp:12-46) for np particles, as
+ * exec/ns2.cpp:191-204 calls it once per step: trilinear interpolation of the
+ * real parts of ux, uy, uz (the physical velocity components, complex128
+ * fields as Vector::inverse leaves them) at each position, with the reference's cell
+ * clamp (an index at dims-1 moves to dims-2), then loc += v K and the periodic
+ * wrap.  vel[np][3] receives the interpolated velocity; energy[np] (may be
+ * NULL) receives Particle::energy() = 0.5 |v|^2 (src/Particle.cpp:48-55).
+ * Same IEEE operations in the same order, no contraction: bit-identical to a
+ * scalar double run of the reference.  Needs nx, ny, nz >= 2. */
+int sh_particles_move(void* loc, void* vel, void* energy, long np, const void* ux, const void* uy,
+                      const void* uz, int nx, int ny, int nz, double K, void* stream);
 
 /* ------------------------------------------------------------------ Navier-Stokes fractional step (exec/ns.cpp:132-217) */
 typedef struct sh_ns_s* sh_ns_t;
@@ -151,6 +181,11 @@ int sh_ns_create(sh_ns_t* handle, int nx, int ny, int nz, double nu, double K, i
 int sh_ns_destroy(sh_ns_t handle);
 /* V0: 3 contiguous Fourier fields, complex128 device memory */
 int sh_ns_set_initial(sh_ns_t handle, const void* V0);
+/* Replace omega0 = curl(V0), formed by sh_ns_set_initial, with the caller's
+ * initial vorticity (3 Fourier fields) before sh_ns_startup: exec/ns.cpp:113-135
+ * and exec/ns2.cpp:113-137 start from the vorticity V0 was built from, whose
+ * mean mode curl(V0) lacks (see sh_vortex_ic_z). */
+int sh_ns_set_vorticity(sh_ns_t handle, const void* omega0);
 /* exec/ns.cpp:132-153: Pi0, Forward-Euler V1, omega1, cross0 */
 int sh_ns_startup(sh_ns_t handle);
 /* nsteps iterations of exec/ns.cpp:167-217 */
@@ -160,6 +195,11 @@ int sh_ns_get_velocity(sh_ns_t handle, void* out_v3);
 int sh_ns_get_vorticity(sh_ns_t handle, void* out_v3);
 /* div.average() of the last step (exec/ns.cpp:209-210).  Synchronous. */
 int sh_ns_div_average(sh_ns_t handle, double* result);
+/* Viscous time discretisation: 0 (default) the Crank-Nicolson step of
+ * exec/ns.cpp:174-187; 1 the backward-Euler step of exec/ns2.cpp:178-189
+ * (lap1 = nu*K*V1.laplacian(), V = Vhalf divided by 1 + nu*K*|k|^2).  The
+ * start-up is the same for both (exec/ns2.cpp:133-137).  Synchronous. */
+int sh_ns_set_scheme(sh_ns_t handle, int scheme);
 /* Stokes limit switch (nonlinear term off), for closed-form checks */
 int sh_ns_set_nonlinear(sh_ns_t handle, int on);
 /* device bytes held by the solver state */

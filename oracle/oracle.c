@@ -447,6 +447,7 @@ struct or_ns {
     size_t F; /* doubles per field */
     double nu, K;
     int nonlinear;
+    int scheme; /* 0: exec/ns.cpp, 1: exec/ns2.cpp */
     double div_avg;
     /* vectors: 3 contiguous fields each */
     double *V0, *V1, *V, *omega0, *omega1, *omega, *cross0, *cross1, *F3, *lap1, *S, *Gg, *Vh;
@@ -467,7 +468,7 @@ or_ns* or_ns_create(int nx, int ny, int nz, double nu, double K) {
     or_ns* s = (or_ns*)calloc(1, sizeof(or_ns));
     s->nx = nx; s->ny = ny; s->nz = nz;
     s->F = 2 * (size_t)nx * ny * nz;
-    s->nu = nu; s->K = K; s->nonlinear = 1;
+    s->nu = nu; s->K = K; s->nonlinear = 1; s->scheme = 0;
     size_t V = 3 * s->F;
     s->V0 = falloc(V); s->V1 = falloc(V); s->V = falloc(V);
     s->omega0 = falloc(V); s->omega1 = falloc(V); s->omega = falloc(V);
@@ -515,6 +516,10 @@ int or_ns_set_initial(or_ns* s, const double* V0) {
             s->nx, s->ny, s->nz);
     return 0;
 }
+
+/* exec/ns.cpp:113-135, exec/ns2.cpp:113-137: the drivers start from the
+ * vorticity they built V0 from, not from curl(V0) */
+void or_ns_set_vorticity(or_ns* s, const double* omega0) { v_copy(s->omega0, omega0, 3 * s->F); }
 
 /* exec/ns.cpp:132-153 */
 int or_ns_startup(or_ns* s) {
@@ -725,4 +730,160 @@ void or_spectrum(const double* V, int nx, int ny, int nz, double* Ek, int nbins)
                     e += V[d * F + 2 * m] * V[d * F + 2 * m] + V[d * F + 2 * m + 1] * V[d * F + 2 * m + 1];
                 Ek[b] += 0.5 * e / ((double)n * (double)n);
             }
+}
+
+/* Enstrophy 0.5 <|curl u|^2> and the divergence norm sqrt(<|div u|^2>) of a
+ * Fourier velocity (numpy ifftn scaling, as or_energy); the curl and the
+ * divergence are those of src/Vector.cpp:92-138 (i k x V, i k . V). */
+void or_vorticity_stats(const double* V, int nx, int ny, int nz, double* enstrophy, double* divnorm) {
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t F = 2 * n;
+    double w2 = 0, d2 = 0;
+#pragma omp parallel for schedule(static) reduction(+ : w2, d2)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                size_t m = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
+                double kk[3] = {or_freq(i, nx), or_freq(j, ny), or_freq(k, nz)};
+                double re[3], im[3];
+                for (int d = 0; d < 3; d++) {
+                    re[d] = V[d * F + 2 * m];
+                    im[d] = V[d * F + 2 * m + 1];
+                }
+                /* |k x u|^2 and |k . u|^2 (the factor i has modulus 1) */
+                for (int d = 0; d < 3; d++) {
+                    int a = (d + 1) % 3, b = (d + 2) % 3;
+                    double cr = kk[a] * re[b] - kk[b] * re[a];
+                    double ci = kk[a] * im[b] - kk[b] * im[a];
+                    w2 += cr * cr + ci * ci;
+                }
+                double dr = kk[0] * re[0] + kk[1] * re[1] + kk[2] * re[2];
+                double di = kk[0] * im[0] + kk[1] * im[1] + kk[2] * im[2];
+                d2 += dr * dr + di * di;
+            }
+    const double nn = (double)n * (double)n;
+    if (enstrophy) *enstrophy = 0.5 * w2 / nn;
+    if (divnorm) *divnorm = sqrt(d2 / nn);
+}
+
+/* ------------------------------------------------------------------ Gaussian vortices */
+static double gaussian2(double x, double y, double xc, double yc, double width) {
+    return exp(-pow(x - xc, 2) / width) * exp(-pow(y - yc, 2) / width);
+}
+
+/* exec/ns.cpp:84-131 (set 0, three vortices) and exec/ns2.cpp:84-130 (set 1,
+ * two vortices): omega0 = (0, 0, w_z), forward() per component; psi =
+ * -w_z/(kx^2+ky^2) where (kx, ky) != 0 and 0 elsewhere; V0 = (-grad_y psi,
+ * grad_x psi, 0).  tmp has one field. */
+int or_ic_vortices(double* V0, double* omega0, double* tmp, int nx, int ny, int nz, int set) {
+    const size_t F = 2 * (size_t)nx * ny * nz;
+    const double width = M_PI / 8, mag = 50, dist = M_PI / 2;
+    const double hx = 2. * M_PI / nx, hy = 2. * M_PI / ny;
+    for (size_t i = 0; i < 3 * F; i++) omega0[i] = V0[i] = 0;
+    double* wz = omega0 + 2 * F;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                size_t m = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
+                double x = hx * i, y = hy * j;
+                double w;
+                if (set == 0)
+                    w = mag * gaussian2(x, y, M_PI - dist, M_PI - dist, width) +
+                        mag * gaussian2(x, y, M_PI + dist, M_PI - dist, width) +
+                        mag * gaussian2(x, y, M_PI, M_PI + dist, width);
+                else
+                    w = mag * gaussian2(x, y, M_PI - dist, M_PI, width) +
+                        mag * gaussian2(x, y, M_PI + dist, M_PI, width);
+                wz[2 * m] = w;
+            }
+    for (int d = 0; d < 3; d++) {
+        int rc = or_forward(omega0 + d * F, tmp, nx, ny, nz);
+        if (rc) return rc;
+    }
+    double* psi = tmp;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                size_t m = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
+                double kx = or_freq(i, nx), ky = or_freq(j, ny);
+                if (kx != 0 || ky != 0) {
+                    double c = -1. / (kx * kx + ky * ky);
+                    psi[2 * m] = c * wz[2 * m];
+                    psi[2 * m + 1] = c * wz[2 * m + 1];
+                } else {
+                    psi[2 * m] = psi[2 * m + 1] = 0;
+                }
+            }
+    /* grad = psi.gradient(); V0[0] = -1.*grad[1]; V0[1] = grad[0]
+     * (gradient written as x -> field 0, y -> field 2, z -> field 1, then moved) */
+    or_gradient(psi, V0, V0 + 2 * F, V0 + F, nx, ny, nz);
+    for (size_t i = 0; i < F; i++) {
+        double gy = V0[2 * F + i];
+        V0[F + i] = V0[i];
+        V0[i] = -1. * gy;
+        V0[2 * F + i] = 0;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ marker particles */
+/* Particle::Particle (src/Particle.cpp:5-10) places each particle uniformly in
+ * [0, 2 pi]^3 with rand(); here coordinate d of particle p is 2 pi U(seed,
+ * 3p + d) (splitmix64, 53-bit), reproducible on every platform. */
+void or_particles_seed(double* loc, long np, uint64_t seed) {
+    for (long p = 0; p < 3 * np; p++) loc[p] = (double)(or_splitmix64(seed, (uint64_t)p) >> 11) * 0x1.0p-53 * 2 * M_PI;
+}
+
+/* Particle::move (src/Particle.cpp:12-46) for np particles, loc and vel [np][3],
+ * U the physical velocity (3 complex fields, real parts read).  Written in
+ * the reference's operation order with contraction off, so that a device
+ * implementation doing the same IEEE operations matches bit for bit.  Cell
+ * indices are clamped to [0, dims-2]: the upper clamp is the reference's,
+ * the lower one only guards positions outside [0, 2 pi), which the
+ * reference's wrap never produces for |v K| < 2 pi. */
+__attribute__((optimize("fp-contract=off")))
+void or_particles_move(double* loc, double* vel, double* energy, long np, const double* U,
+                       int nx, int ny, int nz, double K) {
+    const size_t F = 2 * (size_t)nx * ny * nz;
+    const long dims[3] = {nx, ny, nz};
+    for (long p = 0; p < np; p++) {
+        double* x = loc + 3 * p;
+        double* v = vel + 3 * p;
+        long ind[3];
+        double rd[3];
+        for (int d = 0; d < 3; d++) {
+            double h = 2 * M_PI / dims[d];
+            double q = x[d] / h;
+            long i = q >= 0 ? (q < (double)dims[d] ? (long)q : dims[d]) : 0;
+            if (i + 1 >= dims[d]) i = dims[d] - 2;
+            ind[d] = i;
+            rd[d] = (x[d] - (double)i * h) / h;
+        }
+#define UAT(d, a, b, c) U[(d) * F + 2 * ((size_t)(ind[0] + (a)) + (size_t)nx * (ind[1] + (b)) + (size_t)nx * ny * (ind[2] + (c)))]
+        for (int d = 0; d < 3; d++) {
+            v[d] = UAT(d, 0, 0, 0) * (1 - rd[0]) * (1 - rd[1]) * (1 - rd[2]) +
+                   UAT(d, 1, 0, 0) * rd[0] * (1 - rd[1]) * (1 - rd[2]) +
+                   UAT(d, 1, 1, 0) * rd[0] * rd[1] * (1 - rd[2]) +
+                   UAT(d, 0, 1, 0) * (1 - rd[0]) * rd[1] * (1 - rd[2]) +
+                   UAT(d, 0, 0, 1) * (1 - rd[0]) * (1 - rd[1]) * (rd[2]) +
+                   UAT(d, 1, 0, 1) * rd[0] * (1 - rd[1]) * (rd[2]) +
+                   UAT(d, 1, 1, 1) * rd[0] * rd[1] * (rd[2]) +
+                   UAT(d, 0, 1, 1) * (1 - rd[0]) * rd[1] * (rd[2]);
+        }
+#undef UAT
+        for (int d = 0; d < 3; d++) {
+            x[d] += v[d] * K;
+            if (x[d] >= 2 * M_PI)
+                x[d] = x[d] - 2 * M_PI;
+            else if (x[d] <= 0)
+                x[d] = 2 * M_PI + x[d];
+        }
+        if (energy) {
+            double e = 0;
+            for (int d = 0; d < 3; d++) e += v[d] * v[d];
+            energy[p] = .5 * e;
+        }
+    }
 }

@@ -80,6 +80,10 @@ void or_ns_get_vorticity(const or_ns* s, double* out);
 double or_ns_div_average(const or_ns* s);
 /* disable the cross product (Stokes limit) for closed-form checks */
 void or_ns_set_nonlinear(or_ns* s, int on);
+/* 0: exec/ns.cpp's Crank-Nicolson viscous term; 1: exec/ns2.cpp's backward Euler */
+void or_ns_set_scheme(or_ns* s, int scheme);
+/* replace omega0 = curl(V0) with the caller's initial vorticity (before startup) */
+void or_ns_set_vorticity(or_ns* s, const double* omega0);
 
 /* ---- initial conditions and diagnostics ---- */
 int or_ic_taylor_green(double* V, double* tmp, int nx, int ny, int nz);
@@ -88,6 +92,16 @@ void or_ic_solenoidal(double* V, int nx, int ny, int nz, uint64_t seed, double k
 double or_energy(const double* V, int nx, int ny, int nz);
 /* shell spectrum E(k), k = round(|k|), nbins entries */
 void or_spectrum(const double* V, int nx, int ny, int nz, double* Ek, int nbins);
+/* enstrophy 0.5 <|curl u|^2> and divergence norm sqrt(<|div u|^2>) (either pointer may be NULL) */
+void or_vorticity_stats(const double* V, int nx, int ny, int nz, double* enstrophy, double* divnorm);
+/* Gaussian vortices: set 0 exec/ns.cpp:84-131 (three), set 1 exec/ns2.cpp:84-130
+ * (two); V0 and omega0 are 3 Fourier fields each, tmp one field */
+int or_ic_vortices(double* V0, double* omega0, double* tmp, int nx, int ny, int nz, int set);
+
+/* ---- marker particles (src/Particle.cpp) ---- */
+void or_particles_seed(double* loc, long np, uint64_t seed);
+void or_particles_move(double* loc, double* vel, double* energy, long np, const double* U,
+                       int nx, int ny, int nz, double K);
 
 int or_num_threads(void);
 

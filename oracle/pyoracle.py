@@ -52,6 +52,12 @@ def lib():
             "or_ns_get_vorticity": (None, [vp, _dp]),
             "or_ns_div_average": (d, [vp]),
             "or_ns_set_nonlinear": (None, [vp, i]),
+            "or_ns_set_scheme": (None, [vp, i]),
+            "or_ns_set_vorticity": (None, [vp, _dp]),
+            "or_vorticity_stats": (None, [_dp, i, i, i, _dp, _dp]),
+            "or_ic_vortices": (i, [_dp, _dp, _dp, i, i, i, i]),
+            "or_particles_seed": (None, [_dp, ctypes.c_long, u64]),
+            "or_particles_move": (None, [_dp, _dp, _dp, ctypes.c_long, _dp, i, i, i, d]),
             "or_ic_taylor_green": (i, [_dp, _dp, i, i, i]),
             "or_ic_solenoidal": (None, [_dp, i, i, i, u64, d, d]),
             "or_energy": (d, [_dp, i, i, i]),
@@ -209,6 +215,43 @@ def spectrum(V, nbins):
     return Ek
 
 
+def vorticity_stats(V):
+    """(enstrophy 0.5<|curl u|^2>, divergence norm sqrt(<|div u|^2>))."""
+    V = np.ascontiguousarray(V, np.complex128)
+    e, dn = ctypes.c_double(), ctypes.c_double()
+    lib().or_vorticity_stats(_p(V), *_dims(V), ctypes.byref(e), ctypes.byref(dn))
+    return e.value, dn.value
+
+
+def vortices(dims, which):
+    """Gaussian-vortex IC: which 0 = exec/ns.cpp (three), 1 = exec/ns2.cpp (two).
+    Returns (V0, omega0), Fourier, 3 fields each."""
+    nx, ny, nz = dims
+    V = np.empty((3, nz, ny, nx), np.complex128)
+    W = np.empty_like(V)
+    _check(lib().or_ic_vortices(_p(V), _p(W), _p(np.empty((nz, ny, nx), np.complex128)), nx, ny, nz,
+                                int(which)), "vortices")
+    return V, W
+
+
+def particles_seed(npart, seed):
+    loc = np.empty((npart, 3), np.float64)
+    lib().or_particles_seed(_p(loc), npart, seed)
+    return loc
+
+
+def particles_move(loc, U, K):
+    """Particle::move for every row of loc (modified in place); U is the
+    physical velocity (3, nz, ny, nx) complex128.  Returns (vel, energy)."""
+    assert loc.dtype == np.float64 and loc.flags["C_CONTIGUOUS"]
+    U = np.ascontiguousarray(U, np.complex128)
+    n = loc.shape[0]
+    vel = np.empty((n, 3), np.float64)
+    en = np.empty(n, np.float64)
+    lib().or_particles_move(_p(loc), _p(vel), _p(en), n, _p(U), *_dims(U), float(K))
+    return vel, en
+
+
 def num_threads():
     return lib().or_num_threads()
 
@@ -234,9 +277,17 @@ class NS:
     def set_nonlinear(self, on):
         lib().or_ns_set_nonlinear(self.h, int(bool(on)))
 
+    def set_scheme(self, scheme):
+        """0: exec/ns.cpp (Crank-Nicolson viscous term), 1: exec/ns2.cpp (backward Euler)."""
+        lib().or_ns_set_scheme(self.h, int(scheme))
+
     def set_initial(self, V0):
         V0 = np.ascontiguousarray(V0, np.complex128)
         _check(lib().or_ns_set_initial(self.h, _p(V0)), "ns_set_initial")
+
+    def set_vorticity(self, W0):
+        W0 = np.ascontiguousarray(W0, np.complex128)
+        lib().or_ns_set_vorticity(self.h, _p(W0))
 
     def startup(self):
         _check(lib().or_ns_startup(self.h), "ns_startup")

@@ -94,6 +94,10 @@ class NS:
     def set_nonlinear(self, on):
         lib().sh_ns_set_nonlinear(self.h, int(bool(on)))
 
+    def set_scheme(self, scheme):
+        """0: exec/ns.cpp (Crank-Nicolson viscous term), 1: exec/ns2.cpp (backward Euler)."""
+        lib().sh_ns_set_scheme(self.h, int(scheme))
+
     def startup(self):
         lib().sh_ns_startup(self.h)
 

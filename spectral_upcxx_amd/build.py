@@ -22,17 +22,18 @@ BIN = os.path.join(LIB, "bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HIP_SOURCES = ["runtime.hip", "fft3d.hip", "ops.hip", "ns_step.hip", "tune.hip", "mlp.hip", "les.hip"]
+HIP_SOURCES = ["runtime.hip", "fft3d.hip", "ops.hip", "ns_step.hip", "tune.hip", "mlp.hip", "les.hip", "particles.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     # packed FP32 instructions are kept off (v_pk_* forms are not used)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
 ]
-API_SOURCES = ["Scalar.cpp", "Vector.cpp", "Operators.cpp", "FFT1D.cpp"]
+API_SOURCES = ["Scalar.cpp", "Vector.cpp", "Operators.cpp", "FFT1D.cpp", "Particle.cpp"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + CPP, "-I" + os.path.join(ROOT, "include")]
 EXECS = {
     "ns": "exec/ns.cpp",
+    "ns2": "exec/ns2.cpp",
     "fft": "exec/fft.cpp",
     "ilap": "exec/ilap.cpp",
     "test_operators": "test/operators.cpp",

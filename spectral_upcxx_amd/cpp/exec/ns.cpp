@@ -2,7 +2,8 @@
 //
 //   ns N [steps] [ic] [nu] [K] [mode] [out.bin]
 //     ic   : tg (Taylor-Green) | sol:<seed>:<k0> (random solenoidal, u_rms 1)
-//            | vortex (three Gaussian vortices, exec/ns.cpp:97-131)
+//            | vortex (three Gaussian vortices, exec/ns.cpp:97-131; omega0 is
+//              the vortices' vorticity, as there, not curl(V0))
 //     mode : api   -- the reference's operation sequence through Scalar/Vector/
 //                     Operators (exec/ns.cpp:132-217), one device kernel per op
 //            fused -- the five-kernel step of the C ABI (sh_ns_*)
@@ -32,8 +33,9 @@ static void check(int rc, const char* what) {
     }
 }
 
-// initial velocity V0 in Fourier space
-static void initial_velocity(Vector& V0, const string& ic, array<INT, DIM> dims) {
+// initial velocity V0 in Fourier space; omega0 is set (and true returned) when
+// the IC defines the initial vorticity itself
+static bool initial_velocity(Vector& V0, Vector& omega0_out, const string& ic, array<INT, DIM> dims) {
     const int nx = (int)dims[0], ny = (int)dims[1], nz = (int)dims[2];
     if (ic == "tg") {
         Vector u(dims, true);
@@ -47,6 +49,7 @@ static void initial_velocity(Vector& V0, const string& ic, array<INT, DIM> dims)
                 }
         u.forward();
         V0 = u;
+        return false;
     } else if (ic.rfind("sol", 0) == 0) {
         unsigned long long seed = 2024;
         double k0 = 4.0;
@@ -59,6 +62,7 @@ static void initial_velocity(Vector& V0, const string& ic, array<INT, DIM> dims)
             check(sh_memcpy_d2d(V0[d].deviceDataMutable(), (cdouble*)buf + d * n, n * sizeof(cdouble), nullptr),
                   "copy");
         sh_free(buf);
+        return false;
     } else if (ic == "vortex") {
         // vorticity of three Gaussian vortices, streamfunction, V0 = (-d_y psi, d_x psi, 0)
         Vector omega0(dims);
@@ -83,10 +87,11 @@ static void initial_velocity(Vector& V0, const string& ic, array<INT, DIM> dims)
         Vector grad = psi.gradient();
         V0[0] = -1. * grad[1];
         V0[1] = grad[0];
-    } else {
-        cout << "unknown ic " << ic << endl;
-        exit(2);
+        omega0_out = omega0;
+        return true;
     }
+    cout << "unknown ic " << ic << endl;
+    exit(2);
 }
 
 int main(int argc, char* argv[]) {
@@ -101,13 +106,13 @@ int main(int argc, char* argv[]) {
     array<INT, DIM> dims = {N, N, N};
     const long n = (long)N * N * N;
 
-    Vector V0(dims, false);
-    initial_velocity(V0, ic, dims);
+    Vector V0(dims, false), omega_ic;
+    const bool has_omega = initial_velocity(V0, omega_ic, ic, dims);
 
     Vector V1;
     double wtime = 0, divavg = 0;
     if (mode == "api") {
-        Vector omega0 = V0.curl();
+        Vector omega0 = has_omega ? omega_ic : V0.curl();
         Vector omega1(dims, false), omega(dims, false), V(dims, false), Vhalf(dims, false);
         Scalar Pi0(dims, false), G(dims, false), div(dims, false);
         // start-up: Pi0, Forward Euler, omega1, cross0 (exec/ns.cpp:132-153)
@@ -148,6 +153,16 @@ int main(int argc, char* argv[]) {
         sh_ns_t h = nullptr;
         check(sh_ns_create(&h, (int)N, (int)N, (int)N, nu, K, SH_F64, 1, nullptr), "sh_ns_create");
         check(sh_ns_set_initial(h, v0), "sh_ns_set_initial");
+        if (has_omega) {
+            void* w0 = nullptr;
+            check(sh_malloc(&w0, 3 * n * sizeof(cdouble)), "sh_malloc");
+            for (int d = 0; d < 3; d++)
+                check(sh_memcpy_d2d((cdouble*)w0 + d * n, omega_ic[d].deviceData(), n * sizeof(cdouble), nullptr),
+                      "copy");
+            check(sh_ns_set_vorticity(h, w0), "sh_ns_set_vorticity");
+            check(sh_stream_synchronize(nullptr), "sync");
+            sh_free(w0);
+        }
         check(sh_ns_startup(h), "sh_ns_startup");
         check(sh_stream_synchronize(nullptr), "sync");
         auto t0 = chrono::steady_clock::now();

@@ -41,8 +41,9 @@ int reduce_z(const void* f, long n, int mode, double* result, hipStream_t st);
 template <typename T>
 struct StepCoef {
     T Kh;          // K/2                   (exec/ns.cpp:171)
-    T c1;          // nu*K/2                (exec/ns.cpp:174)
-    double c1d;    // nu*K/2 in FP64 for the CN factor (exec/ns.cpp:186)
+    T c1;          // nu*K/2                (exec/ns.cpp:174; nu*K in exec/ns2.cpp:178)
+    double c1d;    // the same in FP64 for the viscous divide (exec/ns.cpp:186; ns2.cpp:187)
+    int be;        // 1: exec/ns2.cpp's backward-Euler viscous term (V = Vhalf / (1 + nu K k^2))
     T inv_n3;      // 1/(Nx Ny Nz)          (src/Scalar.cpp:223-227)
     int dealias;   // 1: products with some |k_i| > N_i/3 are dropped (2/3 rule);
                    // 2: products scaled by prod_i exp(-36 (|k_i| / (N_i/2))^36)
@@ -300,9 +301,9 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             const T kk3[3] = {k0, k1, k2};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1
+                // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1 (exec/ns2.cpp:185: V = Vhalf)
                 C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
-                C2<T> vv = cadd<T>(vh, L[d]);
+                C2<T> vv = cf.be ? vh : cadd<T>(vh, L[d]);
                 // V[d].laplacian_inverse(1, -nu*K/2): divide by 1 + (nu K/2)|k|^2 (FP64 factor)
                 if (sum != 0) {
                     const double den = 1.0 - (-cf.c1d) * (double)sum;
@@ -516,7 +517,8 @@ struct sh_ns_s {
     double div_avg;
     const void* forcing;  // body force (3 fields per box, precision prec) or null
     int cur_box;          // box whose start-up is running
-    int dealias;          // 2/3-rule truncation of the nonlinear term (0: none, as the reference)
+    int dealias;          // dealiasing of the nonlinear term (0: none, as the reference)
+    int scheme;           // 0: exec/ns.cpp (CN viscous), 1: exec/ns2.cpp (backward Euler viscous)
     std::vector<void*> owned;
 };
 
@@ -628,8 +630,10 @@ int step64(sh_ns_s* s) {
     RC(sh_scale_z(s->Fv, 3.0, s->cr1, V, s->st));
     RC(sh_sub_z(s->Fv, s->Fv, s->cr0, V, s->st));
     RC(sh_scale_z(s->Fv, K / 2, s->Fv, V, s->st));
+    // exec/ns2.cpp:178, 185-189: lap1 = nu*K*V1.laplacian(); V = Vhalf; divide by 1 + nu K |k|^2
+    const double c1 = s->scheme ? nu * K : nu * K / 2;
     RC(vlap(s, s->V1, s->lap1));
-    RC(sh_scale_z(s->lap1, nu * K / 2, s->lap1, V, s->st));
+    RC(sh_scale_z(s->lap1, c1, s->lap1, V, s->st));
     RC(sh_add_z(s->S, s->V1, s->lap1, V, s->st));
     RC(sh_add_z(s->S, s->S, s->Fv, V, s->st));
     RC(sh_divergence_z(F3(s->S, 0), F3(s->S, 1), F3(s->S, 2), s->G, nx, ny, nz, s->st));
@@ -637,8 +641,11 @@ int step64(sh_ns_s* s) {
     RC(sh_gradient_z(s->G, F3(s->Gg, 0), F3(s->Gg, 1), F3(s->Gg, 2), nx, ny, nz, s->st));
     RC(sh_add_z(s->Vh, s->V1, s->Fv, V, s->st));
     RC(sh_sub_z(s->Vh, s->Vh, s->Gg, V, s->st));
-    RC(sh_add_z(s->Vn, s->Vh, s->lap1, V, s->st));
-    for (int d = 0; d < 3; d++) RC(sh_laplacian_inverse_z(F3(s->Vn, d), 1.0, -nu * K / 2, nx, ny, nz, s->st));
+    if (s->scheme)
+        SH_HIP(hipMemcpyAsync(s->Vn, s->Vh, sizeof(double2) * V, hipMemcpyDeviceToDevice, s->st));
+    else
+        RC(sh_add_z(s->Vn, s->Vh, s->lap1, V, s->st));
+    for (int d = 0; d < 3; d++) RC(sh_laplacian_inverse_z(F3(s->Vn, d), 1.0, -c1, nx, ny, nz, s->st));
     RC(curl(s, s->Vn, s->omn));
     RC(sh_divergence_z(F3(s->Vn, 0), F3(s->Vn, 1), F3(s->Vn, 2), s->div, nx, ny, nz, s->st));
     s->have_div = true;
@@ -971,9 +978,11 @@ int upload_coefs_t(sh_ns_s* s) {
     std::vector<StepCoef<T>> h(s->nboxes);
     for (int b = 0; b < s->nboxes; b++) {
         const double nu = s->nus[b];
+        const double c1 = s->scheme ? nu * s->K : nu * s->K / 2;
         h[b].Kh = T(s->K / 2);
-        h[b].c1 = T(nu * s->K / 2);
-        h[b].c1d = nu * s->K / 2;
+        h[b].c1 = T(c1);
+        h[b].c1d = c1;
+        h[b].be = s->scheme;
         h[b].inv_n3 = T(1.0 / ((double)s->nx * s->ny * s->nz));
         h[b].dealias = s->dealias;
     }
@@ -1049,6 +1058,7 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
     s->forcing = nullptr;
     s->cur_box = 0;
     s->dealias = 0;
+    s->scheme = 0;
     const size_t z = sizeof(double2);
     int rc = 0;
     // FP64 state used by the start-up (and by the op-by-op loop)
@@ -1246,6 +1256,15 @@ int sh_ns_set_forcing(sh_ns_t s, const void* f3) {
     return SH_OK;
 }
 
+int sh_ns_set_scheme(sh_ns_t s, int scheme) {
+    if (!s || scheme < 0 || scheme > 1) {
+        set_error("sh_ns_set_scheme: scheme must be 0 (exec/ns.cpp) or 1 (exec/ns2.cpp)");
+        return SH_EINVAL;
+    }
+    s->scheme = scheme;
+    return s->fused ? upload_coefs(s) : SH_OK;
+}
+
 int sh_ns_set_dealias(sh_ns_t s, int mode) {
     if (!s || mode < 0 || mode > 2) {
         set_error("sh_ns_set_dealias: mode must be 0, 1 or 2");
@@ -1265,6 +1284,12 @@ int sh_ns_set_initial(sh_ns_t s, const void* V0) {
     if (!s || !V0) return SH_EINVAL;
     SH_HIP(hipMemcpyAsync(s->V0, V0, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
     return curl(s, s->V0, s->om0);
+}
+
+int sh_ns_set_vorticity(sh_ns_t s, const void* omega0) {
+    if (!s || !omega0) return SH_EINVAL;
+    SH_HIP(hipMemcpyAsync(s->om0, omega0, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
+    return SH_OK;
 }
 
 int sh_ns_startup(sh_ns_t s) {

@@ -325,6 +325,84 @@ __global__ __launch_bounds__(kBlock) void k_spectrum(const double2* __restrict__
     }
 }
 
+// per-block partial sums of |k x u|^2 and |k . u|^2 (curl and divergence of
+// src/Vector.cpp:92-138 without the factor i, which has modulus 1)
+__global__ __launch_bounds__(kBlock) void k_vort_div_partial(const double2* __restrict__ v, Dims d,
+                                                             double* __restrict__ part) {
+    __shared__ double sw[kBlock], sd[kBlock];
+    const long N = d.n();
+    double w2 = 0, d2 = 0;
+    GRID_STRIDE(n, N) {
+        double k[3];
+        wavenumber(n, d, k[0], k[1], k[2]);
+        const double2 u[3] = {v[n], v[N + n], v[2 * N + n]};
+        for (int c = 0; c < 3; c++) {
+            const int a = (c + 1) % 3, b = (c + 2) % 3;
+            const double cr = k[a] * u[b].x - k[b] * u[a].x;
+            const double ci = k[a] * u[b].y - k[b] * u[a].y;
+            w2 += cr * cr + ci * ci;
+        }
+        const double dr = k[0] * u[0].x + k[1] * u[1].x + k[2] * u[2].x;
+        const double di = k[0] * u[0].y + k[1] * u[1].y + k[2] * u[2].y;
+        d2 += dr * dr + di * di;
+    }
+    sw[threadIdx.x] = w2;
+    sd[threadIdx.x] = d2;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sw[threadIdx.x] += sw[threadIdx.x + s];
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sw[0];
+        part[2 * blockIdx.x + 1] = sd[0];
+    }
+}
+
+// Gaussian-vortex vorticity w_z on x_i = 2 pi i / N (exec/ns.cpp:84-112 set 0,
+// exec/ns2.cpp:84-112 set 1); the x and y components are zero
+__device__ __forceinline__ double gauss2(double x, double y, double xc, double yc, double width) {
+    return exp(-((x - xc) * (x - xc)) / width) * exp(-((y - yc) * (y - yc)) / width);
+}
+
+__global__ void k_vortex_vorticity(double2* __restrict__ w, Dims d, int set) {
+    const long N = d.n();
+    const double width = M_PI / 8, mag = 50, dist = M_PI / 2;
+    GRID_STRIDE(n, N) {
+        const int i = (int)(n % d.nx);
+        const int j = (int)((n / d.nx) % d.ny);
+        const double x = (2. * M_PI / d.nx) * i, y = (2. * M_PI / d.ny) * j;
+        double wz;
+        if (set == 0)
+            wz = mag * gauss2(x, y, M_PI - dist, M_PI - dist, width) +
+                 mag * gauss2(x, y, M_PI + dist, M_PI - dist, width) +
+                 mag * gauss2(x, y, M_PI, M_PI + dist, width);
+        else
+            wz = mag * gauss2(x, y, M_PI - dist, M_PI, width) + mag * gauss2(x, y, M_PI + dist, M_PI, width);
+        w[n] = mk<double>(0.0, 0.0);
+        w[N + n] = mk<double>(0.0, 0.0);
+        w[2 * N + n] = mk<double>(wz, 0.0);
+    }
+}
+
+// psi = -w_z/(kx^2+ky^2) where (kx, ky) != 0, else 0; V0 = (-i ky psi, i kx psi, 0)
+__global__ void k_vortex_velocity(const double2* __restrict__ wz, double2* __restrict__ v, Dims d) {
+    const long N = d.n();
+    GRID_STRIDE(n, N) {
+        double k0, k1, k2;
+        wavenumber(n, d, k0, k1, k2);
+        double2 psi = mk<double>(0.0, 0.0);
+        if (k0 != 0 || k1 != 0) psi = rmul<double>(-1. / (k0 * k0 + k1 * k1), wz[n]);
+        const double2 gy = imul<double>(k1, psi);
+        v[n] = mk<double>(-1. * gy.x, -1. * gy.y);
+        v[N + n] = imul<double>(k0, psi);
+        v[2 * N + n] = mk<double>(0.0, 0.0);
+    }
+}
+
 static int check_dims(const Dims& d, const char* who) {
     if (d.nx <= 0 || d.ny <= 0 || d.nz <= 0) {
         set_error(std::string(who) + ": invalid dimensions");
@@ -596,6 +674,60 @@ int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek,
     for (int b = 0; b < nbins; b++) Ek[b] = h[b];
     if (energy) *energy = h[nbins];
     if (dissipation) *dissipation = 2.0 * nu * h[nbins + 1];
+    return SH_OK;
+}
+
+int sh_vorticity_stats_z(const void* v3, int nx, int ny, int nz, double* enstrophy, double* divnorm,
+                         void* stream) {
+    Dims d{nx, ny, nz};
+    SH_REQUIRE(v3, "sh_vorticity_stats_z");
+    int rc = check_dims(d, "sh_vorticity_stats_z");
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    unsigned g = grid_for(d.n());
+    if (g > (unsigned)kRedBlocks) g = kRedBlocks;
+    double* part = nullptr;
+    rc = pool_alloc((void**)&part, sizeof(double) * 2 * g);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vort_div_partial, dim3(g), dim3(kBlock), 0, st, (const double2*)v3, d, part);
+    std::vector<double> h(2 * g);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * g, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    pool_free(part);
+    if (e != hipSuccess) return hip_fail(e, "sh_vorticity_stats_z");
+    // fixed-order host sum of the block partials: the result does not depend on scheduling
+    double w2 = 0, d2 = 0;
+    for (unsigned b = 0; b < g; b++) {
+        w2 += h[2 * b];
+        d2 += h[2 * b + 1];
+    }
+    const double nn = (double)d.n() * (double)d.n();
+    if (enstrophy) *enstrophy = 0.5 * w2 / nn;
+    if (divnorm) *divnorm = sqrt(d2 / nn);
+    return SH_OK;
+}
+
+int sh_vortex_ic_z(void* V0, void* omega0, int nx, int ny, int nz, int set, void* stream) {
+    Dims d{nx, ny, nz};
+    SH_REQUIRE(V0 && omega0, "sh_vortex_ic_z");
+    if (set != 0 && set != 1) {
+        set_error("sh_vortex_ic_z: set must be 0 (exec/ns.cpp) or 1 (exec/ns2.cpp)");
+        return SH_EINVAL;
+    }
+    int rc = check_dims(d, "sh_vortex_ic_z");
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    const long N = d.n();
+    double2* w = (double2*)omega0;
+    hipLaunchKernelGGL(k_vortex_vorticity, dim3(grid_for(N)), dim3(kBlock), 0, st, w, d, set);
+    SH_LAUNCH_CHECK("k_vortex_vorticity");
+    // omega0.forward(): the x and y components are zero and stay zero
+    rc = fft3d<double>(w + 2 * N, 1, N, nx, ny, nz, -1, SH_ZERO_NYQUIST, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vortex_velocity, dim3(grid_for(N)), dim3(kBlock), 0, st, (const double2*)(w + 2 * N),
+                       (double2*)V0, d);
+    SH_LAUNCH_CHECK("k_vortex_velocity");
     return SH_OK;
 }
 

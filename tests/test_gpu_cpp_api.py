@@ -66,30 +66,68 @@ def test_ns_driver_tg16_golden(tmp_path, mode):
     assert rel_l2(_read_v(out, 16), g["step100"]) < 1e-10
 
 
-def test_ns_driver_vortex_ic_vs_oracle(tmp_path):
-    """The reference's own IC (three Gaussian vortices, exec/ns.cpp:97-131),
-    nu = 1e-3, K = 1e-3, 20 steps: API mode against the oracle."""
+@pytest.mark.parametrize("mode", ["api", "fused"])
+def test_ns_driver_vortex_ic_vs_oracle(tmp_path, mode):
+    """The reference's own IC (three Gaussian vortices, exec/ns.cpp:97-131,
+    omega0 kept from the IC), nu = 1e-3, K = 1e-3, 20 steps, against the oracle."""
     n = 32
     out = tmp_path / "v.bin"
-    rc, txt = run(["ns", n, 20, "vortex", 0.001, 0.001, "api", out])
+    rc, txt = run(["ns", n, 20, "vortex", 0.001, 0.001, mode, out])
     assert rc == 0, txt
-    # the same IC on the host
-    x = 2 * np.pi * np.arange(n) / n
-    X, Y = x[None, None, :], x[None, :, None]
-    g = lambda xc, yc: np.exp(-(X - xc) ** 2 / (np.pi / 8)) * np.exp(-(Y - yc) ** 2 / (np.pi / 8))
-    d = np.pi / 2
-    wz = 50 * (g(np.pi - d, np.pi - d) + g(np.pi + d, np.pi - d) + g(np.pi, np.pi + d))
-    wz = O.forward(np.broadcast_to(wz, (n, n, n)).astype(np.complex128))
-    kx = np.array([O.freq(i, n) for i in range(n)])
-    k2 = kx[None, None, :] ** 2 + kx[None, :, None] ** 2
-    psi = np.where(k2 != 0, -1.0 / np.where(k2 != 0, k2, 1) * wz, 0)
-    grad = O.gradient(np.ascontiguousarray(psi))
-    V0 = np.stack([-1.0 * grad[1], grad[0], np.zeros_like(psi)])
+    V0, W0 = O.vortices((n, n, n), 0)
     ref = O.NS((n, n, n), 0.001, 0.001)
     ref.set_initial(V0)
+    ref.set_vorticity(W0)
     ref.startup()
     ref.steps(20)
     assert rel_l2(_read_v(out, n), ref.velocity()) < 1e-10
+
+
+def _read_particles(path):
+    with open(path) as f:
+        assert f.readline().split() == ["x", "y", "z", "E"]
+        return np.loadtxt(f)
+
+
+def test_ns2_driver_vs_oracle_with_particles(tmp_path):
+    """exec/ns2.cpp at 32^3 (two vortices, backward Euler, 1000 particles) for
+    40 steps: both modes against the oracle's velocity; the particle files of
+    the two modes agree and match the oracle's tracers moved through the
+    oracle's fields (printed with 6 significant digits, as the reference)."""
+    n, steps, npart = 32, 40, 1000
+    outs = {}
+    for mode in ("api", "fused"):
+        d = tmp_path / mode
+        d.mkdir()
+        out = tmp_path / ("v_%s.bin" % mode)
+        rc, txt = run(["ns2", n, steps, 0.001, 0.001, mode, npart, d, out])
+        assert rc == 0, txt
+        outs[mode] = (_read_v(out, n), d)
+    V0, W0 = O.vortices((n, n, n), 1)
+    ref = O.NS((n, n, n), 0.001, 0.001)
+    ref.set_scheme(1)
+    ref.set_initial(V0)
+    ref.set_vorticity(W0)
+    ref.startup()
+    loc = O.particles_seed(npart, 1)
+    files = []
+    for t in range(1, steps + 1):
+        ref.steps(1)
+        U = np.stack([O.inverse(c) for c in ref.velocity()])
+        _, en = O.particles_move(loc, U, 0.001)
+        if (t - 1) % max(1, steps // 300) == 0:
+            files.append(np.column_stack([loc, en]))
+    for mode in ("api", "fused"):
+        assert rel_l2(outs[mode][0], ref.velocity()) < 1e-10, mode
+    assert len(files) == steps
+    for i in (0, steps // 2, steps - 1):
+        a = _read_particles(outs["api"][1] / ("p%d.3D" % i))
+        b = _read_particles(outs["fused"][1] / ("p%d.3D" % i))
+        assert a.shape == (npart, 4)
+        # the two modes' fields differ at rounding level; 6 printed digits
+        # can then differ in the last place only
+        assert np.allclose(a, b, rtol=1.1e-5, atol=1e-12), i
+        assert np.allclose(a, files[i], rtol=1.1e-5, atol=1e-12), i
 
 
 def test_ns_driver_api_matches_fused(tmp_path):
