@@ -30,8 +30,12 @@ NU, DT, SEED, K0 = 1.0 / 800.0, 2e-3, 2024, 4.0
 # HBM bytes per launch at 256^3 measured with rocprofv3 PMC (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, factors from a 1 GiB device copy); see
 # profiles/pmc_traffic_ns256.md.  Measured for the kernel variants in this tree.
-MEASURED_TRAFFIC_256 = {"ns_cross_z": 2.416e9, "ns_fourier_x": 5.638e9, "ns_inv_x": 2.416e9,
-                        "ns_y_inv": 3.221e9, "ns_y_fwd": 1.611e9}
+# Keyed by the solver's storage: "full" spectrum or the "half" (kz < nz/2) layout.
+MEASURED_TRAFFIC_256 = {
+    "full": {"ns_cross_z": 2.416e9, "ns_fourier_x": 5.638e9, "ns_inv_x": 2.416e9,
+             "ns_y_inv": 3.221e9, "ns_y_fwd": 1.611e9},
+    "half": {},
+}
 PASSES = {
     "ns_inv_x": 3 + 6,        # read V, write 6 x-transformed fields
     "ns_y_inv": 6 + 6,        # y pass of 6 fields
@@ -81,6 +85,8 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT):
     ns.set_initial(V0)
     del V0
     ns.startup()
+    half = ctypes.c_int(0)
+    L.sh_ns_get_layout(ns.h, ctypes.byref(half))
     if TUNE:
         L.sh_ns_autotune(ns.h, 3)
     ns.steps(warmup)
@@ -105,7 +111,7 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT):
     del V
     ns.close()
     torch.cuda.empty_cache()
-    return t1 - t0, kern, {"div_avg": div, "energy": e.value}
+    return t1 - t0, kern, {"div_avg": div, "energy": e.value, "half": bool(half.value)}
 
 
 def fft_bench(S, n, reps):
@@ -194,7 +200,10 @@ def main():
         state["stable_dt"] = {"dt": a.stable_dt, "steps_per_s": round(a.steps / dts, 3),
                               "final_energy": sts["energy"], "final_div_avg": sts["div_avg"]}
         log("bench: dt=%g: %.2f steps/s, final energy %r" % (a.stable_dt, a.steps / dts, sts["energy"]))
-    pts = float(n) ** 3
+    # grid points each kernel touches per field: the half-spectrum layout
+    # stores the planes kz < nz/2 only
+    layout = "half" if st["half"] else "full"
+    pts = float(n) ** 3 * (0.5 if st["half"] else 1.0)
     kernels = {}
     dom, dom_t = None, -1.0
     for name, (ms, cnt) in kern.items():
@@ -212,13 +221,13 @@ def main():
         ach = k["GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4),
-                "traffic": MEASURED_TRAFFIC_256.get(dom) if n == 256 else None,
+                "traffic": MEASURED_TRAFFIC_256[layout].get(dom) if n == 256 else None,
                 "algo_bytes_per_launch": PASSES[dom] * 16.0 * pts}
     step_bytes = sum(PASSES[k] for k, (ms, cnt) in kern.items() if cnt > 0) * 16.0 * pts
     extra = {"step_algo_GB_fused": round(step_bytes / 1e9, 3),
              "step_GBps_fused": round(step_bytes * value / 1e9, 1),
-             "step_contract_GB": round(1056.0 * pts / 1e9, 3),
-             "step_GBps_contract": round(1056.0 * pts * value / 1e9, 1)}
+             "step_contract_GB": round(1056.0 * n ** 3 / 1e9, 3),
+             "step_GBps_contract": round(1056.0 * n ** 3 * value / 1e9, 1)}
     fp32 = None
     if a.fp32_steps > 0:
         log("bench: FP32 form, %d steps" % a.fp32_steps)
@@ -267,7 +276,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ns_fractional_step_fp64_%d^3" % n, "grid": n, "nu": NU, "dt": DT,
                    "ic": "random solenoidal, E(k)~k^4 exp(-2(k/4)^2), u_rms=1, seed %d" % SEED,
-                   "schedule": "fused (4 kernels/step)"},
+                   "schedule": "fused (4 kernels/step)",
+                   "layout": "half spectrum (kz < nz/2; real fields)" if st["half"] else "full spectrum"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "chosen_form_fp32_steps_per_s": fp32,

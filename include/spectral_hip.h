@@ -202,6 +202,19 @@ int sh_ns_div_average(sh_ns_t handle, double* result);
 int sh_ns_set_scheme(sh_ns_t handle, int scheme);
 /* Stokes limit switch (nonlinear term off), for closed-form checks */
 int sh_ns_set_nonlinear(sh_ns_t handle, int on);
+/* Storage of the fused schedule.  1 (default): when the start-up state (V1,
+ * cross0) is the transform of a real field -- Hermitian, f(-k) = conj f(k),
+ * to 1e-12 relative, as every real initial velocity gives -- only the planes
+ * kz < nz/2 are stepped (kz = nz/2 is zeroed by forward() anyway) and the z
+ * transforms run as half-length complex ones on packed real data; otherwise
+ * the full spectrum.  0: always the full complex spectrum, the reference's
+ * C2C arithmetic (src/Scalar.cpp:400-646) operation for operation.  Both give
+ * the same velocity to rounding on real fields; the half layout moves half
+ * the bytes.  A body force is then read on the planes kz < nz/2 only (the
+ * transform of a real force).  Takes effect at the next start-up. */
+int sh_ns_set_layout(sh_ns_t handle, int mode);
+/* 1 if the solver currently steps the half-spectrum layout, else 0 */
+int sh_ns_get_layout(sh_ns_t handle, int* half);
 /* device bytes held by the solver state */
 int sh_ns_state_bytes(sh_ns_t handle, size_t* bytes);
 
@@ -262,7 +275,8 @@ enum {
     SH_ROLE_Y = 1,          /* y lines (strided pass; also the solver's y passes) */
     SH_ROLE_Z = 2,          /* z lines (strided pass) */
     SH_ROLE_NS_CROSS = 3,   /* the solver's fused z / cross-product kernel */
-    SH_ROLE_NS_FOURIER = 4  /* the solver's fused x / Fourier-update kernel */
+    SH_ROLE_NS_FOURIER = 4, /* the solver's fused x / Fourier-update kernel */
+    SH_ROLE_NS_CROSS_H = 5  /* the fused z kernel of the real-field layout (n = nz) */
 };
 /* time the x, y, z schedules for an nx x ny x nz transform.  Synchronous. */
 int sh_autotune_fft3d(int nx, int ny, int nz, int precision, int reps, void* stream);

@@ -375,6 +375,8 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
     h->clip = clip != 0;
     h->st = as_stream(stream);
     int rc = sh_ns_create(&h->ns, n, n, n, nu, K, SH_F32, 1, stream);
+    // the closure reads the solver's full-spectrum velocity in place
+    if (!rc) rc = sh_ns_set_layout(h->ns, 0);
     void* p;
     const long sizes[4] = {3, 5, 3, 3};
     float2** dst[4] = {&h->F, &h->Z, &h->T, &h->Vc};

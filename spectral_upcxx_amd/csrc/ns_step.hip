@@ -58,6 +58,8 @@ __host__ __device__ __forceinline__ T expfilter(T k, int n) {
     return T(exp(-36.0 * (r16 * r16) * r4));  // r^32 r^4 = r^36
 }
 
+__device__ __forceinline__ double fmax_d(double a, double b) { return a > b ? a : b; }
+
 template <int N>
 struct LdsLine {
     static constexpr int LS = N + N / 16 + 1;
@@ -73,7 +75,7 @@ template <typename T, class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB * 6) void k_ns_inv_x(const C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ W,
                                                               const C2<T>* __restrict__ tw,
-                                                              int ny, int nz, long F, long boff) {
+                                                              int ny, int nz, int nzp, long F, long boff) {
     V += blockIdx.y * boff;
     W += blockIdx.y * boff;
     constexpr int N = P::N;
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(P::TPL* LPB * 6) void k_ns_inv_x(const C2<T>* __res
     const int ll = threadIdx.x / P::TPL;  // 0 .. 6*LPB-1
     const int g = ll / LPB;               // field group: 0-2 velocity, 3-5 vorticity
     const int l = ll % LPB;
-    const long nlines = (long)ny * nz;
+    const long nlines = (long)ny * nzp;  // nzp = nz, or nz/2 in the half-spectrum layout
     const long line = (long)blockIdx.x * LPB + l;
     const bool valid = line < nlines;
     const long lc = valid ? line : 0;
@@ -191,6 +193,181 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
     }
 }
 
+// C in the half-spectrum layout (real physical fields).  A z-line holds
+// X[k], k = 0..M-1, M = nz/2: X[M] is the Nyquist plane forward() zeroes and
+// X[nz-k] = conj X[k].  The length-nz real transforms run as length-M complex
+// ones (P is a length-M plan) on packed pairs of grid points:
+//   inverse: Z[k] = (X[k] + conj X[M-k]) + i e^{2 pi i k/nz} (X[k] - conj X[M-k]);
+//            the length-M inverse of Z is y[2m] + i y[2m+1];
+//   forward: Z = the length-M forward of c[2m] + i c[2m+1];
+//            X[k] = ((Z[k] + conj Z[M-k]) - i e^{-2 pi i k/nz} (Z[k] - conj Z[M-k])) / 2.
+// The cross product acts on the real and the imaginary parts separately (two
+// grid points); the imaginary parts the reference's complex product carries
+// are the rounding noise of real fields and are not formed.
+template <typename T, class P, int LPB>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__ W,
+                                                             const C2<T>* __restrict__ tw,
+                                                             const C2<T>* __restrict__ twn, int nx,
+                                                             int ny, long F, T scale, long boff) {
+    static_assert(P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1),
+                  "fused z kernel needs a symmetric plan");
+    W += blockIdx.y * boff;
+    constexpr int M = P::N;
+    constexpr int CS = LPB + 1;
+    constexpr int E = P::E;
+    __shared__ T sre[M * CS];
+    __shared__ T sim[M * CS];
+    const int c = threadIdx.x % LPB;
+    const int tid = threadIdx.x / LPB;
+    const int tiles = (nx + LPB - 1) / LPB;
+    const int j = blockIdx.x / tiles;
+    const int i = (blockIdx.x % tiles) * LPB + c;
+    const bool valid = i < nx && j < ny;
+    const unsigned es = (unsigned)nx * (unsigned)ny;
+    const unsigned uF = (unsigned)F;
+    const unsigned cb = valid ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
+    auto idx = [&](int pos) { return pos * CS + c; };
+    auto load = [&](int f, C2<T>* x) {
+        for_in<P>(tid, [&](int s, int pos) { x[s] = W[cb + (unsigned)f * uF + (unsigned)pos * es]; });
+    };
+    // X -> Z (X[M-k] is held by another thread: one LDS round trip), the
+    // length-M inverse, and the 1/(Nx Ny Nz) scale
+    auto inv = [&](C2<T>* x) {
+        __syncthreads();
+        for_in<P>(tid, [&](int s, int pos) {
+            sre[idx(pos)] = x[s].x;
+            sim[idx(pos)] = x[s].y;
+        });
+        __syncthreads();
+        for_in<P>(tid, [&](int s, int pos) {
+            const C2<T> a = x[s];
+            const C2<T> b = pos == 0 ? mk<T>(T(0), T(0)) : mk<T>(sre[idx(M - pos)], sim[idx(M - pos)]);
+            const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // X[k] + conj X[M-k]
+            const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // X[k] - conj X[M-k]
+            const C2<T> w = twn[pos];                       // e^{-2 pi i k/nz}
+            const T tx = w.x * dif.x + w.y * dif.y;         // t = conj(w) dif
+            const T ty = w.x * dif.y - w.y * dif.x;
+            x[s] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
+        });
+        __syncthreads();
+        line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+        for (int s = 0; s < E; s++) x[s] = rmul<T>(scale, x[s]);
+    };
+    // elementwise product of two packed pairs of real values
+    auto pmul = [](C2<T> a, C2<T> b) { return mk<T>(a.x * b.x, a.y * b.y); };
+    // r0 = u1 w2 - u2 w1, r1 = u2 w0 - u0 w2, r2 = u0 w1 - u1 w0 (src/Vector.cpp:79-84),
+    // fields in the order w0 w1 u2 w2 u0 u1 (at most 6 live), as kernel C
+    C2<T> w[3][E], o[3][E], x[E], pf[E];
+    load(3, w[0]);
+    load(4, w[1]);
+    inv(w[0]);
+    load(2, pf);
+    inv(w[1]);
+    load(5, w[2]);
+    inv(pf);  // u2
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+        o[0][s] = csub<T>(mk<T>(T(0), T(0)), pmul(pf[s], w[1][s]));
+        o[1][s] = pmul(pf[s], w[0][s]);
+    }
+    load(0, x);
+    inv(w[2]);
+    load(1, pf);
+    inv(x);  // u0
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+        o[1][s] = csub<T>(o[1][s], pmul(x[s], w[2][s]));
+        o[2][s] = pmul(x[s], w[1][s]);
+    }
+    inv(pf);  // u1
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+        o[0][s] = cadd<T>(o[0][s], pmul(pf[s], w[2][s]));
+        o[2][s] = csub<T>(o[2][s], pmul(pf[s], w[0][s]));
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int s = 0; s < E; s++) x[s] = o[d][s];
+        __syncthreads();
+        line_fft<P, -1, T>(x, tid, sre, sim, idx, tw);
+        // Z -> X (Z[M-k] from another thread through LDS)
+        __syncthreads();
+        for_out<P>(tid, [&](int s, int pos) {
+            sre[idx(pos)] = x[s].x;
+            sim[idx(pos)] = x[s].y;
+        });
+        __syncthreads();
+        if (valid)
+            for_out<P>(tid, [&](int s, int pos) {
+                const C2<T> a = x[s];
+                const int mp = pos == 0 ? 0 : M - pos;
+                const C2<T> b = mk<T>(sre[idx(mp)], sim[idx(mp)]);
+                const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // Z[k] + conj Z[M-k]
+                const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // Z[k] - conj Z[M-k]
+                const C2<T> w = twn[pos];
+                const T tx = w.x * dif.x - w.y * dif.y;  // t = w dif
+                const T ty = w.x * dif.y + w.y * dif.x;
+                W[cb + (unsigned)d * uF + (unsigned)pos * es] = mk<T>(T(0.5) * (sum.x + ty), T(0.5) * (sum.y - tx));
+            });
+    }
+}
+
+// expand a half-spectrum field to the full one in place: planes kz > nz/2 from
+// f(-k) = conj f(k), the Nyquist plane kz = nz/2 zeroed (count fields at fstride)
+template <typename T>
+__global__ void k_expand_half(C2<T>* __restrict__ f, int count, long fstride, int nx, int ny, int nz) {
+    const int h = nz / 2;
+    const long plane = (long)nx * ny;
+    const long nup = plane * (nz - h);  // planes h .. nz-1
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nup * count; e += (long)gridDim.x * blockDim.x) {
+        const int fld = (int)(e / nup);
+        const long r = e % nup;
+        const int k = h + (int)(r / plane);
+        const int j = (int)((r % plane) / nx), i = (int)(r % nx);
+        C2<T>* b = f + fld * fstride;
+        C2<T> v = mk<T>(T(0), T(0));
+        if (k != h) {
+            const C2<T> m = b[(long)((nx - i) % nx) + (long)nx * ((ny - j) % ny) + plane * (nz - k)];
+            v = mk<T>(m.x, -m.y);
+        }
+        b[(long)i + (long)nx * j + plane * k] = v;
+    }
+}
+
+// per-block maxima of |f(k) - conj f(-k)| and |f(k)| over count fields (FP64)
+__global__ __launch_bounds__(256) void k_herm_dev(const double2* __restrict__ f, int count, long fstride, int nx,
+                                                  int ny, int nz, double* __restrict__ part) {
+    __shared__ double sd[256], sm[256];
+    const long plane = (long)nx * ny, N = plane * nz;
+    double dmax = 0, fmax = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < N * count; e += (long)gridDim.x * blockDim.x) {
+        const int fld = (int)(e / N);
+        const long r = e % N;
+        const int k = (int)(r / plane), j = (int)((r % plane) / nx), i = (int)(r % nx);
+        const double2* b = f + fld * fstride;
+        const double2 a = b[r];
+        const double2 m = b[(long)((nx - i) % nx) + (long)nx * ((ny - j) % ny) + plane * ((nz - k) % nz)];
+        dmax = fmax_d(dmax, hypot(a.x - m.x, a.y + m.y));
+        fmax = fmax_d(fmax, hypot(a.x, a.y));
+    }
+    sd[threadIdx.x] = dmax;
+    sm[threadIdx.x] = fmax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sd[threadIdx.x] = fmax_d(sd[threadIdx.x], sd[threadIdx.x + s]);
+            sm[threadIdx.x] = fmax_d(sm[threadIdx.x], sm[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sd[0];
+        part[2 * blockIdx.x + 1] = sm[0];
+    }
+}
+
 // E: x-pass forward of the products (-> N1, Nyquist zeroed) and the
 // Fourier-space update of exec/ns.cpp:171-187; writes V1 <- V, N0 <- N1 and
 // per-block sums of |div V| (exec/ns.cpp:209-210).  With EMIT it also runs
@@ -205,7 +382,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
                                                               C2<T>* __restrict__ N0,
                                                               const C2<T>* __restrict__ frc,
                                                               const C2<T>* __restrict__ tw, int ny,
-                                                              int nz, long F,
+                                                              int nz, int nzp, long F,
                                                               const StepCoef<T>* __restrict__ cfs,
                                                               double* __restrict__ divpart,
                                                               long boff, int divstride) {
@@ -226,13 +403,16 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     __shared__ T sim[LPB * LS];
     const int tid = threadIdx.x % P::TPL;
     const int l = threadIdx.x / P::TPL;
-    const long nlines = (long)ny * nz;
+    const long nlines = (long)ny * nzp;
     const long line = (long)blockIdx.x * LPB + l;
     const bool valid = line < nlines;
     const long lc = valid ? line : 0;
     const int jj = (int)(lc % ny), kk = (int)(lc / ny);
     const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
     const bool zline = (jj == ny / 2) || (kk == nz / 2);
+    // half-spectrum layout: a stored plane kz > 0 also stands for the plane -kz,
+    // whose |div V| values are the same (f(-k) = conj f(k))
+    const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
     const long off = lc * N;
     auto idx = [&](int pos) { return l * LS + pos + (pos >> 4); };
     auto load = [&](int f, C2<T>* x) { for_in<P>(tid, [&](int s, int pos) { x[s] = W[f * F + off + pos]; }); };
@@ -315,7 +495,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
             dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
             dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
-            dsum += hypot((double)dv.x, (double)dv.y);
+            dsum += dw * hypot((double)dv.x, (double)dv.y);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 V[d * F + off + pos] = vn[d][s];
@@ -519,6 +699,9 @@ struct sh_ns_s {
     int cur_box;          // box whose start-up is running
     int dealias;          // dealiasing of the nonlinear term (0: none, as the reference)
     int scheme;           // 0: exec/ns.cpp (CN viscous), 1: exec/ns2.cpp (backward Euler viscous)
+    int layout;           // 1: half-spectrum layout when the state is Hermitian; 0: always full
+    bool half;            // the fused state holds only the planes kz < nz/2
+    std::vector<char> box_herm;  // per box: its last start-up state was Hermitian
     std::vector<void*> owned;
 };
 
@@ -664,11 +847,12 @@ int launch_inv_x(sh_ns_s* s) {
     constexpr int LPB = (512 / (6 * P::TPL)) >= 8 ? 8 : (512 / (6 * P::TPL)) >= 4 ? 4 : (512 / (6 * P::TPL)) >= 2 ? 2 : 1;
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
-    const long nlines = (long)s->ny * s->nz;
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long nlines = (long)s->ny * nzp;
     dim3 grid((unsigned)((nlines + LPB - 1) / LPB), (unsigned)s->nboxes), block(P::TPL * LPB * 6);
     ProfScope ps("ns_inv_x", s->st);
     hipLaunchKernelGGL((k_ns_inv_x<T, P, LPB>), grid, block, 0, s->st, (const C2<T>*)s->V1f,
-                       (C2<T>*)s->Wf, tw, s->ny, s->nz, s->N * s->nboxes, s->N);
+                       (C2<T>*)s->Wf, tw, s->ny, s->nz, nzp, s->N * s->nboxes, s->N);
     SH_LAUNCH_CHECK("k_ns_inv_x");
     return SH_OK;
 }
@@ -691,13 +875,42 @@ int launch_cross_z(sh_ns_s* s) {
     return SH_OK;
 }
 
+// kernel C of the half-spectrum layout: P is the length-nz/2 plan
+template <typename T, class P, int LPB>
+int launch_cross_zh(sh_ns_s* s) {
+    const int prec = sizeof(T) == 8 ? SH_F64 : SH_F32;
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, prec);
+    const C2<T>* twn = (const C2<T>*)twiddles(s->nz, prec);
+    if (!tw || !twn) {
+        set_error("ns: twiddle table unavailable (first use inside a graph capture?)");
+        return SH_ENOMEM;
+    }
+    if (2 * P::N != s->nz) {
+        set_error("ns: half-spectrum z kernel of the wrong length");
+        return SH_EINVAL;
+    }
+    if (6 * s->N * s->nboxes > 0xffffffffL) {  // 32-bit element offsets, as kernel C
+        set_error("ns: grid too large for the fused z kernel");
+        return SH_EINVAL;
+    }
+    const int tiles = (s->nx + LPB - 1) / LPB;
+    dim3 grid((unsigned)(tiles * s->ny), (unsigned)s->nboxes), block(P::TPL * LPB);
+    const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
+    ProfScope ps("ns_cross_z", s->st);
+    hipLaunchKernelGGL((k_ns_cross_zh<T, P, LPB>), grid, block, 0, s->st, (C2<T>*)s->Wf, tw, twn, s->nx,
+                       s->ny, s->N * s->nboxes, scale, s->N);
+    SH_LAUNCH_CHECK("k_ns_cross_zh");
+    return SH_OK;
+}
+
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
 template <typename T, class P, int LPB, bool EMIT = true>
 int launch_fourier_x(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
-    const long nlines = (long)s->ny * s->nz;
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long nlines = (long)s->ny * nzp;
     const unsigned nb = (unsigned)((nlines + LPB - 1) / LPB);
     if ((int)nb > s->divblocks) {
         set_error("ns: divergence partial buffer too small");
@@ -706,7 +919,7 @@ int launch_fourier_x(sh_ns_s* s) {
     ProfScope ps("ns_fourier_x", s->st);
     hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
-                       (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, s->N * s->nboxes,
+                       (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, nzp, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
@@ -739,6 +952,18 @@ int launch_fourier_x(sh_ns_s* s) {
     X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4)                               \
     X(256, Q256, 4) X(512, P512, 4) X(512, W512, 2)
 
+// kernel C of the half-spectrum layout: X(nz, length-nz/2 plan, columns per block)
+#define SH_CROSSH_F64_VARIANTS(X)                                                                 \
+    X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
+    X(64, Q32, 32) X(96, Q48, 16)                                                                 \
+    X(128, W64, 16) X(128, P64, 16) X(128, W64, 8) X(256, W128, 16) X(256, W128, 8)               \
+    X(256, Q128, 16) X(256, Q128, 8) X(512, W256, 8) X(512, W256, 4) X(512, Q256, 8) X(512, Q256, 4)
+#define SH_CROSSH_F32_VARIANTS(X)                                                                 \
+    X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
+    X(64, Q32, 32) X(96, Q48, 16)                                                                 \
+    X(128, P64, 16) X(128, W64, 16) X(128, P64, 8) X(256, Q128, 16) X(256, W128, 16)              \
+    X(256, Q128, 8) X(512, Q256, 16) X(512, Q256, 8) X(512, W256, 16) X(512, W256, 8)
+
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
 #define SH_FOURIER_F64_NOEMIT_VARIANTS(X) X(32, Q32, 32) X(64, W64, 16) X(64, W64, 8) X(96, Q96, 10)
@@ -769,6 +994,32 @@ const NsVariants& cross_variants(int n) {
     static const NsVariants none;
     auto it = tab.find(n);
     return it == tab.end() ? none : it->second;
+}
+
+template <typename T>
+const NsVariants& crossh_variants(int n) {
+    static const std::map<int, NsVariants> tab = [] {
+        std::map<int, NsVariants> m;
+#define SH_V(n_, P_, C_)                                \
+    m[n_].names.push_back(#P_ " half columns=" #C_);    \
+    m[n_].fns.push_back(&launch_cross_zh<T, P_, C_>);
+        if constexpr (sizeof(T) == 8) {
+            SH_CROSSH_F64_VARIANTS(SH_V)
+        } else {
+            SH_CROSSH_F32_VARIANTS(SH_V)
+        }
+#undef SH_V
+        return m;
+    }();
+    static const NsVariants none;
+    auto it = tab.find(n);
+    return it == tab.end() ? none : it->second;
+}
+
+// the half-spectrum layout needs an even nz whose half has a fused plan
+bool half_supported(int prec, int nz) {
+    return nz % 2 == 0 && (prec == SH_F64 ? !crossh_variants<double>(nz).fns.empty()
+                                          : !crossh_variants<float>(nz).fns.empty());
 }
 
 template <typename T>
@@ -821,6 +1072,7 @@ int dispatch_inv_x(sh_ns_s* s) {
 }
 template <typename T>
 int dispatch_cross_z(sh_ns_s* s) {
+    if (s->half) return run_variant<T>(crossh_variants<T>(s->nz), s, s->nz, ROLE_CROSS_H);
     return run_variant<T>(cross_variants<T>(s->nz), s, s->nz, ROLE_CROSS);
 }
 
@@ -838,6 +1090,7 @@ int step_fused_front(sh_ns_s* s) {
         return SH_EINVAL;
     }
     const long sxy = (long)s->nx * s->ny;
+    const int nzp = s->half ? s->nz / 2 : s->nz;  // planes the y passes cover
     Epi<T> plain{T(1), 0, 1, -1, -1};
     // kernel A runs once after a (re)start; afterwards kernel E emits its output
     if (!s->w_primed) {
@@ -846,13 +1099,13 @@ int step_fused_front(sh_ns_s* s) {
     }
     {
         ProfScope ps("ns_y_inv", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 6 * s->nboxes, s->N, +1,
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, nzp, sxy, s->nx, 6 * s->nboxes, s->N, +1,
                            plain, s->st, ROLE_Y));
     }
     RC(dispatch_cross_z<T>(s));
     {
         ProfScope ps("ns_y_fwd", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, s->nz, sxy, s->nx, 3 * s->nboxes, s->N, -1,
+        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, nzp, sxy, s->nx, 3 * s->nboxes, s->N, -1,
                            plain, s->st, ROLE_Y));
     }
     return SH_OK;
@@ -892,6 +1145,9 @@ namespace sh {
 int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx);
 
 int ns_variant_count(int precision, int n, int role) {
+    if (role == ROLE_CROSS_H)
+        return precision == SH_F64 ? (int)crossh_variants<double>(n).fns.size()
+                                   : (int)crossh_variants<float>(n).fns.size();
     if (role == ROLE_CROSS)
         return precision == SH_F64 ? (int)cross_variants<double>(n).fns.size()
                                    : (int)cross_variants<float>(n).fns.size();
@@ -900,9 +1156,10 @@ int ns_variant_count(int precision, int n, int role) {
 }
 
 const char* ns_variant_name(int precision, int n, int role, int v) {
-    const NsVariants& vl = role == ROLE_CROSS
-                               ? (precision == SH_F64 ? cross_variants<double>(n) : cross_variants<float>(n))
-                               : (precision == SH_F64 ? fourier_variants<double>(n) : fourier_variants<float>(n));
+    const NsVariants& vl =
+        role == ROLE_CROSS_H ? (precision == SH_F64 ? crossh_variants<double>(n) : crossh_variants<float>(n))
+        : role == ROLE_CROSS ? (precision == SH_F64 ? cross_variants<double>(n) : cross_variants<float>(n))
+                             : (precision == SH_F64 ? fourier_variants<double>(n) : fourier_variants<float>(n));
     return v >= 0 && v < (int)vl.names.size() ? vl.names[v] : "default";
 }
 }  // namespace sh
@@ -938,7 +1195,7 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     SH_HIP(hipEventCreate(&e0));
     SH_HIP(hipEventCreate(&e1));
     int rc = SH_OK;
-    const int roles[3] = {ROLE_Y, ROLE_CROSS, ROLE_FOURIER};
+    const int roles[3] = {ROLE_Y, s->half ? ROLE_CROSS_H : ROLE_CROSS, ROLE_FOURIER};
     const int lens[3] = {s->ny, s->nz, s->nx};
     for (int k = 0; k < 3 && !rc; k++) {
         const int role = roles[k], n = lens[k];
@@ -992,6 +1249,74 @@ int upload_coefs_t(sh_ns_s* s) {
 }
 
 int upload_coefs(sh_ns_s* s) { return s->prec == SH_F64 ? upload_coefs_t<double>(s) : upload_coefs_t<float>(s); }
+
+// complete `count` half-spectrum fields (stride N) to the full spectrum in place
+template <typename T>
+int expand_half(sh_ns_s* s, void* f, int count) {
+    const long nup = (long)s->nx * s->ny * (s->nz - s->nz / 2) * count;
+    hipLaunchKernelGGL(k_expand_half<T>, dim3(egrid(nup)), dim3(256), 0, s->st, (C2<T>*)f, count, s->N, s->nx,
+                       s->ny, s->nz);
+    SH_LAUNCH_CHECK("k_expand_half");
+    return SH_OK;
+}
+
+int expand_state(sh_ns_s* s) {
+    const int nf = 3 * s->nboxes;
+    if (s->prec == SH_F64) {
+        RC(expand_half<double>(s, s->V1f, nf));
+        return expand_half<double>(s, s->N0f, nf);
+    }
+    RC(expand_half<float>(s, s->V1f, nf));
+    return expand_half<float>(s, s->N0f, nf);
+}
+
+// largest |f(k) - conj f(-k)| and |f(k)| over 3 FP64 fields
+int herm_dev(sh_ns_s* s, const double2* f, double* dev, double* mag) {
+    constexpr int g = 1024;
+    double* part = nullptr;
+    RC(pool_alloc((void**)&part, sizeof(double) * 2 * g));
+    hipLaunchKernelGGL(k_herm_dev, dim3(g), dim3(256), 0, s->st, f, 3, s->N, s->nx, s->ny, s->nz, part);
+    std::vector<double> h(2 * g);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * g, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    pool_free(part);
+    if (e != hipSuccess) return hip_fail(e, "herm_dev");
+    double d = 0, m = 0;
+    for (int b = 0; b < g; b++) {
+        if (!(h[2 * b] <= d)) d = h[2 * b];  // NaN propagates
+        if (!(h[2 * b + 1] <= m)) m = h[2 * b + 1];
+    }
+    *dev = d;
+    *mag = m;
+    return SH_OK;
+}
+
+// Choose the storage of the fused state after box b's start-up: the half
+// layout needs the layout switch on, a supported nz, and every box's start
+// (V1, cross0) Hermitian -- V1 to 1e-12 of its size, cross0 to 1e-12 of its
+// size or with K*|deviation| below 1e-12 of V1's size (its step contribution).
+int decide_layout(sh_ns_s* s, int b) {
+    if (!s->fused) return SH_OK;
+    bool herm = false;
+    if (s->layout == 1 && half_supported(s->prec, s->nz)) {
+        double dv, fv, dc, fc;
+        RC(herm_dev(s, s->V1, &dv, &fv));
+        RC(herm_dev(s, s->cr0, &dc, &fc));
+        const double tol = 1e-12;
+        herm = dv <= tol * fv && (dc <= tol * fc || s->K * dc <= tol * fv);
+    }
+    s->box_herm[b] = herm;
+    bool all = true;
+    for (char h : s->box_herm) all = all && h;
+    if (s->half && !all) RC(expand_state(s));
+    s->half = all;
+    s->w_primed = false;
+    // the half-layout z kernel's twiddle tables, built now so that a step
+    // captured into a graph allocates nothing
+    if (s->half && (!twiddles(s->nz / 2, s->prec) || !twiddles(s->nz, s->prec))) return SH_ENOMEM;
+    return SH_OK;
+}
 
 // copy the FP64 start-up result (V1, cross0) into box b of the fused state
 int install_box(sh_ns_s* s, int b) {
@@ -1059,6 +1384,9 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
     s->cur_box = 0;
     s->dealias = 0;
     s->scheme = 0;
+    s->layout = 1;
+    s->half = false;
+    s->box_herm.assign(nboxes, 1);
     const size_t z = sizeof(double2);
     int rc = 0;
     // FP64 state used by the start-up (and by the op-by-op loop)
@@ -1128,6 +1456,7 @@ int sh_ns_set_initial_box(sh_ns_t s, int box, const void* V0) {
     s->nu = nu0;
     s->cur_box = 0;
     if (rc) return rc;
+    RC(decide_layout(s, box));
     s->w_primed = false;
     return install_box(s, box);
 }
@@ -1146,6 +1475,7 @@ int sh_ns_get_velocity_box(sh_ns_t s, int box, void* out) {
             SH_LAUNCH_CHECK("k_c_to_z");
         }
     }
+    if (s->half) return expand_half<double>(s, out, 3);
     return SH_OK;
 }
 
@@ -1161,6 +1491,7 @@ int snapshot_t(sh_ns_s* s, int r, float* out) {
     C2<T>* snap = (C2<T>*)s->snap;
     const long nf = 3L * s->nboxes;
     SH_HIP(hipMemcpyAsync(snap, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
+    if (s->half) RC(expand_half<T>(s, snap, (int)nf));
     RC(fft3d<T>(snap, (int)nf, N, s->nx, s->ny, s->nz, +1, SH_NORMALIZE, s->st));
     unsigned gx = egrid(nc);
     hipLaunchKernelGGL(k_box_filter<T>, dim3(gx, (unsigned)s->nboxes), dim3(256), 0, s->st, snap, N,
@@ -1187,6 +1518,7 @@ int snapshot_force_t(sh_ns_s* s, int r, float* out) {
     C2<T>* u = (C2<T>*)s->snap;
     C2<T>* w = u + nf * N;
     SH_HIP(hipMemcpyAsync(u, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
+    if (s->half) RC(expand_half<T>(s, u, (int)nf));
     hipLaunchKernelGGL(k_curl_boxes<T>, dim3(egrid(N), (unsigned)s->nboxes), dim3(256), 0, s->st, u, w, N,
                        N * s->nboxes, s->nx, s->ny, s->nz);
     SH_LAUNCH_CHECK("k_curl_boxes");
@@ -1300,8 +1632,9 @@ int sh_ns_startup(sh_ns_t s) {
     }
     s->w_primed = false;
     RC(startup64(s));
-    if (s->fused) return install_box(s, 0);
-    return SH_OK;
+    if (!s->fused) return SH_OK;
+    RC(decide_layout(s, 0));
+    return install_box(s, 0);
 }
 
 int sh_ns_steps(sh_ns_t s, int nsteps) {
@@ -1328,6 +1661,27 @@ int sh_ns_get_velocity(sh_ns_t s, void* out) {
         hipLaunchKernelGGL(k_c_to_z, dim3(egrid(n3)), dim3(256), 0, s->st, (const float2*)s->V1f, (double2*)out, n3);
         SH_LAUNCH_CHECK("k_c_to_z");
     }
+    if (s->fused && s->half) return expand_half<double>(s, out, 3);
+    return SH_OK;
+}
+
+int sh_ns_set_layout(sh_ns_t s, int mode) {
+    if (!s || mode < 0 || mode > 1) {
+        set_error("sh_ns_set_layout: mode must be 0 (full spectrum) or 1 (half when Hermitian)");
+        return SH_EINVAL;
+    }
+    s->layout = mode;
+    if (mode == 0 && s->half) {  // leave the half layout at once
+        RC(expand_state(s));
+        s->half = false;
+        s->w_primed = false;
+    }
+    return SH_OK;
+}
+
+int sh_ns_get_layout(sh_ns_t s, int* half) {
+    if (!s || !half) return SH_EINVAL;
+    *half = s->half ? 1 : 0;
     return SH_OK;
 }
 

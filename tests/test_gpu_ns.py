@@ -195,3 +195,94 @@ def test_full_size_256_invariants(hip):
     ns.close()
     del V
     torch.cuda.empty_cache()
+
+
+def _layout(hip, ns):
+    import ctypes
+    h = ctypes.c_int(-1)
+    hip.lib().sh_ns_get_layout(ns.h, ctypes.byref(h))
+    return h.value
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_half_spectrum_layout_matches_full(hip, prec):
+    """A real initial velocity engages the half-spectrum layout; it gives the
+    full-spectrum (C2C) fused step's velocity to rounding, and the oracle's
+    within the FP64 bar."""
+    import torch
+    dims = (32, 32, 32)
+    V0 = torch.empty((3, 32, 32, 32), dtype=torch.complex128, device="cuda")
+    hip.lib().sh_solenoidal_ic_z(hip.ptr(V0), 32, 32, 32, 4242, 4.0, 1.0, hip.stream())
+    out = {}
+    for layout in (0, 1):
+        ns = hip.NS(dims, 1 / 800, 2e-3, hip.SH_F64 if prec == 64 else hip.SH_F32, fused=True)
+        hip.lib().sh_ns_set_layout(ns.h, layout)
+        ns.set_initial(V0)
+        ns.startup()
+        assert _layout(hip, ns) == layout
+        ns.steps(40)
+        out[layout] = (_host(ns.velocity()), ns.div_average())
+        ns.close()
+    assert rel_l2(out[1][0], out[0][0]) < (1e-12 if prec == 64 else 1e-5)
+    # the expanded velocity is Hermitian and its Nyquist plane is zero (the
+    # kz = 0 plane stores k and -k separately: Hermitian to the precision's rounding)
+    v = out[1][0]
+    assert np.abs(v[:, 16]).max() == 0
+    mirror = np.conj(np.roll(v[:, ::-1, ::-1, ::-1], 1, axis=(1, 2, 3)))
+    assert rel_l2(v, mirror) < (1e-14 if prec == 64 else 1e-6)
+    if prec == 64:
+        ref = O.NS(dims, 1 / 800, 2e-3)
+        ref.set_initial(_host(V0))
+        ref.startup()
+        ref.steps(40)
+        assert rel_l2(out[1][0], ref.velocity()) < TOL_STEP
+        # |div V| averages of the two layouts agree (both rounding noise)
+        assert out[1][1] < 1e-12 and out[0][1] < 1e-12
+
+
+def test_non_hermitian_start_keeps_full_layout(hip):
+    """A complex velocity that is not the transform of a real field stays on
+    the full spectrum and matches the op-by-op schedule."""
+    dims = (16, 16, 16)
+    V = O.prng_complex(31, (16, 16, 48)).reshape(3, 16, 16, 16)
+    # divergence-free projection, Nyquist planes zeroed (a legal solver state)
+    kx = np.array([O.freq(i, 16) for i in range(16)])
+    K3 = np.stack(np.broadcast_arrays(kx[None, None, :], kx[None, :, None], kx[:, None, None]))
+    k2 = (K3 ** 2).sum(0)
+    V = V - K3 * ((K3 * V).sum(0) / np.where(k2 == 0, 1, k2))
+    V = np.stack([O.zero_high(c) for c in V])
+    out = []
+    for fused in (False, True):
+        ns = hip.NS(dims, 0.01, 1e-3, hip.SH_F64, fused=fused)
+        ns.set_initial(_dev(V))
+        ns.startup()
+        if fused:
+            assert _layout(hip, ns) == 0
+        ns.steps(20)
+        out.append(_host(ns.velocity()))
+        ns.close()
+    assert rel_l2(out[1], out[0]) < 1e-12
+
+
+def test_layout_switch_mid_run(hip):
+    """Leaving the half layout mid-run (sh_ns_set_layout 0) expands the state
+    in place and continues on the same trajectory."""
+    import torch
+    dims = (32, 32, 32)
+    V0 = torch.empty((3, 32, 32, 32), dtype=torch.complex128, device="cuda")
+    hip.lib().sh_solenoidal_ic_z(hip.ptr(V0), 32, 32, 32, 77, 3.0, 1.0, hip.stream())
+    a = hip.NS(dims, 1 / 400, 2e-3, hip.SH_F64, fused=True)
+    a.set_initial(V0)
+    a.startup()
+    assert _layout(hip, a) == 1
+    a.steps(10)
+    hip.lib().sh_ns_set_layout(a.h, 0)
+    assert _layout(hip, a) == 0
+    a.steps(10)
+    b = hip.NS(dims, 1 / 400, 2e-3, hip.SH_F64, fused=False)
+    b.set_initial(V0)
+    b.startup()
+    b.steps(20)
+    assert rel_l2(_host(a.velocity()), _host(b.velocity())) < 1e-12
+    a.close()
+    b.close()
