@@ -34,7 +34,8 @@ NU, DT, SEED, K0 = 1.0 / 800.0, 2e-3, 2024, 4.0
 MEASURED_TRAFFIC_256 = {
     "full": {"ns_cross_z": 2.416e9, "ns_fourier_x": 5.638e9, "ns_inv_x": 2.416e9,
              "ns_y_inv": 3.221e9, "ns_y_fwd": 1.611e9},
-    "half": {},
+    "half": {"ns_cross_z": 1.208e9, "ns_fourier_x": 2.820e9, "ns_inv_x": 1.208e9,
+             "ns_y_inv": 1.611e9, "ns_y_fwd": 0.805e9},
 }
 PASSES = {
     "ns_inv_x": 3 + 6,        # read V, write 6 x-transformed fields
