@@ -966,9 +966,10 @@ int launch_fourier_x(sh_ns_s* s) {
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
-#define SH_FOURIER_F64_NOEMIT_VARIANTS(X) X(32, Q32, 32) X(64, W64, 16) X(64, W64, 8) X(96, Q96, 10)
+#define SH_FOURIER_F64_NOEMIT_VARIANTS(X) \
+    X(32, Q32, 32) X(64, W64, 16) X(64, W64, 8) X(96, Q96, 10) X(256, W256, 2) X(256, W256, 4) X(256, Q256, 8)
 #define SH_FOURIER_F32_NOEMIT_VARIANTS(X) \
-    X(32, Q32, 32) X(64, P64, 32) X(64, W64, 16) X(64, P64, 8) X(64, W64, 4) X(96, Q96, 10)
+    X(32, Q32, 32) X(64, P64, 32) X(64, W64, 16) X(64, P64, 8) X(64, W64, 4) X(96, Q96, 10) X(256, Q256, 8)
 
 using NsLaunch = int (*)(sh_ns_s*);
 struct NsVariants {

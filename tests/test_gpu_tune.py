@@ -11,7 +11,10 @@ from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
 ROLES_FFT = (0, 1, 2)  # x, y, z
-ROLE_CROSS, ROLE_FOURIER = 3, 4
+ROLE_CROSS, ROLE_FOURIER, ROLE_CROSS_H = 3, 4, 5
+# (role, storage layout the solver must use for the role to run): the full-
+# spectrum z kernel runs with layout 0, the half-spectrum one with layout 1
+NS_ROLES = ((1, 1), (ROLE_CROSS_H, 1), (ROLE_FOURIER, 1), (1, 0), (ROLE_CROSS, 0), (ROLE_FOURIER, 0))
 
 
 def _dev(a):
@@ -97,18 +100,20 @@ def test_ns_variants_vs_oracle(hip, n, prec):
     Vr = ref.velocity()
     tol = 1e-10 if prec == 64 else 1e-5
     try:
-        for role in (1, ROLE_CROSS, ROLE_FOURIER):
+        for role, layout in NS_ROLES:
             for v in range(_count(L, prec, n, role)):
                 L.sh_tune_reset()
                 L.sh_tune_set_variant(prec, n, role, v)
                 ns = hip.NS(dims, 0.01, 1e-3, prec, fused=True)
+                L.sh_ns_set_layout(ns.h, layout)
                 ns.set_initial(_dev(V0))
                 ns.startup()
                 ns.steps(8)
                 err = rel_l2(_host(ns.velocity()), Vr)
                 ns.close()
-                print("fp%d n=%d role=%d variant=%d err=%.2e" % (prec, n, role, v, err), flush=True)
-                assert err < tol, (role, v, err)
+                print("fp%d n=%d role=%d layout=%d variant=%d err=%.2e" % (prec, n, role, layout, v, err),
+                      flush=True)
+                assert err < tol, (role, layout, v, err)
     finally:
         L.sh_tune_reset()
 
@@ -122,8 +127,9 @@ def test_ns_variants_256_agree(hip):
     V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
     L.sh_solenoidal_ic_z(hip.ptr(V0), n, n, n, 9, 4.0, 1.0, hip.stream())
 
-    def run():
+    def run(layout):
         ns = hip.NS((n, n, n), 1 / 800, 5e-4, 64, fused=True)
+        L.sh_ns_set_layout(ns.h, layout)
         ns.set_initial(V0)
         ns.startup()
         ns.steps(3)
@@ -131,20 +137,23 @@ def test_ns_variants_256_agree(hip):
         ns.close()
         return V
 
-    ref = run()
+    refs = {0: run(0), 1: run(1)}
+    # the two layouts agree to rounding
+    assert float(torch.linalg.vector_norm(refs[1] - refs[0]) / torch.linalg.vector_norm(refs[0])) < 1e-13
     try:
-        for role in (1, ROLE_CROSS, ROLE_FOURIER):
+        for role, layout in NS_ROLES:
+            ref = refs[layout]
             for v in range(1, _count(L, 64, n, role)):
                 L.sh_tune_reset()
                 L.sh_tune_set_variant(64, n, role, v)
-                V = run()
+                V = run(layout)
                 err = float(torch.linalg.vector_norm(V - ref) / torch.linalg.vector_norm(ref))
-                print("n=256 role=%d variant=%d err=%.2e" % (role, v, err), flush=True)
-                assert err < 1e-13, (role, v, err)
+                print("n=256 role=%d layout=%d variant=%d err=%.2e" % (role, layout, v, err), flush=True)
+                assert err < 1e-13, (role, layout, v, err)
                 del V
     finally:
         L.sh_tune_reset()
-        del V0, ref
+        del V0, refs
         torch.cuda.empty_cache()
 
 
