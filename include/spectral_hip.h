@@ -215,6 +215,13 @@ int sh_ns_set_nonlinear(sh_ns_t handle, int on);
 int sh_ns_set_layout(sh_ns_t handle, int mode);
 /* 1 if the solver currently steps the half-spectrum layout, else 0 */
 int sh_ns_get_layout(sh_ns_t handle, int* half);
+/* Fused schedule only.  1: the y passes and the fused z kernel of a step run
+ * as two independent halves of the x columns on the solver's stream and a
+ * second (non-blocking, default-priority) stream of its own, joined before
+ * the Fourier update, so that one half's transforms overlap the other's z
+ * kernel; 0 (default): one launch each.  The same arithmetic either way.
+ * sh_ns_autotune times both and keeps the faster. */
+int sh_ns_set_overlap(sh_ns_t handle, int on);
 /* device bytes held by the solver state */
 int sh_ns_state_bytes(sh_ns_t handle, size_t* bytes);
 

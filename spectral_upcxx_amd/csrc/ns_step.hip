@@ -117,9 +117,10 @@ __global__ __launch_bounds__(P::TPL* LPB * 6) void k_ns_inv_x(const C2<T>* __res
 }
 
 // C: z-pass inverse of W[0..6), u x omega, z-pass forward into W[0..3).
+// (W may be offset to a first column; ncx columns of rows nx long are processed)
 template <typename T, class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ W,
-                                                            const C2<T>* __restrict__ tw, int nx,
+                                                            const C2<T>* __restrict__ tw, int nx, int ncx,
                                                             int ny, long F, T scale, long boff) {
     static_assert(P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1),
                   "fused z kernel needs a symmetric plan");
@@ -130,10 +131,10 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
     __shared__ T sim[P::N * CS];
     const int c = threadIdx.x % LPB;
     const int tid = threadIdx.x / LPB;
-    const int tiles = (nx + LPB - 1) / LPB;
+    const int tiles = (ncx + LPB - 1) / LPB;
     const int j = blockIdx.x / tiles;
     const int i = (blockIdx.x % tiles) * LPB + c;
-    const bool valid = i < nx && j < ny;
+    const bool valid = i < ncx && j < ny;
     // 32-bit element offsets from the uniform base W (6 fields of at most
     // 2^31/6 elements; checked by the launcher) halve the address registers
     const unsigned es = (unsigned)nx * (unsigned)ny;
@@ -208,7 +209,7 @@ template <typename T, class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ tw,
                                                              const C2<T>* __restrict__ twn, int nx,
-                                                             int ny, long F, T scale, long boff) {
+                                                             int ncx, int ny, long F, T scale, long boff) {
     static_assert(P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1),
                   "fused z kernel needs a symmetric plan");
     W += blockIdx.y * boff;
@@ -219,10 +220,10 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__
     __shared__ T sim[M * CS];
     const int c = threadIdx.x % LPB;
     const int tid = threadIdx.x / LPB;
-    const int tiles = (nx + LPB - 1) / LPB;
+    const int tiles = (ncx + LPB - 1) / LPB;
     const int j = blockIdx.x / tiles;
     const int i = (blockIdx.x % tiles) * LPB + c;
-    const bool valid = i < nx && j < ny;
+    const bool valid = i < ncx && j < ny;
     const unsigned es = (unsigned)nx * (unsigned)ny;
     const unsigned uF = (unsigned)F;
     const unsigned cb = valid ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
@@ -700,6 +701,11 @@ struct sh_ns_s {
     int dealias;          // dealiasing of the nonlinear term (0: none, as the reference)
     int scheme;           // 0: exec/ns.cpp (CN viscous), 1: exec/ns2.cpp (backward Euler viscous)
     int layout;           // 1: half-spectrum layout when the state is Hermitian; 0: always full
+    int overlap;          // 1: kernels B-D run as two column halves on two streams
+    hipStream_t st2;      // second stream of the overlapped schedule (default priority)
+    hipEvent_t ev_fork, ev_join;
+    hipStream_t cst;      // stream and column range of the B-D launch being issued
+    int cx0, cnx;
     bool half;            // the fused state holds only the planes kz < nz/2
     std::vector<char> box_herm;  // per box: its last start-up state was Hermitian
     std::vector<void*> owned;
@@ -865,12 +871,12 @@ int launch_cross_z(sh_ns_s* s) {
         set_error("ns: grid too large for the fused z kernel");
         return SH_EINVAL;
     }
-    const int tiles = (s->nx + LPB - 1) / LPB;
+    const int tiles = (s->cnx + LPB - 1) / LPB;
     dim3 grid((unsigned)(tiles * s->ny), (unsigned)s->nboxes), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
-    ProfScope ps("ns_cross_z", s->st);
-    hipLaunchKernelGGL((k_ns_cross_z<T, P, LPB>), grid, block, 0, s->st, (C2<T>*)s->Wf, tw, s->nx,
-                       s->ny, s->N * s->nboxes, scale, s->N);
+    ProfScope ps("ns_cross_z", s->cst);
+    hipLaunchKernelGGL((k_ns_cross_z<T, P, LPB>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0, tw, s->nx,
+                       s->cnx, s->ny, s->N * s->nboxes, scale, s->N);
     SH_LAUNCH_CHECK("k_ns_cross_z");
     return SH_OK;
 }
@@ -893,12 +899,12 @@ int launch_cross_zh(sh_ns_s* s) {
         set_error("ns: grid too large for the fused z kernel");
         return SH_EINVAL;
     }
-    const int tiles = (s->nx + LPB - 1) / LPB;
+    const int tiles = (s->cnx + LPB - 1) / LPB;
     dim3 grid((unsigned)(tiles * s->ny), (unsigned)s->nboxes), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
-    ProfScope ps("ns_cross_z", s->st);
-    hipLaunchKernelGGL((k_ns_cross_zh<T, P, LPB>), grid, block, 0, s->st, (C2<T>*)s->Wf, tw, twn, s->nx,
-                       s->ny, s->N * s->nboxes, scale, s->N);
+    ProfScope ps("ns_cross_z", s->cst);
+    hipLaunchKernelGGL((k_ns_cross_zh<T, P, LPB>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0, tw, twn,
+                       s->nx, s->cnx, s->ny, s->N * s->nboxes, scale, s->N);
     SH_LAUNCH_CHECK("k_ns_cross_zh");
     return SH_OK;
 }
@@ -1082,33 +1088,52 @@ int dispatch_fourier_x(sh_ns_s* s) {
     return run_variant<T>(fourier_variants<T>(s->nx), s, s->nx, ROLE_FOURIER);
 }
 
+// kernels B, C, D over the columns [x0, x0 + ncx) on stream st
+template <typename T>
+int step_bcd(sh_ns_s* s, hipStream_t st, int x0, int ncx) {
+    const long sxy = (long)s->nx * s->ny;
+    const int nzp = s->half ? s->nz / 2 : s->nz;  // planes the y passes cover
+    Epi<T> plain{T(1), 0, 1, -1, -1};
+    C2<T>* w = (C2<T>*)s->Wf + x0;
+    s->cst = st;
+    s->cx0 = x0;
+    s->cnx = ncx;
+    {
+        ProfScope ps("ns_y_inv", st);
+        RC(pass_strided<T>(w, s->ny, ncx, 1, nzp, sxy, s->nx, 6 * s->nboxes, s->N, +1, plain, st, ROLE_Y));
+    }
+    RC(dispatch_cross_z<T>(s));
+    {
+        ProfScope ps("ns_y_fwd", st);
+        RC(pass_strided<T>(w, s->ny, ncx, 1, nzp, sxy, s->nx, 3 * s->nboxes, s->N, -1, plain, st, ROLE_Y));
+    }
+    return SH_OK;
+}
+
 // kernels A (after a restart), B, C, D of a step: everything before the
-// Fourier update, which is the only kernel that reads the body force
+// Fourier update, which is the only kernel that reads the body force.  With
+// overlap, B-D run as two independent column halves on two streams, so one
+// half's transform passes overlap the other half's fused z kernel; kernel E
+// waits for both.
 template <typename T>
 int step_fused_front(sh_ns_s* s) {
     if (!s->nonlinear) {
         set_error("ns: the fused schedule always includes the nonlinear term");
         return SH_EINVAL;
     }
-    const long sxy = (long)s->nx * s->ny;
-    const int nzp = s->half ? s->nz / 2 : s->nz;  // planes the y passes cover
-    Epi<T> plain{T(1), 0, 1, -1, -1};
     // kernel A runs once after a (re)start; afterwards kernel E emits its output
     if (!s->w_primed) {
         RC(dispatch_inv_x<T>(s));
         s->w_primed = true;
     }
-    {
-        ProfScope ps("ns_y_inv", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, nzp, sxy, s->nx, 6 * s->nboxes, s->N, +1,
-                           plain, s->st, ROLE_Y));
-    }
-    RC(dispatch_cross_z<T>(s));
-    {
-        ProfScope ps("ns_y_fwd", s->st);
-        RC(pass_strided<T>((C2<T>*)s->Wf, s->ny, s->nx, 1, nzp, sxy, s->nx, 3 * s->nboxes, s->N, -1,
-                           plain, s->st, ROLE_Y));
-    }
+    if (!s->overlap || s->nx < 2) return step_bcd<T>(s, s->st, 0, s->nx);
+    const int h = s->nx / 2;
+    SH_HIP(hipEventRecord(s->ev_fork, s->st));
+    SH_HIP(hipStreamWaitEvent(s->st2, s->ev_fork, 0));
+    RC(step_bcd<T>(s, s->st2, h, s->nx - h));
+    RC(step_bcd<T>(s, s->st, 0, h));
+    SH_HIP(hipEventRecord(s->ev_join, s->st2));
+    SH_HIP(hipStreamWaitEvent(s->st, s->ev_join, 0));
     return SH_OK;
 }
 
@@ -1214,6 +1239,15 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
             }
         }
         if (!rc) tune_set(s->prec, n, role, best, best_ms);
+    }
+    // then the serial or the overlapped (two column halves on two streams) B-D
+    if (!rc) {
+        double ms[2] = {INFINITY, INFINITY};
+        for (int ov = 0; ov < 2 && !rc; ov++) {
+            s->overlap = ov;
+            rc = time_steps(s, reps, e0, e1, &ms[ov], step_cb, ctx);
+        }
+        s->overlap = (!rc && ms[1] < ms[0]) ? 1 : 0;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1388,6 +1422,12 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
     s->layout = 1;
     s->half = false;
     s->box_herm.assign(nboxes, 1);
+    s->overlap = 0;
+    s->st2 = nullptr;
+    s->ev_fork = s->ev_join = nullptr;
+    s->cst = s->st;
+    s->cx0 = 0;
+    s->cnx = nx;
     const size_t z = sizeof(double2);
     int rc = 0;
     // FP64 state used by the start-up (and by the op-by-op loop)
@@ -1414,8 +1454,17 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
         if (!rc) rc = pool_alloc(&s->coefs, sizeof(StepCoef<double>) * nboxes);
         if (!rc) s->owned.push_back(s->coefs);
         if (!rc) rc = upload_coefs(s);
+        // the overlapped schedule's second stream: non-blocking, so it never
+        // waits on the legacy null stream implicitly; default priority
+        if (!rc && hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking) != hipSuccess) rc = SH_EHIP;
+        if (!rc && hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
+        if (!rc && hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
+        if (rc == SH_EHIP) set_error("sh_ns_create: stream or event creation failed");
     }
     if (rc) {
+        if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+        if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+        if (s->st2) (void)hipStreamDestroy(s->st2);
         for (void* q : s->owned) pool_free(q);
         delete s;
         return rc;
@@ -1578,6 +1627,12 @@ int sh_ns_div_average_box(sh_ns_t s, int box, double* result) {
 int sh_ns_destroy(sh_ns_t s) {
     if (!s) return SH_OK;
     (void)hipStreamSynchronize(s->st);
+    if (s->st2) {
+        (void)hipStreamSynchronize(s->st2);
+        (void)hipStreamDestroy(s->st2);
+    }
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     for (void* q : s->owned) pool_free(q);
     delete s;
     return SH_OK;
@@ -1677,6 +1732,19 @@ int sh_ns_set_layout(sh_ns_t s, int mode) {
         s->half = false;
         s->w_primed = false;
     }
+    return SH_OK;
+}
+
+int sh_ns_set_overlap(sh_ns_t s, int on) {
+    if (!s || on < 0 || on > 1) {
+        set_error("sh_ns_set_overlap: 0 or 1");
+        return SH_EINVAL;
+    }
+    if (on && !s->fused) {
+        set_error("sh_ns_set_overlap: only the fused schedule has an overlapped form");
+        return SH_EINVAL;
+    }
+    s->overlap = on;
     return SH_OK;
 }
 

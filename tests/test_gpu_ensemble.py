@@ -99,9 +99,11 @@ def test_snapshot_box_filter(hip, prec):
     hip.lib().sh_ns_destroy(h)
 
 
-def test_graph_replay_matches_eager(hip):
+@pytest.mark.parametrize("overlap", [0, 1])
+def test_graph_replay_matches_eager(hip, overlap):
     """Capture 5 steps + a snapshot; replay 3 times with eager work and
-    synchronisation in between; each replay equals 5 more eager steps."""
+    synchronisation in between; each replay equals 5 more eager steps.  With
+    overlap the capture spans the solver's second stream."""
     import torch
     L = hip.lib()
     n = 32
@@ -111,6 +113,8 @@ def test_graph_replay_matches_eager(hip):
     L.sh_stream_create(ctypes.byref(st))
     hg = _make_ensemble(hip, n, nus, 1e-3, 32, ics, st)
     he = _make_ensemble(hip, n, nus, 1e-3, 32, ics, st)
+    L.sh_ns_set_overlap(hg, overlap)
+    L.sh_ns_set_overlap(he, overlap)
     snap = torch.empty((2, 9, 512), dtype=torch.float32, device="cuda")
     L.sh_ns_snapshot(hg, 4, hip.ptr(snap))
     L.sh_ns_steps(hg, 1)  # twiddles and W ready before capture
