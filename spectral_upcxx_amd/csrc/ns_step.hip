@@ -377,7 +377,10 @@ __global__ __launch_bounds__(256) void k_herm_dev(const double2* __restrict__ f,
 // otherwise do (same slot->position map, so no reshuffle is needed).  The
 // products are read from W[0..3) before any line of W is rewritten, and a
 // block only rewrites its own lines.
-template <typename T, class P, int LPB, bool EMIT>
+// BL: the line's V and N0 stores are issued after the whole update, so no
+// slot's V1/N0 loads wait behind an earlier slot's stores (vector-memory
+// counters count loads and stores together)
+template <typename T, class P, int LPB, bool EMIT, bool BL = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
@@ -442,6 +445,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     for (int d = 0; d < 3; d++)
 #pragma unroll
         for (int s = 0; s < E; s++) vn[d][s] = mk<T>(T(0), T(0));
+    C2<T> nna[BL ? 3 : 1][BL ? E : 1];
     if (valid) {
         for_out<P>(tid, [&](int s, int pos) {
             const T k0 = freq<T>(pos, N);
@@ -499,10 +503,22 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             dsum += dw * hypot((double)dv.x, (double)dv.y);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                V[d * F + off + pos] = vn[d][s];
-                N0[d * F + off + pos] = nn[d];
+                if constexpr (BL) {
+                    nna[d][s] = nn[d];
+                } else {
+                    V[d * F + off + pos] = vn[d][s];
+                    N0[d * F + off + pos] = nn[d];
+                }
             }
         });
+        if constexpr (BL)
+            for_out<P>(tid, [&](int s, int pos) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    V[d * F + off + pos] = vn[d][s];
+                    N0[d * F + off + pos] = nna[d][s];
+                }
+            });
     }
     if constexpr (EMIT) {
         // next step's kernel A: inverse x of V (g = 0..2) and of i k x V (g = 3..5)
@@ -911,7 +927,7 @@ int launch_cross_zh(sh_ns_s* s) {
 
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
-template <typename T, class P, int LPB, bool EMIT = true>
+template <typename T, class P, int LPB, bool EMIT = true, bool BL = false>
 int launch_fourier_x(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
@@ -923,7 +939,7 @@ int launch_fourier_x(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT>), dim3(nb, (unsigned)s->nboxes),
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
                        (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, nzp, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
@@ -969,6 +985,10 @@ int launch_fourier_x(sh_ns_s* s) {
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
     X(128, P64, 16) X(128, W64, 16) X(128, P64, 8) X(256, Q128, 16) X(256, W128, 16)              \
     X(256, Q128, 8) X(512, Q256, 16) X(512, Q256, 8) X(512, W256, 16) X(512, W256, 8)
+
+// kernel E with the line's V and N0 stores after the whole update
+#define SH_FOURIER_F64_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, W256, 2) X(256, W256, 4) X(512, W512, 2)
+#define SH_FOURIER_F32_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
@@ -1036,18 +1056,24 @@ const NsVariants& fourier_variants(int n) {
 #define SH_V(n_, P_, L_)                         \
     m[n_].names.push_back(#P_ " lines=" #L_);    \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_>);
+#define SH_VB(n_, P_, L_)                                       \
+    m[n_].names.push_back(#P_ " lines=" #L_ " deferred stores"); \
+    m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, true, true>);
 #define SH_VN(n_, P_, L_)                                  \
     m[n_].names.push_back(#P_ " lines=" #L_ " no-emit");   \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, false>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
+            SH_FOURIER_F64_BATCHED_VARIANTS(SH_VB)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
+            SH_FOURIER_F32_BATCHED_VARIANTS(SH_VB)
         }
 #undef SH_V
 #undef SH_VN
+#undef SH_VB
         return m;
     }();
     static const NsVariants none;
