@@ -40,6 +40,8 @@ EXECS = {
     "test_3d_fft": "test/3d_fft.cpp",
     "test_freq": "test/freq.cpp",
     "test_cn_heat": "test/cn_heat.cpp",
+    "test_heat": "test/heat.cpp",
+    "test_constructor": "test/constructor.cpp",
 }
 
 LIB_HIP = os.path.join(LIB, "libspectral_hip.so")

@@ -43,6 +43,16 @@ def test_cn_heat_program():
     assert rc == 0 and "cn heat passed" in out, out
 
 
+def test_heat_program():
+    rc, out = run(["test_heat", 32])
+    assert rc == 0 and "heat passed" in out, out
+
+
+def test_constructor_program():
+    rc, out = run(["test_constructor"])
+    assert rc == 0 and "constructor passed" in out, out
+
+
 def test_fft_bench_program():
     rc, out = run(["fft", 64, 3])
     assert rc == 0 and "my man!" in out, out
