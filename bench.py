@@ -3,8 +3,10 @@
 Workload (the headline configuration): plain method, FP64, 256^3 on [0, 2 pi)^3,
 random solenoidal IC (E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1, seed 2024),
 nu = 1/800, dt = 2e-3.  One "step" is one iteration of exec/ns.cpp:167-217
-(fused kernel schedule, identical arithmetic).  Inputs are resident in HBM
-when the timed region starts.
+(fused kernel schedule; the start is real, so the solver steps the
+half-spectrum layout, the planes kz < nz/2 -- the same velocity to rounding).
+Inputs are resident in HBM when the timed region starts.  The same step on
+the full complex spectrum (the reference's C2C arithmetic) is reported beside.
 
 Prints ONE JSON line with the metric, a roofline object for the dominant
 kernel (algorithmic bytes / average launch time from HIP events recorded on
@@ -61,6 +63,7 @@ def parse():
     p.add_argument("--les-steps", type=int, default=200, help="0 skips the LES step-rate line")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fft", action="store_true")
+    p.add_argument("--no-full", action="store_true", help="skip the full-spectrum FP64 line")
     p.add_argument("--cpu-steps", type=int, default=2)
     p.add_argument("--stable-dt", type=float, default=5e-4)
     p.add_argument("--no-tune", action="store_true", help="use the default schedules")
@@ -77,12 +80,13 @@ def prof_total(L, name):
     return ms.value, cnt.value
 
 
-def run_ns(S, n, precision, warmup, steps, profile, dt=DT):
+def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
     import torch
     L = S.lib()
     V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
     L.sh_solenoidal_ic_z(S.ptr(V0), n, n, n, SEED, K0, 1.0, S.stream())
     ns = S.NS((n, n, n), NU, dt, precision, fused=True)
+    L.sh_ns_set_layout(ns.h, layout)
     ns.set_initial(V0)
     del V0
     ns.startup()
@@ -196,6 +200,12 @@ def main():
              "note": "the reference scheme (explicit AB2 advection, no dealiasing) is unstable at "
                      "256^3, dt=2e-3: the CPU oracle also overflows near step 65; per-step cost "
                      "does not depend on the values"}
+    full = None
+    if not a.no_full:
+        # the same step on the full complex spectrum (the reference's C2C arithmetic)
+        dtf, _, _ = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=False, layout=0)
+        full = round(a.steps / dtf, 3)
+        log("bench: full-spectrum layout %.2f steps/s" % full)
     if a.stable_dt > 0:
         dts, _, sts = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=False, dt=a.stable_dt)
         state["stable_dt"] = {"dt": a.stable_dt, "steps_per_s": round(a.steps / dts, 3),
@@ -281,6 +291,7 @@ def main():
                    "layout": "half spectrum (kz < nz/2; real fields)" if st["half"] else "full spectrum"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "full_spectrum_fp64_steps_per_s": full,
         "chosen_form_fp32_steps_per_s": fp32,
         "les_64": les,
         "fft": ffts,
