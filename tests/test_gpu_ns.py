@@ -339,3 +339,23 @@ def test_overlapped_schedule_is_identical(hip, prec, dims):
             out.append(_host(ns.velocity()))
             ns.close()
         assert np.array_equal(out[0], out[1]), layout
+
+
+@pytest.mark.parametrize("dims", [(24, 12, 8), (8, 8, 12), (12, 24, 24)])
+def test_small_grids_layout_choice(hip, dims):
+    """Lengths 8, 12 and 24: nz = 8 and 12 have no half-length fused plan, so
+    the solver keeps the full spectrum; nz = 24 steps the half layout (a
+    length-12 transform of packed pairs).  Both against the oracle."""
+    nx, ny, nz = dims
+    V0 = O.solenoidal(dims, 515, 2.0, 1.0)
+    ref = O.NS(dims, 0.01, 2e-3)
+    ref.set_initial(V0)
+    ref.startup()
+    ref.steps(15)
+    ns = hip.NS(dims, 0.01, 2e-3, hip.SH_F64, fused=True)
+    ns.set_initial(_dev(V0))
+    ns.startup()
+    assert _layout(hip, ns) == (1 if nz == 24 else 0)
+    ns.steps(15)
+    assert rel_l2(_host(ns.velocity()), ref.velocity()) < TOL_STEP
+    ns.close()
