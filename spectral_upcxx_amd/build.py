@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 API_SOURCES = ["Scalar.cpp", "Vector.cpp", "Operators.cpp", "FFT1D.cpp", "Particle.cpp"]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + CPP, "-I" + os.path.join(ROOT, "include")]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", "-I" + CPP, "-I" + os.path.join(ROOT, "include")]
 EXECS = {
     "ns": "exec/ns.cpp",
     "ns2": "exec/ns2.cpp",
@@ -42,6 +42,7 @@ EXECS = {
     "test_cn_heat": "test/cn_heat.cpp",
     "test_heat": "test/heat.cpp",
     "test_constructor": "test/constructor.cpp",
+    "test_particle": "test/particle.cpp",
 }
 
 LIB_HIP = os.path.join(LIB, "libspectral_hip.so")

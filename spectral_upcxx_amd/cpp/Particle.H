@@ -23,7 +23,9 @@ class Particle {
 public:
     /// next position of the process-wide sequence (seed 1, one tracer per call)
     Particle();
-    /// src/Particle.cpp:12-46 on the device
+    /// src/Particle.cpp:12-46 on the device.  Safe to call from several host
+    /// threads at once when the velocity's device copy is current (as after
+    /// Vector::inverse); a host-modified velocity must be uploaded first.
     void move(const Vector& velocity, double K);
     /// 0.5 |v|^2 of the last move (src/Particle.cpp:48-55)
     double energy();

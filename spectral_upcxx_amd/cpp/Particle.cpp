@@ -4,6 +4,7 @@
 // double arithmetic); the host keeps only copies for position()/energy().
 #include "Particle.H"
 
+#include <atomic>
 #include <cstdlib>
 #include <iostream>
 
@@ -24,15 +25,24 @@
 
 namespace {
 
-// process-wide tracer counter for Particle() (the reference's rand() stream)
-long g_next_tracer = 0;
+// process-wide tracer counter for Particle() (the reference's rand() stream);
+// atomic because drivers construct and move tracers from OpenMP threads
+// (exec/ns2.cpp:191-197)
+std::atomic<long> g_next_tracer{0};
 
-// device staging for single-tracer moves: position, velocity, energy
-double* g_one = nullptr;
+// device staging for single-tracer moves (position, velocity, energy), one
+// per host thread so concurrent moves do not share it
+struct Staging {
+    double* p = nullptr;
+    ~Staging() {
+        if (p) sh_free(p);
+    }
+};
+thread_local Staging t_one;
 
 double* staging() {
-    if (!g_one) CHECK(sh_malloc((void**)&g_one, 7 * sizeof(double)));
-    return g_one;
+    if (!t_one.p) CHECK(sh_malloc((void**)&t_one.p, 7 * sizeof(double)));
+    return t_one.p;
 }
 
 }  // namespace

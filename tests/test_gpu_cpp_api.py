@@ -53,6 +53,11 @@ def test_constructor_program():
     assert rc == 0 and "constructor passed" in out, out
 
 
+def test_particle_program():
+    rc, out = run(["test_particle"])
+    assert rc == 0 and "particle passed" in out, out
+
+
 def test_fft_bench_program():
     rc, out = run(["fft", 64, 3])
     assert rc == 0 and "my man!" in out, out
