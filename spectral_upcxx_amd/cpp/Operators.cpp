@@ -22,28 +22,28 @@
 
 Scalar operator*(const Scalar& a, const Scalar& b) {
     if (a.isPhysical() != b.isPhysical()) ERROR("space mismatch");
-    Scalar ret(a.getDims(), a.isPhysical());
+    Scalar ret(a.getDims(), a.isPhysical(), Scalar::NoFill());
     CHECK(sh_mul_z(ret.deviceDataMutable(), a.deviceData(), b.deviceData(), (long)a.localSize(), nullptr));
     return ret;
 }
 
 Scalar operator-(const Scalar& a, const Scalar& b) {
     if (a.isPhysical() != b.isPhysical()) ERROR("space mismatch");
-    Scalar ret(a.getDims(), a.isPhysical());
+    Scalar ret(a.getDims(), a.isPhysical(), Scalar::NoFill());
     CHECK(sh_sub_z(ret.deviceDataMutable(), a.deviceData(), b.deviceData(), (long)a.localSize(), nullptr));
     return ret;
 }
 
 Scalar operator+(const Scalar& a, const Scalar& b) {
     if (a.isPhysical() != b.isPhysical()) ERROR("space mismatch");
-    Scalar ret(a.getDims(), a.isPhysical());
+    Scalar ret(a.getDims(), a.isPhysical(), Scalar::NoFill());
     CHECK(sh_add_z(ret.deviceDataMutable(), a.deviceData(), b.deviceData(), (long)a.localSize(), nullptr));
     return ret;
 }
 
 Vector operator-(const Vector& a, const Vector& b) {
     if (a.isPhysical() != b.isPhysical()) ERROR("space mismatch");
-    Vector ret(a.getDims(), a.isPhysical());
+    Vector ret(a.getDims(), a.isPhysical(), Scalar::NoFill());
     for (int d = 0; d < DIM; d++)
         CHECK(sh_sub_z(ret[d].deviceDataMutable(), a[d].deviceData(), b[d].deviceData(),
                        (long)a[0].localSize(), nullptr));
@@ -52,7 +52,7 @@ Vector operator-(const Vector& a, const Vector& b) {
 
 Vector operator+(const Vector& a, const Vector& b) {
     if (a.isPhysical() != b.isPhysical()) ERROR("space mismatch");
-    Vector ret(a.getDims(), a.isPhysical());
+    Vector ret(a.getDims(), a.isPhysical(), Scalar::NoFill());
     for (int d = 0; d < DIM; d++)
         CHECK(sh_add_z(ret[d].deviceDataMutable(), a[d].deviceData(), b[d].deviceData(),
                        (long)a[0].localSize(), nullptr));
@@ -60,13 +60,13 @@ Vector operator+(const Vector& a, const Vector& b) {
 }
 
 Scalar operator*(double alpha, const Scalar& s) {
-    Scalar ret(s.getDims(), s.isPhysical());
+    Scalar ret(s.getDims(), s.isPhysical(), Scalar::NoFill());
     CHECK(sh_scale_z(ret.deviceDataMutable(), alpha, s.deviceData(), (long)s.localSize(), nullptr));
     return ret;
 }
 
 Vector operator*(double alpha, const Vector& v) {
-    Vector ret(v.getDims(), v.isPhysical());
+    Vector ret(v.getDims(), v.isPhysical(), Scalar::NoFill());
     for (int d = 0; d < DIM; d++)
         CHECK(sh_scale_z(ret[d].deviceDataMutable(), alpha, v[d].deviceData(), (long)v[0].localSize(),
                          nullptr));

@@ -24,6 +24,16 @@ public:
     Scalar(const Scalar& scalar);
     /// deep copy; keeps the target's space flag if it is initialised (src/Scalar.cpp:59-66)
     void operator=(const Scalar& scalar);
+    /// move from a temporary (the results of operations): takes its device
+    /// buffer instead of copying it
+    Scalar(Scalar&& scalar) noexcept;
+    /// assignment from a temporary: the same result as the copy (the target
+    /// keeps its space flag if it is initialised) without the device copy
+    void operator=(Scalar&& scalar);
+    /// tag: a field whose every element is written before it is read, so the
+    /// zero fill of a new Scalar is skipped (operation results)
+    struct NoFill {};
+    Scalar(std::array<INT, DIM> N, bool physical, NoFill);
     void init(std::array<INT, DIM> N, bool physical = true);
     void set(std::array<INT, DIM> index, cdouble val);
     cdouble get(std::array<INT, DIM> index) const;
@@ -77,6 +87,7 @@ public:
     cdouble* deviceDataMutable();
 
 private:
+    void init_impl(std::array<INT, DIM> N, bool physical, bool fill);
     void transform(int dir);
     void getIndex(std::array<INT, DIM> index, INT& rank, INT& loc) const;
     void host_current() const;

@@ -3,6 +3,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <utility>
 
 #include "DataObjects.H"
 #include "spectral_hip.h"
@@ -36,6 +37,55 @@ Scalar::Scalar() : m_Nz(0), m_Ny(0), m_N(0), m_dSize(0), m_dims{0, 0, 0}, m_pdim
 
 Scalar::Scalar(std::array<INT, DIM> N, bool physical) : Scalar() { init(N, physical); }
 
+Scalar::Scalar(std::array<INT, DIM> N, bool physical, NoFill) : Scalar() { init_impl(N, physical, false); }
+
+Scalar::Scalar(Scalar&& o) noexcept
+    : m_Nz(o.m_Nz), m_Ny(o.m_Ny), m_N(o.m_N), m_dSize(o.m_dSize), m_dims(o.m_dims), m_pdims(o.m_pdims),
+      m_dev(o.m_dev), m_host(std::move(o.m_host)), m_where(o.m_where), m_fft_x(o.m_fft_x), m_fft_y(o.m_fft_y),
+      m_fft_z(o.m_fft_z), m_fourier(o.m_fourier), m_initialized(o.m_initialized) {
+    o.m_dev = nullptr;
+    o.m_where = 0;
+    o.m_initialized = false;  // the count of live Scalars moves with the buffer
+}
+
+void Scalar::operator=(Scalar&& o) {
+    if (&o == this) return;
+    if (!o.m_initialized) {
+        *this = static_cast<const Scalar&>(o);
+        return;
+    }
+    if (!m_initialized) {
+        // as the copy: the target takes the source's dimensions and flag (and
+        // here its buffer; the count of live Scalars moves with it)
+        m_Nz = o.m_Nz;
+        m_Ny = o.m_Ny;
+        m_N = o.m_N;
+        m_dSize = o.m_dSize;
+        m_dims = o.m_dims;
+        m_pdims = o.m_pdims;
+        m_dev = o.m_dev;
+        m_host = std::move(o.m_host);
+        m_where = o.m_where;
+        m_fft_x = o.m_fft_x;
+        m_fft_y = o.m_fft_y;
+        m_fft_z = o.m_fft_z;
+        m_fourier = o.m_fourier;
+        m_initialized = true;
+        o.m_dev = nullptr;
+        o.m_where = 0;
+        o.m_initialized = false;
+        return;
+    }
+    if (o.m_dims != m_dims) {
+        ERROR("assignment between different dimensions");
+    }
+    // exchange the buffers (and their host mirrors); the target keeps its
+    // flag, the temporary frees the old buffer
+    std::swap(m_dev, o.m_dev);
+    std::swap(m_host, o.m_host);
+    std::swap(m_where, o.m_where);
+}
+
 Scalar::~Scalar() {
     if (m_initialized) {
         sh_free(m_dev);
@@ -46,20 +96,22 @@ Scalar::~Scalar() {
 }
 
 Scalar::Scalar(const Scalar& scalar) : Scalar() {
-    init(scalar.getDims(), scalar.isPhysical());
+    init_impl(scalar.getDims(), scalar.isPhysical(), false);
     CHECK(sh_memcpy_d2d(m_dev, scalar.deviceData(), m_dSize * sizeof(cdouble), nullptr));
     m_where = 0;
 }
 
 void Scalar::operator=(const Scalar& scalar) {
-    if (!m_initialized) init(scalar.getDims(), scalar.isPhysical());
+    if (!m_initialized) init_impl(scalar.getDims(), scalar.isPhysical(), false);
     if (scalar.getDims() != m_dims) ERROR("assignment between different dimensions");
     if (&scalar == this) return;
     CHECK(sh_memcpy_d2d(m_dev, scalar.deviceData(), m_dSize * sizeof(cdouble), nullptr));
     m_where = 0;
 }
 
-void Scalar::init(std::array<INT, DIM> N, bool physical) {
+void Scalar::init(std::array<INT, DIM> N, bool physical) { init_impl(N, physical, true); }
+
+void Scalar::init_impl(std::array<INT, DIM> N, bool physical, bool fill) {
     if (m_initialized) {
         sh_free(m_dev);
         m_nscalars--;
@@ -77,7 +129,7 @@ void Scalar::init(std::array<INT, DIM> N, bool physical) {
     void* p = nullptr;
     CHECK(sh_malloc(&p, m_dSize * sizeof(cdouble)));
     m_dev = static_cast<cdouble*>(p);
-    CHECK(sh_memset(m_dev, 0, m_dSize * sizeof(cdouble), nullptr));
+    if (fill) CHECK(sh_memset(m_dev, 0, m_dSize * sizeof(cdouble), nullptr));
     m_host.clear();
     m_where = 0;
     m_fft_x.init((int)m_dims[0], 1, m_dev);
@@ -199,7 +251,7 @@ void Scalar::inverse(Scalar& a_scalar) const {
 
 Vector Scalar::gradient() const {
     if (!isFourier()) ERROR("must begin gradient in fourier space");
-    Vector v(m_dims, false);
+    Vector v(m_dims, false, NoFill());
     CHECK(sh_gradient_z(deviceData(), v[0].deviceDataMutable(), v[1].deviceDataMutable(),
                         v[2].deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], nullptr));
     return v;
@@ -207,7 +259,7 @@ Vector Scalar::gradient() const {
 
 Scalar Scalar::laplacian() const {
     if (!isFourier()) ERROR("must begin laplacian in fourier space");
-    Scalar lap(m_dims, false);
+    Scalar lap(m_dims, false, NoFill());
     CHECK(sh_laplacian_z(deviceData(), lap.deviceDataMutable(), (int)m_dims[0], (int)m_dims[1],
                          (int)m_dims[2], nullptr));
     return lap;

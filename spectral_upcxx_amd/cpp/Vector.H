@@ -6,6 +6,8 @@ class Vector {
 public:
     Vector();
     Vector(std::array<INT, DIM> dims, bool physical = true);
+    /// components whose every element is written before it is read (no zero fill)
+    Vector(std::array<INT, DIM> dims, bool physical, Scalar::NoFill);
     void init(std::array<INT, DIM> dims, bool physical = true);
     Scalar& operator[](int a_i);
     const Scalar& operator[](int a_i) const;

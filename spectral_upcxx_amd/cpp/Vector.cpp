@@ -25,6 +25,12 @@ Vector::Vector() : m_dims{0, 0, 0}, m_N(0) {}
 
 Vector::Vector(std::array<INT, DIM> dims, bool physical) { init(dims, physical); }
 
+Vector::Vector(std::array<INT, DIM> dims, bool physical, Scalar::NoFill) : m_dims(dims) {
+    for (int i = 0; i < DIM; i++) m_vector[i] = Scalar(m_dims, physical, Scalar::NoFill());
+    m_N = 1;
+    for (int i = 0; i < DIM; i++) m_N *= m_dims[i];
+}
+
 void Vector::init(std::array<INT, DIM> dims, bool physical) {
     m_dims = dims;
     for (int i = 0; i < DIM; i++) m_vector[i].init(m_dims, physical);
@@ -65,7 +71,7 @@ Vector Vector::cross(const Vector& a_v) const {
     Vector v;
     inverse(u);
     a_v.inverse(v);
-    Vector ret(m_dims, true);
+    Vector ret(m_dims, true, Scalar::NoFill());
     CHECK(sh_cross_pointwise_z(u[0].deviceData(), u[1].deviceData(), u[2].deviceData(),
                                v[0].deviceData(), v[1].deviceData(), v[2].deviceData(),
                                ret[0].deviceDataMutable(), ret[1].deviceDataMutable(),
@@ -76,7 +82,7 @@ Vector Vector::cross(const Vector& a_v) const {
 
 Vector Vector::curl() const {
     if (!isFourier()) ERROR("must start in fourier space");
-    Vector c(m_dims, false);
+    Vector c(m_dims, false, Scalar::NoFill());
     CHECK(sh_curl_z(m_vector[0].deviceData(), m_vector[1].deviceData(), m_vector[2].deviceData(),
                     c[0].deviceDataMutable(), c[1].deviceDataMutable(), c[2].deviceDataMutable(),
                     (int)m_dims[0], (int)m_dims[1], (int)m_dims[2], nullptr));
@@ -85,7 +91,7 @@ Vector Vector::curl() const {
 
 Scalar Vector::divergence() const {
     if (!isFourier()) ERROR("must start in fourier space");
-    Scalar div(m_dims, false);
+    Scalar div(m_dims, false, Scalar::NoFill());
     CHECK(sh_divergence_z(m_vector[0].deviceData(), m_vector[1].deviceData(), m_vector[2].deviceData(),
                           div.deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2],
                           nullptr));
@@ -94,7 +100,7 @@ Scalar Vector::divergence() const {
 
 Vector Vector::laplacian() const {
     if (!isFourier()) ERROR("must start in fourier space");
-    Vector lap(m_dims, false);
+    Vector lap(m_dims, false, Scalar::NoFill());
     CHECK(sh_vlaplacian_z(m_vector[0].deviceData(), m_vector[1].deviceData(), m_vector[2].deviceData(),
                           lap[0].deviceDataMutable(), lap[1].deviceDataMutable(),
                           lap[2].deviceDataMutable(), (int)m_dims[0], (int)m_dims[1], (int)m_dims[2],
