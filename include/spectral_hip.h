@@ -364,6 +364,14 @@ int sh_les_set_dealias(sh_les_t handle, int mode);
  * then the start-up step (exec/ns.cpp:132-153) with the closure force */
 int sh_les_set_initial_physical(sh_les_t handle, const void* ubar3);
 int sh_les_steps(sh_les_t handle, int nsteps);
+/* lag = 0 (default): step t is forced by f(V^t), the closure running beside
+ * the solver's kernels A-D.  lag = 1: step t is forced by the extrapolation
+ * 2 f(V^{t-1}) - f(V^{t-2}) (second order in time, as the AB2 term it joins),
+ * so the closure of V^t runs beside the whole solver step t.  Switching on
+ * restarts the history at the current state (the next step is forced by
+ * f(V)).  Any split of a run into sh_les_steps calls, and graph replays of a
+ * call, give the same sequence. */
+int sh_les_set_lag(sh_les_t handle, int lag);
 /* tune the solver's and the closure transforms' schedules on this grid
  * (after the initial condition is set; the state is kept).  Synchronous. */
 int sh_les_autotune(sh_les_t handle, int reps);
@@ -373,6 +381,10 @@ int sh_les_get_velocity(sh_les_t handle, void* out_v3);
 int sh_les_stress(sh_les_t handle, void* out_tau6);
 /* the body force -div tau of the current velocity, 3 Fourier complex64 fields */
 int sh_les_force(sh_les_t handle, void* out_f3);
+/* the force the next step applies with the lagged closure (3 Fourier
+ * complex64 fields); with the synchronous closure, the one the last step
+ * applied */
+int sh_les_pending_force(sh_les_t handle, void* out_f3);
 /* network inputs of a physical filtered velocity ubar (float [3][n^3]):
  * X (float [9][ldx]) = d ubar_i / d x_j by spectral derivatives on the n^3
  * grid (Nyquist modes dropped), the same features the LES feeds the model */

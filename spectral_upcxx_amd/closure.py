@@ -100,8 +100,9 @@ class LES:
     """FP32 coarse-grid solver with the network's stress as body force
     (model=None: the no-model solve)."""
 
-    def __init__(self, n, nu, K, model=None, clip=True, strm=None, dealias=0):
-        """dealias: 0 none (the reference scheme), 1 the 2/3 rule, 2 the exponential filter"""
+    def __init__(self, n, nu, K, model=None, clip=True, strm=None, dealias=0, lag=0):
+        """dealias: 0 none (the reference scheme), 1 the 2/3 rule, 2 the exponential filter;
+        lag: 0 synchronous closure, 1 lagged closure (sh_les_set_lag)"""
         self.n = n
         self.h = ctypes.c_void_p()
         self.stream = stream() if strm is None else strm
@@ -109,6 +110,11 @@ class LES:
                             int(bool(clip)), self.stream)
         if dealias:
             lib().sh_les_set_dealias(self.h, int(dealias))
+        if lag:
+            lib().sh_les_set_lag(self.h, int(lag))
+
+    def set_lag(self, lag):
+        lib().sh_les_set_lag(self.h, int(lag))
 
     def close(self):
         if self.h:
@@ -140,4 +146,10 @@ class LES:
         import torch
         f = torch.empty((3, self.n, self.n, self.n), dtype=torch.complex64, device="cuda")
         lib().sh_les_force(self.h, ptr(f))
+        return f
+
+    def pending_force(self):
+        import torch
+        f = torch.empty((3, self.n, self.n, self.n), dtype=torch.complex64, device="cuda")
+        lib().sh_les_pending_force(self.h, ptr(f))
         return f

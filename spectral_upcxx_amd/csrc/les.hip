@@ -8,10 +8,14 @@
 //   2  inverse 3D transform of Z (5 fields)
 //   3  BF16 MFMA network on the packed physical gradients -> packed stresses
 //      (tau_xx + i tau_yy, tau_zz + i tau_xy, tau_xz + i tau_yz)
-//   4  forward 3D transform of the stresses (3 fields, not 6)
-//   5  unpack by Hermitian symmetry, f_i = -sum_j i k_j tau_ij
+//   4  forward z and y passes of the stresses (3 fields, not 6)
+//   5  the x pass on pairs of mirror lines, fused with the unpacking by
+//      Hermitian symmetry and f_i = -sum_j i k_j tau_ij
 // Nyquist wavenumbers count as zero in the derivatives, so every packed field
 // is the transform of real fields and the packing is exact.
+// With the lagged closure (sh_les_set_lag) step t+1 is forced by an
+// extrapolation from the closures of steps t and t-1, and the closure chain
+// runs beside the solver's whole step instead of its kernels A-D.
 #include <cmath>
 
 #include "fft_core.hpp"
@@ -118,6 +122,104 @@ __global__ void k_les_div_unpack(const float2* __restrict__ Pt, long ts, float2*
             F[a * fs + p] = nyq ? make_float2(0.f, 0.f) : make_float2(-s.x, -s.y);
         }
     }
+}
+
+// x-line pair q of an n x n grid of (y, z) lines: the lines (j, k) and
+// (-j, -k) hold the modes k and -k.  Pairs 0 .. (n/2-1) n - 1 are the rows
+// k = 1 .. n/2-1 (every j), the last 2 (n/2 + 1) the rows k = 0 and k = n/2
+// with j <= n/2 (those rows are their own mirror).  n even.
+__device__ __forceinline__ void pair_line(long q, int n, int& j, int& k) {
+    const long mid = (long)(n / 2 - 1) * n;
+    if (q < mid) {
+        k = 1 + (int)(q / n);
+        j = (int)(q % n);
+        return;
+    }
+    const int r = (int)(q - mid);
+    k = r <= n / 2 ? 0 : n / 2;
+    j = r <= n / 2 ? r : r - (n / 2 + 1);
+}
+
+// The last pass of the stress transform fused with the unpacking and the
+// divergence: the forward x pass of the packed stresses P_c (c = 0..2, y and
+// z already forward-transformed) on a line and its mirror line, then
+//   tau_{2c} = (P_c(k) + conj P_c(-k)) / 2, tau_{2c+1} = (P_c(k) - conj P_c(-k)) / 2i,
+//   f_a = -sum_b i k_b tau_ab   (zero on the Nyquist planes).
+// mode 0: F = f;  1: F = H = f;  2 (lagged closure): F = 2 f - H, H = f, so F
+// extrapolates the force of the next step from this one and the previous one.
+template <class P, int PPB>
+__global__ __launch_bounds__(P::TPL * 6 * PPB) void k_les_x_div(const float2* __restrict__ T,
+                                                                float2* __restrict__ F,
+                                                                float2* __restrict__ H,
+                                                                const float2* __restrict__ tw, int n, int mode) {
+    constexpr int N = P::N;
+    constexpr int LS = N + N / 16 + 1;
+    __shared__ float sre[6 * PPB * LS];
+    __shared__ float sim[6 * PPB * LS];
+    const int tid = threadIdx.x % P::TPL;
+    const int ll = threadIdx.x / P::TPL;  // 0 .. 6 PPB - 1: pair ll / 6, group ll % 6
+    const int g = ll % 6;                 // 0..2: field g of the line, 3..5: field g - 3 of its mirror
+    const long N3 = (long)n * n * n;
+    const long npairs = (long)n * n / 2 + 2;
+    const long q0 = (long)blockIdx.x * PPB + ll / 6;
+    int j, k;
+    pair_line(q0 < npairs ? q0 : 0, n, j, k);
+    const long line = g < 3 ? (long)k * n + j : (long)((n - k) % n) * n + (n - j) % n;
+    const float2* src = T + (g % 3) * N3 + line * N;
+    float2 x[P::E];
+    for_in<P>(tid, [&](int s, int pos) { x[s] = src[pos]; });
+    auto idx = [&](int grp, int pos) { return grp * LS + pos + (pos >> 4); };
+    auto lidx = [&](int pos) { return idx(ll, pos); };
+    line_fft<P, -1, float>(x, tid, sre, sim, lidx, tw);
+    __syncthreads();
+    for_out<P>(tid, [&](int s, int pos) {
+        sre[lidx(pos)] = x[s].x;
+        sim[lidx(pos)] = x[s].y;
+    });
+    __syncthreads();
+    const int pair[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // xx yy zz xy xz yz
+    for (int o = threadIdx.x; o < PPB * 2 * N; o += blockDim.x) {
+        const int p2 = o / (2 * N), side = (o / N) % 2, i = o % N;
+        const long q = (long)blockIdx.x * PPB + p2;
+        if (q >= npairs) continue;
+        int jj, kk;
+        pair_line(q, n, jj, kk);
+        const int jm = (n - jj) % n, km = (n - kk) % n;
+        if (side == 1 && jm == jj && km == kk) continue;  // a line that is its own mirror
+        const int lj = side ? jm : jj, lk = side ? km : kk;
+        const int im = (N - i) % N;
+        float2 t[6];
+        for (int c = 0; c < 3; c++) {
+            const int ga = 6 * p2 + 3 * side + c, gb = 6 * p2 + 3 * (1 - side) + c;
+            const float2 a = make_float2(sre[idx(ga, i)], sim[idx(ga, i)]);
+            const float2 b = make_float2(sre[idx(gb, im)], sim[idx(gb, im)]);
+            t[2 * c] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+            t[2 * c + 1] = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+        }
+        const bool nyq = 2 * i == n || 2 * lj == n || 2 * lk == n;
+        const float kk3[3] = {kd(i, n), kd(lj, n), kd(lk, n)};
+        const long p = ((long)lk * n + lj) * N + i;
+        for (int a = 0; a < 3; a++) {
+            float2 s = make_float2(0.f, 0.f);
+            for (int b = 0; b < 3; b++) {
+                const float2 d = ik(kk3[b], t[pair[a][b]]);
+                s.x += d.x;
+                s.y += d.y;
+            }
+            const float2 f = nyq ? make_float2(0.f, 0.f) : make_float2(-s.x, -s.y);
+            if (mode == 2) {
+                const float2 h = H[a * N3 + p];
+                F[a * N3 + p] = make_float2(2.f * f.x - h.x, 2.f * f.y - h.y);
+            } else {
+                F[a * N3 + p] = f;
+            }
+            if (mode >= 1) H[a * N3 + p] = f;
+        }
+    }
+}
+
+__global__ void k_copy_c(const float2* __restrict__ a, float2* __restrict__ b, long n) {
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) b[p] = a[p];
 }
 
 // ---------------------------------------------------------------- force-matching loss
@@ -266,12 +368,8 @@ int launch_grad_x(const float2* V, float2* Z, int n, hipStream_t st) {
     return SH_OK;
 }
 
-// physical packed gradient fields Z (5 complex64 fields) of V (3 Fourier
-// fields, n^3 each): the fused derivative + x pass, then the y and z passes
-// (the z pass scales by 1/n^3)
-int les_gradients(const float2* V, float2* Z, int n, hipStream_t st) {
-    const long sxy = (long)n * n, N = sxy * n;
-    ProfScope ps("les_fft_inv", st);
+// the fused derivative + x pass of the gradient fields (reads V, writes Z)
+int les_grad_x(const float2* V, float2* Z, int n, hipStream_t st) {
     int rc = SH_EUNSUPPORTED;
     switch (n) {
 #define SH_CASE(n_, P_, L_, S_)                   \
@@ -281,14 +379,59 @@ int les_gradients(const float2* V, float2* Z, int n, hipStream_t st) {
         SH_FOR_EACH_FFT_SIZE(SH_CASE)
 #undef SH_CASE
     }
-    if (rc) {
-        if (rc == SH_EUNSUPPORTED) set_error("les: unsupported grid size");
-        return rc;
-    }
+    if (rc == SH_EUNSUPPORTED) set_error("les: unsupported grid size");
+    return rc;
+}
+
+// the y and z passes of the gradient fields (the z pass scales by 1/n^3)
+int les_grad_yz(float2* Z, int n, hipStream_t st) {
+    const long sxy = (long)n * n, N = sxy * n;
     Epi<float> plain{1.f, 0, 1, -1, -1};
     RC(pass_strided<float>(Z, n, n, 1, n, sxy, n, 5, N, +1, plain, st, ROLE_Y));
     Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
     return pass_strided<float>(Z, n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z);
+}
+
+// physical packed gradient fields Z (5 complex64 fields) of V (3 Fourier
+// fields, n^3 each)
+int les_gradients(const float2* V, float2* Z, int n, hipStream_t st) {
+    ProfScope ps("les_fft_inv", st);
+    RC(les_grad_x(V, Z, n, st));
+    return les_grad_yz(Z, n, st);
+}
+
+template <class P>
+int launch_x_div(const float2* T, float2* F, float2* H, int n, int mode, hipStream_t st) {
+    constexpr int PPB = 256 / (6 * P::TPL) > 0 ? 256 / (6 * P::TPL) : 1;
+    const float2* tw = (const float2*)twiddles(P::N, SH_F32);
+    if (!tw) return SH_ENOMEM;
+    const long npairs = (long)n * n / 2 + 2;
+    hipLaunchKernelGGL((k_les_x_div<P, PPB>), dim3((unsigned)((npairs + PPB - 1) / PPB)), dim3(P::TPL * 6 * PPB), 0,
+                       st, T, F, H, tw, n, mode);
+    SH_LAUNCH_CHECK("k_les_x_div");
+    return SH_OK;
+}
+
+// f = -div tau from the packed physical stresses T (3 fields, overwritten):
+// forward z and y passes, then the x pass fused with the Hermitian unpacking
+// and the divergence (k_les_x_div, `mode` as there)
+int les_force_from_stress(float2* T, float2* F, float2* H, int n, int mode, hipStream_t st) {
+    const long sxy = (long)n * n, N = sxy * n;
+    ProfScope ps("les_fft_fwd", st);
+    Epi<float> plain{1.f, 0, 1, -1, -1};
+    RC(pass_strided<float>(T, n, n, 1, n, n, sxy, 3, N, -1, plain, st, ROLE_Z));
+    RC(pass_strided<float>(T, n, n, 1, n, sxy, n, 3, N, -1, plain, st, ROLE_Y));
+    int rc = SH_EUNSUPPORTED;
+    switch (n) {
+#define SH_CASE(n_, P_, L_, S_)                                 \
+    case n_:                                                    \
+        rc = launch_x_div<P_>(T, F, H, n, mode, st);           \
+        break;
+        SH_FOR_EACH_FFT_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    if (rc == SH_EUNSUPPORTED) set_error("les: unsupported grid size");
+    return rc;
 }
 
 }  // namespace sh
@@ -302,56 +445,105 @@ struct sh_les_s {
     int clip;
     hipStream_t st;
     sh_ns_t ns;
-    float2 *F, *Z, *T, *Vc;  // force (3), packed gradients (5), packed stresses (3), FP32 velocity (3)
-    double2* V64;            // 3 fields for the initial condition
-    // the closure runs on `side` while the solver's kernels A-D run on st;
-    // kernel E waits for ev_force, the next closure for ev_state
+    // force slots (3 fields each; slot 0 is the one the solver reads between
+    // calls), packed gradients (5), packed stresses (3), FP32 velocity (3),
+    // the previous step's force for the lagged closure (3)
+    float2 *F, *F1, *Z, *T, *Vc, *H;
+    double2* V64;  // 3 fields for the initial condition
+    int lag;       // 1: the closure of step t forces step t+1 (extrapolated)
+    bool started;  // an initial condition has been set
+    // the closure runs on `side` while the solver's kernels run on st; with
+    // the synchronous closure kernel E waits for ev_force, with the lagged
+    // one only for ev_read (the closure has read V); the next closure waits
+    // for ev_state
     hipStream_t side;
-    hipEvent_t ev_state, ev_force;
+    hipEvent_t ev_state, ev_force, ev_read;
 };
 
 namespace {
 
-int sgs_force(sh_les_s* h, const float2* V, hipStream_t st) {
+// f = -div tau(V) into F (mode and H as k_les_x_div); ev_read, if given, is
+// recorded once V has been read
+int sgs_force(sh_les_s* h, const float2* V, float2* F, float2* H, int mode, hipStream_t st, hipEvent_t ev_read) {
     const long N = h->N;
-    RC(les_gradients(V, h->Z, h->n, st));
+    {
+        ProfScope ps("les_fft_inv", st);
+        RC(les_grad_x(V, h->Z, h->n, st));
+        if (ev_read) SH_HIP(hipEventRecord(ev_read, st));
+        RC(les_grad_yz(h->Z, h->n, st));
+    }
     {
         ProfScope ps("les_mlp", st);
         RC(mlp_infer_packed(h->model, h->Z, N, N, h->T, N, h->clip, st));
     }
-    {
-        ProfScope ps("les_fft_fwd", st);
-        RC(fft3d<float>(h->T, 3, N, h->n, h->n, h->n, -1, 0, st));
-    }
-    hipLaunchKernelGGL(k_les_div_unpack, dim3(lgrid(N)), dim3(256), 0, st, h->T, N, h->F, N, h->n);
-    SH_LAUNCH_CHECK("k_les_div_unpack");
-    return SH_OK;
+    return les_force_from_stress(h->T, F, H, h->n, mode, st);
 }
 
-// one LES step: the closure of V^n on the side stream, concurrent with the
-// solver's kernels A-D (which read V and the work fields and write only the
-// latter); kernel E reads the force and then overwrites V, after which the
-// next closure may start
+// one LES step with the synchronous closure: the closure of V^t on the side
+// stream, concurrent with the solver's kernels A-D (which read V and the work
+// fields and write only the latter); kernel E reads the force and then
+// overwrites V, after which the next closure may start
 int les_step(sh_les_s* h) {
     if (!h->model) return sh_ns_steps(h->ns, 1);
     const float2* V = (const float2*)ns_fused_velocity(h->ns);
     SH_HIP(hipEventRecord(h->ev_state, h->st));
     SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
-    RC(sgs_force(h, V, h->side));
+    RC(sgs_force(h, V, h->F, nullptr, 0, h->side, nullptr));
     SH_HIP(hipEventRecord(h->ev_force, h->side));
     RC(ns_step_phase(h->ns, 0));
     SH_HIP(hipStreamWaitEvent(h->st, h->ev_force, 0));
     return ns_step_phase(h->ns, 1);
 }
 
-int les_step_cb(void* p) { return les_step((sh_les_s*)p); }
+// nsteps steps with the lagged closure.  Step t reads the force slot t % 2,
+// which holds 2 f(V^{t-1}) - f(V^{t-2}) (second order in time, like the AB2
+// term it is added to); the closure of V^t runs on the side stream
+// concurrently with the solver's whole step t and writes slot (t+1) % 2.
+// Kernel E of step t waits only for the closure's first kernel, the reader
+// of V.  The call ends with the side stream joined and the pending force in
+// slot 0, so any number of steps per call (and graph replays of a call)
+// continue the same sequence.
+int les_steps_lagged(sh_les_s* h, int nsteps) {
+    float2* slot[2] = {h->F, h->F1};
+    for (int t = 0; t < nsteps; t++) {
+        const float2* V = (const float2*)ns_fused_velocity(h->ns);
+        SH_HIP(hipEventRecord(h->ev_state, h->st));
+        SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
+        RC(sgs_force(h, V, slot[(t + 1) % 2], h->H, 2, h->side, h->ev_read));
+        RC(sh_ns_set_forcing(h->ns, slot[t % 2]));
+        RC(ns_step_phase(h->ns, 0));
+        SH_HIP(hipStreamWaitEvent(h->st, h->ev_read, 0));
+        RC(ns_step_phase(h->ns, 1));
+    }
+    SH_HIP(hipEventRecord(h->ev_force, h->side));
+    SH_HIP(hipStreamWaitEvent(h->st, h->ev_force, 0));
+    if (nsteps % 2) {
+        hipLaunchKernelGGL(k_copy_c, dim3(lgrid(3 * h->N)), dim3(256), 0, h->st, h->F1, h->F, 3 * h->N);
+        SH_LAUNCH_CHECK("k_copy_c");
+    }
+    return sh_ns_set_forcing(h->ns, h->F);
+}
+
+int les_run(sh_les_s* h, int nsteps) {
+    if (h->model && h->lag) return les_steps_lagged(h, nsteps);
+    for (int t = 0; t < nsteps; t++) RC(les_step(h));
+    return SH_OK;
+}
+
+// a timing unit for the solver's schedule tuner: one synchronous step, or two
+// lagged ones (the lagged call ends with a join)
+int les_step_cb(void* p) {
+    sh_les_s* h = (sh_les_s*)p;
+    return les_run(h, h->model && h->lag ? 2 : 1);
+}
 
 void free_les(sh_les_s* h) {
     if (h->ev_state) (void)hipEventDestroy(h->ev_state);
     if (h->ev_force) (void)hipEventDestroy(h->ev_force);
+    if (h->ev_read) (void)hipEventDestroy(h->ev_read);
     if (h->side) (void)hipStreamDestroy(h->side);
     if (h->ns) sh_ns_destroy(h->ns);
-    float2* bufs[4] = {h->F, h->Z, h->T, h->Vc};
+    float2* bufs[6] = {h->F, h->F1, h->Z, h->T, h->Vc, h->H};
     for (float2* b : bufs)
         if (b) pool_free(b);
     if (h->V64) pool_free(h->V64);
@@ -378,9 +570,9 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
     // the closure reads the solver's full-spectrum velocity in place
     if (!rc) rc = sh_ns_set_layout(h->ns, 0);
     void* p;
-    const long sizes[4] = {3, 5, 3, 3};
-    float2** dst[4] = {&h->F, &h->Z, &h->T, &h->Vc};
-    for (int i = 0; i < 4; i++) {
+    const long sizes[6] = {3, 3, 5, 3, 3, 3};
+    float2** dst[6] = {&h->F, &h->F1, &h->Z, &h->T, &h->Vc, &h->H};
+    for (int i = 0; i < 6; i++) {
         p = nullptr;
         if (!rc) rc = pool_alloc(&p, sizeof(float2) * h->N * sizes[i]);
         *dst[i] = (float2*)p;
@@ -393,6 +585,7 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
     if (!rc && hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) rc = SH_EHIP;
     if (!rc && hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
     if (!rc && hipEventCreateWithFlags(&h->ev_force, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
+    if (!rc && hipEventCreateWithFlags(&h->ev_read, hipEventDisableTiming) != hipSuccess) rc = SH_EHIP;
     if (rc) {
         if (rc == SH_EHIP) set_error("sh_les_create: stream or event creation failed");
         free_les(h);
@@ -422,15 +615,30 @@ int sh_les_set_initial_physical(sh_les_t h, const void* ubar3) {
     if (h->model) {
         hipLaunchKernelGGL(k_z_to_c2, dim3(lgrid(3 * N)), dim3(256), 0, h->st, h->V64, h->Vc, 3 * N);
         SH_LAUNCH_CHECK("k_z_to_c2");
-        RC(sgs_force(h, h->Vc, h->st));
+        // the force of the start (the lagged closure's history starts there too)
+        RC(sgs_force(h, h->Vc, h->F, h->H, 1, h->st, nullptr));
     }
+    h->started = true;
     return sh_ns_startup(h->ns);
 }
 
 int sh_les_steps(sh_les_t h, int nsteps) {
     if (!h || nsteps < 0) return SH_EINVAL;
-    for (int t = 0; t < nsteps; t++) RC(les_step(h));
-    return SH_OK;
+    return les_run(h, nsteps);
+}
+
+int sh_les_set_lag(sh_les_t h, int lag) {
+    if (!h || lag < 0 || lag > 1) {
+        set_error("sh_les_set_lag: lag must be 0 (synchronous closure) or 1 (lagged, extrapolated)");
+        return SH_EINVAL;
+    }
+    if (lag && !h->lag && h->model && h->started) {
+        // restart the history at the current state: the next step is forced
+        // by f(V), as after a start
+        RC(sgs_force(h, (const float2*)ns_fused_velocity(h->ns), h->F, h->H, 1, h->st, nullptr));
+    }
+    h->lag = lag;
+    return sh_ns_set_forcing(h->ns, h->model ? h->F : nullptr);
 }
 
 int sh_les_set_dealias(sh_les_t h, int mode) {
@@ -442,8 +650,28 @@ int sh_les_autotune(sh_les_t h, int reps) {
     if (!h || reps <= 0) return SH_EINVAL;
     RC(sh_autotune_fft3d(h->n, h->n, h->n, SH_F32, reps, h->st));
     // the solver's schedules timed on the whole LES step (closure overlapped
-    // on the side stream), which is what they run in
-    return ns_autotune_with(h->ns, reps, &les_step_cb, h);
+    // on the side stream), which is what they run in.  The solver restores
+    // its own state; the lagged closure's pending force and history are
+    // saved and restored here.
+    const bool keep = h->model && h->lag;
+    void* save = nullptr;
+    const size_t fb = sizeof(float2) * 3 * h->N;
+    if (keep) {
+        RC(pool_alloc(&save, 2 * fb));
+        SH_HIP(hipMemcpyAsync(save, h->F, fb, hipMemcpyDeviceToDevice, h->st));
+        SH_HIP(hipMemcpyAsync((char*)save + fb, h->H, fb, hipMemcpyDeviceToDevice, h->st));
+    }
+    int rc = ns_autotune_with(h->ns, reps, &les_step_cb, h);
+    if (keep) {
+        if (hipMemcpyAsync(h->F, save, fb, hipMemcpyDeviceToDevice, h->st) != hipSuccess ||
+            hipMemcpyAsync(h->H, (char*)save + fb, fb, hipMemcpyDeviceToDevice, h->st) != hipSuccess ||
+            hipStreamSynchronize(h->st) != hipSuccess) {
+            set_error("sh_les_autotune: restoring the closure state failed");
+            if (!rc) rc = SH_EHIP;
+        }
+        pool_free(save);
+    }
+    return rc;
 }
 
 int sh_les_get_velocity(sh_les_t h, void* out_v3) {
@@ -465,7 +693,16 @@ int sh_les_force(sh_les_t h, void* out_f3) {
         set_error("sh_les_force: invalid argument (an LES with a model)");
         return SH_EINVAL;
     }
-    RC(sgs_force(h, (const float2*)ns_fused_velocity(h->ns), h->st));
+    // straight into the caller's buffer: the force slots hold the lagged
+    // closure's pending force
+    return sgs_force(h, (const float2*)ns_fused_velocity(h->ns), (float2*)out_f3, nullptr, 0, h->st, nullptr);
+}
+
+int sh_les_pending_force(sh_les_t h, void* out_f3) {
+    if (!h || !out_f3 || !h->model) {
+        set_error("sh_les_pending_force: invalid argument (an LES with a model)");
+        return SH_EINVAL;
+    }
     SH_HIP(hipMemcpyAsync(out_f3, h->F, sizeof(float2) * 3 * h->N, hipMemcpyDeviceToDevice, h->st));
     return SH_OK;
 }

@@ -495,8 +495,8 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
             out[t] = spectrum(physical_to_fourier(uf), nu, nb)
         return out
 
-    def trajectory(ub0, mdl, clip, dealias):
-        les = closure.LES(n, nu, les_dt, mdl, clip=clip, dealias=dealias)
+    def trajectory(ub0, mdl, clip, dealias, lag=0):
+        les = closure.LES(n, nu, les_dt, mdl, clip=clip, dealias=dealias, lag=lag)
         les.set_initial(ub0)
         cur = every  # after the start-up step the LES is at DNS step `every`
         Es = {}
@@ -562,23 +562,25 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         ub0 = te["snaps"][0][:3].reshape(3, n, n, n).contiguous()
         Fs = fdns_spectra(te["snaps"])
         res = {"seed": te["seed"], "model": compare(trajectory(ub0, model, clip, dealias), Fs),
+               "model_lagged": compare(trajectory(ub0, model, clip, dealias, lag=1), Fs),
                "no_model": compare(trajectory(ub0, None, False, dealias), Fs),
                "no_model_plain": compare(trajectory(ub0, None, False, False), Fs)}
         evals.append(res)
         log("les seed %d: " % te["seed"] + ", ".join(
-            "t=%.2f model %.3f no-model %.3f" % (a["t"], a[key], b[key]) for a, b in zip(res["model"], res["no_model"])))
+            "t=%.2f model %.3f lagged %.3f no-model %.3f" % (a["t"], a[key], c[key], b[key])
+            for a, b, c in zip(res["model"], res["no_model"], res["model_lagged"])))
     # ---- speed: LES steps/s with each step's launches replayed from one graph
     speed = {}
     ub0 = test[0]["snaps"][0][:3].reshape(3, n, n, n).contiguous()
     st = ctypes.c_void_p()
     L.sh_stream_create(ctypes.byref(st))
-    for name, mdl in (("model", model), ("no_model", None)):
+    for name, mdl, lag in (("model", model, 0), ("model_lagged", model, 1), ("no_model", None, 0)):
         m2 = None
         if mdl is not None:
             m2 = closure.MLP(seed=net_seed, strm=st)
             m2.set_params(model.params())
             m2.set_scaling(in_scale, out_scale * best_gain)
-        les = closure.LES(n, nu, les_dt, m2, clip=clip, strm=st, dealias=dealias)
+        les = closure.LES(n, nu, les_dt, m2, clip=clip, strm=st, dealias=dealias, lag=lag)
         torch.cuda.synchronize()
         les.set_initial(ub0)
         les.steps(2)
@@ -614,11 +616,15 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         "evals": evals,
         "final_%s_model_mean" % key: sum(r["model"][-1][key] for r in evals) / len(evals),
         "final_%s_no_model_mean" % key: sum(r["no_model"][-1][key] for r in evals) / len(evals),
+        "final_%s_model_lagged_mean" % key: sum(r["model_lagged"][-1][key] for r in evals) / len(evals),
         "mean_rel_err_model": sum(score(r["model"]) for r in evals) / len(evals),
+        "mean_rel_err_model_lagged": sum(score(r["model_lagged"]) for r in evals) / len(evals),
         "mean_rel_err_no_model": sum(score(r["no_model"]) for r in evals) / len(evals),
         "mean_rel_err_no_model_plain": sum(score(r["no_model_plain"]) for r in evals) / len(evals),
         "dns_fp64_steps_per_s": dns_sps, "les_model_steps_per_s": speed["model"],
         "les_no_model_steps_per_s": speed["no_model"],
+        "les_model_lagged_steps_per_s": speed["model_lagged"],
+        "les_lagged_vs_dns_steps_ratio": speed["model_lagged"] / dns_sps,
         "les_vs_dns_steps_ratio": speed["model"] / dns_sps,
         "les_vs_dns_sim_time_ratio": speed["model"] * les_dt / (dns_sps * dt_dns),
         "param_checksum": float(abs(model_params).sum()),
@@ -626,9 +632,10 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
 
 
 def les_speed(n=64, steps=200, reps=5, clip=True, seed=7100, nu=1.0 / 800, dt=2e-3):
-    """LES step rate with and without the closure (untrained network with a
-    small output scale; the cost does not depend on the weights), each
-    `steps` steps replayed from one hipGraph."""
+    """LES step rate with the synchronous closure, with the lagged closure
+    (sh_les_set_lag) and without a closure (untrained network with a small
+    output scale; the cost does not depend on the weights), each `steps` steps
+    replayed from one hipGraph."""
     torch = _torch()
     from . import closure
     L = lib()
@@ -639,15 +646,15 @@ def les_speed(n=64, steps=200, reps=5, clip=True, seed=7100, nu=1.0 / 800, dt=2e
     st = ctypes.c_void_p()
     L.sh_stream_create(ctypes.byref(st))
     out = {"form": "les_speed", "grid": n, "steps": steps, "reps": reps, "clip": clip}
-    for name in ("model", "no_model"):
+    for name in ("model", "model_lagged", "no_model"):
         m = None
-        if name == "model":
+        if name != "no_model":
             m = closure.MLP(seed=3, strm=st)
             m.set_scaling([1.0] * 9, [1e-3] * 6)
-        les = closure.LES(n, nu, dt, m, clip=clip, strm=st)
+        les = closure.LES(n, nu, dt, m, clip=clip, strm=st, lag=int(name == "model_lagged"))
         torch.cuda.synchronize()
         les.set_initial(ub)
-        if name == "model":
+        if m is not None:
             les.autotune(2)
         les.steps(2)
         L.sh_stream_synchronize(st)

@@ -354,6 +354,109 @@ def test_les_graph_replay_matches_eager(hip):
     L.sh_stream_destroy(st)
 
 
+def test_les_lag_extrapolates_the_closure(hip):
+    """Lagged closure: after a step from V^m the force pending for the next
+    step is 2 f(V^m) - f(V^{m-1}), f as sh_les_force computes it, bit for bit;
+    switching the lag on restarts the history at f(V)."""
+    n = 32
+    m, les = _les_with_model(hip, n, _params(15), np.full(9, 0.3, np.float32), np.full(6, 0.05, np.float32),
+                             True)
+    les.set_initial(_dev(_physical_field(n, 71).astype(np.float32)))
+    les.steps(2)
+    les.set_lag(1)
+    f32 = lambda t: _host(t).view(np.float32)
+    a = f32(les.force())
+    assert np.array_equal(f32(les.pending_force()), a)
+    les.steps(1)
+    assert np.array_equal(f32(les.pending_force()), a)  # 2 f(V) - f(V)
+    b = f32(les.force())
+    assert not np.array_equal(a, b)
+    les.steps(1)
+    assert np.array_equal(f32(les.pending_force()), np.float32(2) * b - a)
+    c = f32(les.force())
+    les.steps(1)
+    assert np.array_equal(f32(les.pending_force()), np.float32(2) * c - b)
+    les.close()
+    m.close()
+
+
+def test_les_lag_split_and_graph_replay(hip):
+    """The lagged run does not depend on how it is split into calls (an odd
+    call moves the pending force back to slot 0) and graph replays of an
+    odd-length call continue it exactly."""
+    from spectral_upcxx_amd import closure
+    L = hip.lib()
+    n = 32
+    st = ctypes.c_void_p()
+    L.sh_stream_create(ctypes.byref(st))
+    P = _params(16)
+    u = _dev(_physical_field(n, 81).astype(np.float32))
+    ms, runs = [], []
+    for _ in range(3):
+        m = closure.MLP(seed=1, strm=st)
+        m.set_params(P)
+        m.set_scaling(np.full(9, 0.3, np.float32), np.full(6, 0.02, np.float32))
+        ms.append(m)
+        runs.append(closure.LES(n, 0.01, 1e-3, m, clip=True, strm=st, lag=1))
+    L.sh_stream_synchronize(st)
+    for r in runs:
+        r.set_initial(u)
+        r.steps(1)
+    a, b, g = runs
+    a.steps(7)
+    for _ in range(7):
+        b.steps(1)
+    L.sh_stream_synchronize(st)
+    va = _host(a.velocity())
+    assert np.array_equal(va, _host(b.velocity()))
+    g.steps(1)
+    L.sh_stream_synchronize(st)
+    L.sh_graph_begin(st)
+    g.steps(3)
+    gr = ctypes.c_void_p()
+    L.sh_graph_end(st, ctypes.byref(gr))
+    for _ in range(2):
+        L.sh_graph_launch(gr, st)
+        L.sh_stream_synchronize(st)
+    assert np.array_equal(_host(g.velocity()), va)
+    assert np.array_equal(_host(g.pending_force()), _host(a.pending_force()))
+    L.sh_graph_destroy(gr)
+    for r in runs:
+        r.close()
+    for m in ms:
+        m.close()
+    L.sh_stream_destroy(st)
+
+
+def test_les_lag_is_second_order(hip):
+    """The lagged closure's extrapolation error is O(dt^2): halving dt divides
+    the gap between the lagged and the synchronous run by about 4 (a plain
+    one-step lag would give 2).  32^3, nu = 0.01, a random network, t = 0.32
+    at dt = 0.01 and 0.005 (the extrapolated explicit force has a smaller
+    stability region than the synchronous one: this closure is unstable with
+    it at dt = 0.02)."""
+    from spectral_upcxx_amd import closure
+    n = 32
+    P = _params(17)
+    u = _dev(_physical_field(n, 91).astype(np.float32))
+    T = 0.32
+    gaps = []
+    for dt in (0.01, 0.005):
+        res = []
+        for lag in (0, 1):
+            m = _mlp(hip, P, np.full(9, 0.3, np.float32), np.full(6, 0.05, np.float32))
+            les = closure.LES(n, 0.01, dt, m, clip=False, lag=lag)
+            les.set_initial(u)
+            les.steps(int(round(T / dt)))
+            res.append(_host(les.velocity()))
+            les.close()
+            m.close()
+        gaps.append(rel_l2(res[1], res[0]))
+    print("lag gaps %.3e %.3e ratio %.2f" % (gaps[0], gaps[1], gaps[0] / gaps[1]), flush=True)
+    assert 1e-5 < gaps[0] < 1e-3  # well above the FP32 rounding of the runs
+    assert gaps[0] / gaps[1] > 3.2
+
+
 def _kolmogorov(n, A=0.7, k=2):
     """u = (A sin(k y), 0, 0) in Fourier space (complex128, 3 fields)."""
     x = 2 * np.pi * np.arange(n) / n
