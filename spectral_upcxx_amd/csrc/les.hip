@@ -8,7 +8,8 @@
 //   2  inverse 3D transform of Z (5 fields)
 //   3  BF16 MFMA network on the packed physical gradients -> packed stresses
 //      (tau_xx + i tau_yy, tau_zz + i tau_xy, tau_xz + i tau_yz)
-//   4  forward z and y passes of the stresses (3 fields, not 6)
+//   4  forward z and y passes of the stresses (3 fields, not 6); the z passes
+//      of steps 2 and 4 and the network run as one kernel (k_les_z_mlp)
 //   5  the x pass on pairs of mirror lines, fused with the unpacking by
 //      Hermitian symmetry and f_i = -sum_j i k_j tau_ij
 // Nyquist wavenumbers count as zero in the derivatives, so every packed field
@@ -19,6 +20,7 @@
 #include <cmath>
 
 #include "fft_core.hpp"
+#include "mlp_core.hpp"
 
 namespace sh {
 
@@ -36,6 +38,7 @@ int mlp_adam(sh_mlp_t h, float lr, hipStream_t st);
 void* mlp_aux(sh_mlp_t h, size_t bytes);
 float* mlp_grad_buffer(sh_mlp_t h);
 const float* mlp_scales(sh_mlp_t h);
+const float* mlp_params(sh_mlp_t h);
 hipStream_t mlp_stream(sh_mlp_t h);
 
 // wavenumber of index i on an n-point axis, with the Nyquist index as zero
@@ -215,6 +218,162 @@ __global__ __launch_bounds__(P::TPL * 6 * PPB) void k_les_x_div(const float2* __
             }
             if (mode >= 1) H[a * N3 + p] = f;
         }
+    }
+}
+
+// The closure's z passes and the network in one kernel, for LPB x-columns of
+// one y-row (every z), with 8 thread groups of one wave each: the inverse z
+// transforms of the 5 packed gradient fields (groups 0-4, scaled by 1/n^3),
+// the network on the block's LPB*n grid points with all 8 waves (staged in
+// LDS as [feature][sample], sample pos*LPB + column), and the forward z
+// transforms of the 3 packed stress fields (groups 0-2).  The other groups
+// transform zeros in one shared scratch area, so that every thread meets the
+// same barriers.  The BF16 weight operands are formed once per block in LDS.
+// Same plan, scale and network arithmetic as the separate z passes and
+// k_mlp_infer, so the same results bit for bit.
+// forward2 (mlp_core.hpp) with the weight operands and biases read from LDS
+// at each use: operand o of lane l at W[64 o + l] (W1^T tile mt at o = mt,
+// W2^T tile (mt, kb) at 4 + 4 mt + kb, W3^T tile kb at 20 + kb), bias tile k
+// of lane group q at B[4 k + q] (b1: k = mt, b2: 4 + mt, b3: 8).  The same
+// operations in the same order, so the same results.
+__device__ __forceinline__ void forward2_lds(const s16x4* W, const f32x4* B, int lane, int q, s16x4 xa, s16x4 xb,
+                                             f32x4& ya, f32x4& yb) {
+    s16x4 ha[4], hb[4];
+    for (int mt = 0; mt < 4; mt++) {
+        const s16x4 w = W[mt * 64 + lane];
+        const f32x4 b = B[4 * mt + q];
+        ha[mt] = to_bf(tanh4(mfma(w, xa, b)));
+        hb[mt] = to_bf(tanh4(mfma(w, xb, b)));
+    }
+    s16x4 ga[4], gb[4];
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 a = B[4 * (4 + mt) + q], c = a;
+        for (int kb = 0; kb < 4; kb++) {
+            const s16x4 w = W[(4 + 4 * mt + kb) * 64 + lane];
+            a = mfma(w, ha[kb], a);
+            c = mfma(w, hb[kb], c);
+        }
+        ga[mt] = to_bf(tanh4(a));
+        gb[mt] = to_bf(tanh4(c));
+    }
+    ya = B[4 * 8 + q];
+    yb = ya;
+    for (int kb = 0; kb < 4; kb++) {
+        const s16x4 w = W[(20 + kb) * 64 + lane];
+        ya = mfma(w, ga[kb], ya);
+        yb = mfma(w, gb[kb], yb);
+    }
+}
+
+template <class P>
+struct ZMlp {
+    static constexpr int gcd64(int t) { return t % 64 == 0 ? 64 : t % 32 == 0 ? 32 : t % 16 == 0 ? 16 : t % 8 == 0 ? 8 : t % 4 == 0 ? 4 : t % 2 == 0 ? 2 : 1; }
+    static constexpr int LPB = 64 / gcd64(P::TPL);  // P::TPL * LPB is a multiple of 64
+    static constexpr int CS = LPB + 1;
+    static constexpr int S = LPB * P::N;            // samples of a block (a multiple of 32 for N % 4 == 0)
+    static constexpr int GS = S + 16;               // row stride of the sample array
+    static constexpr int THREADS = 8 * P::TPL * LPB;  // 8 groups
+    static constexpr long LDS = 4L * (2 * 6 * P::N * CS + 9 * GS) + 8L * 24 * 64 + 16L * 36;
+    static constexpr bool OK = P::N % 4 == 0 && LDS <= 65536 && THREADS <= 1024;
+};
+
+template <class P>
+__global__ __launch_bounds__(ZMlp<P>::THREADS) void k_les_z_mlp(const float2* __restrict__ Z, float2* __restrict__ T,
+                                                                const float2* __restrict__ tw, int n,
+                                                                const float* __restrict__ Pm,
+                                                                const float* __restrict__ sc, int clip) {
+    using Z_ = ZMlp<P>;
+    constexpr int N = P::N, LPB = Z_::LPB, CS = Z_::CS, S = Z_::S, GS = Z_::GS;
+    constexpr int GT = P::TPL * LPB;  // threads per group (64)
+    __shared__ float sre[6 * N * CS];
+    __shared__ float sim[6 * N * CS];
+    __shared__ float G[9 * GS];
+    __shared__ s16x4 Wop[24 * 64];
+    __shared__ f32x4 Bop[9 * 4];
+    const int g = threadIdx.x / GT;
+    const int c = (threadIdx.x % GT) % LPB, tid = (threadIdx.x % GT) / LPB;
+    const int tiles = (n + LPB - 1) / LPB;
+    const int j = blockIdx.x / tiles;
+    const int i = (blockIdx.x % tiles) * LPB + c;
+    const bool valid = i < n && j < n;
+    const long N3 = (long)n * n * n, es = (long)n * n;
+    const long cb = valid ? (long)i + (long)j * n : 0;
+    float* gre = sre + (g < 5 ? g : 5) * N * CS;
+    float* gim = sim + (g < 5 ? g : 5) * N * CS;
+    auto idx = [&](int pos) { return pos * CS + c; };
+    // operand o of lane l at Wop[64 o + l], as load_fwd forms them: W1^T tile
+    // mt at o = mt, W2^T tile (mt, kb) at 4 + 4 mt + kb, W3^T tile kb at 20 + kb
+    for (int e = threadIdx.x; e < 24 * 64; e += blockDim.x) {
+        const int o = e / 64, l = e % 64, r = l % 16, q = l / 16;
+        float v[4];
+        for (int jj = 0; jj < 4; jj++) {
+            if (o < 4)
+                v[jj] = Pm[OFF_W1 + (4 * q + jj) * 64 + 16 * o + r];
+            else if (o < 20)
+                v[jj] = Pm[OFF_W2 + (16 * ((o - 4) % 4) + 4 * q + jj) * 64 + 16 * ((o - 4) / 4) + r];
+            else
+                v[jj] = Pm[OFF_W3 + (16 * (o - 20) + 4 * q + jj) * 16 + r];
+        }
+        Wop[e] = pack4(v[0], v[1], v[2], v[3]);
+    }
+    // bias tile k (b1 tiles 0-3, b2 tiles 4-7, b3 8) of lane group q at Bop[4 k + q]
+    if (threadIdx.x < 36) {
+        const int k = threadIdx.x / 4, q = threadIdx.x % 4;
+        const int base = k < 4 ? OFF_B1 + 16 * k : k < 8 ? OFF_B2 + 16 * (k - 4) : OFF_B3;
+        f32x4 b;
+        for (int jj = 0; jj < 4; jj++) b[jj] = Pm[base + 4 * q + jj];
+        Bop[threadIdx.x] = b;
+    }
+    {  // inverse z transforms of the gradients -> samples
+        float2 x[P::E];
+        for_in<P>(tid, [&](int s, int pos) { x[s] = g < 5 ? Z[g * N3 + cb + pos * es] : make_float2(0.f, 0.f); });
+        line_fft<P, 1, float>(x, tid, gre, gim, idx, tw);
+        const float scale = (float)(1.0 / (double)N3);
+        if (g < 5)
+            for_out<P>(tid, [&](int s, int pos) {
+                const float2 v = rmul<float>(scale, x[s]);
+                G[(2 * g) * GS + pos * LPB + c] = v.x;
+                if (g < 4) G[(2 * g + 1) * GS + pos * LPB + c] = v.y;
+            });
+    }
+    __syncthreads();
+    {  // the network, two 16-sample tiles per trip and wave; stresses over rows 0-5
+        const int lane = threadIdx.x % 64, r = lane % 16, q = lane / 16;
+        const int wave = threadIdx.x / 64, nw = blockDim.x / 64;
+        const f32x4 sc_in = load_scale_in(sc, lane);
+        f32x4 sc_out;
+        for (int jj = 0; jj < 4; jj++) sc_out[jj] = 4 * q + jj < kOut ? sc[kIn + 4 * q + jj] : 0.f;
+        auto raw = [&](int s0) {
+            f32x4 xf;
+            for (int jj = 0; jj < 4; jj++) xf[jj] = 4 * q + jj < kIn ? G[(4 * q + jj) * GS + s0 + r] : 0.f;
+            return xf;
+        };
+        for (int s00 = wave * 32; s00 < S; s00 += nw * 32) {
+            const f32x4 ra = raw(s00), rb = raw(s00 + 16);
+            f32x4 ya, yb;
+            // the operands are read from LDS at each use (not held in ~84
+            // VGPRs): the laundered lane index keeps the reads in the loop
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            forward2_lds(Wop, Bop, ln, q, to_bf(ra * sc_in), to_bf(rb * sc_in), ya, yb);
+            for (int half = 0; half < 2; half++) {
+                f32x4 y = (half ? yb : ya) * sc_out;
+                if (clip) y = clip_backscatter(y, half ? rb : ra, r);
+                if (q < 2)
+                    for (int jj = 0; jj < 4; jj++)
+                        if (4 * q + jj < kOut) G[(4 * q + jj) * GS + s00 + 16 * half + r] = y[jj];
+            }
+        }
+    }
+    __syncthreads();
+    {  // forward z transforms of tau_{2g} + i tau_{2g+1}, g < 3
+        float2 x[P::E];
+        for_in<P>(tid, [&](int s, int pos) {
+            const int sm = pos * LPB + c;
+            x[s] = g < 3 ? make_float2(G[(2 * g) * GS + sm], G[(2 * g + 1) * GS + sm]) : make_float2(0.f, 0.f);
+        });
+        line_fft<P, -1, float>(x, tid, gre, gim, idx, tw);
+        if (valid && g < 3) for_out<P>(tid, [&](int s, int pos) { T[g * N3 + cb + pos * es] = x[s]; });
     }
 }
 
@@ -412,14 +571,49 @@ int launch_x_div(const float2* T, float2* F, float2* H, int n, int mode, hipStre
     return SH_OK;
 }
 
-// f = -div tau from the packed physical stresses T (3 fields, overwritten):
-// forward z and y passes, then the x pass fused with the Hermitian unpacking
-// and the divergence (k_les_x_div, `mode` as there)
-int les_force_from_stress(float2* T, float2* F, float2* H, int n, int mode, hipStream_t st) {
+template <class P>
+int launch_z_mlp(const float2* Z, float2* T, int n, const float* Pm, const float* sc, int clip, hipStream_t st) {
+    if constexpr (!ZMlp<P>::OK) {
+        return SH_EUNSUPPORTED;
+    } else {
+        const float2* tw = (const float2*)twiddles(P::N, SH_F32);
+        if (!tw) return SH_ENOMEM;
+        const int tiles = (n + ZMlp<P>::LPB - 1) / ZMlp<P>::LPB;
+        hipLaunchKernelGGL((k_les_z_mlp<P>), dim3((unsigned)(tiles * n)), dim3(ZMlp<P>::THREADS), 0, st, Z, T, tw,
+                           n, Pm, sc, clip);
+        SH_LAUNCH_CHECK("k_les_z_mlp");
+        return SH_OK;
+    }
+}
+
+// gradients after their x and y passes -> packed stresses after their z
+// pass: one fused kernel where its LDS fits (n = 8, 16, 32, 64), else the
+// z pass, the network and the z pass as separate launches
+int les_z_mlp(sh_mlp_t m, const float2* Z, float2* T, int n, int clip, hipStream_t st) {
+    int rc = SH_EUNSUPPORTED;
+    switch (n) {
+#define SH_CASE(n_, P_, L_, S_)                                                       \
+    case n_:                                                                          \
+        rc = launch_z_mlp<P_>(Z, T, n, mlp_params(m), mlp_scales(m), clip, st);      \
+        break;
+        SH_FOR_EACH_FFT_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    if (rc != SH_EUNSUPPORTED) return rc;
     const long sxy = (long)n * n, N = sxy * n;
-    ProfScope ps("les_fft_fwd", st);
+    Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
+    RC(pass_strided<float>(const_cast<float2*>(Z), n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z));
+    RC(mlp_infer_packed(m, Z, N, N, T, N, clip, st));
     Epi<float> plain{1.f, 0, 1, -1, -1};
-    RC(pass_strided<float>(T, n, n, 1, n, n, sxy, 3, N, -1, plain, st, ROLE_Z));
+    return pass_strided<float>(T, n, n, 1, n, n, sxy, 3, N, -1, plain, st, ROLE_Z);
+}
+
+// f = -div tau from the packed stresses T after their z pass (3 fields,
+// overwritten): the forward y pass, then the x pass fused with the Hermitian
+// unpacking and the divergence (k_les_x_div, `mode` as there)
+int les_force_yx(float2* T, float2* F, float2* H, int n, int mode, hipStream_t st) {
+    const long sxy = (long)n * n, N = sxy * n;
+    Epi<float> plain{1.f, 0, 1, -1, -1};
     RC(pass_strided<float>(T, n, n, 1, n, sxy, n, 3, N, -1, plain, st, ROLE_Y));
     int rc = SH_EUNSUPPORTED;
     switch (n) {
@@ -465,18 +659,21 @@ namespace {
 // f = -div tau(V) into F (mode and H as k_les_x_div); ev_read, if given, is
 // recorded once V has been read
 int sgs_force(sh_les_s* h, const float2* V, float2* F, float2* H, int mode, hipStream_t st, hipEvent_t ev_read) {
-    const long N = h->N;
+    const int n = h->n;
+    const long sxy = (long)n * n;
     {
         ProfScope ps("les_fft_inv", st);
-        RC(les_grad_x(V, h->Z, h->n, st));
+        RC(les_grad_x(V, h->Z, n, st));
         if (ev_read) SH_HIP(hipEventRecord(ev_read, st));
-        RC(les_grad_yz(h->Z, h->n, st));
+        Epi<float> plain{1.f, 0, 1, -1, -1};
+        RC(pass_strided<float>(h->Z, n, n, 1, n, sxy, n, 5, h->N, +1, plain, st, ROLE_Y));
     }
     {
         ProfScope ps("les_mlp", st);
-        RC(mlp_infer_packed(h->model, h->Z, N, N, h->T, N, h->clip, st));
+        RC(les_z_mlp(h->model, h->Z, h->T, n, h->clip, st));
     }
-    return les_force_from_stress(h->T, F, H, h->n, mode, st);
+    ProfScope ps("les_fft_fwd", st);
+    return les_force_yx(h->T, F, H, n, mode, st);
 }
 
 // one LES step with the synchronous closure: the closure of V^t on the side
