@@ -229,8 +229,11 @@ __global__ __launch_bounds__(P::TPL * 6 * PPB) void k_les_x_div(const float2* __
 // transforms of the 3 packed stress fields (groups 0-2).  The other groups
 // transform zeros in one shared scratch area, so that every thread meets the
 // same barriers.  The BF16 weight operands are formed once per block in LDS.
-// Same plan, scale and network arithmetic as the separate z passes and
-// k_mlp_infer, so the same results bit for bit.
+// The separate launches with the default z plan (P below) and k_mlp_infer
+// do the same operations; where the compiler rounds the transforms the same
+// way in both (measured at n = 16 and 32) the results are the same bit for
+// bit, otherwise they agree to FP32 rounding (a few BF16 activations round
+// the other way).
 // forward2 (mlp_core.hpp) with the weight operands and biases read from LDS
 // at each use: operand o of lane l at W[64 o + l] (W1^T tile mt at o = mt,
 // W2^T tile (mt, kb) at 4 + 4 mt + kb, W3^T tile kb at 20 + kb), bias tile k
