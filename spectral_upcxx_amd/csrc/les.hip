@@ -227,7 +227,7 @@ __global__ __launch_bounds__(P::TPL * 6 * PPB) void k_les_x_div(const float2* __
 // the network on the block's LPB*n grid points with all 8 waves (staged in
 // LDS as [feature][sample], sample pos*LPB + column), and the forward z
 // transforms of the 3 packed stress fields (groups 0-2).  The other groups
-// transform zeros in one shared scratch area, so that every thread meets the
+// transform zeros in scratch of their own, so that every thread meets the
 // same barriers.  The BF16 weight operands are formed once per block in LDS.
 // The separate launches with the default z plan (P below) and k_mlp_infer
 // do the same operations; where the compiler rounds the transforms the same
@@ -276,8 +276,9 @@ struct ZMlp {
     static constexpr int S = LPB * P::N;            // samples of a block (a multiple of 32 for N % 4 == 0)
     static constexpr int GS = S + 16;               // row stride of the sample array
     static constexpr int THREADS = 8 * P::TPL * LPB;  // 8 groups
-    static constexpr long LDS = 4L * (2 * 6 * P::N * CS + 9 * GS) + 8L * 24 * 64 + 16L * 36;
-    static constexpr bool OK = P::N % 4 == 0 && LDS <= 65536 && THREADS <= 1024;
+    static constexpr long LDS = 4L * (2 * 8 * P::N * CS + 9 * GS) + 8L * 24 * 64 + 16L * 36;
+    // two blocks per CU (160 KiB of LDS)
+    static constexpr bool OK = P::N % 4 == 0 && LDS <= 80 * 1024 && THREADS <= 1024;
 };
 
 template <class P>
@@ -288,8 +289,8 @@ __global__ __launch_bounds__(ZMlp<P>::THREADS) void k_les_z_mlp(const float2* __
     using Z_ = ZMlp<P>;
     constexpr int N = P::N, LPB = Z_::LPB, CS = Z_::CS, S = Z_::S, GS = Z_::GS;
     constexpr int GT = P::TPL * LPB;  // threads per group (64)
-    __shared__ float sre[6 * N * CS];
-    __shared__ float sim[6 * N * CS];
+    __shared__ float sre[8 * N * CS];
+    __shared__ float sim[8 * N * CS];
     __shared__ float G[9 * GS];
     __shared__ s16x4 Wop[24 * 64];
     __shared__ f32x4 Bop[9 * 4];
@@ -301,8 +302,8 @@ __global__ __launch_bounds__(ZMlp<P>::THREADS) void k_les_z_mlp(const float2* __
     const bool valid = i < n && j < n;
     const long N3 = (long)n * n * n, es = (long)n * n;
     const long cb = valid ? (long)i + (long)j * n : 0;
-    float* gre = sre + (g < 5 ? g : 5) * N * CS;
-    float* gim = sim + (g < 5 ? g : 5) * N * CS;
+    float* gre = sre + g * N * CS;
+    float* gim = sim + g * N * CS;
     auto idx = [&](int pos) { return pos * CS + c; };
     // operand o of lane l at Wop[64 o + l], as load_fwd forms them: W1^T tile
     // mt at o = mt, W2^T tile (mt, kb) at 4 + 4 mt + kb, W3^T tile kb at 20 + kb
