@@ -173,3 +173,60 @@ def test_zero_high(hip):
     t = _dev(x)
     hip.lib().sh_zero_high_z(hip.ptr(t), *dims, hip.stream())
     assert np.array_equal(_host(t), O.zero_high(x))
+
+
+@pytest.mark.parametrize("dims", [(256, 256, 256), (512, 256, 128)])
+def test_operator_identities_full_size(hip, dims):
+    """At the benchmark sizes, on random fields generated on the device:
+    curl(grad f) = 0, div(curl V) = 0, lap f = div(grad f), vlap V is lap per
+    component, lap^-1 (lap f) = f off k = 0, and the Crank-Nicolson solve
+    (a, b) = (1, -c) inverts 1 - c lap (src/Scalar.cpp:245-334,
+    src/Vector.cpp:92-181).  Both sides round their products in different
+    orders, so the bars are a few ulps of the magnitudes involved."""
+    import torch
+    L, st, P = hip.lib(), hip.stream(), hip.ptr
+    nx, ny, nz = dims
+    kmax2 = (nx / 2) ** 2 + (ny / 2) ** 2 + (nz / 2) ** 2
+    f = torch.empty((nz, ny, nx), dtype=torch.complex128, device="cuda")
+    V = torch.empty((3, nz, ny, nx), dtype=torch.complex128, device="cuda")
+    L.sh_prng_complex_z(P(f), f.numel(), 101, st)
+    L.sh_prng_complex_z(P(V), V.numel(), 102, st)
+    nrm = lambda t: float(torch.linalg.vector_norm(t))
+    g = torch.empty_like(V)
+    L.sh_gradient_z(P(f), P(g[0]), P(g[1]), P(g[2]), nx, ny, nz, st)
+    c = torch.empty_like(V)
+    L.sh_curl_z(P(g[0]), P(g[1]), P(g[2]), P(c[0]), P(c[1]), P(c[2]), nx, ny, nz, st)
+    torch.cuda.synchronize()
+    assert nrm(c) < 1e-15 * kmax2 * nrm(f)
+    L.sh_curl_z(P(V[0]), P(V[1]), P(V[2]), P(c[0]), P(c[1]), P(c[2]), nx, ny, nz, st)
+    d = torch.empty_like(f)
+    L.sh_divergence_z(P(c[0]), P(c[1]), P(c[2]), P(d), nx, ny, nz, st)
+    torch.cuda.synchronize()
+    assert nrm(d) < 1e-15 * kmax2 * nrm(V)
+    lap = torch.empty_like(f)
+    L.sh_laplacian_z(P(f), P(lap), nx, ny, nz, st)
+    L.sh_divergence_z(P(g[0]), P(g[1]), P(g[2]), P(d), nx, ny, nz, st)
+    torch.cuda.synchronize()
+    assert nrm(d - lap) < 1e-15 * nrm(lap)
+    vl = torch.empty_like(V)
+    L.sh_vlaplacian_z(P(V[0]), P(V[1]), P(V[2]), P(vl[0]), P(vl[1]), P(vl[2]), nx, ny, nz, st)
+    for a in range(3):
+        L.sh_laplacian_z(P(V[a]), P(lap), nx, ny, nz, st)
+        torch.cuda.synchronize()
+        assert torch.equal(vl[a], lap), a
+    L.sh_laplacian_z(P(f), P(lap), nx, ny, nz, st)
+    back = lap.clone()
+    L.sh_laplacian_inverse_z(P(back), 0.0, 1.0, nx, ny, nz, st)
+    torch.cuda.synchronize()
+    assert back[0, 0, 0] == 0  # lap f has no mean and laplacian_inverse skips |k|^2 = 0
+    want = f.clone()
+    want[0, 0, 0] = 0
+    assert nrm(back - want) < 1e-15 * nrm(want)
+    cfac = 0.37 / kmax2
+    u = f.clone()
+    L.sh_laplacian_inverse_z(P(u), 1.0, -cfac, nx, ny, nz, st)
+    L.sh_laplacian_z(P(u), P(lap), nx, ny, nz, st)
+    torch.cuda.synchronize()
+    assert nrm(u - cfac * lap - f) < 1e-15 * nrm(f)
+    del f, V, g, c, d, lap, vl, back, want, u
+    torch.cuda.empty_cache()
