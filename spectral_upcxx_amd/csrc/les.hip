@@ -57,10 +57,13 @@ __device__ __forceinline__ float expfilter_f(float k, int n) {
 // Step 1 (Z_m = g_{2m} + i g_{2m+1}, g_{3a+b} = i k_b V_a) fused with the x
 // pass of step 2: block (x, m) forms packed field m of LPB x-lines on load
 // (the derivative factors of line (j, k) and of each position) and
-// transforms it back along x in registers and LDS.
+// transforms it back along x in registers and LDS.  half = 1: V holds only
+// the planes k < n/2 (the solver's half-spectrum layout of a real field);
+// a line with k > n/2 is read from its mirror line (-j, -k) as
+// V(kx, ky, kz) = conj V(-kx, -ky, -kz), and the plane k = n/2 is zero.
 template <class P, int LPB>
 __global__ __launch_bounds__(P::TPL* LPB) void k_les_grad_x(const float2* __restrict__ V, float2* __restrict__ Z,
-                                                            const float2* __restrict__ tw, int n) {
+                                                            const float2* __restrict__ tw, int n, int half) {
     constexpr int N = P::N;
     constexpr int LS = N + N / 16 + 1;
     __shared__ float sre[LPB * LS];
@@ -76,18 +79,30 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_les_grad_x(const float2* __rest
     // components 2m and 2m+1 of g_{3a+b} = i k_b V_a (the second is absent for m = 4)
     const int a0 = (2 * m) / 3, b0 = (2 * m) % 3, a1 = (2 * m + 1) / 3, b1 = (2 * m + 1) % 3;
     const bool two = m < 4;
-    const float ky = kd((int)(lc % n), n), kz = kd((int)(lc / n), n);
-    const float2* v0 = V + a0 * N3 + lc * N;
-    const float2* v1 = V + (two ? a1 : a0) * N3 + lc * N;
+    const int jl = (int)(lc % n), kl = (int)(lc / n);
+    const float ky = kd(jl, n), kz = kd(kl, n);
+    // where the line's values are read: itself, or (half layout, k > n/2)
+    // its mirror line, conjugated with kx reversed; nothing on the plane n/2
+    const bool mirror = half && 2 * kl > n;
+    const bool zero = half && 2 * kl == n;
+    const long src = mirror ? (long)(n - kl) * n + (n - jl) % n : lc;
+    const float2* v0 = V + a0 * N3 + src * N;
+    const float2* v1 = V + (two ? a1 : a0) * N3 + src * N;
+    auto vget = [&](const float2* v, int pos) {
+        if (zero) return make_float2(0.f, 0.f);
+        if (!mirror) return v[pos];
+        const float2 w = v[(N - pos) % N];
+        return make_float2(w.x, -w.y);
+    };
     float2 x[P::E];
     for_in<P>(tid, [&](int s, int pos) {
         const float kx = kd(pos, N);
         const float kb0 = b0 == 0 ? kx : b0 == 1 ? ky : kz;
-        const float2 g0 = ik(kb0, v0[pos]);
+        const float2 g0 = ik(kb0, vget(v0, pos));
         float2 z = g0;
         if (two) {
             const float kb1 = b1 == 0 ? kx : b1 == 1 ? ky : kz;
-            const float2 g1 = ik(kb1, v1[pos]);
+            const float2 g1 = ik(kb1, vget(v1, pos));
             z = make_float2(g0.x - g1.y, g0.y + g1.x);
         }
         x[s] = z;
@@ -521,23 +536,23 @@ static unsigned lgrid(long n) {
     } while (0)
 
 template <class P, int LPB>
-int launch_grad_x(const float2* V, float2* Z, int n, hipStream_t st) {
+int launch_grad_x(const float2* V, float2* Z, int n, int half, hipStream_t st) {
     const float2* tw = (const float2*)twiddles(P::N, SH_F32);
     if (!tw) return SH_ENOMEM;
     const long nlines = (long)n * n;
     hipLaunchKernelGGL((k_les_grad_x<P, LPB>), dim3((unsigned)((nlines + LPB - 1) / LPB), 5), dim3(P::TPL * LPB), 0,
-                       st, V, Z, tw, n);
+                       st, V, Z, tw, n, half);
     SH_LAUNCH_CHECK("k_les_grad_x");
     return SH_OK;
 }
 
 // the fused derivative + x pass of the gradient fields (reads V, writes Z)
-int les_grad_x(const float2* V, float2* Z, int n, hipStream_t st) {
+int les_grad_x(const float2* V, float2* Z, int n, int half, hipStream_t st) {
     int rc = SH_EUNSUPPORTED;
     switch (n) {
 #define SH_CASE(n_, P_, L_, S_)                   \
     case n_:                                      \
-        rc = launch_grad_x<P_, L_>(V, Z, n, st); \
+        rc = launch_grad_x<P_, L_>(V, Z, n, half, st); \
         break;
         SH_FOR_EACH_FFT_SIZE(SH_CASE)
 #undef SH_CASE
@@ -557,9 +572,9 @@ int les_grad_yz(float2* Z, int n, hipStream_t st) {
 
 // physical packed gradient fields Z (5 complex64 fields) of V (3 Fourier
 // fields, n^3 each)
-int les_gradients(const float2* V, float2* Z, int n, hipStream_t st) {
+int les_gradients(const float2* V, float2* Z, int n, int half, hipStream_t st) {
     ProfScope ps("les_fft_inv", st);
-    RC(les_grad_x(V, Z, n, st));
+    RC(les_grad_x(V, Z, n, half, st));
     return les_grad_yz(Z, n, st);
 }
 
@@ -660,14 +675,23 @@ struct sh_les_s {
 
 namespace {
 
-// f = -div tau(V) into F (mode and H as k_les_x_div); ev_read, if given, is
-// recorded once V has been read
-int sgs_force(sh_les_s* h, const float2* V, float2* F, float2* H, int mode, hipStream_t st, hipEvent_t ev_read) {
+// 1 when the solver steps the half-spectrum layout (its velocity holds the
+// planes kz < n/2 only)
+int les_half(sh_les_s* h) {
+    int half = 0;
+    return sh_ns_get_layout(h->ns, &half) == SH_OK && half ? 1 : 0;
+}
+
+// f = -div tau(V) into F, every plane (mode and H as k_les_x_div); half: V is
+// in the half-spectrum layout; ev_read, if given, is recorded once V has
+// been read
+int sgs_force(sh_les_s* h, const float2* V, int half, float2* F, float2* H, int mode, hipStream_t st,
+              hipEvent_t ev_read) {
     const int n = h->n;
     const long sxy = (long)n * n;
     {
         ProfScope ps("les_fft_inv", st);
-        RC(les_grad_x(V, h->Z, n, st));
+        RC(les_grad_x(V, h->Z, n, half, st));
         if (ev_read) SH_HIP(hipEventRecord(ev_read, st));
         Epi<float> plain{1.f, 0, 1, -1, -1};
         RC(pass_strided<float>(h->Z, n, n, 1, n, sxy, n, 5, h->N, +1, plain, st, ROLE_Y));
@@ -689,7 +713,7 @@ int les_step(sh_les_s* h) {
     const float2* V = (const float2*)ns_fused_velocity(h->ns);
     SH_HIP(hipEventRecord(h->ev_state, h->st));
     SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
-    RC(sgs_force(h, V, h->F, nullptr, 0, h->side, nullptr));
+    RC(sgs_force(h, V, les_half(h), h->F, nullptr, 0, h->side, nullptr));
     SH_HIP(hipEventRecord(h->ev_force, h->side));
     RC(ns_step_phase(h->ns, 0));
     SH_HIP(hipStreamWaitEvent(h->st, h->ev_force, 0));
@@ -706,11 +730,12 @@ int les_step(sh_les_s* h) {
 // continue the same sequence.
 int les_steps_lagged(sh_les_s* h, int nsteps) {
     float2* slot[2] = {h->F, h->F1};
+    const int half = les_half(h);
     for (int t = 0; t < nsteps; t++) {
         const float2* V = (const float2*)ns_fused_velocity(h->ns);
         SH_HIP(hipEventRecord(h->ev_state, h->st));
         SH_HIP(hipStreamWaitEvent(h->side, h->ev_state, 0));
-        RC(sgs_force(h, V, slot[(t + 1) % 2], h->H, 2, h->side, h->ev_read));
+        RC(sgs_force(h, V, half, slot[(t + 1) % 2], h->H, 2, h->side, h->ev_read));
         RC(sh_ns_set_forcing(h->ns, slot[t % 2]));
         RC(ns_step_phase(h->ns, 0));
         SH_HIP(hipStreamWaitEvent(h->st, h->ev_read, 0));
@@ -767,9 +792,9 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
     h->model = model;
     h->clip = clip != 0;
     h->st = as_stream(stream);
+    // the solver steps the half-spectrum layout once its start is real; the
+    // closure reads its velocity in place in either layout
     int rc = sh_ns_create(&h->ns, n, n, n, nu, K, SH_F32, 1, stream);
-    // the closure reads the solver's full-spectrum velocity in place
-    if (!rc) rc = sh_ns_set_layout(h->ns, 0);
     void* p;
     const long sizes[6] = {3, 3, 5, 3, 3, 3};
     float2** dst[6] = {&h->F, &h->F1, &h->Z, &h->T, &h->Vc, &h->H};
@@ -817,7 +842,7 @@ int sh_les_set_initial_physical(sh_les_t h, const void* ubar3) {
         hipLaunchKernelGGL(k_z_to_c2, dim3(lgrid(3 * N)), dim3(256), 0, h->st, h->V64, h->Vc, 3 * N);
         SH_LAUNCH_CHECK("k_z_to_c2");
         // the force of the start (the lagged closure's history starts there too)
-        RC(sgs_force(h, h->Vc, h->F, h->H, 1, h->st, nullptr));
+        RC(sgs_force(h, h->Vc, 0, h->F, h->H, 1, h->st, nullptr));
     }
     h->started = true;
     return sh_ns_startup(h->ns);
@@ -836,7 +861,7 @@ int sh_les_set_lag(sh_les_t h, int lag) {
     if (lag && !h->lag && h->model && h->started) {
         // restart the history at the current state: the next step is forced
         // by f(V), as after a start
-        RC(sgs_force(h, (const float2*)ns_fused_velocity(h->ns), h->F, h->H, 1, h->st, nullptr));
+        RC(sgs_force(h, (const float2*)ns_fused_velocity(h->ns), les_half(h), h->F, h->H, 1, h->st, nullptr));
     }
     h->lag = lag;
     return sh_ns_set_forcing(h->ns, h->model ? h->F : nullptr);
@@ -885,7 +910,7 @@ int sh_les_stress(sh_les_t h, void* out_tau6) {
         set_error("sh_les_stress: invalid argument (an LES with a model)");
         return SH_EINVAL;
     }
-    RC(les_gradients((const float2*)ns_fused_velocity(h->ns), h->Z, h->n, h->st));
+    RC(les_gradients((const float2*)ns_fused_velocity(h->ns), h->Z, h->n, les_half(h), h->st));
     return mlp_infer_packed_planar(h->model, h->Z, h->N, h->N, (float*)out_tau6, h->N, h->clip, h->st);
 }
 
@@ -896,7 +921,8 @@ int sh_les_force(sh_les_t h, void* out_f3) {
     }
     // straight into the caller's buffer: the force slots hold the lagged
     // closure's pending force
-    return sgs_force(h, (const float2*)ns_fused_velocity(h->ns), (float2*)out_f3, nullptr, 0, h->st, nullptr);
+    return sgs_force(h, (const float2*)ns_fused_velocity(h->ns), les_half(h), (float2*)out_f3, nullptr, 0, h->st,
+                     nullptr);
 }
 
 int sh_les_pending_force(sh_les_t h, void* out_f3) {
@@ -924,7 +950,7 @@ int sh_les_features(int n, const void* ubar3, void* X, long ldx, void* stream) {
         rc = hipGetLastError() == hipSuccess ? SH_OK : SH_EHIP;
     }
     if (!rc) rc = fft3d<float>(v, 3, N, n, n, n, -1, 0, st);
-    if (!rc) rc = les_gradients((const float2*)v, (float2*)z, n, st);
+    if (!rc) rc = les_gradients((const float2*)v, (float2*)z, n, 0, st);
     if (!rc) {
         hipLaunchKernelGGL(k_unpack_planar, dim3(lgrid(N)), dim3(256), 0, st, (const float2*)z, N, (float*)X, ldx, N);
         rc = hipGetLastError() == hipSuccess ? SH_OK : SH_EHIP;
@@ -968,7 +994,7 @@ int force_loss_grad(sh_mlp_t m, int n, const float* X, const float2* ft, int dea
                        part + kResidBlocks);
     SH_LAUNCH_CHECK("k_sum_partials");
     // dL/dtau by the adjoint of the divergence, then back through the network
-    RC(les_gradients(R, Z, n, st));
+    RC(les_gradients(R, Z, n, 0, st));
     hipLaunchKernelGGL(k_force_dy, dim3(lgrid(N)), dim3(256), 0, st, Z, N, D, N, N, (float)(2.0 * weight / Nd),
                        mlp_scales(m) + 9);
     SH_LAUNCH_CHECK("k_force_dy");
