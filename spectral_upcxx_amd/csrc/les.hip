@@ -114,6 +114,81 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_les_grad_x(const float2* __rest
     for_out<P>(tid, [&](int s, int pos) { zb[pos] = x[s]; });
 }
 
+// Steps 1 and 2's x and y passes in one kernel for one z-plane k of one packed
+// field m (block (k, m)): Z_m formed on load as k_les_grad_x does (with its
+// half-layout mirror reads), the inverse x transforms of the plane's n rows,
+// then the inverse y transforms of its n columns, through the plane held in
+// LDS (element (y, x) at y*LS + x + x/16); the plane is stored row by row.
+// Rows run with a line's threads adjacent, columns with adjacent columns, so
+// both phases address LDS without bank conflicts.
+template <class P>
+__global__ __launch_bounds__(P::TPL* P::N) void k_les_grad_xy(const float2* __restrict__ V, float2* __restrict__ Z,
+                                                              const float2* __restrict__ tw, int n, int half) {
+    constexpr int N = P::N;
+    constexpr int LS = N + N / 16 + 1;
+    __shared__ float pre[N * LS];
+    __shared__ float pim[N * LS];
+    const long N3 = (long)N * N * N;
+    const int kl = blockIdx.x, m = blockIdx.y;
+    auto at = [](int y, int x) { return y * LS + x + (x >> 4); };
+    float2 x[P::E];
+    {  // the x transform of row j = line (j, kl)
+        const int tid = threadIdx.x % P::TPL, jl = threadIdx.x / P::TPL;
+        const int a0 = (2 * m) / 3, b0 = (2 * m) % 3, a1 = (2 * m + 1) / 3, b1 = (2 * m + 1) % 3;
+        const bool two = m < 4;
+        const float ky = kd(jl, n), kz = kd(kl, n);
+        const bool mirror = half && 2 * kl > n;
+        const bool zero = half && 2 * kl == n;
+        const long src = mirror ? (long)(n - kl) * n + (n - jl) % n : (long)kl * n + jl;
+        const float2* v0 = V + a0 * N3 + src * N;
+        const float2* v1 = V + (two ? a1 : a0) * N3 + src * N;
+        auto vget = [&](const float2* v, int pos) {
+            if (zero) return make_float2(0.f, 0.f);
+            if (!mirror) return v[pos];
+            const float2 w = v[(N - pos) % N];
+            return make_float2(w.x, -w.y);
+        };
+        for_in<P>(tid, [&](int s, int pos) {
+            const float kx = kd(pos, N);
+            const float kb0 = b0 == 0 ? kx : b0 == 1 ? ky : kz;
+            const float2 g0 = ik(kb0, vget(v0, pos));
+            float2 z = g0;
+            if (two) {
+                const float kb1 = b1 == 0 ? kx : b1 == 1 ? ky : kz;
+                const float2 g1 = ik(kb1, vget(v1, pos));
+                z = make_float2(g0.x - g1.y, g0.y + g1.x);
+            }
+            x[s] = z;
+        });
+        auto ridx = [&](int pos) { return at(jl, pos); };
+        line_fft<P, 1, float>(x, tid, pre, pim, ridx, tw);
+        __syncthreads();
+        for_out<P>(tid, [&](int s, int pos) {
+            pre[ridx(pos)] = x[s].x;
+            pim[ridx(pos)] = x[s].y;
+        });
+    }
+    __syncthreads();
+    {  // the y transform of column i, in place in the plane
+        const int col = threadIdx.x % N, tid = threadIdx.x / N;
+        auto cidx = [&](int pos) { return at(pos, col); };
+        for_in<P>(tid, [&](int s, int pos) { x[s] = make_float2(pre[cidx(pos)], pim[cidx(pos)]); });
+        __syncthreads();  // a column's threads are in different waves: all read before any writes
+        line_fft<P, 1, float>(x, tid, pre, pim, cidx, tw);
+        __syncthreads();
+        for_out<P>(tid, [&](int s, int pos) {
+            pre[cidx(pos)] = x[s].x;
+            pim[cidx(pos)] = x[s].y;
+        });
+    }
+    __syncthreads();
+    float2* zp = Z + m * N3 + (long)kl * N * N;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int y = e / N, xx = e % N;
+        zp[e] = make_float2(pre[at(y, xx)], pim[at(y, xx)]);
+    }
+}
+
 // f_a = -sum_b i k_b tau_ab from the packed stress transforms P_c:
 // tau_{2c} = (P_c(k) + conj P_c(-k)) / 2, tau_{2c+1} = (P_c(k) - conj P_c(-k)) / 2i
 __global__ void k_les_div_unpack(const float2* __restrict__ Pt, long ts, float2* __restrict__ F, long fs, int n) {
@@ -561,21 +636,48 @@ int les_grad_x(const float2* V, float2* Z, int n, int half, hipStream_t st) {
     return rc;
 }
 
-// the y and z passes of the gradient fields (the z pass scales by 1/n^3)
-int les_grad_yz(float2* Z, int n, hipStream_t st) {
-    const long sxy = (long)n * n, N = sxy * n;
+template <class P>
+int launch_grad_xy(const float2* V, float2* Z, int n, int half, hipStream_t st) {
+    constexpr bool OK = 2L * 4 * P::N * (P::N + P::N / 16 + 1) <= 65536 && P::TPL * P::N <= 1024;
+    if constexpr (!OK) {
+        return SH_EUNSUPPORTED;
+    } else {
+        const float2* tw = (const float2*)twiddles(P::N, SH_F32);
+        if (!tw) return SH_ENOMEM;
+        hipLaunchKernelGGL((k_les_grad_xy<P>), dim3((unsigned)n, 5), dim3(P::TPL * P::N), 0, st, V, Z, tw, n, half);
+        SH_LAUNCH_CHECK("k_les_grad_xy");
+        return SH_OK;
+    }
+}
+
+// the gradient fields after their x and y passes: one plane kernel where the
+// plane fits in LDS (n <= 64), else the x pass and the y pass
+int les_grad_xy(const float2* V, float2* Z, int n, int half, hipStream_t st) {
+    int rc = SH_EUNSUPPORTED;
+    switch (n) {
+#define SH_CASE(n_, P_, L_, S_)                               \
+    case n_:                                                  \
+        rc = launch_grad_xy<P_>(V, Z, n, half, st);           \
+        break;
+        SH_FOR_EACH_FFT_SIZE(SH_CASE)
+#undef SH_CASE
+    }
+    if (rc != SH_EUNSUPPORTED) return rc;
+    RC(les_grad_x(V, Z, n, half, st));
+    const long sxy = (long)n * n;
     Epi<float> plain{1.f, 0, 1, -1, -1};
-    RC(pass_strided<float>(Z, n, n, 1, n, sxy, n, 5, N, +1, plain, st, ROLE_Y));
-    Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
-    return pass_strided<float>(Z, n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z);
+    return pass_strided<float>(Z, n, n, 1, n, sxy, n, 5, sxy * n, +1, plain, st, ROLE_Y);
 }
 
 // physical packed gradient fields Z (5 complex64 fields) of V (3 Fourier
-// fields, n^3 each)
+// fields, n^3 each; half: V in the half-spectrum layout)
 int les_gradients(const float2* V, float2* Z, int n, int half, hipStream_t st) {
     ProfScope ps("les_fft_inv", st);
-    RC(les_grad_x(V, Z, n, half, st));
-    return les_grad_yz(Z, n, st);
+    RC(les_grad_xy(V, Z, n, half, st));
+    // the z pass, scaled by 1/n^3
+    const long sxy = (long)n * n, N = sxy * n;
+    Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
+    return pass_strided<float>(Z, n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z);
 }
 
 template <class P>
@@ -688,13 +790,10 @@ int les_half(sh_les_s* h) {
 int sgs_force(sh_les_s* h, const float2* V, int half, float2* F, float2* H, int mode, hipStream_t st,
               hipEvent_t ev_read) {
     const int n = h->n;
-    const long sxy = (long)n * n;
     {
         ProfScope ps("les_fft_inv", st);
-        RC(les_grad_x(V, h->Z, n, half, st));
+        RC(les_grad_xy(V, h->Z, n, half, st));
         if (ev_read) SH_HIP(hipEventRecord(ev_read, st));
-        Epi<float> plain{1.f, 0, 1, -1, -1};
-        RC(pass_strided<float>(h->Z, n, n, 1, n, sxy, n, 5, h->N, +1, plain, st, ROLE_Y));
     }
     {
         ProfScope ps("les_mlp", st);
