@@ -73,6 +73,25 @@ def test_host_argument_errors_without_gpu():
     with pytest.raises(RuntimeError):
         L.sh_gradient_z(None, None, None, None, 8, 8, 8, None)
     assert "null" in L.error() or "invalid" in L.error()
+    # the solver, closure and tracer options reject what they do not define
+    with pytest.raises(RuntimeError, match="scheme must be"):
+        L.sh_ns_set_scheme(None, 2)
+    with pytest.raises(RuntimeError, match="mode must be"):
+        L.sh_ns_set_layout(None, 1)
+    with pytest.raises(RuntimeError, match="lag must be"):
+        L.sh_les_set_lag(None, 1)
+    with pytest.raises(RuntimeError, match="lag must be"):
+        L.sh_les_set_lag(None, 2)
+    with pytest.raises(RuntimeError, match="invalid argument"):
+        L.sh_les_pending_force(None, None)
+    with pytest.raises(RuntimeError, match="invalid argument"):
+        L.sh_les_force(None, None)
+    with pytest.raises(RuntimeError, match="invalid arguments"):
+        L.sh_particles_seed(None, 10, 1, 0, None)
+    with pytest.raises(RuntimeError, match="invalid arguments"):
+        L.sh_particles_move(None, None, None, -1, None, None, None, 8, 8, 8, 0.1, None)
+    with pytest.raises(RuntimeError, match="null argument"):
+        L.sh_vortex_ic_z(None, None, 8, 8, 8, 1, None)
 
 
 def test_cpp_api_built():
