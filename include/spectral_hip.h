@@ -372,6 +372,10 @@ int sh_les_steps(sh_les_t handle, int nsteps);
  * f(V)).  Any split of a run into sh_les_steps calls, and graph replays of a
  * call, give the same sequence. */
 int sh_les_set_lag(sh_les_t handle, int lag);
+/* mixed model: tau = tau_network (after clipping) - 2 (Cs Delta)^2 |S| S, the
+ * Smagorinsky stress of the resolved strain rate S = (g + g^T)/2, |S| =
+ * sqrt(2 S:S), Delta = 2 pi / n.  cs = 0 (default) is the network alone. */
+int sh_les_set_smagorinsky(sh_les_t handle, double cs);
 /* tune the solver's and the closure transforms' schedules on this grid
  * (after the initial condition is set; the state is kept).  Synchronous. */
 int sh_les_autotune(sh_les_t handle, int reps);

@@ -223,6 +223,17 @@ def clip(tau, g):
     return out
 
 
+def smagorinsky(tau, g, c):
+    """tau - 2 c |S| S: the Smagorinsky stress of the strain rate
+    S = (g + g^T)/2 added to tau ([n, 6], xx yy zz xy xz yz), |S| =
+    sqrt(2 S:S), c = (Cs Delta)^2.  Textbook eddy-viscosity closure, no
+    counterpart in the reference."""
+    s = np.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                  0.5 * (g[:, 5] + g[:, 7])], 1)
+    ss = (s[:, :3] ** 2).sum(1) + 2 * (s[:, 3:] ** 2).sum(1)
+    return tau - 2 * c * np.sqrt(2 * ss)[:, None] * s
+
+
 def _k(n):
     k = np.fft.fftfreq(n, 1.0 / n)
     k[n // 2] = 0.0  # the Nyquist wavenumber counts as zero in derivatives

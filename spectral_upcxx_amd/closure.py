@@ -116,6 +116,10 @@ class LES:
     def set_lag(self, lag):
         lib().sh_les_set_lag(self.h, int(lag))
 
+    def set_smagorinsky(self, cs):
+        """mixed model: the network's stress plus the Smagorinsky stress with constant cs"""
+        lib().sh_les_set_smagorinsky(self.h, float(cs))
+
     def close(self):
         if self.h:
             lib().sh_les_destroy(self.h)

@@ -26,8 +26,9 @@ namespace sh {
 
 template <typename T>
 int fft3d(void* data, int count, long fstride, int nx, int ny, int nz, int dir, int flags, void* stream);
-int mlp_infer_packed(sh_mlp_t h, const float2* X, long ld, long n, float2* Y, long ldy, int clip, hipStream_t st);
-int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float* Y, long ldy, int clip,
+int mlp_infer_packed(sh_mlp_t h, const float2* X, long ld, long n, float2* Y, long ldy, int clip, float smag,
+                     hipStream_t st);
+int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float* Y, long ldy, int clip, float smag,
                             hipStream_t st);
 void* ns_fused_velocity(sh_ns_t s);
 int ns_step_phase(sh_ns_t s, int phase);
@@ -375,7 +376,8 @@ template <class P>
 __global__ __launch_bounds__(ZMlp<P>::THREADS) void k_les_z_mlp(const float2* __restrict__ Z, float2* __restrict__ T,
                                                                 const float2* __restrict__ tw, int n,
                                                                 const float* __restrict__ Pm,
-                                                                const float* __restrict__ sc, int clip) {
+                                                                const float* __restrict__ sc, int clip,
+                                                                float smag) {
     using Z_ = ZMlp<P>;
     constexpr int N = P::N, LPB = Z_::LPB, CS = Z_::CS, S = Z_::S, GS = Z_::GS;
     constexpr int GT = P::TPL * LPB;  // threads per group (64)
@@ -453,6 +455,7 @@ __global__ __launch_bounds__(ZMlp<P>::THREADS) void k_les_z_mlp(const float2* __
             for (int half = 0; half < 2; half++) {
                 f32x4 y = (half ? yb : ya) * sc_out;
                 if (clip) y = clip_backscatter(y, half ? rb : ra, r);
+                if (smag > 0.f) y = smagorinsky_add(y, half ? rb : ra, r, smag);
                 if (q < 2)
                     for (int jj = 0; jj < 4; jj++)
                         if (4 * q + jj < kOut) G[(4 * q + jj) * GS + s00 + 16 * half + r] = y[jj];
@@ -693,7 +696,8 @@ int launch_x_div(const float2* T, float2* F, float2* H, int n, int mode, hipStre
 }
 
 template <class P>
-int launch_z_mlp(const float2* Z, float2* T, int n, const float* Pm, const float* sc, int clip, hipStream_t st) {
+int launch_z_mlp(const float2* Z, float2* T, int n, const float* Pm, const float* sc, int clip, float smag,
+                 hipStream_t st) {
     if constexpr (!ZMlp<P>::OK) {
         return SH_EUNSUPPORTED;
     } else {
@@ -701,7 +705,7 @@ int launch_z_mlp(const float2* Z, float2* T, int n, const float* Pm, const float
         if (!tw) return SH_ENOMEM;
         const int tiles = (n + ZMlp<P>::LPB - 1) / ZMlp<P>::LPB;
         hipLaunchKernelGGL((k_les_z_mlp<P>), dim3((unsigned)(tiles * n)), dim3(ZMlp<P>::THREADS), 0, st, Z, T, tw,
-                           n, Pm, sc, clip);
+                           n, Pm, sc, clip, smag);
         SH_LAUNCH_CHECK("k_les_z_mlp");
         return SH_OK;
     }
@@ -710,12 +714,12 @@ int launch_z_mlp(const float2* Z, float2* T, int n, const float* Pm, const float
 // gradients after their x and y passes -> packed stresses after their z
 // pass: one fused kernel where its LDS fits (n = 8, 16, 32, 64), else the
 // z pass, the network and the z pass as separate launches
-int les_z_mlp(sh_mlp_t m, const float2* Z, float2* T, int n, int clip, hipStream_t st) {
+int les_z_mlp(sh_mlp_t m, const float2* Z, float2* T, int n, int clip, float smag, hipStream_t st) {
     int rc = SH_EUNSUPPORTED;
     switch (n) {
 #define SH_CASE(n_, P_, L_, S_)                                                       \
     case n_:                                                                          \
-        rc = launch_z_mlp<P_>(Z, T, n, mlp_params(m), mlp_scales(m), clip, st);      \
+        rc = launch_z_mlp<P_>(Z, T, n, mlp_params(m), mlp_scales(m), clip, smag, st); \
         break;
         SH_FOR_EACH_FFT_SIZE(SH_CASE)
 #undef SH_CASE
@@ -724,7 +728,7 @@ int les_z_mlp(sh_mlp_t m, const float2* Z, float2* T, int n, int clip, hipStream
     const long sxy = (long)n * n, N = sxy * n;
     Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
     RC(pass_strided<float>(const_cast<float2*>(Z), n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z));
-    RC(mlp_infer_packed(m, Z, N, N, T, N, clip, st));
+    RC(mlp_infer_packed(m, Z, N, N, T, N, clip, smag, st));
     Epi<float> plain{1.f, 0, 1, -1, -1};
     return pass_strided<float>(T, n, n, 1, n, n, sxy, 3, N, -1, plain, st, ROLE_Z);
 }
@@ -758,6 +762,7 @@ struct sh_les_s {
     long N;
     sh_mlp_t model;
     int clip;
+    float smag;  // (Cs Delta)^2 of the Smagorinsky stress added to the network's, 0: none
     hipStream_t st;
     sh_ns_t ns;
     // force slots (3 fields each; slot 0 is the one the solver reads between
@@ -797,7 +802,7 @@ int sgs_force(sh_les_s* h, const float2* V, int half, float2* F, float2* H, int 
     }
     {
         ProfScope ps("les_mlp", st);
-        RC(les_z_mlp(h->model, h->Z, h->T, n, h->clip, st));
+        RC(les_z_mlp(h->model, h->Z, h->T, n, h->clip, h->smag, st));
     }
     ProfScope ps("les_fft_fwd", st);
     return les_force_yx(h->T, F, H, n, mode, st);
@@ -952,6 +957,16 @@ int sh_les_steps(sh_les_t h, int nsteps) {
     return les_run(h, nsteps);
 }
 
+int sh_les_set_smagorinsky(sh_les_t h, double cs) {
+    if (!h || !(cs >= 0) || !h->model) {
+        set_error("sh_les_set_smagorinsky: invalid argument (an LES with a model, Cs >= 0)");
+        return SH_EINVAL;
+    }
+    const double delta = 2.0 * M_PI / h->n;  // the coarse grid spacing, the filter width
+    h->smag = (float)((cs * delta) * (cs * delta));
+    return SH_OK;
+}
+
 int sh_les_set_lag(sh_les_t h, int lag) {
     if (!h || lag < 0 || lag > 1) {
         set_error("sh_les_set_lag: lag must be 0 (synchronous closure) or 1 (lagged, extrapolated)");
@@ -1010,7 +1025,7 @@ int sh_les_stress(sh_les_t h, void* out_tau6) {
         return SH_EINVAL;
     }
     RC(les_gradients((const float2*)ns_fused_velocity(h->ns), h->Z, h->n, les_half(h), h->st));
-    return mlp_infer_packed_planar(h->model, h->Z, h->N, h->N, (float*)out_tau6, h->N, h->clip, h->st);
+    return mlp_infer_packed_planar(h->model, h->Z, h->N, h->N, (float*)out_tau6, h->N, h->clip, h->smag, h->st);
 }
 
 int sh_les_force(sh_les_t h, void* out_f3) {

@@ -17,7 +17,7 @@ namespace sh {
 template <bool PACKED_IN, bool PACKED_OUT>
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, const float* __restrict__ X,
                                                    long ld, long n, const float* __restrict__ sc,
-                                                   float* __restrict__ Y, long ldy, int clip) {
+                                                   float* __restrict__ Y, long ldy, int clip, float smag) {
     const int lane = threadIdx.x % 64;
     const int r = lane % 16, q = lane / 16;
     const long wave = (blockIdx.x * (long)blockDim.x + threadIdx.x) / 64;
@@ -44,6 +44,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P, 
             const long s = s00 + 16 * half + r;
             f32x4 y = (half ? yb : ya) * sc_out;
             if (clip) y = clip_backscatter(y, half ? rb : ra, r);
+            if (smag > 0.f) y = smagorinsky_add(y, half ? rb : ra, r, smag);
             if (s < n && q < 2) {
                 if constexpr (PACKED_OUT) {
                     float2* Y2 = reinterpret_cast<float2*>(Y);
@@ -265,24 +266,24 @@ struct sh_mlp_s {
 
 namespace sh {
 // the LES feeds packed complex gradient fields and takes packed stresses
-int mlp_infer_packed(sh_mlp_t h, const float2* X, long ld, long n, float2* Y, long ldy, int clip,
+int mlp_infer_packed(sh_mlp_t h, const float2* X, long ld, long n, float2* Y, long ldy, int clip, float smag,
                      hipStream_t st) {
     hipLaunchKernelGGL((k_mlp_infer<true, true>), dim3(infer_blocks(n)), dim3(256), 0, st, h->P,
-                       (const float*)X, ld, n, h->scale, (float*)Y, ldy, clip);
+                       (const float*)X, ld, n, h->scale, (float*)Y, ldy, clip, smag);
     SH_LAUNCH_CHECK("k_mlp_infer");
     return SH_OK;
 }
-int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float* Y, long ldy, int clip,
+int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float* Y, long ldy, int clip, float smag,
                             hipStream_t st) {
     hipLaunchKernelGGL((k_mlp_infer<true, false>), dim3(infer_blocks(n)), dim3(256), 0, st, h->P,
-                       (const float*)X, ld, n, h->scale, Y, ldy, clip);
+                       (const float*)X, ld, n, h->scale, Y, ldy, clip, smag);
     SH_LAUNCH_CHECK("k_mlp_infer");
     return SH_OK;
 }
 // planar features in, packed stresses out (the force-matching loss)
 int mlp_infer_planar_packed(sh_mlp_t h, const float* X, long ld, long n, float2* Y, long ldy, hipStream_t st) {
     hipLaunchKernelGGL((k_mlp_infer<false, true>), dim3(infer_blocks(n)), dim3(256), 0, st, h->P, X, ld, n,
-                       h->scale, (float*)Y, ldy, 0);
+                       h->scale, (float*)Y, ldy, 0, 0.f);
     SH_LAUNCH_CHECK("k_mlp_infer");
     return SH_OK;
 }
@@ -424,7 +425,7 @@ int sh_mlp_forward(sh_mlp_t h, const void* X, long ld, long n, void* Y, long ldy
     }
     if (n == 0) return SH_OK;
     hipLaunchKernelGGL((k_mlp_infer<false, false>), dim3(infer_blocks(n)), dim3(256), 0, h->st, h->P,
-                       (const float*)X, ld, n, h->scale, (float*)Y, ldy, 0);
+                       (const float*)X, ld, n, h->scale, (float*)Y, ldy, 0, 0.f);
     SH_LAUNCH_CHECK("k_mlp_infer");
     return SH_OK;
 }

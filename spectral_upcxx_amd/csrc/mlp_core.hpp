@@ -235,4 +235,23 @@ __device__ __forceinline__ f32x4 clip_backscatter(f32x4 y, f32x4 raw, int r) {
     return ts > 0.f ? f32x4{0, 0, 0, 0} : y;
 }
 
+// Smagorinsky stress added to one D tile of stresses: y_o -= 2 c |S| S_o for
+// the outputs o = 4q + j < 6 this lane holds (xx yy zz xy xz yz), c =
+// (Cs Delta)^2, S the strain rate of the sample's nine unscaled gradients
+// (gathered across the lanes as clip_backscatter does), |S| = sqrt(2 S:S).
+// Every lane gathers, so the whole wave must call it.
+__device__ __forceinline__ f32x4 smagorinsky_add(f32x4 y, f32x4 raw, int r, float c) {
+    const int q = (threadIdx.x % 64) / 16;
+    float g[9];
+    for (int f = 0; f < 9; f++) g[f] = __shfl(raw[f & 3], r + 16 * (f >> 2), 64);
+    const float s[6] = {g[0], g[4], g[8], 0.5f * (g[1] + g[3]), 0.5f * (g[2] + g[6]), 0.5f * (g[5] + g[7])};
+    const float ss = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2.f * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+    const float nut = c * sqrtf(2.f * ss);
+    for (int j = 0; j < 4; j++) {
+        const float so = q == 0 ? s[j] : (q == 1 && j < 2) ? s[4 + j] : 0.f;
+        y[j] -= 2.f * nut * so;
+    }
+    return y;
+}
+
 }  // namespace sh

@@ -300,7 +300,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
               variants=((True, 1), (False, 1), (True, 2), (False, 2)), eval_dt=0.5, kcut=16, net_seed=5,
               gain_candidates=(0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0), force_iters=2000, force_lr=1e-3,
               force_variants=(("force", False, 0.0, 1), ("force_exp", False, 0.0, 2)),
-              log=None):
+              cs_candidates=(0.0, 0.05, 0.1), log=None):
     """Learned-closure form: FP64 DNS at n_dns^3 box-filtered by `ratio`
     gives training pairs (filtered velocity gradients -> stresses); the
     network is trained on the GPU; the FP32 LES on the coarse grid with and
@@ -495,8 +495,10 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
             out[t] = spectrum(physical_to_fourier(uf), nu, nb)
         return out
 
-    def trajectory(ub0, mdl, clip, dealias, lag=0):
+    def trajectory(ub0, mdl, clip, dealias, lag=0, cs=0.0):
         les = closure.LES(n, nu, les_dt, mdl, clip=clip, dealias=dealias, lag=lag)
+        if mdl is not None and cs > 0:
+            les.set_smagorinsky(cs)
         les.set_initial(ub0)
         cur = every  # after the start-up step the LES is at DNS step `every`
         Es = {}
@@ -541,18 +543,22 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         for clip_v, dealias_v in variants:
             for gain in gain_candidates:
                 model.set_scaling(in_scale, out_scale * gain)
-                s = sum(score(compare(trajectory(ub, model, clip_v, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
-                scores[(cname, clip_v, dealias_v, gain)] = s
+                for cs in cs_candidates:
+                    s = sum(score(compare(trajectory(ub, model, clip_v, dealias_v, cs=cs), Fs))
+                            for ub, Fs in calib) / len(calib)
+                    scores[(cname, clip_v, dealias_v, gain, cs)] = s
             best_c = min((k for k in scores if k[:3] == (cname, clip_v, dealias_v)), key=scores.get)
-            log("calibration %s clip=%d dealias=%d: best gain %.2f mean rel err %.4f" % (
-                cname, clip_v, dealias_v, best_c[3], scores[best_c]))
+            log("calibration %s clip=%d dealias=%d: best gain %.2f Cs %.2f mean rel err %.4f" % (
+                cname, clip_v, dealias_v, best_c[3], best_c[4], scores[best_c]))
     for dealias_v in sorted({d for _, d in variants}):
         s = sum(score(compare(trajectory(ub, None, False, dealias_v), Fs)) for ub, Fs in calib) / len(calib)
         scores[("no_model", dealias_v)] = s
         log("calibration no-model dealias=%d: mean rel err %.4f" % (dealias_v, s))
     best = min((k for k in scores if k[0] != "no_model"), key=scores.get)
-    chosen, clip, dealias, best_gain = best
-    log("chosen: %s clip=%d dealias=%d gain %.2f" % best)
+    chosen, clip, dealias, best_gain, best_cs = best
+    log("chosen: %s clip=%d dealias=%d gain %.2f Cs %.2f" % best)
+    # the best choice without the eddy-viscosity term, for comparison
+    best0 = min((k for k in scores if k[0] != "no_model" and k[4] == 0), key=scores.get)
     model.set_params(params[chosen])
     model.set_scaling(in_scale, out_scale * best_gain)
     for tr in train:
@@ -561,10 +567,17 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
     for te in test:
         ub0 = te["snaps"][0][:3].reshape(3, n, n, n).contiguous()
         Fs = fdns_spectra(te["snaps"])
-        res = {"seed": te["seed"], "model": compare(trajectory(ub0, model, clip, dealias), Fs),
-               "model_lagged": compare(trajectory(ub0, model, clip, dealias, lag=1), Fs),
+        res = {"seed": te["seed"]}
+        if best_cs > 0:
+            model.set_params(params[best0[0]])
+            model.set_scaling(in_scale, out_scale * best0[3])
+            res["model_cs0"] = compare(trajectory(ub0, model, best0[1], best0[2]), Fs)
+            model.set_params(params[chosen])
+            model.set_scaling(in_scale, out_scale * best_gain)
+        res.update({"model": compare(trajectory(ub0, model, clip, dealias, cs=best_cs), Fs),
+               "model_lagged": compare(trajectory(ub0, model, clip, dealias, lag=1, cs=best_cs), Fs),
                "no_model": compare(trajectory(ub0, None, False, dealias), Fs),
-               "no_model_plain": compare(trajectory(ub0, None, False, False), Fs)}
+               "no_model_plain": compare(trajectory(ub0, None, False, False), Fs)})
         evals.append(res)
         log("les seed %d: " % te["seed"] + ", ".join(
             "t=%.2f model %.3f lagged %.3f no-model %.3f" % (a["t"], a[key], c[key], b[key])
@@ -581,6 +594,8 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
             m2.set_params(model.params())
             m2.set_scaling(in_scale, out_scale * best_gain)
         les = closure.LES(n, nu, les_dt, m2, clip=clip, strm=st, dealias=dealias, lag=lag)
+        if m2 is not None and best_cs > 0:
+            les.set_smagorinsky(best_cs)
         torch.cuda.synchronize()
         les.set_initial(ub0)
         les.steps(2)
@@ -609,6 +624,7 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         "dt_les": les_dt, "t_end": t_end, "k0": k0, "train_seeds": [t["seed"] for t in train],
         "test_seeds": [t["seed"] for t in test], "dns_all_finite": all(t["finite"] for t in train + test),
         "training_samples": M, "iters": iters, "batch": batch, "lr": lr, "train_s": train_s,
+        "smagorinsky_cs": best_cs, "best_without_smagorinsky": str(best0),
         "loss_history": losses, "force_matching": force_log, "network": chosen, "clip": clip, "dealias": dealias,
         "a_priori_corr_by_component": [round(c, 4) for c in corr[:6]],
         "a_priori_corr_mean": sum(corr) / len(corr),
@@ -619,6 +635,10 @@ def les_study(n_dns=256, ratio=4, nu=1.0 / 800, dt_dns=5e-4, les_dt=2e-3, t_end=
         "final_%s_model_lagged_mean" % key: sum(r["model_lagged"][-1][key] for r in evals) / len(evals),
         "mean_rel_err_model": sum(score(r["model"]) for r in evals) / len(evals),
         "mean_rel_err_model_lagged": sum(score(r["model_lagged"]) for r in evals) / len(evals),
+        "mean_rel_err_model_cs0": (sum(score(r["model_cs0"]) for r in evals) / len(evals)
+                                   if best_cs > 0 else None),
+        "final_%s_model_cs0_mean" % key: (sum(r["model_cs0"][-1][key] for r in evals) / len(evals)
+                                          if best_cs > 0 else None),
         "mean_rel_err_no_model": sum(score(r["no_model"]) for r in evals) / len(evals),
         "mean_rel_err_no_model_plain": sum(score(r["no_model_plain"]) for r in evals) / len(evals),
         "dns_fp64_steps_per_s": dns_sps, "les_model_steps_per_s": speed["model"],

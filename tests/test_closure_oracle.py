@@ -238,3 +238,28 @@ def test_box_filter_direct_sums():
     assert np.isclose(out[3 + 3, c], (blk[0] * blk[1]).mean() - blk[0].mean() * blk[1].mean(), rtol=1e-12)
     # the stress trace is a sum of variances
     assert np.all(out[3] + out[4] + out[5] >= 0)
+
+
+def test_smagorinsky_closed_forms():
+    """The mixed model's eddy-viscosity term: a pure shear g_xy = gamma gives
+    S_xy = gamma/2, |S| = |gamma|, so tau_xy = -c |gamma| gamma and nothing
+    else; for trace-free gradients the added stress is trace-free and its
+    work on the strain is tau:S = -c |S|^3 (S:S = |S|^2 / 2)."""
+    rng = np.random.default_rng(3)
+    c = 0.013
+    gam = rng.normal(size=5)
+    g = np.zeros((5, 9))
+    g[:, 1] = gam
+    t = C.smagorinsky(np.zeros((5, 6)), g, c)
+    assert np.allclose(t[:, 3], -c * np.abs(gam) * gam, rtol=1e-14, atol=0)
+    assert not t[:, [0, 1, 2, 4, 5]].any()
+    g = rng.normal(size=(200, 9))
+    g[:, 8] = -(g[:, 0] + g[:, 4])
+    base = rng.normal(size=(200, 6))
+    t = C.smagorinsky(base, g, c) - base
+    assert np.abs(t[:, :3].sum(1)).max() < 1e-13
+    s = np.stack([g[:, 0], g[:, 4], g[:, 8], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                  0.5 * (g[:, 5] + g[:, 7])], 1)
+    work = (t[:, :3] * s[:, :3]).sum(1) + 2 * (t[:, 3:] * s[:, 3:]).sum(1)
+    mag = np.sqrt(2 * ((s[:, :3] ** 2).sum(1) + 2 * (s[:, 3:] ** 2).sum(1)))
+    assert np.allclose(work, -c * mag ** 3, rtol=1e-12)

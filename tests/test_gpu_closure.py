@@ -255,18 +255,23 @@ def _les_with_model(hip, n, P, sc_in, sc_out, clip, nu=0.01, K=1e-3):
     return m, les
 
 
-@pytest.mark.parametrize("clip,n", [(False, 32), (True, 32), (True, 16), (True, 64), (True, 24)])
-def test_les_stress_and_force_match_oracle(hip, clip, n):
+@pytest.mark.parametrize("clip,n,cs", [(False, 32, 0), (True, 32, 0), (True, 16, 0), (True, 64, 0), (True, 24, 0),
+                                       (True, 32, 0.17), (True, 24, 0.17)])
+def test_les_stress_and_force_match_oracle(hip, clip, n, cs):
     """n = 8..64 run the fused z/network kernel, n = 24 the separate launches.
     The stress comes from the separate launches; the force pipeline is
     checked on it to 1e-5 where the fused kernel rounds its transforms as
     they do (n = 16, 32: the same bits).  At n = 64 the two kernels' FP32
     transforms round differently and a few BF16 activations flip, which moves
-    the force by about 1.2e-4; a wrong sample would move it by > 1e-3."""
+    the force by about 1.2e-4; a wrong sample would move it by > 1e-3.
+    cs > 0: the mixed model, the Smagorinsky stress added after clipping
+    (fused kernel at 32, separate launches at 24)."""
     P = _params(12)
     sc_in = np.full(9, 0.3, np.float32)
     sc_out = np.full(6, 0.05, np.float32)
     m, les = _les_with_model(hip, n, P, sc_in, sc_out, clip)
+    if cs:
+        les.set_smagorinsky(cs)
     u = _physical_field(n, 41)
     les.set_initial(_dev(u.astype(np.float32)))
     V = _host(les.velocity())
@@ -276,9 +281,18 @@ def test_les_stress_and_force_match_oracle(hip, clip, n):
     if clip:
         tau = C.clip(tau, g)
         assert (np.abs(tau).sum(1) == 0).mean() > 0.05  # some points are clipped
+    clipped = np.abs(tau).sum(1) == 0
+    if cs:
+        tau = C.smagorinsky(tau, g, (cs * 2 * np.pi / n) ** 2)
     got = _host(les.stress())
     keep = np.ones(n ** 3, dtype=bool)
-    if clip:
+    if clip and cs:
+        # compare where the clip decisions agree (the eddy-viscosity part is
+        # the same either way; it is what remains at clipped points)
+        net = got.T - (C.smagorinsky(np.zeros((n ** 3, 6)), g, (cs * 2 * np.pi / n) ** 2))
+        keep = (np.abs(net).sum(1) < 1e-3 * np.abs(got).max()) == clipped
+        assert (~keep).mean() < 1e-3
+    elif clip:
         # the clip decision may differ where tau:S rounds across zero
         zg = np.abs(got.T).sum(1) == 0
         zo = np.abs(tau).sum(1) == 0
