@@ -59,7 +59,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_contig(C2<T>* __restrict__ 
     for_out<P>(tid, [&](int slot, int pos) { base[pos] = epilogue<T, P>(x[slot], pos, zline, ep); });
 }
 
-template <typename T, class P, int LPB, int SIGN>
+template <typename T, class P, int LPB, int SIGN, bool NT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_fft_strided(C2<T>* __restrict__ data,
                                                              const C2<T>* __restrict__ tw,
                                                              int ncols, long cdist, int nouter,
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_strided(C2<T>* __restrict__
     line_fft<P, SIGN, T>(x, tid, sre, sim, idx, tw);
     if (!valid) return;
     const bool zline = (ep.flags & 2) && (col == ep.za || o == ep.zb);
-    for_out<P>(tid, [&](int slot, int pos) { base[pos * es] = epilogue<T, P>(x[slot], pos, zline, ep); });
+    for_out<P>(tid, [&](int slot, int pos) { st_line<NT, T>(base + pos * es, epilogue<T, P>(x[slot], pos, zline, ep)); });
 }
 
 // ---------------------------------------------------------------- launchers
@@ -105,7 +105,7 @@ static int launch_contig(C2<T>* data, long nlines, long ldist, int count, long f
     return SH_OK;
 }
 
-template <typename T, class P, int LPB>
+template <typename T, class P, int LPB, bool NT = false>
 static int launch_strided(C2<T>* data, int ncols, long cdist, int nouter, long odist, long es,
                           int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
@@ -117,10 +117,10 @@ static int launch_strided(C2<T>* data, int ncols, long cdist, int nouter, long o
     dim3 grid((unsigned)(tiles * nouter), (unsigned)count);
     dim3 block(P::TPL * LPB);
     if (dir < 0)
-        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, -1>), grid, block, 0, st, data, tw, ncols, cdist,
+        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, -1, NT>), grid, block, 0, st, data, tw, ncols, cdist,
                            nouter, odist, es, fstride, ep);
     else
-        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, 1>), grid, block, 0, st, data, tw, ncols, cdist,
+        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, 1, NT>), grid, block, 0, st, data, tw, ncols, cdist,
                            nouter, odist, es, fstride, ep);
     SH_LAUNCH_CHECK("k_fft_strided");
     return SH_OK;
@@ -164,6 +164,11 @@ const VariantList<StridedLaunch<T>>& strided_variants(int n) {
     m[n_].fns.push_back(&launch_strided<T, P_, C_>);
         SH_STRIDED_VARIANTS(SH_V)
 #undef SH_V
+#define SH_VT(n_, P_, C_)                                             \
+    m[n_].names.push_back(#P_ " columns=" #C_ " nontemporal stores"); \
+    m[n_].fns.push_back(&launch_strided<T, P_, C_, true>);
+        SH_STRIDED_NT_VARIANTS(SH_VT)
+#undef SH_VT
         return m;
     }();
     static const VariantList<StridedLaunch<T>> none;

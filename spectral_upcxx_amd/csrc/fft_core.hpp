@@ -197,6 +197,20 @@ __device__ __forceinline__ void lds_get(C2<T>* x, int tid, const T* re, const T*
     }
 }
 
+// a complex store; NT: with the nontemporal hint (one 16- or 8-byte vector
+// store that does not allocate in the caches: the line is not read again
+// before it would be evicted)
+template <bool NT, typename T>
+__device__ __forceinline__ void st_line(C2<T>* p, C2<T> v) {
+    if constexpr (NT) {
+        typedef T v2 __attribute__((ext_vector_type(2)));
+        const v2 w = {v.x, v.y};
+        __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 // Whole-line transform.  On entry x holds the stage-0 inputs (see for_in),
 // on exit the outputs (see for_out).  If the LDS arrays were used before the
 // call, the caller must __syncthreads() first.
@@ -374,6 +388,8 @@ inline bool fft_size_supported(int n) {
 using X512 = FftPlan<512, 32, 16, 2, 16>;
 
 // strided pass: X(n, plan, columns per block)
+// the same with nontemporal stores (X(n, plan, columns per block))
+#define SH_STRIDED_NT_VARIANTS(X) X(256, Q256, 16) X(256, Q256, 8) X(256, W256, 16) X(512, X512, 16) X(512, P512, 8)
 #define SH_STRIDED_VARIANTS(X) \
     X(64, P64, 16)             \
     X(64, P64, 32)             \

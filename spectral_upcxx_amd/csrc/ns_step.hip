@@ -205,7 +205,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_z(C2<T>* __restrict__ 
 // The cross product acts on the real and the imaginary parts separately (two
 // grid points); the imaginary parts the reference's complex product carries
 // are the rounding noise of real fields and are not formed.
-template <typename T, class P, int LPB>
+template <typename T, class P, int LPB, bool NT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ tw,
                                                              const C2<T>* __restrict__ twn, int nx,
@@ -310,7 +310,8 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__
                 const C2<T> w = twn[pos];
                 const T tx = w.x * dif.x - w.y * dif.y;  // t = w dif
                 const T ty = w.x * dif.y + w.y * dif.x;
-                W[cb + (unsigned)d * uF + (unsigned)pos * es] = mk<T>(T(0.5) * (sum.x + ty), T(0.5) * (sum.y - tx));
+                st_line<NT, T>(W + (cb + (unsigned)d * uF + (unsigned)pos * es),
+                               mk<T>(T(0.5) * (sum.x + ty), T(0.5) * (sum.y - tx)));
             });
     }
 }
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(256) void k_herm_dev(const double2* __restrict__ f,
 // BL: the line's V and N0 stores are issued after the whole update, so no
 // slot's V1/N0 loads wait behind an earlier slot's stores (vector-memory
 // counters count loads and stores together)
-template <typename T, class P, int LPB, bool EMIT, bool BL = false>
+template <typename T, class P, int LPB, bool EMIT, bool BL = false, bool NT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
@@ -506,8 +507,8 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
                 if constexpr (BL) {
                     nna[d][s] = nn[d];
                 } else {
-                    V[d * F + off + pos] = vn[d][s];
-                    N0[d * F + off + pos] = nn[d];
+                    st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
+                    st_line<NT, T>(N0 + d * F + off + pos, nn[d]);
                 }
             }
         });
@@ -515,8 +516,8 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             for_out<P>(tid, [&](int s, int pos) {
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    V[d * F + off + pos] = vn[d][s];
-                    N0[d * F + off + pos] = nna[d][s];
+                    st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
+                    st_line<NT, T>(N0 + d * F + off + pos, nna[d][s]);
                 }
             });
     }
@@ -537,7 +538,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             });
             __syncthreads();
             line_fft<P, 1, T>(x, tid, sre, sim, idx, tw);
-            if (valid) for_out<P>(tid, [&](int s, int pos) { W[g * F + off + pos] = x[s]; });
+            if (valid) for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + g * F + off + pos, x[s]); });
         }
     }
     // block sum of |div V| (FP64), deterministic; reuses the FFT's LDS
@@ -898,7 +899,7 @@ int launch_cross_z(sh_ns_s* s) {
 }
 
 // kernel C of the half-spectrum layout: P is the length-nz/2 plan
-template <typename T, class P, int LPB>
+template <typename T, class P, int LPB, bool NT = false>
 int launch_cross_zh(sh_ns_s* s) {
     const int prec = sizeof(T) == 8 ? SH_F64 : SH_F32;
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, prec);
@@ -919,7 +920,7 @@ int launch_cross_zh(sh_ns_s* s) {
     dim3 grid((unsigned)(tiles * s->ny), (unsigned)s->nboxes), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
     ProfScope ps("ns_cross_z", s->cst);
-    hipLaunchKernelGGL((k_ns_cross_zh<T, P, LPB>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0, tw, twn,
+    hipLaunchKernelGGL((k_ns_cross_zh<T, P, LPB, NT>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0, tw, twn,
                        s->nx, s->cnx, s->ny, s->N * s->nboxes, scale, s->N);
     SH_LAUNCH_CHECK("k_ns_cross_zh");
     return SH_OK;
@@ -927,7 +928,7 @@ int launch_cross_zh(sh_ns_s* s) {
 
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
-template <typename T, class P, int LPB, bool EMIT = true, bool BL = false>
+template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false>
 int launch_fourier_x(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
@@ -939,7 +940,7 @@ int launch_fourier_x(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL>), dim3(nb, (unsigned)s->nboxes),
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
                        (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, nzp, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
@@ -980,6 +981,9 @@ int launch_fourier_x(sh_ns_s* s) {
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
     X(128, W64, 16) X(128, P64, 16) X(128, W64, 8) X(256, W128, 16) X(256, W128, 8)               \
     X(256, Q128, 16) X(256, Q128, 8) X(512, W256, 8) X(512, W256, 4) X(512, Q256, 8) X(512, Q256, 4)
+// kernel C with nontemporal stores of the three products
+#define SH_CROSSH_F64_NT_VARIANTS(X) X(256, W128, 16) X(256, W128, 8)
+#define SH_CROSSH_F32_NT_VARIANTS(X) X(256, Q128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -989,6 +993,9 @@ int launch_fourier_x(sh_ns_s* s) {
 // kernel E with the line's V and N0 stores after the whole update
 #define SH_FOURIER_F64_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, W256, 2) X(256, W256, 4) X(512, W512, 2)
 #define SH_FOURIER_F32_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, Q256, 8) X(256, W256, 4)
+// kernel E with nontemporal stores (V, N0 and the emitted fields)
+#define SH_FOURIER_F64_NT_VARIANTS(X) X(256, W256, 2) X(256, W256, 4)
+#define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
@@ -1030,11 +1037,17 @@ const NsVariants& crossh_variants(int n) {
 #define SH_V(n_, P_, C_)                                \
     m[n_].names.push_back(#P_ " half columns=" #C_);    \
     m[n_].fns.push_back(&launch_cross_zh<T, P_, C_>);
+#define SH_VT(n_, P_, C_)                                               \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " nontemporal stores");  \
+    m[n_].fns.push_back(&launch_cross_zh<T, P_, C_, true>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSSH_F64_VARIANTS(SH_V)
+            SH_CROSSH_F64_NT_VARIANTS(SH_VT)
         } else {
             SH_CROSSH_F32_VARIANTS(SH_V)
+            SH_CROSSH_F32_NT_VARIANTS(SH_VT)
         }
+#undef SH_VT
 #undef SH_V
         return m;
     }();
@@ -1062,15 +1075,21 @@ const NsVariants& fourier_variants(int n) {
 #define SH_VN(n_, P_, L_)                                  \
     m[n_].names.push_back(#P_ " lines=" #L_ " no-emit");   \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, false>);
+#define SH_VT(n_, P_, L_)                                             \
+    m[n_].names.push_back(#P_ " lines=" #L_ " nontemporal stores");   \
+    m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, true, false, true>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F64_BATCHED_VARIANTS(SH_VB)
+            SH_FOURIER_F64_NT_VARIANTS(SH_VT)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F32_BATCHED_VARIANTS(SH_VB)
+            SH_FOURIER_F32_NT_VARIANTS(SH_VT)
         }
+#undef SH_VT
 #undef SH_V
 #undef SH_VN
 #undef SH_VB
