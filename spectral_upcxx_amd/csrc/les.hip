@@ -712,9 +712,10 @@ int launch_z_mlp(const float2* Z, float2* T, int n, const float* Pm, const float
 }
 
 // gradients after their x and y passes -> packed stresses after their z
-// pass: one fused kernel where its LDS fits (n = 8, 16, 32, 64), else the
-// z pass, the network and the z pass as separate launches
-int les_z_mlp(sh_mlp_t m, const float2* Z, float2* T, int n, int clip, float smag, hipStream_t st) {
+// pass: one fused kernel where its LDS fits (n = 8, 16, 32, 64; Z is only
+// read), else the z pass (in place in Z), the network and the z pass as
+// separate launches
+int les_z_mlp(sh_mlp_t m, float2* Z, float2* T, int n, int clip, float smag, hipStream_t st) {
     int rc = SH_EUNSUPPORTED;
     switch (n) {
 #define SH_CASE(n_, P_, L_, S_)                                                       \
@@ -727,7 +728,7 @@ int les_z_mlp(sh_mlp_t m, const float2* Z, float2* T, int n, int clip, float sma
     if (rc != SH_EUNSUPPORTED) return rc;
     const long sxy = (long)n * n, N = sxy * n;
     Epi<float> last{(float)(1.0 / (double)N), 1, 1, -1, -1};
-    RC(pass_strided<float>(const_cast<float2*>(Z), n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z));
+    RC(pass_strided<float>(Z, n, n, 1, n, n, sxy, 5, N, +1, last, st, ROLE_Z));
     RC(mlp_infer_packed(m, Z, N, N, T, N, clip, smag, st));
     Epi<float> plain{1.f, 0, 1, -1, -1};
     return pass_strided<float>(T, n, n, 1, n, n, sxy, 3, N, -1, plain, st, ROLE_Z);
