@@ -447,6 +447,37 @@ def test_les_lag_split_and_graph_replay(hip):
     L.sh_stream_destroy(st)
 
 
+def test_les_autotune_keeps_the_lagged_state(hip):
+    """sh_les_autotune times lagged steps on the live LES; afterwards the
+    velocity and the pending force are the ones before, and the run goes on
+    within rounding of a twin that was not tuned."""
+    from spectral_upcxx_amd import closure
+    n = 32
+    P = _params(18)
+    u = _dev(_physical_field(n, 95).astype(np.float32))
+    runs, ms = [], []
+    for _ in range(2):
+        m = _mlp(hip, P, np.full(9, 0.3, np.float32), np.full(6, 0.05, np.float32))
+        les = closure.LES(n, 0.01, 2e-3, m, clip=True, lag=1)
+        les.set_initial(u)
+        les.steps(3)
+        runs.append(les)
+        ms.append(m)
+    a, b = runs
+    v0, f0 = _host(a.velocity()), _host(a.pending_force())
+    a.autotune(2)
+    assert np.array_equal(_host(a.velocity()), v0)
+    assert np.array_equal(_host(a.pending_force()), f0)
+    a.steps(5)
+    b.steps(5)
+    va, vb = _host(a.velocity()), _host(b.velocity())
+    assert np.isfinite(va).all() and rel_l2(va, vb) < 1e-5
+    for r in runs:
+        r.close()
+    for m in ms:
+        m.close()
+
+
 def test_les_lag_is_second_order(hip):
     """The lagged closure's extrapolation error is O(dt^2): halving dt divides
     the gap between the lagged and the synchronous run by about 4 (a plain
