@@ -86,6 +86,8 @@ def test_host_argument_errors_without_gpu():
         L.sh_les_pending_force(None, None)
     with pytest.raises(RuntimeError, match="invalid argument"):
         L.sh_les_force(None, None)
+    with pytest.raises(RuntimeError, match="Cs >= 0"):
+        L.sh_les_set_smagorinsky(None, 0.1)
     with pytest.raises(RuntimeError, match="invalid arguments"):
         L.sh_particles_seed(None, 10, 1, 0, None)
     with pytest.raises(RuntimeError, match="invalid arguments"):
