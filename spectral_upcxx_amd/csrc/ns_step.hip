@@ -968,7 +968,7 @@ int launch_fourier_x(sh_ns_s* s) {
     X(8, P8, 256) X(12, Q12, 64) X(16, P16, 64) X(24, Q24, 32) X(32, Q32, 32) X(48, Q48, 21)      \
     X(64, W64, 16) X(64, P64, 32) X(64, W64, 8) X(64, P64, 8)                                     \
     X(96, Q96, 10) X(128, W128, 8) X(256, W256, 4) X(256, Q256, 8)                                \
-    X(256, W256, 2) X(512, W512, 2) X(512, P512, 4)
+    X(256, W256, 2) X(256, W256, 1) X(512, W512, 2) X(512, P512, 4)
 #define SH_FOURIER_F32_VARIANTS(X)                                                                \
     X(8, P8, 256) X(12, Q12, 64) X(16, P16, 64) X(24, Q24, 32) X(32, Q32, 32) X(48, Q48, 21)      \
     X(64, P64, 32) X(64, W64, 16) X(64, P64, 16) X(64, P64, 8) X(64, W64, 8) X(64, W64, 4)       \
@@ -994,7 +994,7 @@ int launch_fourier_x(sh_ns_s* s) {
 #define SH_FOURIER_F64_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, W256, 2) X(256, W256, 4) X(512, W512, 2)
 #define SH_FOURIER_F32_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, Q256, 8) X(256, W256, 4)
 // kernel E with nontemporal stores (V, N0 and the emitted fields)
-#define SH_FOURIER_F64_NT_VARIANTS(X) X(256, W256, 2) X(256, W256, 4)
+#define SH_FOURIER_F64_NT_VARIANTS(X) X(256, W256, 2) X(256, W256, 4) X(256, W256, 1)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
