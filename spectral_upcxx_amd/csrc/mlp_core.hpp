@@ -105,9 +105,12 @@ __device__ void load_bias(const float* __restrict__ P, FwdB& b, int lane) {
 // tanh x = sign(x) (1 - e) / (1 + e), e = exp(-2|x|) = 2^(-2 log2(e) |x|), on
 // the hardware exp2 and reciprocal (about 1e-7 absolute error, far below the
 // BF16 rounding that follows)
+// tanh x = 1 - 2 / (1 + e^{2x}): five instructions (two of them
+// transcendental), saturating to +-1 without a branch (e^{2x} overflows to
+// +inf, whose reciprocal is 0); the absolute error is a few FP32 ulps of 1
 __device__ __forceinline__ float tanh_fast(float x) {
-    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(x));
-    return copysignf((1.f - e) * __builtin_amdgcn_rcpf(1.f + e), x);
+    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);
+    return fmaf(-2.f, __builtin_amdgcn_rcpf(1.f + e), 1.f);
 }
 
 __device__ __forceinline__ f32x4 tanh4(f32x4 v) {

@@ -301,9 +301,14 @@ def test_les_stress_and_force_match_oracle(hip, clip, n, cs):
     assert rel_l2(got.T[keep], tau[keep]) < 2e-3
     # the force pipeline (packed transforms, Hermitian unpacking, -div)
     # applied to the device's own stress field
+    # The force kernel recomputes the stress after its own z transform, whose
+    # FP32 rounding can differ in the last bit from the separate pass behind
+    # les.stress(); with the clip on, a point whose tau:S rounds across zero
+    # then takes the other decision (as counted above), which moves the force
+    # by ~1e-5 of its norm.  Without the clip the two agree to 1e-5.
     f = _host(les.force())
     fo = C.sgs_force(got.astype(np.float64).reshape(6, n, n, n))
-    assert rel_l2(f, fo) < (5e-4 if n == 64 else 1e-5)
+    assert rel_l2(f, fo) < (5e-4 if n == 64 else 1e-4 if clip else 1e-5)
     les.close()
     m.close()
 
