@@ -62,6 +62,13 @@ int sh_memcpy_d2h(void* dst, const void* src, size_t bytes);
 /* stream-ordered device copy (the deep copies of src/Scalar.cpp:52-66) */
 int sh_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int sh_memset(void* dst, int value, size_t bytes, void* stream);
+/* Streaming device copy (16-byte vectors per lane, nontemporal loads and
+ * stores) used as the measured HBM bandwidth ceiling beside the kernels'
+ * rooflines; kernel class "stream_copy" for sh_prof_query.  bytes a multiple
+ * of 16, pointers 16-byte aligned; blocks = 0 launches one trip per lane,
+ * otherwise that many 256-lane blocks loop over the buffer.  No counterpart
+ * in the reference. */
+int sh_stream_copy(void* dst, const void* src, size_t bytes, int blocks, void* stream);
 int sh_stream_synchronize(void* stream);
 
 /* ------------------------------------------------------------------ FFT */
@@ -122,6 +129,9 @@ int sh_add_z(void* out, const void* x, const void* y, long n, void* stream);
 int sh_sub_z(void* out, const void* x, const void* y, long n, void* stream);
 /* out = alpha * x (double * Scalar / double * Vector) */
 int sh_scale_z(void* out, double alpha, const void* x, long n, void* stream);
+/* x[i*stride] /= divisor for i < n, in place: the divide loop of
+ * FFT1D::inverse (src/FFT1D.cpp:59-66) as one strided launch */
+int sh_div_strided_z(void* x, double divisor, long n, long stride, void* stream);
 /* Scalar::average: mean |f| (src/Scalar.cpp:15-22).  Synchronous. */
 int sh_average_z(const void* f, long n, double* result, void* stream);
 /* sum of |f|^2 over n elements, FP64 accumulation.  Synchronous. */
@@ -135,9 +145,13 @@ int sh_taylor_green_physical_z(void* v3, int nx, int ny, int nz, void* stream);
 /* random solenoidal Fourier velocity, E(k) ~ k^4 exp(-2 (k/k0)^2), scaled to u_rms */
 int sh_solenoidal_ic_z(void* v3, int nx, int ny, int nz, uint64_t seed, double k0, double urms,
                        void* stream);
-/* shell energy spectrum E(k) of a Fourier velocity (FP64 sums), nbins host doubles.
- * Also returns total energy 0.5<|u|^2> and dissipation 2 nu sum k^2 E if the
- * pointers are non-null.  Synchronous. */
+/* shell energy spectrum of a Fourier velocity: Ek[b] = sum over the modes with
+ * round(|k|) = b of e(k) = 0.5 |u(k)|^2 / (Nx Ny Nz)^2, b < nbins (at most
+ * 2048; nbins may be 0).  Also the total energy sum e(k) = 0.5<|u|^2> and the
+ * dissipation 2 nu sum |k|^2 e(k) if the pointers are non-null.  FP64 sums in
+ * an order fixed by the grid size alone (no atomics): repeated calls give the
+ * same bits.  Synchronous.  No counterpart in the reference (its diagnostic
+ * is the divergence check of exec/ns.cpp:209-210). */
 int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek, int nbins,
                   double* energy, double* dissipation, void* stream);
 /* enstrophy 0.5<|curl u|^2> and divergence norm sqrt(<|div u|^2>) of a Fourier
@@ -190,7 +204,9 @@ int sh_ns_set_vorticity(sh_ns_t handle, const void* omega0);
 int sh_ns_startup(sh_ns_t handle);
 /* nsteps iterations of exec/ns.cpp:167-217 */
 int sh_ns_steps(sh_ns_t handle, int nsteps);
-/* current velocity V1 into 3 contiguous complex128 device fields */
+/* current velocity V1 into 3 contiguous complex128 device fields.  These
+ * three single-domain getters return SH_EINVAL on an ensemble handle (more
+ * than one box): use the *_box calls there. */
 int sh_ns_get_velocity(sh_ns_t handle, void* out_v3);
 int sh_ns_get_vorticity(sh_ns_t handle, void* out_v3);
 /* div.average() of the last step (exec/ns.cpp:209-210).  Synchronous. */

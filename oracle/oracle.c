@@ -4,10 +4,12 @@
  * Checker and timed CPU baseline only; the HIP product never calls it.
  * Every function cites the reference source it restates (paths relative to
  * the reference root).  The 1D engine is a project-written Stockham
- * mixed-radix (4, 2, 3) FFT standing in for FFTW, which is not installed
- * anywhere this code runs; the reference's FFTW calls are plain unnormalised
- * C2C transforms (src/FFT1D.cpp:72-101), which is what this computes.
- * Twiddles are evaluated in long double and rounded once.
+ * mixed-radix FFT standing in for FFTW, which is not installed anywhere this
+ * code runs; the reference's FFTW calls are plain unnormalised C2C transforms
+ * of any length (src/FFT1D.cpp:14-54, 95-101), which is what this computes:
+ * radices 4, 2, 3 with dedicated butterflies, and every other prime factor
+ * (5, 7, 11, ..., a prime length itself) as a direct DFT stage summed in long
+ * double.  Twiddles are evaluated in long double and rounded once.
  */
 #include "oracle.h"
 
@@ -57,16 +59,22 @@ typedef struct {
     double* w; /* w[2m], w[2m+1] = cos, -sin of 2 pi m / n  (forward twiddles) */
 } plan1d;
 
-#define MAX_PLANS 64
+#define MAX_PLANS 256
 static plan1d g_plans[MAX_PLANS];
 static int g_nplans = 0;
 
+/* 4s, 2s, 3s, then the remaining prime factors in increasing order (each a
+ * direct-DFT stage); any n >= 1 factors */
 static int factorize(int n, int* radix) {
     int ns = 0;
+    if (n < 1) return -1;
     while (n % 4 == 0) { radix[ns++] = 4; n /= 4; }
     while (n % 2 == 0) { radix[ns++] = 2; n /= 2; }
     while (n % 3 == 0) { radix[ns++] = 3; n /= 3; }
-    return n == 1 ? ns : -1;
+    for (int p = 5; n > 1 && (long)p * p <= n; p += 2)
+        while (n % p == 0) { radix[ns++] = p; n /= p; }
+    if (n > 1) radix[ns++] = n;
+    return ns;
 }
 
 static const plan1d* get_plan(int n) {
@@ -94,6 +102,47 @@ static const plan1d* get_plan(int n) {
     return found;
 }
 
+/* One Stockham stage of a radix R other than 2, 3, 4 (5, 7 or any larger
+ * prime factor): the twiddled inputs, then the direct length-R DFT
+ * X[q] = sum_r x[r] exp(sign 2 pi i r q / R) with its constants taken from the
+ * length-n table (exp(2 pi i m / R) = w[m n / R]) and the sums in long
+ * double. */
+static void stage_generic(const plan1d* p, const double* in, double* out, int R, int Ns,
+                          double ws) {
+    const int n = p->n;
+    const int m = n / R;
+    const int tws = n / (Ns * R);
+    const int step = n / R;
+    long double xr[R], xi[R];
+    for (int j = 0; j < m; j++) {
+        const int k = j % Ns;
+        for (int r = 0; r < R; r++) {
+            double a = in[2 * (j + r * m)], b = in[2 * (j + r * m) + 1];
+            if (Ns > 1 && r > 0) {
+                const int t = k * r * tws;
+                const double wr = p->w[2 * t], wi = ws * p->w[2 * t + 1];
+                const double c = a * wr - b * wi;
+                b = a * wi + b * wr;
+                a = c;
+            }
+            xr[r] = a;
+            xi[r] = b;
+        }
+        const int base = (j / Ns) * Ns * R + k;
+        for (int q = 0; q < R; q++) {
+            long double sr = 0, si = 0;
+            for (int r = 0; r < R; r++) {
+                const int t = (int)(((long)r * q) % R) * step;
+                const long double wr = p->w[2 * t], wi = ws * p->w[2 * t + 1];
+                sr += xr[r] * wr - xi[r] * wi;
+                si += xr[r] * wi + xi[r] * wr;
+            }
+            out[2 * (base + q * Ns)] = (double)sr;
+            out[2 * (base + q * Ns) + 1] = (double)si;
+        }
+    }
+}
+
 /* Stockham autosort pass over one contiguous line; x is overwritten with the
  * result, y is scratch of the same size. */
 static void fft_line(const plan1d* p, double* x, double* y, int dir) {
@@ -107,6 +156,12 @@ static void fft_line(const plan1d* p, double* x, double* y, int dir) {
         const int R = p->radix[st];
         const int m = n / R;
         const int tws = n / (Ns * R);
+        if (R > 4) {
+            stage_generic(p, in, out, R, Ns, ws);
+            double* t = in; in = out; out = t;
+            Ns *= R;
+            continue;
+        }
         for (int j = 0; j < m; j++) {
             const int k = j % Ns;
             double xr[4], xi[4];
@@ -161,7 +216,8 @@ static void fft_line(const plan1d* p, double* x, double* y, int dir) {
     if (in != x) memcpy(x, in, sizeof(double) * 2 * (size_t)n);
 }
 
-/* FFT1D::transform(ptr, N, stride, DIR) -- src/FFT1D.cpp:118-142, 144-151 */
+/* FFT1D::transform(ptr, DIR) with the plan of FFT1D::init(N, stride) --
+ * src/FFT1D.cpp:14-54 (plans of any N), 95-101 (execution) */
 int or_fft1d(double* x, int n, long stride, int dir) {
     const plan1d* p = get_plan(n);
     if (!p) return -1;
@@ -204,15 +260,84 @@ static void strided_block(const plan1d* p, double* base, int n, long es, int nb,
     }
 }
 
-/* Scalar::transform(DIR) at one rank -- src/Scalar.cpp:400-646.
- * Phase order as the reference: columns (y, :425-431), rows (x, :485-502)
- * with the row copy into tmp (the p=1 branch of the pencil put, :495-496),
- * z on tmp (:547-553), transpose back (slab copies, :582-596). */
+/* Virtual slab ranks of the 3D transform (1 = one rank, OpenMP threads over
+ * every loop).  With R > 1 ranks, R threads each own a z-slab of f and a
+ * y-slab of tmp and run their phases serially, with the reference's slab
+ * puts between them as copies (src/Scalar.cpp:446-503, 582-596). */
+static int g_ranks = 1;
+
+void or_set_ranks(int ranks) { g_ranks = ranks < 1 ? 1 : ranks; }
+int or_get_ranks(void) { return g_ranks; }
+
+/* Scalar::transform(DIR) at p = R ranks, SLABS variant -- src/Scalar.cpp:400-646,
+ * rank r holding planes [r Nz/R, (r+1) Nz/R) of f (its m_local) and the
+ * y-slab tmp + r Nx (Ny/R) Nz (its tlocal):
+ *   columns (y lines on the local planes, :420-431),
+ *   rows (x lines) plane by plane, each plane's row block j sent to rank j at
+ *     loc = r (Ny/R) Nx (Nz/R) + k Nx (Ny/R) (:451-474),
+ *   barrier, z lines on tlocal at stride Nx (Ny/R) (:530-553),
+ *   barrier, transpose back: slab k of tlocal to rank k/(Nz/R) at
+ *     dest = r (Ny/R) Nx + (k mod Nz/R) Nx Ny (:570-596). */
+#ifdef _OPENMP
+static int transform3d_ranks(const plan1d* px, const plan1d* py, const plan1d* pz, double* f,
+                             double* tmp, int nx, int ny, int nz, int dir, int R) {
+    const long sxy = (long)nx * ny;
+    const int nzp = nz / R, nyp = ny / R;
+    const int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+#pragma omp parallel num_threads(R)
+    {
+        const int r = omp_get_thread_num();
+        double* buf = (double*)malloc(sizeof(double) * 2 * (size_t)nmax * (OR_BLK + 2));
+        double* loc = f + 2 * (size_t)r * nzp * sxy;
+        double* tl = tmp + 2 * (size_t)r * nyp * nx * nz;
+        for (int k = 0; k < nzp; k++)
+            for (int i0 = 0; i0 < nx; i0 += OR_BLK) {
+                int nb = nx - i0 < OR_BLK ? nx - i0 : OR_BLK;
+                strided_block(py, loc + 2 * ((size_t)k * sxy + i0), ny, nx, nb, dir, buf);
+            }
+        for (int k = 0; k < nzp; k++) {
+            for (int j = 0; j < ny; j++) fft_line(px, loc + 2 * ((size_t)k * sxy + (size_t)j * nx), buf, dir);
+            for (int q = 0; q < R; q++) {
+                const double* src = loc + 2 * ((size_t)k * sxy + (size_t)q * nx * nyp);
+                double* dst = tmp + 2 * ((size_t)q * nyp * nx * nz + (size_t)r * nyp * nx * nzp +
+                                         (size_t)k * nx * nyp);
+                memcpy(dst, src, sizeof(double) * 2 * (size_t)nx * nyp);
+            }
+        }
+#pragma omp barrier
+        for (int j = 0; j < nyp; j++)
+            for (int i0 = 0; i0 < nx; i0 += OR_BLK) {
+                int nb = nx - i0 < OR_BLK ? nx - i0 : OR_BLK;
+                strided_block(pz, tl + 2 * ((size_t)j * nx + i0), nz, (long)nx * nyp, nb, dir, buf);
+            }
+#pragma omp barrier
+        for (int k = 0; k < nz; k++) {
+            const double* slab = tl + 2 * (size_t)k * nyp * nx;
+            double* dst = f + 2 * ((size_t)(k / nzp) * nzp * sxy + (size_t)r * nyp * nx +
+                                   (size_t)(k % nzp) * sxy);
+            memcpy(dst, slab, sizeof(double) * 2 * (size_t)nx * nyp);
+        }
+        free(buf);
+    }
+    return 0;
+}
+#endif
+
+/* Scalar::transform(DIR) -- src/Scalar.cpp:400-646.  At one rank the phase
+ * order is the reference's: columns (y, :425-431), rows (x, :485-502) with the
+ * row copy into tmp (the p=1 branch of the pencil put, :495-496), z on tmp
+ * (:547-553), transpose back (slab copies, :582-596), OpenMP threads over the
+ * lines of each phase.  With or_set_ranks(R), R | Ny and R | Nz, the slab
+ * ranks above. */
 int or_transform3d(double* f, double* tmp, int nx, int ny, int nz, int dir) {
     const plan1d* px = get_plan(nx);
     const plan1d* py = get_plan(ny);
     const plan1d* pz = get_plan(nz);
     if (!px || !py || !pz) return -1;
+#ifdef _OPENMP
+    if (g_ranks > 1 && ny % g_ranks == 0 && nz % g_ranks == 0)
+        return transform3d_ranks(px, py, pz, f, tmp, nx, ny, nz, dir, g_ranks);
+#endif
     const long sxy = (long)nx * ny;
     const int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
 #pragma omp parallel
@@ -415,7 +540,8 @@ int or_cross(const double* a, const double* b, double* out, double* work, int nx
     return 0;
 }
 
-/* Scalar::average -- src/Scalar.cpp:15-22: mean of |f| over the local data */
+/* Scalar::average -- src/Scalar.cpp:15-22: sum of abs(m_local[i]) over the
+ * local data divided by localSize() (at one rank: every element) */
 double or_average(const double* f, size_t n) {
     double acc = 0;
 #pragma omp parallel for schedule(static) reduction(+ : acc)
@@ -714,6 +840,29 @@ double or_energy(const double* V, int nx, int ny, int nz) {
     return 0.5 * e2 / ((double)n * (double)n);
 }
 
+/* Dissipation rate 2 nu sum_modes |k|^2 e(k) of a Fourier velocity, e(k) the
+ * mode's kinetic energy 0.5 |u(k)|^2 / N^2 (numpy ifftn scaling, as
+ * or_energy); for a solenoidal field it equals 2 nu times the enstrophy. */
+double or_dissipation(const double* V, int nx, int ny, int nz, double nu) {
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t F = 2 * n;
+    double acc = 0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                size_t m = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
+                double kx = or_freq(i, nx), ky = or_freq(j, ny), kz = or_freq(k, nz);
+                double e = 0;
+                for (int d = 0; d < 3; d++)
+                    e += V[d * F + 2 * m] * V[d * F + 2 * m] + V[d * F + 2 * m + 1] * V[d * F + 2 * m + 1];
+                acc += (kx * kx + ky * ky + kz * kz) * 0.5 * e;
+            }
+    return 2.0 * nu * acc / ((double)n * (double)n);
+}
+
+/* Shell spectrum E(b) = sum of the mode energies e(k) (as or_dissipation)
+ * over the modes with round(|k|) = b, b < nbins (modes beyond are dropped). */
 void or_spectrum(const double* V, int nx, int ny, int nz, double* Ek, int nbins) {
     const size_t n = (size_t)nx * ny * nz;
     const size_t F = 2 * n;

@@ -34,6 +34,11 @@ void or_prng_complex(uint64_t seed, int nx, int ny, int nz, double* out);
 /* ---- 1D / 3D FFT ---- */
 /* in-place strided 1D FFT of length n (FFT1D::transform, src/FFT1D.cpp:95-101) */
 int or_fft1d(double* x, int n, long stride, int dir);
+/* virtual slab ranks of the 3D transform (src/Scalar.cpp:446-626): 1 (default)
+ * runs every phase with all OpenMP threads; R > 1 runs R rank threads, each on
+ * its own slab with the slab puts as copies (used when R divides Ny and Nz) */
+void or_set_ranks(int ranks);
+int or_get_ranks(void);
 /* Scalar::transform(DIR) (src/Scalar.cpp:400-646); tmp has the size of f */
 int or_transform3d(double* f, double* tmp, int nx, int ny, int nz, int dir);
 /* Scalar::forward: transform(-1) + zeroHighModes (src/Scalar.cpp:208-215) */
@@ -92,6 +97,8 @@ void or_ic_solenoidal(double* V, int nx, int ny, int nz, uint64_t seed, double k
 double or_energy(const double* V, int nx, int ny, int nz);
 /* shell spectrum E(k), k = round(|k|), nbins entries */
 void or_spectrum(const double* V, int nx, int ny, int nz, double* Ek, int nbins);
+/* dissipation rate 2 nu sum |k|^2 e(k) (e as or_energy's per-mode terms) */
+double or_dissipation(const double* V, int nx, int ny, int nz, double nu);
 /* enstrophy 0.5 <|curl u|^2> and divergence norm sqrt(<|div u|^2>) (either pointer may be NULL) */
 void or_vorticity_stats(const double* V, int nx, int ny, int nz, double* enstrophy, double* divnorm);
 /* Gaussian vortices: set 0 exec/ns.cpp:84-131 (three), set 1 exec/ns2.cpp:84-130

@@ -12,9 +12,20 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+# the same source built with -march=native on the host that runs it (make native)
+NATIVE_LIB_PATH = os.path.join(HERE, "build", "native", "liboracle.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
+
+
+def use_library(path):
+    """Load the oracle from `path` (e.g. NATIVE_LIB_PATH) instead of the
+    portable build; only before the first call into the oracle."""
+    global LIB_PATH
+    if _lib is not None and path != LIB_PATH:
+        raise RuntimeError("the oracle library is already loaded from %s" % LIB_PATH)
+    LIB_PATH = path
 
 
 def lib():
@@ -62,6 +73,9 @@ def lib():
             "or_ic_solenoidal": (None, [_dp, i, i, i, u64, d, d]),
             "or_energy": (d, [_dp, i, i, i]),
             "or_spectrum": (None, [_dp, i, i, i, _dp, i]),
+            "or_dissipation": (d, [_dp, i, i, i, d]),
+            "or_set_ranks": (None, [i]),
+            "or_get_ranks": (i, []),
             "or_num_threads": (i, []),
         }
         for name, (res, args) in sig.items():
@@ -213,6 +227,21 @@ def spectrum(V, nbins):
     Ek = np.zeros(nbins)
     lib().or_spectrum(_p(V), *_dims(V), _p(Ek), nbins)
     return Ek
+
+
+def dissipation(V, nu):
+    """2 nu sum_modes |k|^2 e(k) (e(k) = 0.5 |u(k)|^2 / N^2)."""
+    V = np.ascontiguousarray(V, np.complex128)
+    return lib().or_dissipation(_p(V), *_dims(V), float(nu))
+
+
+def set_ranks(r):
+    """Virtual slab ranks of the 3D transform (1 = OpenMP threads over each phase)."""
+    lib().or_set_ranks(int(r))
+
+
+def get_ranks():
+    return lib().or_get_ranks()
 
 
 def vorticity_stats(V):

@@ -1,5 +1,5 @@
 // FFT1D.cpp -- plan handle over sh_fft1d_z2z (reference: src/FFT1D.cpp:14-101).
-// No planning step is needed: the transform lengths are compiled kernels.
+// No planning step is needed: lengths are dispatched to compiled kernels.
 #include "FFT1D.H"
 
 #include <cstdlib>
@@ -28,12 +28,8 @@ void FFT1D::forward(cdouble* input, int N, int stride) { transform(input, N, str
 
 void FFT1D::inverse(cdouble* input, int N, int stride) {
     transform(input, N, stride, 1);
-    // divide by N along the stride (one run when contiguous)
-    if (stride == 1) {
-        fft1d_check(sh_scale_z(input, 1.0 / N, input, N, nullptr));
-    } else {
-        for (int i = 0; i < N; i++) fft1d_check(sh_scale_z(input + (long)i * stride, 1.0 / N, input + (long)i * stride, 1, nullptr));
-    }
+    // input[i*stride] /= N for every i (src/FFT1D.cpp:62-65), one launch
+    fft1d_check(sh_div_strided_z(input, (double)N, N, stride, nullptr));
 }
 
 void FFT1D::transform(cdouble* input, int N, int stride, int dir) {

@@ -1751,8 +1751,18 @@ int sh_ns_steps(sh_ns_t s, int nsteps) {
     return SH_OK;
 }
 
+// the single-domain getters; an ensemble's boxes are field-major (field d of
+// box b at (d*nboxes + b)*N), so they are read through the *_box calls
+static int reject_ensemble(sh_ns_t s, const char* who) {
+    if (s->nboxes == 1) return SH_OK;
+    set_error(std::string(who) + ": the handle holds " + std::to_string(s->nboxes) +
+              " boxes; use the *_box call with a box index");
+    return SH_EINVAL;
+}
+
 int sh_ns_get_velocity(sh_ns_t s, void* out) {
     if (!s || !out) return SH_EINVAL;
+    RC(reject_ensemble(s, "sh_ns_get_velocity"));
     const long n3 = 3 * s->N;
     if (!s->fused) {
         SH_HIP(hipMemcpyAsync(out, s->V1, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
@@ -1801,6 +1811,7 @@ int sh_ns_get_layout(sh_ns_t s, int* half) {
 
 int sh_ns_get_vorticity(sh_ns_t s, void* out) {
     if (!s || !out) return SH_EINVAL;
+    RC(reject_ensemble(s, "sh_ns_get_vorticity"));
     if (!s->fused) {
         SH_HIP(hipMemcpyAsync(out, s->om1, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
         return SH_OK;
@@ -1812,6 +1823,7 @@ int sh_ns_get_vorticity(sh_ns_t s, void* out) {
 
 int sh_ns_div_average(sh_ns_t s, double* result) {
     if (!s || !result) return SH_EINVAL;
+    RC(reject_ensemble(s, "sh_ns_div_average"));
     if (!s->have_div) {
         *result = 0;
         return SH_OK;

@@ -6,6 +6,7 @@
 // per thread per grid-stride trip, 16-byte loads, wavenumbers recomputed from
 // the linear index (Scalar::freq) instead of being read from memory.
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "sh_common.hpp"
@@ -163,6 +164,13 @@ __global__ void k_sub(double2* __restrict__ o, const double2* __restrict__ x,
 __global__ void k_scale(double2* __restrict__ o, double a, const double2* __restrict__ x, long N) {
     GRID_STRIDE(n, N) o[n] = rmul(a, x[n]);
 }
+// x[n * stride] /= d for n < N (the divide loop of FFT1D::inverse)
+__global__ void k_div_strided(double2* __restrict__ x, double d, long N, long stride) {
+    GRID_STRIDE(n, N) {
+        const double2 v = x[n * stride];
+        x[n * stride] = mk<double>(v.x / d, v.y / d);
+    }
+}
 
 // ---------------------------------------------------------------- reductions (deterministic two-pass)
 static constexpr int kRedBlocks = 1024;
@@ -295,33 +303,76 @@ __global__ void k_solenoidal(double2* __restrict__ v, Dims d, uint64_t seed, dou
     }
 }
 
-// shell spectrum: per-block LDS histogram, then global atomics
-__global__ __launch_bounds__(kBlock) void k_spectrum(const double2* __restrict__ v, Dims d,
-                                                     double norm, double* __restrict__ bins,
-                                                     int nbins, double* __restrict__ tot) {
-    extern __shared__ double sb[];  // nbins + 2
-    for (int b = threadIdx.x; b < nbins + 2; b += blockDim.x) sb[b] = 0;
+// Shell spectrum, deterministic (the same bits on every call): each wave
+// walks 64-element chunks of the field; within a chunk the lanes of each
+// distinct shell are summed by an xor butterfly (identical on every lane, as
+// a + b == b + a) and lane 0 adds that sum to the wave's own LDS histogram, so
+// every bin accumulates in a fixed order.  Entries nbins, nbins + 1 of a row
+// hold the totals of e and |k|^2 e.  The block adds its waves' rows in wave
+// order into its partial row; k_spectrum_final adds the rows in block order.
+static constexpr int kSpecWaves = kBlock / 64;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_spectrum_partial(const double2* __restrict__ v, Dims d,
+                                                             double norm, int nbins,
+                                                             double* __restrict__ part) {
+    extern __shared__ double sh[];  // kSpecWaves rows of nbins + 2
+    const int W = nbins + 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* hist = sh + wave * W;
+    for (int b = threadIdx.x; b < kSpecWaves * W; b += blockDim.x) sh[b] = 0;
     __syncthreads();
     const long N = d.n();
+    const long stride = (long)gridDim.x * blockDim.x;
     double e_acc = 0, d_acc = 0;
-    GRID_STRIDE(n, N) {
-        double k0, k1, k2;
-        wavenumber(n, d, k0, k1, k2);
-        const double q = k0 * k0 + k1 * k1 + k2 * k2;
-        const double2 a = v[n], b = v[N + n], c = v[2 * N + n];
-        const double e = 0.5 * (a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y + c.x * c.x + c.y * c.y) * norm;
-        e_acc += e;
-        d_acc += q * e;
-        const int bin = (int)floor(sqrt(q) + 0.5);
-        if (bin < nbins) atomicAdd(&sb[bin], e);
+    for (long base = (long)blockIdx.x * blockDim.x + 64 * wave; base < N; base += stride) {
+        const long n = base + lane;
+        double e = 0;
+        int bin = -1;
+        if (n < N) {
+            double k0, k1, k2;
+            wavenumber(n, d, k0, k1, k2);
+            const double q = k0 * k0 + k1 * k1 + k2 * k2;
+            const double2 a = v[n], b = v[N + n], c = v[2 * N + n];
+            e = 0.5 * (a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y + c.x * c.x + c.y * c.y) * norm;
+            e_acc += e;
+            d_acc += q * e;
+            const int sb = (int)floor(sqrt(q) + 0.5);
+            if (sb < nbins) bin = sb;
+        }
+        unsigned long long pend = __ballot(bin >= 0);
+        while (pend) {
+            const int b = __shfl(bin, __ffsll(pend) - 1, 64);
+            const double x = wave_sum(bin == b ? e : 0.0);
+            if (lane == 0) hist[b] += x;
+            pend &= ~__ballot(bin == b);
+        }
     }
-    atomicAdd(&sb[nbins], e_acc);
-    atomicAdd(&sb[nbins + 1], d_acc);
+    e_acc = wave_sum(e_acc);
+    d_acc = wave_sum(d_acc);
+    if (lane == 0) {
+        hist[nbins] = e_acc;
+        hist[nbins + 1] = d_acc;
+    }
     __syncthreads();
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) atomicAdd(&bins[b], sb[b]);
-    if (threadIdx.x == 0) {
-        atomicAdd(&tot[0], sb[nbins]);
-        atomicAdd(&tot[1], sb[nbins + 1]);
+    for (int b = threadIdx.x; b < W; b += blockDim.x) {
+        double acc = 0;
+        for (int w = 0; w < kSpecWaves; w++) acc += sh[w * W + b];
+        part[(long)blockIdx.x * W + b] = acc;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_spectrum_final(const double* __restrict__ part, int np, int W,
+                                                           double* __restrict__ out) {
+    for (int b = threadIdx.x; b < W; b += blockDim.x) {
+        double acc = 0;
+        for (int r = 0; r < np; r++) acc += part[(long)r * W + b];
+        out[b] = acc;
     }
 }
 
@@ -409,6 +460,39 @@ static int check_dims(const Dims& d, const char* who) {
         return SH_EINVAL;
     }
     return ensure_init();
+}
+
+// spectrum launch shape: depends on the field size only, so the summation
+// order (and the result) is the same on every device and call
+static constexpr int kSpecMaxBins = 2048;
+static unsigned spectrum_grid(long n) {
+    unsigned g = grid_for(n);
+    return g > (unsigned)kRedBlocks ? (unsigned)kRedBlocks : g;
+}
+
+// partial rows into buf[0 .. g*W), their sum into buf[g*W ..) and then into
+// host[0 .. W); synchronous
+static int spectrum_run(const double2* v3, const Dims& d, int nbins, double* buf, unsigned g,
+                        double* host, hipStream_t st) {
+    const int W = nbins + 2;
+    const double norm = 1.0 / ((double)d.n() * (double)d.n());
+    hipLaunchKernelGGL(k_spectrum_partial, dim3(g), dim3(kBlock), sizeof(double) * kSpecWaves * W, st, v3, d,
+                       norm, nbins, buf);
+    hipLaunchKernelGGL(k_spectrum_final, dim3(1), dim3(kBlock), 0, st, (const double*)buf, (int)g, W,
+                       buf + (size_t)g * W);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host, buf + (size_t)g * W, sizeof(double) * W, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "sh_spectrum_z");
+    return SH_OK;
+}
+
+static void spectrum_outputs(const double* h, int nbins, double nu, double* Ek, double* energy,
+                             double* dissipation) {
+    for (int b = 0; b < nbins; b++) Ek[b] = h[b];
+    if (energy) *energy = h[nbins];
+    if (dissipation) *dissipation = 2.0 * nu * h[nbins + 1];
 }
 
 }  // namespace sh
@@ -579,6 +663,21 @@ int sh_scale_z(void* out, double alpha, const void* x, long n, void* stream) {
     return SH_OK;
 }
 
+int sh_div_strided_z(void* x, double divisor, long n, long stride, void* stream) {
+    SH_REQUIRE(x, "sh_div_strided_z");
+    if (n < 0 || stride < 1) {
+        set_error("sh_div_strided_z: n >= 0 and stride >= 1");
+        return SH_EINVAL;
+    }
+    if (n == 0) return SH_OK;
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_div_strided, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream), (double2*)x,
+                       divisor, n, stride);
+    SH_LAUNCH_CHECK("k_div_strided");
+    return SH_OK;
+}
+
 int sh_average_z(const void* f, long n, double* result, void* stream) {
     SH_REQUIRE(f && result, "sh_average_z");
     if (n <= 0) {
@@ -653,27 +752,23 @@ int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek,
                   double* energy, double* dissipation, void* stream) {
     Dims d{nx, ny, nz};
     SH_REQUIRE(v3, "sh_spectrum_z");
-    if (nbins < 0 || (nbins > 0 && !Ek)) return SH_EINVAL;
+    if (nbins < 0 || nbins > kSpecMaxBins || (nbins > 0 && !Ek)) {
+        set_error("sh_spectrum_z: nbins must be 0.." + std::to_string(kSpecMaxBins) + " (Ek non-null if > 0)");
+        return SH_EINVAL;
+    }
     int rc = check_dims(d, "sh_spectrum_z");
     if (rc) return rc;
     hipStream_t st = as_stream(stream);
+    const unsigned g = spectrum_grid(d.n());
+    const int W = nbins + 2;
     double* buf = nullptr;
-    rc = pool_alloc((void**)&buf, sizeof(double) * (nbins + 2));
+    rc = pool_alloc((void**)&buf, sizeof(double) * ((size_t)g + 1) * W);
     if (rc) return rc;
-    SH_HIP(hipMemsetAsync(buf, 0, sizeof(double) * (nbins + 2), st));
-    const double norm = 1.0 / ((double)d.n() * (double)d.n());
-    unsigned g = grid_for(d.n());
-    if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_spectrum, dim3(g), dim3(kBlock), sizeof(double) * (nbins + 2), st,
-                       (const double2*)v3, d, norm, buf, nbins, buf + nbins);
-    SH_LAUNCH_CHECK("k_spectrum");
-    std::vector<double> h(nbins + 2);
-    SH_HIP(hipMemcpyAsync(h.data(), buf, sizeof(double) * (nbins + 2), hipMemcpyDeviceToHost, st));
-    SH_HIP(hipStreamSynchronize(st));
+    std::vector<double> h(W);
+    rc = spectrum_run((const double2*)v3, d, nbins, buf, g, h.data(), st);
     pool_free(buf);
-    for (int b = 0; b < nbins; b++) Ek[b] = h[b];
-    if (energy) *energy = h[nbins];
-    if (dissipation) *dissipation = 2.0 * nu * h[nbins + 1];
+    if (rc) return rc;
+    spectrum_outputs(h.data(), nbins, nu, Ek, energy, dissipation);
     return SH_OK;
 }
 

@@ -280,6 +280,53 @@ int sh_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
     return SH_OK;
 }
 
+}  // extern "C"
+
+namespace sh {
+// Streaming copy, the bandwidth reference for the kernels' rooflines: every
+// lane moves 16-byte vectors with nontemporal loads and stores (no cache
+// allocation, as the passes that stream whole fields), 4 vectors in flight
+// per lane per trip.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+static constexpr int kCopyBlock = 256, kCopyUnroll = 4;
+
+__global__ __launch_bounds__(kCopyBlock) void k_stream_copy(const u32x4* __restrict__ src,
+                                                            u32x4* __restrict__ dst, long n) {
+    const long stride = (long)gridDim.x * kCopyBlock * kCopyUnroll;
+    for (long i = (long)blockIdx.x * kCopyBlock * kCopyUnroll + threadIdx.x; i < n; i += stride) {
+        u32x4 v[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; u++)
+            if (i + u * kCopyBlock < n) v[u] = __builtin_nontemporal_load(src + i + u * kCopyBlock);
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; u++)
+            if (i + u * kCopyBlock < n) __builtin_nontemporal_store(v[u], dst + i + u * kCopyBlock);
+    }
+}
+}  // namespace sh
+
+extern "C" {
+
+int sh_stream_copy(void* dst, const void* src, size_t bytes, int blocks, void* stream) {
+    if (bytes == 0) return SH_OK;
+    if (!dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16 || blocks < 0) {
+        set_error("sh_stream_copy: 16-byte aligned pointers, a multiple of 16 bytes, blocks >= 0");
+        return SH_EINVAL;
+    }
+    const long n = (long)(bytes / 16);
+    const long per = (long)kCopyBlock * kCopyUnroll;
+    long g = blocks > 0 ? blocks : (n + per - 1) / per;
+    if (g > 0x7fffffffL) g = 0x7fffffffL;
+    hipStream_t st = as_stream(stream);
+    {
+        ProfScope ps("stream_copy", st);
+        hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(kCopyBlock), 0, st, (const u32x4*)src,
+                           (u32x4*)dst, n);
+    }
+    SH_LAUNCH_CHECK("k_stream_copy");
+    return SH_OK;
+}
+
 int sh_memset(void* dst, int value, size_t bytes, void* stream) {
     if (bytes == 0) return SH_OK;
     if (!dst) return SH_EINVAL;

@@ -62,6 +62,15 @@ def test_ensemble_boxes_match_single_and_oracle(hip):
         d = ctypes.c_double()
         hip.lib().sh_ns_div_average_box(h, b, ctypes.byref(d))
         assert d.value < 1e-12
+    # the single-domain getters refuse an ensemble handle (its boxes are
+    # field-major; a 3N copy would mix boxes)
+    import torch
+    buf = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    for call in (lambda: hip.lib().sh_ns_get_velocity(h, hip.ptr(buf)),
+                 lambda: hip.lib().sh_ns_get_vorticity(h, hip.ptr(buf)),
+                 lambda: hip.lib().sh_ns_div_average(h, ctypes.byref(ctypes.c_double()))):
+        with pytest.raises(RuntimeError, match="box"):
+            call()
     hip.lib().sh_ns_destroy(h)
 
 
