@@ -137,6 +137,8 @@ def test_ensemble_64_96_graph_vs_single_and_oracle(hip):
     try:
         snaps = [torch.empty((steps // every,) + s, dtype=torch.float32, device="cuda")
                  for s in ens.snapshot_shapes(ratio)]
+        ens.snapshot([s[0] for s in snaps], ratio)  # the snapshot scratch exists before capture
+        L.sh_stream_synchronize(ens.stream)
         g = ens.capture(steps, every, ratio, snaps)
         L.sh_graph_launch(g, ens.stream)
         L.sh_stream_synchronize(ens.stream)
