@@ -154,6 +154,15 @@ int sh_solenoidal_ic_z(void* v3, int nx, int ny, int nz, uint64_t seed, double k
  * is the divergence check of exec/ns.cpp:209-210). */
 int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek, int nbins,
                   double* energy, double* dissipation, void* stream);
+/* sh_spectrum_z without any device allocation, for callers that cannot
+ * allocate (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The same kernels and
+ * the same results, bit for bit, as sh_spectrum_z; the partial sums live in a
+ * static device array of 1025 x 512 doubles (4 MiB per device, held until the
+ * process ends), so nbins is at most 510 (SH_EINVAL beyond).  Calls that
+ * overlap on one device race on that array.  Synchronous. */
+int sh_spectrum_z_noalloc(const void* v3, int nx, int ny, int nz, double nu, double* Ek, int nbins,
+                          double* energy, double* dissipation, void* stream);
 /* enstrophy 0.5<|curl u|^2> and divergence norm sqrt(<|div u|^2>) of a Fourier
  * velocity, curl and divergence as src/Vector.cpp:92-138 (either output may
  * be NULL; FP64 block sums added in a fixed order).  Synchronous. */

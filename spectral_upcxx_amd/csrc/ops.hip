@@ -6,6 +6,7 @@
 // per thread per grid-stride trip, 16-byte loads, wavenumbers recomputed from
 // the linear index (Scalar::freq) instead of being read from memory.
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -495,6 +496,42 @@ static void spectrum_outputs(const double* h, int nbins, double nu, double* Ek, 
     if (dissipation) *dissipation = 2.0 * nu * h[nbins + 1];
 }
 
+// ---------------------------------------------------------------- spectrum without allocation
+// The partial rows of sh_spectrum_z_noalloc live in `scratch`, a file-scope
+// device array (each device holds its own copy until the process ends).  Host
+// code never uses the name `scratch` as an address: its device address is
+// taken once per device with hipGetSymbolAddress and kept in g_scratch_dev.
+// Every entry of the rows a call reads is written by that call's kernels
+// first, so nothing needs clearing between calls; calls that overlap on one
+// device race on the array.
+static constexpr int kSpecNoallocMaxBins = 510;
+static constexpr size_t kSpecScratchDoubles = (size_t)(kRedBlocks + 1) * (kSpecNoallocMaxBins + 2);
+static __device__ __align__(256) double scratch[kSpecScratchDoubles];
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_dev;
+
+// the current device's address of `scratch` (taken on first use per device)
+static int scratch_address(double** out) {
+    int dev = 0;
+    SH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "spectrum scratch: hipGetSymbolAddress");
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return SH_OK;
+}
+
+// sh_init's hook: take the address on the device being initialised
+int spectrum_scratch_init() {
+    double* p = nullptr;
+    return scratch_address(&p);
+}
+
 }  // namespace sh
 
 using namespace sh;
@@ -769,6 +806,28 @@ int sh_spectrum_z(const void* v3, int nx, int ny, int nz, double nu, double* Ek,
     pool_free(buf);
     if (rc) return rc;
     spectrum_outputs(h.data(), nbins, nu, Ek, energy, dissipation);
+    return SH_OK;
+}
+
+int sh_spectrum_z_noalloc(const void* v3, int nx, int ny, int nz, double nu, double* Ek, int nbins,
+                          double* energy, double* dissipation, void* stream) {
+    Dims d{nx, ny, nz};
+    SH_REQUIRE(v3, "sh_spectrum_z_noalloc");
+    if (nbins < 0 || nbins > kSpecNoallocMaxBins || (nbins > 0 && !Ek)) {
+        set_error("sh_spectrum_z_noalloc: nbins must be 0.." + std::to_string(kSpecNoallocMaxBins) +
+                  " (the size of its static scratch; Ek non-null if > 0)");
+        return SH_EINVAL;
+    }
+    int rc = check_dims(d, "sh_spectrum_z_noalloc");
+    if (rc) return rc;
+    double* buf = nullptr;
+    rc = scratch_address(&buf);
+    if (rc) return rc;
+    const unsigned g = spectrum_grid(d.n());  // at most kRedBlocks partial rows
+    double h[kSpecNoallocMaxBins + 2];
+    rc = spectrum_run((const double2*)v3, d, nbins, buf, g, h, as_stream(stream));
+    if (rc) return rc;
+    spectrum_outputs(h, nbins, nu, Ek, energy, dissipation);
     return SH_OK;
 }
 

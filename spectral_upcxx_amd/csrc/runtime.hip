@@ -13,6 +13,8 @@
 
 namespace sh {
 
+int spectrum_scratch_init();  // ops.hip: device address of the spectrum's static scratch
+
 static thread_local std::string g_err;
 
 void set_error(const std::string& msg) { g_err = msg; }
@@ -231,7 +233,9 @@ int sh_init(int device) {
         return SH_EINVAL;
     }
     SH_HIP(hipSetDevice(device));
-    return ensure_init();
+    int rc = ensure_init();
+    if (rc) return rc;
+    return spectrum_scratch_init();
 }
 
 int sh_device_count(int* count) {
