@@ -13,7 +13,13 @@
  * A "vector" argument is three such fields; a `v3` pointer means the three
  * fields are contiguous (component d at v3 + d*Nx*Ny*Nz elements).
  *
- * Supported FFT lengths per axis: 8 12 16 24 32 48 64 96 128 256 512.
+ * FFT lengths per axis: any N from 1 to 2048 whose prime factors are 2, 3,
+ * 5, 7, 11, 13, and any other N up to 1024 (as FFTW plans any length,
+ * src/FFT1D.cpp:14-54).  Compiled register/LDS plans: 8 10 12 14 16 20 24 28
+ * 32 40 48 56 64 80 96 100 128 256 512 1024; the other smooth lengths run a
+ * mixed-radix kernel with the radix list as an argument, the rest Bluestein's
+ * chirp-z transform.  The fused solver schedule has kernels for 8 12 16 24 32
+ * 48 64 96 128 256 512 per axis; the op-by-op schedule runs every length.
  * Sign convention (src/FFT1D.cpp:72-101): dir = -1 gives exp(-2 pi i jk/N)
  * (forward), dir = +1 gives exp(+2 pi i jk/N), unnormalised.
  */
@@ -30,7 +36,7 @@ extern "C" {
 enum {
     SH_OK = 0,
     SH_EINVAL = -1,      /* bad argument (null pointer, size, direction) */
-    SH_EUNSUPPORTED = -2, /* FFT length not in the supported list */
+    SH_EUNSUPPORTED = -2, /* FFT length (or fused-schedule grid) not supported */
     SH_EHIP = -3,        /* HIP runtime error */
     SH_ENOMEM = -4       /* device allocation failed */
 };

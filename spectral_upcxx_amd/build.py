@@ -22,7 +22,7 @@ BIN = os.path.join(LIB, "bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HIP_SOURCES = ["runtime.hip", "fft3d.hip", "ops.hip", "ns_step.hip", "tune.hip", "mlp.hip", "les.hip", "particles.hip"]
+HIP_SOURCES = ["runtime.hip", "fft3d.hip", "fft_any.hip", "ops.hip", "ns_step.hip", "tune.hip", "mlp.hip", "les.hip", "particles.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     # packed FP32 instructions are kept off (v_pk_* forms are not used)

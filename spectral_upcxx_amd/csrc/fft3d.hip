@@ -196,8 +196,7 @@ int pass_contig(C2<T>* data, int n, long nlines, long ldist, int count, long fst
         SH_FOR_EACH_FFT_SIZE(SH_CASE)
 #undef SH_CASE
         default:
-            set_error("unsupported FFT length " + std::to_string(n));
-            return SH_EUNSUPPORTED;
+            return pass_any<T>(data, n, nlines, nlines, ldist, 0, 1, count, fstride, dir, ep, false, st);
     }
 }
 
@@ -218,8 +217,8 @@ int pass_strided(C2<T>* data, int n, int ncols, long cdist, int nouter, long odi
         SH_FOR_EACH_FFT_SIZE(SH_CASE)
 #undef SH_CASE
         default:
-            set_error("unsupported FFT length " + std::to_string(n));
-            return SH_EUNSUPPORTED;
+            return pass_any<T>(data, n, (long)ncols * nouter, ncols, cdist, odist, es, count, fstride, dir, ep, true,
+                               st);
     }
 }
 
@@ -257,7 +256,7 @@ static int fft1d(void* data, int n, long stride, long dist, long batch, int dir,
         set_error("sh_fft1d: invalid argument");
         return SH_EINVAL;
     }
-    if (!fft_size_supported(n)) {
+    if (!fft_length_supported(n)) {
         set_error("sh_fft1d: unsupported length " + std::to_string(n));
         return SH_EUNSUPPORTED;
     }
@@ -282,7 +281,7 @@ int fft3d(void* data, int count, long fstride, int nx, int ny, int nz, int dir, 
         set_error("sh_fft3d: invalid argument");
         return SH_EINVAL;
     }
-    if (!fft_size_supported(nx) || !fft_size_supported(ny) || !fft_size_supported(nz)) {
+    if (!fft_length_supported(nx) || !fft_length_supported(ny) || !fft_length_supported(nz)) {
         set_error("sh_fft3d: unsupported dimensions " + std::to_string(nx) + "x" +
                   std::to_string(ny) + "x" + std::to_string(nz));
         return SH_EUNSUPPORTED;

@@ -35,6 +35,62 @@ constexpr double kSin32[16] = {0.0, 0.19509032201612828, 0.3826834323650898, 0.5
                                0.3826834323650898, 0.19509032201612828};
 constexpr double kSqrtHalf = 0.7071067811865476;
 constexpr double kSqrt3Half = 0.8660254037844386;
+// cos, sin(2 pi m / R), m = 0..R/2, for the odd radices 5 and 7 (extended
+// precision, rounded once)
+constexpr double kCos5[3] = {1.0, 0.30901699437494745, -0.8090169943749475};
+constexpr double kSin5[3] = {0.0, 0.9510565162951535, 0.5877852522924731};
+constexpr double kCos7[4] = {1.0, 0.6234898018587335, -0.2225209339563144, -0.9009688679024191};
+constexpr double kSin7[4] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.4338837391175581};
+
+template <int R>
+__host__ __device__ constexpr double odd_cos(int m) {
+    return R == 5 ? kCos5[m] : kCos7[m];
+}
+template <int R>
+__host__ __device__ constexpr double odd_sin(int m) {
+    return R == 5 ? kSin5[m] : kSin7[m];
+}
+
+// In-register DFT of an odd prime size R (5 or 7), SIGN = +-1:
+// t_k = x_k + x_{R-k}, d_k = x_k - x_{R-k} (k = 1..R/2);
+// X_0 = x_0 + sum t_k; for q = 1..R/2
+//   a_q = x_0 + sum_k cos(2 pi kq/R) t_k,  b_q = SIGN sum_k sin(2 pi kq/R) d_k,
+//   X_q = a_q + i b_q,  X_{R-q} = a_q - i b_q.
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft_odd(C2<T>* x) {
+    constexpr int H = R / 2;
+    C2<T> t[H + 1], d[H + 1];
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        t[k] = cadd<T>(x[k], x[R - k]);
+        d[k] = csub<T>(x[k], x[R - k]);
+    }
+    const C2<T> x0 = x[0];
+    C2<T> s0 = x0;
+#pragma unroll
+    for (int k = 1; k <= H; k++) s0 = cadd<T>(s0, t[k]);
+#pragma unroll
+    for (int q = 1; q <= H; q++) {
+        T ar = x0.x, ai = x0.y, br = T(0), bi = T(0);
+#pragma unroll
+        for (int k = 1; k <= H; k++) {
+            const int m = (k * q) % R;
+            const T c = T(odd_cos<R>(m <= H ? m : R - m));
+            const T sn = m <= H ? T(odd_sin<R>(m)) : -T(odd_sin<R>(R - m));
+            ar += c * t[k].x;
+            ai += c * t[k].y;
+            br += sn * d[k].x;
+            bi += sn * d[k].y;
+        }
+        if (SIGN < 0) {
+            br = -br;
+            bi = -bi;
+        }
+        x[q] = mk<T>(ar - bi, ai + br);
+        x[R - q] = mk<T>(ar + bi, ai - br);
+    }
+    x[0] = s0;
+}
 
 // a * exp(SIGN 2 pi i K / R), R | 32, 0 <= K < R/2
 template <typename T, int R, int K, int SIGN>
@@ -91,6 +147,8 @@ struct Dft {
             x[0] = cadd<T>(x[0], t);
             x[1] = cadd<T>(m, d);
             x[2] = csub<T>(m, d);
+        } else if constexpr (R == 5 || R == 7) {
+            dft_odd<T, R, SIGN>(x);
         } else if constexpr (R == 4) {
             const C2<T> a = cadd<T>(x[0], x[2]), b = csub<T>(x[0], x[2]);
             const C2<T> c = cadd<T>(x[1], x[3]), e = csub<T>(x[1], x[3]);
@@ -289,19 +347,38 @@ using P96 = FftPlan<96, 12, 8, 4, 3>;
 using P128 = FftPlan<128, 8, 16, 8>;
 using P256 = FftPlan<256, 16, 16, 16>;
 using P512 = FftPlan<512, 64, 8, 8, 8>;
+// lengths with the odd radices 5 and 7, and 1024
+using P10 = FftPlan<10, 2, 5, 2>;
+using P14 = FftPlan<14, 2, 7, 2>;
+using P20 = FftPlan<20, 4, 5, 4>;
+using P28 = FftPlan<28, 4, 7, 4>;
+using P40 = FftPlan<40, 8, 5, 8>;
+using P56 = FftPlan<56, 8, 7, 8>;
+using P80 = FftPlan<80, 16, 5, 16>;
+using P100 = FftPlan<100, 20, 5, 5, 4>;
+using P1024 = FftPlan<1024, 64, 16, 4, 16>;
 
 #define SH_FOR_EACH_FFT_SIZE(X) \
     X(8, P8, 64, 16)            \
+    X(10, P10, 64, 32)          \
     X(12, P12, 64, 16)          \
+    X(14, P14, 64, 32)          \
     X(16, P16, 64, 16)          \
+    X(20, P20, 64, 16)          \
     X(24, P24, 64, 16)          \
+    X(28, P28, 64, 16)          \
     X(32, P32, 64, 16)          \
+    X(40, P40, 32, 16)          \
     X(48, P48, 16, 16)          \
+    X(56, P56, 32, 16)          \
     X(64, P64, 32, 16)          \
+    X(80, P80, 16, 16)          \
     X(96, P96, 16, 16)          \
+    X(100, P100, 16, 16)        \
     X(128, P128, 32, 16)        \
     X(256, P256, 16, 16)        \
-    X(512, P512, 4, 8)
+    X(512, P512, 4, 8)          \
+    X(1024, P1024, 2, 2)
 
 // Plans for the fused solver kernels: at most 9 complex values per thread
 // (several fields are held at once), and first radix == last radix, so the
@@ -374,6 +451,27 @@ inline bool fft_size_supported(int n) {
     switch (n) {
 #define SH_CASE(n_, P_, L_, C_) case n_:
         SH_FOR_EACH_FFT_SIZE(SH_CASE)
+#undef SH_CASE
+        return true;
+        default:
+            return false;
+    }
+}
+
+// lengths with a compiled plan, a mixed-radix plan (fft_any.hip) or a
+// Bluestein transform: every length the passes accept
+bool fft_length_supported(int n);
+// the pass over lengths without a compiled plan (fft_any.hip): line L of the
+// batch starts at (L % a_n) a_d + (L / a_n) b_d, elements at stride es;
+// strided selects the column-fastest lane order
+template <typename T>
+int pass_any(C2<T>* data, int n, long nlines, long a_n, long a_d, long b_d, long es, int count, long fstride,
+             int dir, const Epi<T>& ep, bool strided, hipStream_t st);
+
+inline bool fused_size_supported(int n) {
+    switch (n) {
+#define SH_CASE(n_, P_, C_) case n_:
+        SH_FOR_EACH_FUSED_SIZE(SH_CASE)
 #undef SH_CASE
         return true;
         default:

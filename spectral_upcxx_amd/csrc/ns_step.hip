@@ -1431,7 +1431,12 @@ int ns_create(sh_ns_t* handle, int nx, int ny, int nz, int nboxes, const double*
         return SH_EINVAL;
     }
     const double nu = nus[0];
-    if (!fft_size_supported(nx) || !fft_size_supported(ny) || !fft_size_supported(nz)) {
+    if (fused && !(fused_size_supported(nx) && fused_size_supported(ny) && fused_size_supported(nz))) {
+        set_error("sh_ns_create: the fused schedule has no kernels for this grid (axis lengths 8 12 16 24 32 "
+                  "48 64 96 128 256 512); the op-by-op schedule (fused = 0) runs any supported length");
+        return SH_EUNSUPPORTED;
+    }
+    if (!fft_length_supported(nx) || !fft_length_supported(ny) || !fft_length_supported(nz)) {
         set_error("sh_ns_create: unsupported grid size");
         return SH_EUNSUPPORTED;
     }

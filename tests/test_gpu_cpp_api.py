@@ -63,6 +63,34 @@ def test_fft_bench_program():
     assert rc == 0 and "my man!" in out, out
 
 
+@pytest.mark.parametrize("n", [20, 40, 17])
+def test_programs_any_length(n):
+    """the drop-in API at lengths outside the power-of-two family: radix-5
+    plans (20, 40) and Bluestein (17): the fft driver's round trip, the
+    restated test/3d_fft.cpp and test/operators.cpp"""
+    rc, out = run(["fft", n, 2])
+    assert rc == 0 and "my man!" in out, out
+    rc, out = run(["test_3d_fft", n])
+    assert rc == 0 and "my man!" in out, out
+    rc, out = run(["test_operators", n])
+    assert rc == 0, out
+    assert "failed" not in out and out.count("passed") >= 14
+
+
+def test_ns_driver_any_grid_vs_oracle(tmp_path):
+    """exec/ns.cpp through the API on a 20^3 grid (op-by-op), Taylor-Green,
+    20 steps, against the oracle"""
+    n = 20
+    out = tmp_path / "v.bin"
+    rc, txt = run(["ns", n, 20, "tg", 0.01, 0.001, "api", out])
+    assert rc == 0, txt
+    ref = O.NS((n, n, n), 0.01, 0.001)
+    ref.set_initial(O.taylor_green((n, n, n)))
+    ref.startup()
+    ref.steps(20)
+    assert rel_l2(_read_v(out, n), ref.velocity()) < 1e-10
+
+
 def test_ilap_program():
     rc, out = run(["ilap", 32, 3])
     assert rc == 0 and "min time" in out, out

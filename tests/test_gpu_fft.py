@@ -120,8 +120,78 @@ def test_fft3d_c2c_fp32(hip):
     assert rel_l2(_host(t).astype(np.complex128), ref) < 2e-6
 
 
-def test_unsupported_size_is_an_error(hip):
+# lengths of every path: compiled plans with radix 5 / 7 and 1024; the
+# mixed-radix kernel (factors 8 4 2 3 5 7 11 13, up to 2048); Bluestein (a
+# larger prime factor, up to 1024)
+ANY_COMPILED = [10, 14, 20, 28, 40, 56, 80, 100, 1024]
+ANY_MIXED = [6, 11, 13, 22, 30, 44, 60, 77, 120, 143, 1000, 2048]
+ANY_BLUESTEIN = [17, 31, 101, 127, 509, 1009]
+
+
+@pytest.mark.parametrize("n", ANY_COMPILED + ANY_MIXED + ANY_BLUESTEIN)
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fft1d_any_length_vs_oracle(hip, n, prec):
+    """FFT1D of any length (src/FFT1D.cpp:14-54 plans FFTW for any N): contiguous
+    and strided batches against the oracle (itself pinned by brute-force DFTs
+    and numpy), both signs."""
     import torch
-    t = torch.zeros((7, 7, 7), dtype=torch.complex128, device="cuda")
+    batch = 5
+    x = O.prng_complex(n + 3, (n * batch, 1, 1)).reshape(batch, n)
+    tol = TOL if prec == 64 else 3e-6
+    dt = np.complex128 if prec == 64 else np.complex64
+    fn = hip.lib().sh_fft1d_z2z if prec == 64 else hip.lib().sh_fft1d_c2c
+    for direction in (-1, 1):
+        ref = np.stack([O.fft1d(row, direction) for row in x])
+        t = _dev(x.astype(dt))
+        fn(hip.ptr(t), n, 1, n, batch, direction, hip.stream())
+        err = rel_l2(_host(t).astype(np.complex128), ref)
+        assert err < tol, ("contiguous", n, direction, err)
+        # the same lines stored column-wise (stride batch, distance 1)
+        t = _dev(np.ascontiguousarray(x.T).astype(dt))
+        fn(hip.ptr(t), n, batch, 1, batch, direction, hip.stream())
+        err = rel_l2(_host(t).T.astype(np.complex128), ref)
+        assert err < tol, ("strided", n, direction, err)
+
+
+@pytest.mark.parametrize("dims", [(10, 20, 40), (100, 80, 14), (11, 13, 17), (30, 22, 31), (6, 1024, 2)])
+def test_fft3d_any_dims_vs_oracle(hip, dims):
+    x = O.prng_complex(sum(dims), dims)
+    for direction in (-1, 1):
+        t = _dev(x)
+        hip.fft3d(t, direction, 0)
+        assert rel_l2(_host(t), O.transform3d(x, direction)) < TOL, (dims, direction)
+    # Scalar::forward / inverse: Nyquist planes at index n/2 (floor for odd n)
+    t = _dev(x)
+    hip.forward(t)
+    assert rel_l2(_host(t), O.forward(x)) < TOL
+    hip.inverse(t)
+    assert rel_l2(_host(t), O.inverse(O.forward(x))) < TOL
+
+
+def test_ns_op_by_op_any_grid_vs_oracle(hip):
+    """exec/ns.cpp's step on a 20 x 20 x 20 Taylor-Green grid (radix-5
+    plans), the op-by-op schedule, 10 steps against the oracle; the fused
+    schedule reports that it has no kernels for the grid."""
+    dims = (20, 20, 20)
+    V0 = O.taylor_green(dims)
+    ref = O.NS(dims, 0.01, 1e-3)
+    ref.set_initial(V0)
+    ref.startup()
+    ref.steps(10)
+    ns = hip.NS(dims, 0.01, 1e-3, hip.SH_F64, fused=False)
+    ns.set_initial(_dev(V0))
+    ns.startup()
+    ns.steps(10)
+    assert rel_l2(_host(ns.velocity()), ref.velocity()) < 1e-10
+    ns.close()
+    with pytest.raises(RuntimeError, match="fused"):
+        hip.NS(dims, 0.01, 1e-3, hip.SH_F64, fused=True)
+
+
+def test_unsupported_size_is_an_error(hip):
+    """a length beyond both the mixed-radix kernel (2048) and Bluestein (1024):
+    the prime 2053"""
+    import torch
+    t = torch.zeros((2053,), dtype=torch.complex128, device="cuda")
     with pytest.raises(RuntimeError, match="unsupported"):
-        hip.fft3d(t, -1, 0)
+        hip.lib().sh_fft1d_z2z(hip.ptr(t), 2053, 1, 2053, 1, -1, hip.stream())
