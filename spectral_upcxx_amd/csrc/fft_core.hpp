@@ -269,37 +269,50 @@ __device__ __forceinline__ void st_line(C2<T>* p, C2<T> v) {
     }
 }
 
+// Block barriers for the LDS exchanges.  SyncBlock is __syncthreads().
+// SyncLds waits for this wave's LDS operations only and meets the other
+// waves: global loads -- and loads into LDS (global_load_lds) for another
+// buffer -- stay in flight across it, which a workgroup fence would drain.
+// It orders LDS traffic only, so a kernel that uses it manages the waits for
+// its loads into LDS itself.
+struct SyncBlock {
+    __device__ static __forceinline__ void run() { __syncthreads(); }
+};
+struct SyncLds {
+    __device__ static __forceinline__ void run() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+};
+
 // Whole-line transform.  On entry x holds the stage-0 inputs (see for_in),
 // on exit the outputs (see for_out).  If the LDS arrays were used before the
-// call, the caller must __syncthreads() first.
-template <class P, int SIGN, typename T, class Idx>
+// call, the caller must synchronise first.
+template <class P, int SIGN, typename T, class Idx, class Sync = SyncBlock>
 __device__ __forceinline__ void line_fft(C2<T>* x, int tid, T* re, T* im, const Idx& idx,
                                          const C2<T>* __restrict__ tw) {
     stage_bfly<P, 0, SIGN, T>(x, tid, tw);
     if constexpr (P::S > 1) {
         lds_put<P, 0, T>(x, tid, re, im, idx);
-        __syncthreads();
+        Sync::run();
         lds_get<P, 1, T>(x, tid, re, im, idx);
         stage_bfly<P, 1, SIGN, T>(x, tid, tw);
     }
     if constexpr (P::S > 2) {
-        __syncthreads();
+        Sync::run();
         lds_put<P, 1, T>(x, tid, re, im, idx);
-        __syncthreads();
+        Sync::run();
         lds_get<P, 2, T>(x, tid, re, im, idx);
         stage_bfly<P, 2, SIGN, T>(x, tid, tw);
     }
     if constexpr (P::S > 3) {
-        __syncthreads();
+        Sync::run();
         lds_put<P, 2, T>(x, tid, re, im, idx);
-        __syncthreads();
+        Sync::run();
         lds_get<P, 3, T>(x, tid, re, im, idx);
         stage_bfly<P, 3, SIGN, T>(x, tid, tw);
     }
     if constexpr (P::S > 4) {
-        __syncthreads();
+        Sync::run();
         lds_put<P, 3, T>(x, tid, re, im, idx);
-        __syncthreads();
+        Sync::run();
         lds_get<P, 4, T>(x, tid, re, im, idx);
         stage_bfly<P, 4, SIGN, T>(x, tid, tw);
     }

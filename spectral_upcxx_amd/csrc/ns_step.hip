@@ -316,6 +316,214 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__
     }
 }
 
+// C of the half-spectrum layout with its inputs streamed into LDS (FP64).
+// The same arithmetic as k_ns_cross_zh; what changes is how the six input
+// fields arrive.  Each block loops over column tiles (a persistent grid of
+// resident blocks) and keeps three field slots in LDS that gfx950's direct
+// global->LDS loads (global_load_lds_dwordx4, no registers) fill two fields
+// ahead: while a tile's field is transformed, the next two are in flight, and
+// during a tile's three forward transforms the next tile's first three are.
+// The FFT exchanges synchronise with SyncLds, which does not drain those
+// loads, and each slot is read only after an explicit wait for its own loads
+// (vmcnt counts this wave's loads into LDS and its stores, in issue order)
+// and a barrier.  The half-spectrum packing reads X[k] and X[M-k] straight
+// from the slot, so it needs no exchange of its own.  The twiddle tables are
+// copied into LDS once per block, so the only vector-memory operations left
+// are the slot loads and the product stores.
+template <int N>
+__device__ __forceinline__ void vm_wait_barrier() {
+    asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
+template <typename T>
+__device__ __forceinline__ unsigned lds_offset(const T* p) {
+    return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)p;
+}
+
+template <class P, int LPB, bool NT = false>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict__ W,
+                                                             const double2* __restrict__ twg,
+                                                             const double2* __restrict__ twng, int nx,
+                                                             int ncx, int ny, long F, double scale, long boff,
+                                                             int ntile) {
+    using T = double;
+    constexpr int M = P::N;
+    constexpr int R0 = P::radix(0);
+    constexpr int Q = M / R0;
+    constexpr int CS = LPB + 1;
+    constexpr int E = P::E;
+    constexpr int NTH = P::TPL * LPB;
+    constexpr int NW = NTH / 64;
+    constexpr int RPI = 64 / LPB;            // rows of a slot one wave instruction fills
+    constexpr int IPW = M / (RPI * NW);      // slot loads per wave (and per lane) per field
+    constexpr unsigned ROWB = LPB * 16;      // bytes per slot row
+    static_assert(R0 == 4 && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == 4 && P::nb(P::S - 1) == 1,
+                  "the slot reads assume one radix-4 butterfly per thread in the first and last stage");
+    static_assert(NTH % 64 == 0 && 64 % LPB == 0 && M % (RPI * NW) == 0 && (R0 - 1) * Q * ROWB < 65536,
+                  "slot geometry");
+    __shared__ double2 slot[3][M * LPB];
+    __shared__ T sre[M * CS];
+    __shared__ T sim[M * CS];
+    __shared__ double2 tw[M];
+    __shared__ double2 twn[M];
+    W += blockIdx.y * boff;
+    for (int t = threadIdx.x; t < M; t += NTH) {
+        tw[t] = twg[t];
+        twn[t] = twng[t];
+    }
+    __syncthreads();
+    const int c = threadIdx.x % LPB;
+    const int tid = threadIdx.x / LPB;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int tiles_x = (ncx + LPB - 1) / LPB;
+    const unsigned es = (unsigned)nx * (unsigned)ny;
+    const unsigned uF = (unsigned)F;
+    auto idx = [&](int pos) { return pos * CS + c; };
+    // field f of tile t into slot sl: lane -> (column lane % LPB, row lane / LPB)
+    // of each RPI-row group; the LDS destination of a wave instruction is
+    // uniform (M0) and the lanes land 16 bytes apart
+    auto fetch = [&](int t, int f, int sl) {
+        const int j = t / tiles_x;
+        const int i = (t % tiles_x) * LPB + lane % LPB;
+        const unsigned cb = i < ncx ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
+#pragma unroll
+        for (int k = 0; k < IPW; k++) {
+            const int row0 = (wave * IPW + k) * RPI;
+            const double2* src = W + (cb + (unsigned)f * uF + (unsigned)(row0 + lane / LPB) * es);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)&slot[sl][row0 * LPB], 16,
+                                             0, 0);
+        }
+    };
+    // X[k] and X[M-k] of this thread's first-stage positions k = tid + r Q
+    // from slot sl (X[M] = 0, the Nyquist plane), packed into Z and
+    // inverse-transformed, scaled by 1/(Nx Ny Nz)
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    auto inv = [&](int sl, C2<T>* x) {
+        const unsigned base = lds_offset(&slot[sl][0]);
+        const unsigned own = base + (unsigned)(tid * LPB + c) * 16u;
+        const unsigned mir = base + (unsigned)((tid == 0 ? Q : Q - tid) * LPB + c) * 16u;
+        d2v a0, a1, a2, a3, m0, m1, m2, m3;
+        asm volatile(
+            "ds_read_b128 %0, %8\n\t"
+            "ds_read_b128 %1, %8 offset:%10\n\t"
+            "ds_read_b128 %2, %8 offset:%11\n\t"
+            "ds_read_b128 %3, %8 offset:%12\n\t"
+            "ds_read_b128 %4, %9 offset:%12\n\t"
+            "ds_read_b128 %5, %9 offset:%11\n\t"
+            "ds_read_b128 %6, %9 offset:%10\n\t"
+            "ds_read_b128 %7, %9\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(m0), "=&v"(m1), "=&v"(m2), "=&v"(m3)
+            : "v"(own), "v"(mir), "n"(Q * ROWB), "n"(2 * Q * ROWB), "n"(3 * Q * ROWB)
+            : "memory");
+        const d2v av[4] = {a0, a1, a2, a3};
+        const d2v mv[4] = {m0, m1, m2, m3};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int pos = tid + r * Q;
+            const C2<T> a = mk<T>(av[r].x, av[r].y);
+            const C2<T> b = pos == 0 ? mk<T>(T(0), T(0)) : mk<T>(mv[r].x, mv[r].y);
+            const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // X[k] + conj X[M-k]
+            const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // X[k] - conj X[M-k]
+            const C2<T> w = twn[pos];                       // e^{-2 pi i k/nz}
+            const T tx = w.x * dif.x + w.y * dif.y;         // t = conj(w) dif
+            const T ty = w.x * dif.y - w.y * dif.x;
+            x[r] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
+        }
+        SyncLds::run();
+        line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+        for (int s2 = 0; s2 < E; s2++) x[s2] = rmul<T>(scale, x[s2]);
+    };
+    auto pmul = [](C2<T> a, C2<T> b) { return mk<T>(a.x * b.x, a.y * b.y); };
+    // fields in the order w0 w1 u2 w2 u0 u1 (3 4 2 5 0 1), slots 0 1 2 0 1 2
+    int t = blockIdx.x;
+    if (t < ntile) {
+        fetch(t, 3, 0);
+        fetch(t, 4, 1);
+        fetch(t, 2, 2);
+    }
+    for (; t < ntile; t += gridDim.x) {
+        const int tn = t + gridDim.x;
+        const bool more = tn < ntile;
+        const int j = t / tiles_x;
+        const int i = (t % tiles_x) * LPB + c;
+        const bool valid = i < ncx;
+        const unsigned cb = valid ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
+        C2<T> w[3][E], o[3][E], x[E];
+        // at each wait, the loads issued after the awaited field's are those of
+        // the next two fields (and, for this tile's first three, the previous
+        // tile's stores, which the wait may drain as well)
+        vm_wait_barrier<2 * IPW>();
+        inv(0, w[0]);
+        fetch(t, 5, 0);
+        vm_wait_barrier<2 * IPW>();
+        inv(1, w[1]);
+        fetch(t, 0, 1);
+        vm_wait_barrier<2 * IPW>();
+        inv(2, x);  // u2
+#pragma unroll
+        for (int s2 = 0; s2 < E; s2++) {
+            o[0][s2] = csub<T>(mk<T>(T(0), T(0)), pmul(x[s2], w[1][s2]));
+            o[1][s2] = pmul(x[s2], w[0][s2]);
+        }
+        fetch(t, 1, 2);
+        vm_wait_barrier<2 * IPW>();
+        inv(0, w[2]);
+        if (more) fetch(tn, 3, 0);
+        if (more)
+            vm_wait_barrier<2 * IPW>();
+        else
+            vm_wait_barrier<IPW>();
+        inv(1, x);  // u0
+#pragma unroll
+        for (int s2 = 0; s2 < E; s2++) {
+            o[1][s2] = csub<T>(o[1][s2], pmul(x[s2], w[2][s2]));
+            o[2][s2] = pmul(x[s2], w[1][s2]);
+        }
+        if (more) fetch(tn, 4, 1);
+        if (more)
+            vm_wait_barrier<2 * IPW>();
+        else
+            vm_wait_barrier<0>();
+        inv(2, x);  // u1
+#pragma unroll
+        for (int s2 = 0; s2 < E; s2++) {
+            o[0][s2] = cadd<T>(o[0][s2], pmul(x[s2], w[2][s2]));
+            o[2][s2] = csub<T>(o[2][s2], pmul(x[s2], w[0][s2]));
+        }
+        if (more) fetch(tn, 2, 2);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+#pragma unroll
+            for (int s2 = 0; s2 < E; s2++) x[s2] = o[d][s2];
+            SyncLds::run();
+            line_fft<P, -1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            // Z -> X (Z[M-k] from another thread through LDS)
+            SyncLds::run();
+            for_out<P>(tid, [&](int s2, int pos) {
+                sre[idx(pos)] = x[s2].x;
+                sim[idx(pos)] = x[s2].y;
+            });
+            SyncLds::run();
+            if (valid)
+                for_out<P>(tid, [&](int s2, int pos) {
+                    const C2<T> a = x[s2];
+                    const int mp = pos == 0 ? 0 : M - pos;
+                    const C2<T> b = mk<T>(sre[idx(mp)], sim[idx(mp)]);
+                    const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // Z[k] + conj Z[M-k]
+                    const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // Z[k] - conj Z[M-k]
+                    const C2<T> wv = twn[pos];
+                    const T tx = wv.x * dif.x - wv.y * dif.y;  // t = w dif
+                    const T ty = wv.x * dif.y + wv.y * dif.x;
+                    st_line<NT, T>(W + (cb + (unsigned)d * uF + (unsigned)pos * es),
+                                   mk<T>(T(0.5) * (sum.x + ty), T(0.5) * (sum.y - tx)));
+                });
+        }
+    }
+}
+
 // expand a half-spectrum field to the full one in place: planes kz > nz/2 from
 // f(-k) = conj f(k), the Nyquist plane kz = nz/2 zeroed (count fields at fstride)
 template <typename T>
@@ -926,6 +1134,52 @@ int launch_cross_zh(sh_ns_s* s) {
     return SH_OK;
 }
 
+// kernel C of the half-spectrum layout with LDS-streamed inputs (FP64): a
+// persistent grid of the resident blocks looping over the column tiles
+template <typename T, class P, int LPB, bool NT = false>
+int launch_cross_zd(sh_ns_s* s) {
+    static_assert(sizeof(T) == 8, "FP64 only");
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
+    const C2<T>* twn = (const C2<T>*)twiddles(s->nz, SH_F64);
+    if (!tw || !twn) {
+        set_error("ns: twiddle table unavailable (first use inside a graph capture?)");
+        return SH_ENOMEM;
+    }
+    if (2 * P::N != s->nz) {
+        set_error("ns: half-spectrum z kernel of the wrong length");
+        return SH_EINVAL;
+    }
+    if (6 * s->N * s->nboxes > 0xffffffffL) {  // 32-bit element offsets, as kernel C
+        set_error("ns: grid too large for the fused z kernel");
+        return SH_EINVAL;
+    }
+    // resident blocks: CUs x blocks per CU (LDS-bound: one 16-column block,
+    // two 8-column ones)
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, v = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT>, P::TPL * LPB, 0) !=
+                hipSuccess ||
+            per <= 0)
+            per = 1;
+        resident = v * per;
+    }
+    const int tiles = (s->cnx + LPB - 1) / LPB;
+    const int ntile = tiles * s->ny;
+    const int g = ntile < resident ? ntile : resident;
+    dim3 grid((unsigned)g, (unsigned)s->nboxes), block(P::TPL * LPB);
+    const double scale = 1.0 / ((double)s->nx * (double)s->ny * (double)s->nz);
+    ProfScope ps("ns_cross_z", s->cst);
+    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT>), grid, block, 0, s->cst, (double2*)s->Wf + s->cx0,
+                       (const double2*)tw, (const double2*)twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale,
+                       s->N, ntile);
+    SH_LAUNCH_CHECK("k_ns_cross_zd");
+    return SH_OK;
+}
+
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
 template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false>
@@ -983,6 +1237,10 @@ int launch_fourier_x(sh_ns_s* s) {
     X(256, Q128, 16) X(256, Q128, 8) X(512, W256, 8) X(512, W256, 4) X(512, Q256, 8) X(512, Q256, 4)
 // kernel C with nontemporal stores of the three products
 #define SH_CROSSH_F64_NT_VARIANTS(X) X(256, W128, 16) X(256, W128, 8)
+// kernel C with its inputs streamed into LDS (k_ns_cross_zd), plain and
+// nontemporal stores: X(nz, length-nz/2 plan, columns per block)
+#define SH_CROSSH_F64_LDS_VARIANTS(X) \
+    X(64, Q32, 16) X(64, Q32, 8) X(128, W64, 16) X(128, W64, 8) X(256, W128, 16) X(256, W128, 8)
 #define SH_CROSSH_F32_NT_VARIANTS(X) X(256, Q128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
@@ -1040,13 +1298,20 @@ const NsVariants& crossh_variants(int n) {
 #define SH_VT(n_, P_, C_)                                               \
     m[n_].names.push_back(#P_ " half columns=" #C_ " nontemporal stores");  \
     m[n_].fns.push_back(&launch_cross_zh<T, P_, C_, true>);
+#define SH_VD(n_, P_, C_)                                                        \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed");                  \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_>);                                 \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed nontemporal stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSSH_F64_VARIANTS(SH_V)
             SH_CROSSH_F64_NT_VARIANTS(SH_VT)
+            SH_CROSSH_F64_LDS_VARIANTS(SH_VD)
         } else {
             SH_CROSSH_F32_VARIANTS(SH_V)
             SH_CROSSH_F32_NT_VARIANTS(SH_VT)
         }
+#undef SH_VD
 #undef SH_VT
 #undef SH_V
         return m;

@@ -86,8 +86,7 @@ def test_fft_variants_512_round_trip(hip):
         torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("n", [64, 96])
-@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n,prec", [(64, 64), (64, 32), (96, 64), (96, 32), (128, 64)])
 def test_ns_variants_vs_oracle(hip, n, prec):
     import torch
     L = hip.lib()
