@@ -318,6 +318,70 @@ __device__ __forceinline__ void line_fft(C2<T>* x, int tid, T* re, T* im, const 
     }
 }
 
+// The same transform exchanging interleaved complex values through one LDS
+// array (one 16-byte access per value in FP64 instead of two 8-byte ones):
+// fewer LDS instructions, and with consecutive lanes on consecutive values
+// (idx column-fastest) the accesses stay free of bank conflicts.
+template <class P, int s, typename T, class Idx>
+__device__ __forceinline__ void lds_put_c(const C2<T>* x, int tid, C2<T>* buf, const Idx& idx) {
+    constexpr int R = P::radix(s), Ns = P::ns(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+            const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[idx(base + r * Ns)] = x[b * R + r];
+        }
+    }
+}
+
+template <class P, int s, typename T, class Idx>
+__device__ __forceinline__ void lds_get_c(C2<T>* x, int tid, const C2<T>* buf, const Idx& idx) {
+    constexpr int R = P::radix(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+#pragma unroll
+            for (int r = 0; r < R; r++) x[b * R + r] = buf[idx(j + r * (P::N / R))];
+        }
+    }
+}
+
+template <class P, int SIGN, typename T, class Idx, class Sync = SyncBlock>
+__device__ __forceinline__ void line_fft_c(C2<T>* x, int tid, C2<T>* buf, const Idx& idx,
+                                           const C2<T>* __restrict__ tw) {
+    stage_bfly<P, 0, SIGN, T>(x, tid, tw);
+    if constexpr (P::S > 1) {
+        lds_put_c<P, 0, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 1, T>(x, tid, buf, idx);
+        stage_bfly<P, 1, SIGN, T>(x, tid, tw);
+    }
+    if constexpr (P::S > 2) {
+        Sync::run();
+        lds_put_c<P, 1, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 2, T>(x, tid, buf, idx);
+        stage_bfly<P, 2, SIGN, T>(x, tid, tw);
+    }
+    if constexpr (P::S > 3) {
+        Sync::run();
+        lds_put_c<P, 2, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 3, T>(x, tid, buf, idx);
+        stage_bfly<P, 3, SIGN, T>(x, tid, tw);
+    }
+    if constexpr (P::S > 4) {
+        Sync::run();
+        lds_put_c<P, 3, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 4, T>(x, tid, buf, idx);
+        stage_bfly<P, 4, SIGN, T>(x, tid, tw);
+    }
+}
+
 // f(slot, pos) for every stage-0 input this thread owns
 template <class P, class F>
 __device__ __forceinline__ void for_in(int tid, F&& f) {

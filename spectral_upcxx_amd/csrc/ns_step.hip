@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "fft_core.hpp"
@@ -340,7 +341,7 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
     return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)p;
 }
 
-template <class P, int LPB, bool NT = false>
+template <class P, int LPB, bool NT = false, bool IL = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict__ W,
                                                              const double2* __restrict__ twg,
                                                              const double2* __restrict__ twng, int nx,
@@ -362,8 +363,11 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
     static_assert(NTH % 64 == 0 && 64 % LPB == 0 && M % (RPI * NW) == 0 && (R0 - 1) * Q * ROWB < 65536,
                   "slot geometry");
     __shared__ double2 slot[3][M * LPB];
-    __shared__ T sre[M * CS];
-    __shared__ T sim[M * CS];
+    // the FFT exchange: split re / im arrays with padded rows, or (IL)
+    // interleaved complex values, rows of LPB columns
+    __shared__ __align__(16) T sre[IL ? 2 * M * LPB : M * CS];
+    __shared__ T sim[IL ? 1 : M * CS];
+    double2* const sx = reinterpret_cast<double2*>(sre);
     __shared__ double2 tw[M];
     __shared__ double2 twn[M];
     W += blockIdx.y * boff;
@@ -378,7 +382,14 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
     const int tiles_x = (ncx + LPB - 1) / LPB;
     const unsigned es = (unsigned)nx * (unsigned)ny;
     const unsigned uF = (unsigned)F;
-    auto idx = [&](int pos) { return pos * CS + c; };
+    auto idx = [&](int pos) { return IL ? pos * LPB + c : pos * CS + c; };
+    auto fft = [&](auto sign, C2<T>* v) {
+        constexpr int SG = decltype(sign)::value;
+        if constexpr (IL)
+            line_fft_c<P, SG, T, decltype(idx), SyncLds>(v, tid, sx, idx, tw);
+        else
+            line_fft<P, SG, T, decltype(idx), SyncLds>(v, tid, sre, sim, idx, tw);
+    };
     // field f of tile t into slot sl: lane -> (column lane % LPB, row lane / LPB)
     // of each RPI-row group; the LDS destination of a wave instruction is
     // uniform (M0) and the lanes land 16 bytes apart
@@ -432,7 +443,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
             x[r] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
         }
         SyncLds::run();
-        line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+        fft(std::integral_constant<int, 1>{}, x);
 #pragma unroll
         for (int s2 = 0; s2 < E; s2++) x[s2] = rmul<T>(scale, x[s2]);
     };
@@ -499,19 +510,23 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
 #pragma unroll
             for (int s2 = 0; s2 < E; s2++) x[s2] = o[d][s2];
             SyncLds::run();
-            line_fft<P, -1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            fft(std::integral_constant<int, -1>{}, x);
             // Z -> X (Z[M-k] from another thread through LDS)
             SyncLds::run();
             for_out<P>(tid, [&](int s2, int pos) {
-                sre[idx(pos)] = x[s2].x;
-                sim[idx(pos)] = x[s2].y;
+                if constexpr (IL) {
+                    sx[idx(pos)] = x[s2];
+                } else {
+                    sre[idx(pos)] = x[s2].x;
+                    sim[idx(pos)] = x[s2].y;
+                }
             });
             SyncLds::run();
             if (valid)
                 for_out<P>(tid, [&](int s2, int pos) {
                     const C2<T> a = x[s2];
                     const int mp = pos == 0 ? 0 : M - pos;
-                    const C2<T> b = mk<T>(sre[idx(mp)], sim[idx(mp)]);
+                    const C2<T> b = IL ? sx[idx(mp)] : mk<T>(sre[idx(mp)], sim[idx(mp)]);
                     const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // Z[k] + conj Z[M-k]
                     const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // Z[k] - conj Z[M-k]
                     const C2<T> wv = twn[pos];
@@ -1136,7 +1151,7 @@ int launch_cross_zh(sh_ns_s* s) {
 
 // kernel C of the half-spectrum layout with LDS-streamed inputs (FP64): a
 // persistent grid of the resident blocks looping over the column tiles
-template <typename T, class P, int LPB, bool NT = false>
+template <typename T, class P, int LPB, bool NT = false, bool IL = false>
 int launch_cross_zd(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
@@ -1161,7 +1176,7 @@ int launch_cross_zd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT>, P::TPL * LPB, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL>, P::TPL * LPB, 0) !=
                 hipSuccess ||
             per <= 0)
             per = 1;
@@ -1173,7 +1188,7 @@ int launch_cross_zd(sh_ns_s* s) {
     dim3 grid((unsigned)g, (unsigned)s->nboxes), block(P::TPL * LPB);
     const double scale = 1.0 / ((double)s->nx * (double)s->ny * (double)s->nz);
     ProfScope ps("ns_cross_z", s->cst);
-    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT>), grid, block, 0, s->cst, (double2*)s->Wf + s->cx0,
+    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL>), grid, block, 0, s->cst, (double2*)s->Wf + s->cx0,
                        (const double2*)tw, (const double2*)twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale,
                        s->N, ntile);
     SH_LAUNCH_CHECK("k_ns_cross_zd");
@@ -1302,7 +1317,9 @@ const NsVariants& crossh_variants(int n) {
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed");                  \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_>);                                 \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed nontemporal stores"); \
-    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true>);
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true>);                           \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSSH_F64_VARIANTS(SH_V)
             SH_CROSSH_F64_NT_VARIANTS(SH_VT)
