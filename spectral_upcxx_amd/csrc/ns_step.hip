@@ -779,6 +779,207 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     if (threadIdx.x == 0) divpart[blockIdx.x] = red[0];
 }
 
+// E with its inputs streamed into LDS (FP64, one wave per x-line, a
+// persistent grid of one-wave blocks looping over the lines; no body force).
+// The same arithmetic as k_ns_fourier_x with EMIT.  A line's three products
+// and its V1 and N0 arrive in LDS by direct global->LDS loads issued a line
+// ahead: the next line's products once this line's product transforms have
+// read theirs, the next line's V1 and N0 once this line's update has read
+// its own -- so the loads are in flight during the update, the emitted
+// inverse transforms and the next product transforms.  All reads of the
+// landed data are inline-asm LDS reads that wait for themselves, the twiddles
+// come from an LDS copy, and the FFT exchanges synchronise with SyncLds: the
+// only vector-memory operations left are the loads into LDS and the stores,
+// and each wait counts them explicitly (vmcnt counts them in issue order).
+template <class P, bool NT = true>
+__global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
+                                                      double2* __restrict__ N0, const double2* __restrict__ twg,
+                                                      int ny, int nz, int nzp, long F,
+                                                      const StepCoef<double>* __restrict__ cfs,
+                                                      double* __restrict__ divpart, long boff, int divstride) {
+    using T = double;
+    constexpr int N = P::N;
+    constexpr int LS = LdsLine<N>::LS;
+    constexpr int E = P::E;
+    constexpr int R0 = P::radix(0);
+    constexpr int Q = N / R0;
+    constexpr int PIECES = N / 64;  // 64-value pieces of a line (one load into LDS each)
+    static_assert(P::TPL == 64 && R0 == 4 && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == 4 &&
+                      P::nb(P::S - 1) == 1 && N % 64 == 0 && 3 * Q * 16 < 65536,
+                  "one wave per line, one radix-4 butterfly per lane in the first and last stage");
+    const StepCoef<T> cf = cfs[blockIdx.y];
+    W += blockIdx.y * boff;
+    V += blockIdx.y * boff;
+    N0 += blockIdx.y * boff;
+    divpart += (long)blockIdx.y * divstride;
+    __shared__ double2 xs[3][N];   // the products of the line
+    __shared__ double2 vs[6][N];   // its V1 (0..2) and N0 (3..5)
+    __shared__ T sre[LS];
+    __shared__ T sim[LS];
+    __shared__ double2 tw[N];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N; t += 64) tw[t] = twg[t];
+    __syncthreads();
+    const long nlines = (long)ny * nzp;
+    auto idx = [&](int pos) { return pos + (pos >> 4); };
+    auto fetch = [&](double2* slots, const double2* src, int nf, long line) {
+        for (int f = 0; f < nf; f++)
+#pragma unroll
+            for (int k = 0; k < PIECES; k++)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * 64 + tid),
+                    (__attribute__((address_space(3))) void*)(slots + f * N + k * 64), 16, 0, 0);
+    };
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    // this lane's first-stage positions tid + r Q of a landed field
+    auto read4 = [&](const double2* slot, C2<T>* x) {
+        const unsigned a = lds_offset(slot + tid);
+        d2v v0, v1, v2, v3;
+        asm volatile(
+            "ds_read_b128 %0, %4\n\t"
+            "ds_read_b128 %1, %4 offset:%5\n\t"
+            "ds_read_b128 %2, %4 offset:%6\n\t"
+            "ds_read_b128 %3, %4 offset:%7\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+            : "v"(a), "n"(Q * 16), "n"(2 * Q * 16), "n"(3 * Q * 16)
+            : "memory");
+        x[0] = mk<T>(v0.x, v0.y);
+        x[1] = mk<T>(v1.x, v1.y);
+        x[2] = mk<T>(v2.x, v2.y);
+        x[3] = mk<T>(v3.x, v3.y);
+    };
+    double dsum = 0;
+    long line = blockIdx.x;
+    if (line < nlines) {
+        fetch(&xs[0][0], W, 3, line);
+        fetch(&vs[0][0], V, 3, line);
+        fetch(&vs[3][0], N0, 3, line);
+    }
+    for (; line < nlines; line += gridDim.x) {
+        const long nxt = line + gridDim.x;
+        const bool more = nxt < nlines;
+        const int jj = (int)(line % ny), kk = (int)(line / ny);
+        const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
+        const bool zline = (jj == ny / 2) || (kk == nz / 2);
+        const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
+        const long off = line * N;
+        // products of this line landed: issued after them are this line's V1 and
+        // N0 (6 fields) and, from the second line on, more (the previous line's
+        // stores), so a wait for at most 6 fields' loads outstanding is enough
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(6 * PIECES) : "memory");
+        C2<T> n1[3][E], x[E];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            read4(&xs[d][0], x);
+            if (d > 0) SyncLds::run();
+            line_fft<P, -1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+            for (int s2 = 0; s2 < E; s2++) n1[d][s2] = x[s2];
+        }
+        if (more) fetch(&xs[0][0], W, 3, nxt);
+        // V1 and N0 of this line: issued after them, the previous line's stores
+        // and the next line's products (none of either on a block's only line)
+        if (more)
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        C2<T> v1s[3][E], n0s[3][E], vn[3][E];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            read4(&vs[d][0], v1s[d]);
+            read4(&vs[3 + d][0], n0s[d]);
+        }
+        if (more) {
+            fetch(&vs[0][0], V, 3, nxt);
+            fetch(&vs[3][0], N0, 3, nxt);
+        }
+        for_out<P>(tid, [&](int s, int pos) {
+            const T k0 = freq<T>(pos, N);
+            const bool z = zline || pos == N / 2 ||
+                           (cf.dealias == 1 && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
+            const T sig = cf.dealias == 2 ? expfilter<T>(k0, N) * expfilter<T>(k1, ny) * expfilter<T>(k2, nz) : T(1);
+            C2<T> nn[3], v1[3], f[3], L[3], S[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                nn[d] = z ? mk<T>(T(0), T(0)) : n1[d][s];
+                if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
+                v1[d] = v1s[d][s];
+                const C2<T> n0 = n0s[d][s];
+                // F = K/2*(3.*cross1 - cross0)
+                f[d] = rmul<T>(cf.Kh, csub<T>(rmul<T>(T(3), nn[d]), n0));
+            }
+            // lap1 = nu*K/2*V1.laplacian() = c1 * (-(k.k) V1)
+            const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                L[d] = rmul<T>(cf.c1, rmul<T>(q, v1[d]));
+                S[d] = cadd<T>(cadd<T>(v1[d], L[d]), f[d]);  // (V1 + lap1) + F
+            }
+            // G = div(S); G.laplacian_inverse() (a=0, b=1)
+            C2<T> G = mk<T>(T(0), T(0));
+            G = cadd<T>(G, imul<T>(k0, S[0]));
+            G = cadd<T>(G, imul<T>(k1, S[1]));
+            G = cadd<T>(G, imul<T>(k2, S[2]));
+            T sum = 0;
+            sum += k0 * k0;
+            sum += k1 * k1;
+            sum += k2 * k2;
+            if (sum != 0) {
+                const T den = T(0) - T(1) * sum;
+                G.x /= den;
+                G.y /= den;
+            }
+            const T kk3[3] = {k0, k1, k2};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1 (exec/ns2.cpp:185: V = Vhalf)
+                C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
+                C2<T> vv = cf.be ? vh : cadd<T>(vh, L[d]);
+                // V[d].laplacian_inverse(1, -nu*K/2): divide by 1 + (nu K/2)|k|^2 (FP64 factor)
+                if (sum != 0) {
+                    const double den = 1.0 - (-cf.c1d) * (double)sum;
+                    vv = mk<T>(T((double)vv.x / den), T((double)vv.y / den));
+                }
+                vn[d][s] = vv;
+            }
+            C2<T> dv = mk<T>(T(0), T(0));
+            dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
+            dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
+            dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
+            dsum += dw * hypot((double)dv.x, (double)dv.y);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
+                st_line<NT, T>(N0 + d * F + off + pos, nn[d]);
+            }
+        });
+        // the next step's kernel A: inverse x of V (g = 0..2) and of i k x V (g = 3..5)
+#pragma unroll
+        for (int g = 0; g < 6; g++) {
+            for_out<P>(tid, [&](int s, int pos) {
+                const T k0 = freq<T>(pos, N);
+                if (g < 3)
+                    x[s] = vn[g][s];
+                else if (g == 3)
+                    x[s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
+                else if (g == 4)
+                    x[s] = csub<T>(imul<T>(k2, vn[0][s]), imul<T>(k0, vn[2][s]));
+                else
+                    x[s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
+            });
+            SyncLds::run();
+            line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + g * F + off + pos, x[s]); });
+        }
+        SyncLds::run();
+    }
+    // this block's sum of |div V| over its lines (FP64; lanes in a fixed order)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, 64);
+    if (tid == 0) divpart[blockIdx.x] = dsum;
+}
+
 // ---------------------------------------------------------------- conversion kernels
 __global__ void k_z_to_c(const double2* __restrict__ a, float2* __restrict__ b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -1219,6 +1420,43 @@ int launch_fourier_x(sh_ns_s* s) {
     return SH_OK;
 }
 
+// kernel E with LDS-streamed inputs: a persistent grid of one-wave blocks, as
+// many as are resident, at most one per line; with a body force the
+// register-loaded kernel of the same plan runs instead
+template <typename T, class P, bool NT = true>
+int launch_fourier_xd(sh_ns_s* s) {
+    static_assert(sizeof(T) == 8, "FP64 only");
+    if (s->forcing) return launch_fourier_x<T, P, 1, true, false, NT>(s);
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
+    if (!tw) return SH_ENOMEM;
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long nlines = (long)s->ny * nzp;
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, v = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT>, 64, 0) != hipSuccess ||
+            per <= 0)
+            per = 1;
+        resident = v * per;
+    }
+    const unsigned nb = (unsigned)(nlines < resident ? nlines : resident);
+    if ((int)nb > s->divblocks) {
+        set_error("ns: divergence partial buffer too small");
+        return SH_EINVAL;
+    }
+    ProfScope ps("ns_fourier_x", s->st);
+    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT>), dim3(nb, (unsigned)s->nboxes), dim3(64), 0, s->st,
+                       (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz, nzp,
+                       s->N * s->nboxes, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
+    SH_LAUNCH_CHECK("k_ns_fourier_xd");
+    s->div_nb = (int)nb;
+    s->w_primed = true;
+    return SH_OK;
+}
+
 // ---------------------------------------------------------------- tunable fused schedules
 // X(n, plan, columns or lines per block); the first entry of a length is its
 // default.  Kernel C holds up to seven fields per thread and kernel E three
@@ -1268,6 +1506,8 @@ int launch_fourier_x(sh_ns_s* s) {
 #define SH_FOURIER_F32_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, Q256, 8) X(256, W256, 4)
 // kernel E with nontemporal stores (V, N0 and the emitted fields)
 #define SH_FOURIER_F64_NT_VARIANTS(X) X(256, W256, 2) X(256, W256, 4) X(256, W256, 1)
+// kernel E with its inputs streamed into LDS (k_ns_fourier_xd): X(n, plan, nontemporal stores)
+#define SH_FOURIER_F64_LDS_VARIANTS(X) X(256, W256, true) X(256, W256, false)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -1360,17 +1600,22 @@ const NsVariants& fourier_variants(int n) {
 #define SH_VT(n_, P_, L_)                                             \
     m[n_].names.push_back(#P_ " lines=" #L_ " nontemporal stores");   \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, true, false, true>);
+#define SH_VS(n_, P_, NT_)                                                       \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_); \
+    m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F64_BATCHED_VARIANTS(SH_VB)
             SH_FOURIER_F64_NT_VARIANTS(SH_VT)
+            SH_FOURIER_F64_LDS_VARIANTS(SH_VS)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F32_BATCHED_VARIANTS(SH_VB)
             SH_FOURIER_F32_NT_VARIANTS(SH_VT)
         }
+#undef SH_VS
 #undef SH_VT
 #undef SH_V
 #undef SH_VN
