@@ -779,6 +779,78 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     if (threadIdx.x == 0) divpart[blockIdx.x] = red[0];
 }
 
+// A one-wave line transform whose LDS exchanges are inline asm: interleaved
+// complex values (one ds_write/ds_read_b128 each) at the padded index
+// pos + pos/16 of a buffer at LDS address buf.  The compiler sees no LDS access
+// in it, so it inserts no wait for loads into LDS in flight meanwhile (which
+// it would drain at once, vmcnt(0), before any access it cannot tell apart
+// from their targets); the caller orders the buffer against those loads.  The
+// block is one wave, whose LDS instructions execute in issue order, so a put
+// needs no wait before the get that follows it; each get waits for its reads.
+typedef double xa_d2 __attribute__((ext_vector_type(2)));
+
+template <class P, int s>
+__device__ __forceinline__ void xa_put(const C2<double>* x, int tid, unsigned buf) {
+    constexpr int R = P::radix(s), Ns = P::ns(s);
+    const int base = (tid / Ns) * Ns * R + (tid % Ns);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int p = base + r * Ns;
+        const unsigned a = buf + 16u * (unsigned)(p + (p >> 4));
+        const xa_d2 v = {x[r].x, x[r].y};
+        asm volatile("ds_write_b128 %0, %1" ::"v"(a), "v"(v) : "memory");
+    }
+}
+
+// the values tid + r N/4 (r = 0..3) of the next radix-4 stage
+template <class P>
+__device__ __forceinline__ void xa_get(C2<double>* x, int tid, unsigned buf) {
+    constexpr int Q = P::N / 4;
+    constexpr int D = 16 * (Q + Q / 16);
+    static_assert(Q % 16 == 0 && 3 * D < 65536, "constant offsets between a lane's four values");
+    const unsigned a = buf + 16u * (unsigned)(tid + (tid >> 4));
+    xa_d2 v0, v1, v2, v3;
+    asm volatile(
+        "ds_read_b128 %0, %4\n\t"
+        "ds_read_b128 %1, %4 offset:%5\n\t"
+        "ds_read_b128 %2, %4 offset:%6\n\t"
+        "ds_read_b128 %3, %4 offset:%7\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+        : "v"(a), "n"(D), "n"(2 * D), "n"(3 * D)
+        : "memory");
+    x[0] = mk<double>(v0.x, v0.y);
+    x[1] = mk<double>(v1.x, v1.y);
+    x[2] = mk<double>(v2.x, v2.y);
+    x[3] = mk<double>(v3.x, v3.y);
+}
+
+template <class P, int SIGN>
+__device__ __forceinline__ void line_fft_xa(C2<double>* x, int tid, unsigned buf,
+                                            const C2<double>* __restrict__ tw) {
+    static_assert(P::TPL == 64 && P::S == 4 && P::radix(0) == 4 && P::radix(1) == 4 && P::radix(2) == 4 &&
+                      P::radix(3) == 4 && P::N == 256,
+                  "four radix-4 stages, one butterfly per lane");
+    stage_bfly<P, 0, SIGN, double>(x, tid, tw);
+    xa_put<P, 0>(x, tid, buf);
+    xa_get<P>(x, tid, buf);
+    stage_bfly<P, 1, SIGN, double>(x, tid, tw);
+    xa_put<P, 1>(x, tid, buf);
+    xa_get<P>(x, tid, buf);
+    stage_bfly<P, 2, SIGN, double>(x, tid, tw);
+    xa_put<P, 2>(x, tid, buf);
+    xa_get<P>(x, tid, buf);
+    stage_bfly<P, 3, SIGN, double>(x, tid, tw);
+}
+
+// The exchange buffer of a line transform with LDS of its own (allocated for
+// a kernel only when the kernel calls this).
+template <int LS>
+__device__ __forceinline__ double* xd_exchange() {
+    __shared__ double ex[2 * LS];
+    return ex;
+}
+
 // E with its inputs streamed into LDS (FP64, one wave per x-line, a
 // persistent grid of one-wave blocks looping over the lines; no body force).
 // The same arithmetic as k_ns_fourier_x with EMIT.  A line's three products
@@ -791,7 +863,13 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
 // come from an LDS copy, and the FFT exchanges synchronise with SyncLds: the
 // only vector-memory operations left are the loads into LDS and the stores,
 // and each wait counts them explicitly (vmcnt counts them in issue order).
-template <class P, bool NT = true>
+// SHARE: the exchanges of the product transforms use the product slots (read
+// into registers first) and those of the emitted inverse transforms the N0
+// slots, whose next loads are then issued after the emitted transforms; the
+// exchanges are the inline-asm ones of line_fft_xa.  The block's LDS shrinks
+// from 45.3 KB to 40 KB, so four one-wave blocks fit on a compute unit (one
+// per SIMD) instead of three.
+template <class P, bool NT = true, bool SHARE = false>
 __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
                                                       double2* __restrict__ N0, const double2* __restrict__ twg,
                                                       int ny, int nz, int nzp, long F,
@@ -807,16 +885,24 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
     static_assert(P::TPL == 64 && R0 == 4 && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == 4 &&
                       P::nb(P::S - 1) == 1 && N % 64 == 0 && 3 * Q * 16 < 65536,
                   "one wave per line, one radix-4 butterfly per lane in the first and last stage");
+    static_assert(!SHARE || (N + N / 16) * sizeof(double2) <= 3 * N * sizeof(double2), "an exchange fits in three slots");
+    // vector-memory operations a line issues after its products' loads (the
+    // next V1, the 6 + 6 field stores and, when SHARE, the next N0), capped at
+    // what vmcnt encodes
+    constexpr int AFTER_X = (SHARE ? 6 * PIECES : 3 * PIECES) + 12 * E;
+    constexpr int TOP_WAIT = SHARE ? (AFTER_X < 63 ? AFTER_X : 63) : 6 * PIECES;
     const StepCoef<T> cf = cfs[blockIdx.y];
     W += blockIdx.y * boff;
     V += blockIdx.y * boff;
     N0 += blockIdx.y * boff;
     divpart += (long)blockIdx.y * divstride;
     __shared__ double2 xs[3][N];   // the products of the line
-    __shared__ double2 vs[6][N];   // its V1 (0..2) and N0 (3..5)
-    __shared__ T sre[LS];
-    __shared__ T sim[LS];
+    __shared__ double2 v1l[3][N];  // its V1
+    __shared__ double2 n0l[3][N];  // its N0
     __shared__ double2 tw[N];
+    T* sre = nullptr;  // the exchange of the transforms when it has LDS of its own
+    if constexpr (!SHARE) sre = xd_exchange<LS>();
+    T* const sim = SHARE ? nullptr : sre + LS;
     const int tid = threadIdx.x;
     for (int t = tid; t < N; t += 64) tw[t] = twg[t];
     __syncthreads();
@@ -853,8 +939,10 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
     long line = blockIdx.x;
     if (line < nlines) {
         fetch(&xs[0][0], W, 3, line);
-        fetch(&vs[0][0], V, 3, line);
-        fetch(&vs[3][0], N0, 3, line);
+        fetch(&v1l[0][0], V, 3, line);
+        fetch(&n0l[0][0], N0, 3, line);
+        // the first line's products (TOP_WAIT may exceed what follows them here)
+        if constexpr (SHARE) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(6 * PIECES) : "memory");
     }
     for (; line < nlines; line += gridDim.x) {
         const long nxt = line + gridDim.x;
@@ -865,17 +953,26 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
         const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
         const long off = line * N;
         // products of this line landed: issued after them are this line's V1 and
-        // N0 (6 fields) and, from the second line on, more (the previous line's
-        // stores), so a wait for at most 6 fields' loads outstanding is enough
-        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(6 * PIECES) : "memory");
+        // N0 and, from the second line on, the previous line's stores (and, when
+        // SHARE, only V1 ahead of the stores and N0 after them)
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(TOP_WAIT) : "memory");
         C2<T> n1[3][E], x[E];
+        if constexpr (SHARE) {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            read4(&xs[d][0], x);
-            if (d > 0) SyncLds::run();
-            line_fft<P, -1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            for (int d = 0; d < 3; d++) read4(&xs[d][0], n1[d]);
+            // the exchanges reuse the product slots; their last reads have
+            // completed before the next products are loaded there
 #pragma unroll
-            for (int s2 = 0; s2 < E; s2++) n1[d][s2] = x[s2];
+            for (int d = 0; d < 3; d++) line_fft_xa<P, -1>(n1[d], tid, lds_offset(&xs[0][0]), tw);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                read4(&xs[d][0], x);
+                if (d > 0) SyncLds::run();
+                line_fft<P, -1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+#pragma unroll
+                for (int s2 = 0; s2 < E; s2++) n1[d][s2] = x[s2];
+            }
         }
         if (more) fetch(&xs[0][0], W, 3, nxt);
         // V1 and N0 of this line: issued after them, the previous line's stores
@@ -887,12 +984,12 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
         C2<T> v1s[3][E], n0s[3][E], vn[3][E];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            read4(&vs[d][0], v1s[d]);
-            read4(&vs[3 + d][0], n0s[d]);
+            read4(&v1l[d][0], v1s[d]);
+            read4(&n0l[d][0], n0s[d]);
         }
         if (more) {
-            fetch(&vs[0][0], V, 3, nxt);
-            fetch(&vs[3][0], N0, 3, nxt);
+            fetch(&v1l[0][0], V, 3, nxt);
+            if constexpr (!SHARE) fetch(&n0l[0][0], N0, 3, nxt);
         }
         for_out<P>(tid, [&](int s, int pos) {
             const T k0 = freq<T>(pos, N);
@@ -968,11 +1065,19 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                 else
                     x[s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
             });
-            SyncLds::run();
-            line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            if constexpr (SHARE) {
+                line_fft_xa<P, 1>(x, tid, lds_offset(&n0l[0][0]), tw);  // in the N0 slots, read above
+            } else {
+                SyncLds::run();
+                line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            }
             for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + g * F + off + pos, x[s]); });
         }
-        SyncLds::run();
+        if constexpr (SHARE) {
+            if (more) fetch(&n0l[0][0], N0, 3, nxt);
+        } else {
+            SyncLds::run();
+        }
     }
     // this block's sum of |div V| over its lines (FP64; lanes in a fixed order)
 #pragma unroll
@@ -1423,7 +1528,7 @@ int launch_fourier_x(sh_ns_s* s) {
 // kernel E with LDS-streamed inputs: a persistent grid of one-wave blocks, as
 // many as are resident, at most one per line; with a body force the
 // register-loaded kernel of the same plan runs instead
-template <typename T, class P, bool NT = true>
+template <typename T, class P, bool NT = true, bool SHARE = false>
 int launch_fourier_xd(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     if (s->forcing) return launch_fourier_x<T, P, 1, true, false, NT>(s);
@@ -1437,7 +1542,7 @@ int launch_fourier_xd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT>, 64, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, SHARE>, 64, 0) != hipSuccess ||
             per <= 0)
             per = 1;
         resident = v * per;
@@ -1448,7 +1553,7 @@ int launch_fourier_xd(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT>), dim3(nb, (unsigned)s->nboxes), dim3(64), 0, s->st,
+    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, SHARE>), dim3(nb, (unsigned)s->nboxes), dim3(64), 0, s->st,
                        (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz, nzp,
                        s->N * s->nboxes, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_xd");
@@ -1506,8 +1611,10 @@ int launch_fourier_xd(sh_ns_s* s) {
 #define SH_FOURIER_F32_BATCHED_VARIANTS(X) X(64, W64, 16) X(256, Q256, 8) X(256, W256, 4)
 // kernel E with nontemporal stores (V, N0 and the emitted fields)
 #define SH_FOURIER_F64_NT_VARIANTS(X) X(256, W256, 2) X(256, W256, 4) X(256, W256, 1)
-// kernel E with its inputs streamed into LDS (k_ns_fourier_xd): X(n, plan, nontemporal stores)
-#define SH_FOURIER_F64_LDS_VARIANTS(X) X(256, W256, true) X(256, W256, false)
+// kernel E with its inputs streamed into LDS (k_ns_fourier_xd): X(n, plan,
+// nontemporal stores, exchanges in the consumed slots)
+#define SH_FOURIER_F64_LDS_VARIANTS(X) \
+    X(256, W256, true, false) X(256, W256, false, false) X(256, W256, true, true) X(256, W256, false, true)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -1600,9 +1707,9 @@ const NsVariants& fourier_variants(int n) {
 #define SH_VT(n_, P_, L_)                                             \
     m[n_].names.push_back(#P_ " lines=" #L_ " nontemporal stores");   \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, true, false, true>);
-#define SH_VS(n_, P_, NT_)                                                       \
-    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_); \
-    m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_>);
+#define SH_VS(n_, P_, NT_, SH_)                                                                     \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_ ", shared exchange=" #SH_); \
+    m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_, SH_>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
