@@ -349,6 +349,50 @@ __device__ __forceinline__ void lds_get_c(C2<T>* x, int tid, const C2<T>* buf, c
     }
 }
 
+// XOR-swizzled index of a line of 16-byte values (a permutation of 0..N-1
+// for N a multiple of 64): bits 0..3 of p flipped by bits 3..5, so that the
+// puts and gets of the radix-4 and radix-2 stages of a line held by one wave
+// (32 or 64 threads) and the reads of a whole stage touch distinct LDS banks
+// in groups of 8 and of 16 lanes; positions r 64 apart keep their offsets
+struct SwzIdx {
+    __device__ __forceinline__ int operator()(int p) const {
+        const int h = p >> 3;
+        return p ^ ((h & 3) | (((h ^ (h >> 1)) & 1) << 2) | ((h & 4) << 1));
+    }
+};
+
+// Stage-major twiddle table: for each stage s with Ns > 1 the factors
+// w^(k r step_s), r = 1..R-1, k = 0..Ns-1, at sm_off(s) + (r - 1) Ns + k, so
+// the threads of a stage read consecutive entries (no bank conflicts); the
+// factors are those stage_bfly reads from the natural table
+template <class P, int s>
+__host__ __device__ constexpr int sm_off() {
+    if constexpr (s <= 1)
+        return 0;
+    else
+        return sm_off<P, s - 1>() + (P::ns(s - 1) > 1 ? (P::radix(s - 1) - 1) * P::ns(s - 1) : 0);
+}
+template <class P>
+__host__ __device__ constexpr int sm_size() {
+    return sm_off<P, P::S>();
+}
+
+// the stage-major table from the natural one (entries t0, t0 + stride, ...)
+template <class P, int s, typename T>
+__device__ __forceinline__ void sm_fill(C2<T>* dst, const C2<T>* __restrict__ tw, int t0, int stride) {
+    if constexpr (s < P::S) {
+        constexpr int R = P::radix(s), Ns = P::ns(s);
+        if constexpr (Ns > 1) {
+            constexpr int step = P::N / (Ns * R);
+            for (int e = t0; e < (R - 1) * Ns; e += stride) {
+                const int r = e / Ns + 1, k = e % Ns;
+                dst[sm_off<P, s>() + e] = tw[k * r * step];
+            }
+        }
+        sm_fill<P, s + 1, T>(dst, tw, t0, stride);
+    }
+}
+
 template <class P, int SIGN, typename T, class Idx, class Sync = SyncBlock>
 __device__ __forceinline__ void line_fft_c(C2<T>* x, int tid, C2<T>* buf, const Idx& idx,
                                            const C2<T>* __restrict__ tw) {

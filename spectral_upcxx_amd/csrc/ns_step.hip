@@ -702,11 +702,12 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             sum += k0 * k0;
             sum += k1 * k1;
             sum += k2 * k2;
-            if (sum != 0) {
-                const T den = T(0) - T(1) * sum;
-                G.x /= den;
-                G.y /= den;
-            }
+            // one reciprocal per point for each factor, shared by the components
+            // (within an ulp of dividing each component)
+            const T rg = sum != 0 ? T(1) / (T(0) - T(1) * sum) : T(1);
+            G.x *= rg;
+            G.y *= rg;
+            const double rv = 1.0 / (1.0 - (-cf.c1d) * (double)sum);  // 1 where sum == 0
             const T kk3[3] = {k0, k1, k2};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -714,17 +715,14 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
                 C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
                 C2<T> vv = cf.be ? vh : cadd<T>(vh, L[d]);
                 // V[d].laplacian_inverse(1, -nu*K/2): divide by 1 + (nu K/2)|k|^2 (FP64 factor)
-                if (sum != 0) {
-                    const double den = 1.0 - (-cf.c1d) * (double)sum;
-                    vv = mk<T>(T((double)vv.x / den), T((double)vv.y / den));
-                }
+                vv = mk<T>(T((double)vv.x * rv), T((double)vv.y * rv));
                 vn[d][s] = vv;
             }
             C2<T> dv = mk<T>(T(0), T(0));
             dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
             dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
             dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
-            dsum += dw * hypot((double)dv.x, (double)dv.y);
+            dsum += dw * sqrt((double)dv.x * (double)dv.x + (double)dv.y * (double)dv.y);  // |div V| (no overflow here)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 if constexpr (BL) {
@@ -780,11 +778,12 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
 }
 
 // A one-wave line transform whose LDS exchanges are inline asm: interleaved
-// complex values (one ds_write/ds_read_b128 each) at the padded index
-// pos + pos/16 of a buffer at LDS address buf.  The compiler sees no LDS access
-// in it, so it inserts no wait for loads into LDS in flight meanwhile (which
-// it would drain at once, vmcnt(0), before any access it cannot tell apart
-// from their targets); the caller orders the buffer against those loads.  The
+// complex values (one ds_write/ds_read_b128 each) at the swizzled index
+// SwzIdx of a buffer of N values at LDS address buf, and the twiddles from a
+// stage-major table at LDS address twb.  The compiler sees no LDS access in
+// it, so it inserts no wait for loads into LDS in flight meanwhile (which it
+// would drain at once, vmcnt(0), before any access it cannot tell apart from
+// their targets); the caller orders the buffer against those loads.  The
 // block is one wave, whose LDS instructions execute in issue order, so a put
 // needs no wait before the get that follows it; each get waits for its reads.
 typedef double xa_d2 __attribute__((ext_vector_type(2)));
@@ -795,52 +794,161 @@ __device__ __forceinline__ void xa_put(const C2<double>* x, int tid, unsigned bu
     const int base = (tid / Ns) * Ns * R + (tid % Ns);
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const int p = base + r * Ns;
-        const unsigned a = buf + 16u * (unsigned)(p + (p >> 4));
+        const unsigned a = buf + 16u * (unsigned)SwzIdx()(base + r * Ns);
         const xa_d2 v = {x[r].x, x[r].y};
         asm volatile("ds_write_b128 %0, %1" ::"v"(a), "v"(v) : "memory");
     }
 }
 
-// the values tid + r N/4 (r = 0..3) of the next radix-4 stage
-template <class P>
-__device__ __forceinline__ void xa_get(C2<double>* x, int tid, unsigned buf) {
+// the values tid + r N/4 (r = 0..3) of radix-4 stage s and, in the same asm
+// block (one wait for both), this lane's twiddles of that stage (k = tid mod Ns)
+template <class P, int s>
+__device__ __forceinline__ void xa_get(C2<double>* x, C2<double>* w, int tid, unsigned buf, unsigned twb) {
     constexpr int Q = P::N / 4;
-    constexpr int D = 16 * (Q + Q / 16);
-    static_assert(Q % 16 == 0 && 3 * D < 65536, "constant offsets between a lane's four values");
-    const unsigned a = buf + 16u * (unsigned)(tid + (tid >> 4));
-    xa_d2 v0, v1, v2, v3;
+    constexpr int D = 16 * Q;
+    constexpr int Ns = P::ns(s);
+    static_assert(Q % 64 == 0 && 3 * D < 65536 && 32 * Ns < 65536,
+                  "constant offsets between a lane's four values and three twiddles");
+    const unsigned a = buf + 16u * (unsigned)SwzIdx()(tid);
+    const unsigned t = twb + 16u * (unsigned)(sm_off<P, s>() + tid % Ns);
+    xa_d2 v0, v1, v2, v3, w1, w2, w3;
     asm volatile(
-        "ds_read_b128 %0, %4\n\t"
-        "ds_read_b128 %1, %4 offset:%5\n\t"
-        "ds_read_b128 %2, %4 offset:%6\n\t"
-        "ds_read_b128 %3, %4 offset:%7\n\t"
+        "ds_read_b128 %0, %7\n\t"
+        "ds_read_b128 %1, %7 offset:%9\n\t"
+        "ds_read_b128 %2, %7 offset:%10\n\t"
+        "ds_read_b128 %3, %7 offset:%11\n\t"
+        "ds_read_b128 %4, %8\n\t"
+        "ds_read_b128 %5, %8 offset:%12\n\t"
+        "ds_read_b128 %6, %8 offset:%13\n\t"
         "s_waitcnt lgkmcnt(0)"
-        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
-        : "v"(a), "n"(D), "n"(2 * D), "n"(3 * D)
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3), "=&v"(w1), "=&v"(w2), "=&v"(w3)
+        : "v"(a), "v"(t), "n"(D), "n"(2 * D), "n"(3 * D), "n"(16 * Ns), "n"(32 * Ns)
         : "memory");
     x[0] = mk<double>(v0.x, v0.y);
     x[1] = mk<double>(v1.x, v1.y);
     x[2] = mk<double>(v2.x, v2.y);
     x[3] = mk<double>(v3.x, v3.y);
+    w[0] = mk<double>(w1.x, w1.y);
+    w[1] = mk<double>(w2.x, w2.y);
+    w[2] = mk<double>(w3.x, w3.y);
+}
+
+// radix-4 butterfly of a stage after the first: twiddles w (conjugated for the
+// inverse sign), then the 4-point DFT -- stage_bfly's arithmetic
+template <int SIGN>
+__device__ __forceinline__ void xa_bfly(C2<double>* x, const C2<double>* w) {
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+        C2<double> t = w[r - 1];
+        if constexpr (SIGN > 0) t.y = -t.y;
+        x[r] = cmul<double>(x[r], t);
+    }
+    Dft<double, 4, SIGN>::run(x);
+}
+
+// tws: the stage-major twiddle table (sm_fill)
+template <class P, int SIGN>
+__device__ __forceinline__ void line_fft_xa(C2<double>* x, int tid, unsigned buf,
+                                            const C2<double>* __restrict__ tws) {
+    static_assert(P::TPL == 64 && P::S == 4 && P::radix(0) == 4 && P::radix(1) == 4 && P::radix(2) == 4 &&
+                      P::radix(3) == 4 && P::N == 256 && P::ns(0) == 1,
+                  "four radix-4 stages, one butterfly per lane");
+    const unsigned twb = lds_offset(tws);
+    C2<double> w[3];
+    stage_bfly<P, 0, SIGN, double>(x, tid, tws);  // no twiddles in the first stage
+    xa_put<P, 0>(x, tid, buf);
+    xa_get<P, 1>(x, w, tid, buf, twb);
+    xa_bfly<SIGN>(x, w);
+    xa_put<P, 1>(x, tid, buf);
+    xa_get<P, 2>(x, w, tid, buf, twb);
+    xa_bfly<SIGN>(x, w);
+    xa_put<P, 2>(x, tid, buf);
+    xa_get<P, 3>(x, w, tid, buf, twb);
+    xa_bfly<SIGN>(x, w);
+}
+
+// Three lines transformed together by one wave, stage by stage (three times
+// the independent arithmetic between LDS round trips): line d exchanges in
+// the buffer at LDS address buf + d * 16 N, all with the same twiddles.
+template <class P, int s>
+__device__ __forceinline__ void xa_put3(const C2<double> (&x)[3][4], int tid, unsigned buf) {
+    constexpr int R = P::radix(s), Ns = P::ns(s);
+    const int base = (tid / Ns) * Ns * R + (tid % Ns);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned a = buf + 16u * (unsigned)SwzIdx()(base + r * Ns);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const xa_d2 v = {x[d][r].x, x[d][r].y};
+            asm volatile("ds_write_b128 %0, %1 offset:%2" ::"v"(a), "v"(v), "n"(d * 16 * P::N) : "memory");
+        }
+    }
+}
+
+template <class P, int s>
+__device__ __forceinline__ void xa_get3(C2<double> (&x)[3][4], C2<double>* w, int tid, unsigned buf,
+                                        unsigned twb) {
+    constexpr int Q = P::N / 4;
+    constexpr int D = 16 * Q;
+    constexpr int Ns = P::ns(s);
+    static_assert(Q % 64 == 0 && 2 * 16 * P::N + 3 * D < 65536 && 32 * Ns < 65536, "constant offsets");
+    const unsigned a0 = buf + 16u * (unsigned)SwzIdx()(tid);
+    const unsigned a1 = a0 + 16u * P::N, a2 = a1 + 16u * P::N;
+    const unsigned t = twb + 16u * (unsigned)(sm_off<P, s>() + tid % Ns);
+    xa_d2 v[3][4], w1, w2, w3;
+    asm volatile(
+        "ds_read_b128 %0, %15\n\t"
+        "ds_read_b128 %1, %15 offset:%19\n\t"
+        "ds_read_b128 %2, %15 offset:%20\n\t"
+        "ds_read_b128 %3, %15 offset:%21\n\t"
+        "ds_read_b128 %4, %16\n\t"
+        "ds_read_b128 %5, %16 offset:%19\n\t"
+        "ds_read_b128 %6, %16 offset:%20\n\t"
+        "ds_read_b128 %7, %16 offset:%21\n\t"
+        "ds_read_b128 %8, %17\n\t"
+        "ds_read_b128 %9, %17 offset:%19\n\t"
+        "ds_read_b128 %10, %17 offset:%20\n\t"
+        "ds_read_b128 %11, %17 offset:%21\n\t"
+        "ds_read_b128 %12, %18\n\t"
+        "ds_read_b128 %13, %18 offset:%22\n\t"
+        "ds_read_b128 %14, %18 offset:%23\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v[0][0]), "=&v"(v[0][1]), "=&v"(v[0][2]), "=&v"(v[0][3]), "=&v"(v[1][0]), "=&v"(v[1][1]),
+          "=&v"(v[1][2]), "=&v"(v[1][3]), "=&v"(v[2][0]), "=&v"(v[2][1]), "=&v"(v[2][2]), "=&v"(v[2][3]),
+          "=&v"(w1), "=&v"(w2), "=&v"(w3)
+        : "v"(a0), "v"(a1), "v"(a2), "v"(t), "n"(D), "n"(2 * D), "n"(3 * D), "n"(16 * Ns), "n"(32 * Ns)
+        : "memory");
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) x[d][r] = mk<double>(v[d][r].x, v[d][r].y);
+    w[0] = mk<double>(w1.x, w1.y);
+    w[1] = mk<double>(w2.x, w2.y);
+    w[2] = mk<double>(w3.x, w3.y);
 }
 
 template <class P, int SIGN>
-__device__ __forceinline__ void line_fft_xa(C2<double>* x, int tid, unsigned buf,
-                                            const C2<double>* __restrict__ tw) {
+__device__ __forceinline__ void line_fft_xa3(C2<double> (&x)[3][4], int tid, unsigned buf,
+                                             const C2<double>* __restrict__ tws) {
     static_assert(P::TPL == 64 && P::S == 4 && P::radix(0) == 4 && P::radix(1) == 4 && P::radix(2) == 4 &&
-                      P::radix(3) == 4 && P::N == 256,
+                      P::radix(3) == 4 && P::N == 256 && P::ns(0) == 1,
                   "four radix-4 stages, one butterfly per lane");
-    stage_bfly<P, 0, SIGN, double>(x, tid, tw);
-    xa_put<P, 0>(x, tid, buf);
-    xa_get<P>(x, tid, buf);
-    stage_bfly<P, 1, SIGN, double>(x, tid, tw);
-    xa_put<P, 1>(x, tid, buf);
-    xa_get<P>(x, tid, buf);
-    stage_bfly<P, 2, SIGN, double>(x, tid, tw);
-    xa_put<P, 2>(x, tid, buf);
-    xa_get<P>(x, tid, buf);
-    stage_bfly<P, 3, SIGN, double>(x, tid, tw);
+    const unsigned twb = lds_offset(tws);
+    C2<double> w[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) stage_bfly<P, 0, SIGN, double>(x[d], tid, tws);  // no twiddles
+    xa_put3<P, 0>(x, tid, buf);
+    xa_get3<P, 1>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xa_bfly<SIGN>(x[d], w);
+    xa_put3<P, 1>(x, tid, buf);
+    xa_get3<P, 2>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xa_bfly<SIGN>(x[d], w);
+    xa_put3<P, 2>(x, tid, buf);
+    xa_get3<P, 3>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xa_bfly<SIGN>(x[d], w);
 }
 
 // The exchange buffer of a line transform with LDS of its own (allocated for
@@ -866,10 +974,13 @@ __device__ __forceinline__ double* xd_exchange() {
 // SHARE: the exchanges of the product transforms use the product slots (read
 // into registers first) and those of the emitted inverse transforms the N0
 // slots, whose next loads are then issued after the emitted transforms; the
-// exchanges are the inline-asm ones of line_fft_xa.  The block's LDS shrinks
+// exchanges are the inline-asm ones of line_fft_xa (swizzled exchange index,
+// stage-major twiddles).  The block's LDS shrinks
 // from 45.3 KB to 40 KB, so four one-wave blocks fit on a compute unit (one
-// per SIMD) instead of three.
-template <class P, bool NT = true, bool SHARE = false>
+// per SIMD) instead of three.  G3 (with SHARE): the three product transforms,
+// and the emitted ones in two groups of three, run together stage by stage
+// (line_fft_xa3).
+template <class P, bool NT = true, bool SHARE = false, bool G3 = false>
 __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
                                                       double2* __restrict__ N0, const double2* __restrict__ twg,
                                                       int ny, int nz, int nzp, long F,
@@ -885,7 +996,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
     static_assert(P::TPL == 64 && R0 == 4 && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == 4 &&
                       P::nb(P::S - 1) == 1 && N % 64 == 0 && 3 * Q * 16 < 65536,
                   "one wave per line, one radix-4 butterfly per lane in the first and last stage");
-    static_assert(!SHARE || (N + N / 16) * sizeof(double2) <= 3 * N * sizeof(double2), "an exchange fits in three slots");
+    static_assert(!SHARE || sm_size<P>() <= N, "stage-major twiddles fit the table");
     // vector-memory operations a line issues after its products' loads (the
     // next V1, the 6 + 6 field stores and, when SHARE, the next N0), capped at
     // what vmcnt encodes
@@ -904,7 +1015,10 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
     if constexpr (!SHARE) sre = xd_exchange<LS>();
     T* const sim = SHARE ? nullptr : sre + LS;
     const int tid = threadIdx.x;
-    for (int t = tid; t < N; t += 64) tw[t] = twg[t];
+    if constexpr (SHARE)
+        sm_fill<P, 1, T>(tw, twg, tid, 64);  // stage-major, for line_fft_xa
+    else
+        for (int t = tid; t < N; t += 64) tw[t] = twg[t];
     __syncthreads();
     const long nlines = (long)ny * nzp;
     auto idx = [&](int pos) { return pos + (pos >> 4); };
@@ -962,8 +1076,12 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             for (int d = 0; d < 3; d++) read4(&xs[d][0], n1[d]);
             // the exchanges reuse the product slots; their last reads have
             // completed before the next products are loaded there
+            if constexpr (G3) {
+                line_fft_xa3<P, -1>(n1, tid, lds_offset(&xs[0][0]), tw);
+            } else {
 #pragma unroll
-            for (int d = 0; d < 3; d++) line_fft_xa<P, -1>(n1[d], tid, lds_offset(&xs[0][0]), tw);
+                for (int d = 0; d < 3; d++) line_fft_xa<P, -1>(n1[d], tid, lds_offset(&xs[0][0]), tw);
+            }
         } else {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -1022,11 +1140,12 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             sum += k0 * k0;
             sum += k1 * k1;
             sum += k2 * k2;
-            if (sum != 0) {
-                const T den = T(0) - T(1) * sum;
-                G.x /= den;
-                G.y /= den;
-            }
+            // one reciprocal per point for each factor, shared by the components
+            // (within an ulp of dividing each component)
+            const T rg = sum != 0 ? T(1) / (T(0) - T(1) * sum) : T(1);
+            G.x *= rg;
+            G.y *= rg;
+            const double rv = 1.0 / (1.0 - (-cf.c1d) * (double)sum);  // 1 where sum == 0
             const T kk3[3] = {k0, k1, k2};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -1034,17 +1153,14 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                 C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
                 C2<T> vv = cf.be ? vh : cadd<T>(vh, L[d]);
                 // V[d].laplacian_inverse(1, -nu*K/2): divide by 1 + (nu K/2)|k|^2 (FP64 factor)
-                if (sum != 0) {
-                    const double den = 1.0 - (-cf.c1d) * (double)sum;
-                    vv = mk<T>(T((double)vv.x / den), T((double)vv.y / den));
-                }
+                vv = mk<T>(T((double)vv.x * rv), T((double)vv.y * rv));
                 vn[d][s] = vv;
             }
             C2<T> dv = mk<T>(T(0), T(0));
             dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
             dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
             dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
-            dsum += dw * hypot((double)dv.x, (double)dv.y);
+            dsum += dw * sqrt((double)dv.x * (double)dv.x + (double)dv.y * (double)dv.y);  // |div V| (no overflow here)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
@@ -1052,8 +1168,30 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             }
         });
         // the next step's kernel A: inverse x of V (g = 0..2) and of i k x V (g = 3..5)
+        if constexpr (G3) {
+            static_assert(SHARE && E == 4, "grouped transforms in the N0 slots");
+            C2<T> x3[3][E];
 #pragma unroll
-        for (int g = 0; g < 6; g++) {
+            for (int h = 0; h < 2; h++) {
+                for_out<P>(tid, [&](int s, int pos) {
+                    const T k0 = freq<T>(pos, N);
+                    if (h == 0) {
+#pragma unroll
+                        for (int d = 0; d < 3; d++) x3[d][s] = vn[d][s];
+                    } else {
+                        x3[0][s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
+                        x3[1][s] = csub<T>(imul<T>(k2, vn[0][s]), imul<T>(k0, vn[2][s]));
+                        x3[2][s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
+                    }
+                });
+                line_fft_xa3<P, 1>(x3, tid, lds_offset(&n0l[0][0]), tw);  // in the N0 slots, read above
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+                    for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + (3 * h + d) * F + off + pos, x3[d][s]); });
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < (G3 ? 0 : 6); g++) {
             for_out<P>(tid, [&](int s, int pos) {
                 const T k0 = freq<T>(pos, N);
                 if (g < 3)
@@ -1528,7 +1666,7 @@ int launch_fourier_x(sh_ns_s* s) {
 // kernel E with LDS-streamed inputs: a persistent grid of one-wave blocks, as
 // many as are resident, at most one per line; with a body force the
 // register-loaded kernel of the same plan runs instead
-template <typename T, class P, bool NT = true, bool SHARE = false>
+template <typename T, class P, bool NT = true, bool SHARE = false, bool G3 = false>
 int launch_fourier_xd(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     if (s->forcing) return launch_fourier_x<T, P, 1, true, false, NT>(s);
@@ -1542,7 +1680,7 @@ int launch_fourier_xd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, SHARE>, 64, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, SHARE, G3>, 64, 0) != hipSuccess ||
             per <= 0)
             per = 1;
         resident = v * per;
@@ -1553,7 +1691,7 @@ int launch_fourier_xd(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, SHARE>), dim3(nb, (unsigned)s->nboxes), dim3(64), 0, s->st,
+    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, SHARE, G3>), dim3(nb, (unsigned)s->nboxes), dim3(64), 0, s->st,
                        (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz, nzp,
                        s->N * s->nboxes, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_xd");
@@ -1612,9 +1750,10 @@ int launch_fourier_xd(sh_ns_s* s) {
 // kernel E with nontemporal stores (V, N0 and the emitted fields)
 #define SH_FOURIER_F64_NT_VARIANTS(X) X(256, W256, 2) X(256, W256, 4) X(256, W256, 1)
 // kernel E with its inputs streamed into LDS (k_ns_fourier_xd): X(n, plan,
-// nontemporal stores, exchanges in the consumed slots)
-#define SH_FOURIER_F64_LDS_VARIANTS(X) \
-    X(256, W256, true, false) X(256, W256, false, false) X(256, W256, true, true) X(256, W256, false, true)
+// nontemporal stores, exchanges in the consumed slots, grouped transforms)
+#define SH_FOURIER_F64_LDS_VARIANTS(X)                                                        \
+    X(256, W256, true, false, false) X(256, W256, false, false, false) X(256, W256, true, true, false) \
+    X(256, W256, false, true, false) X(256, W256, true, true, true)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -1707,9 +1846,10 @@ const NsVariants& fourier_variants(int n) {
 #define SH_VT(n_, P_, L_)                                             \
     m[n_].names.push_back(#P_ " lines=" #L_ " nontemporal stores");   \
     m[n_].fns.push_back(&launch_fourier_x<T, P_, L_, true, false, true>);
-#define SH_VS(n_, P_, NT_, SH_)                                                                     \
-    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_ ", shared exchange=" #SH_); \
-    m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_, SH_>);
+#define SH_VS(n_, P_, NT_, SH_, G3_)                                                               \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_ ", shared exchange=" #SH_ \
+                          ", grouped=" #G3_);                                                          \
+    m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_, SH_, G3_>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
