@@ -330,7 +330,10 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__
 // and a barrier.  The half-spectrum packing reads X[k] and X[M-k] straight
 // from the slot, so it needs no exchange of its own.  The twiddle tables are
 // copied into LDS once per block, so the only vector-memory operations left
-// are the slot loads and the product stores.
+// are the slot loads and the product stores.  HALF = false: the same kernel
+// for the full spectrum (the plain C2C method, k_ns_cross_z's arithmetic): a
+// z-line is one length-nz transform, with no packing, the products are full
+// complex products, and the forward transforms are stored as they are.
 template <int N>
 __device__ __forceinline__ void vm_wait_barrier() {
     asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(N) : "memory");
@@ -341,7 +344,7 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
     return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)p;
 }
 
-template <class P, int LPB, bool NT = false, bool IL = false>
+template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict__ W,
                                                              const double2* __restrict__ twg,
                                                              const double2* __restrict__ twng, int nx,
@@ -369,11 +372,11 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
     __shared__ T sim[IL ? 1 : M * CS];
     double2* const sx = reinterpret_cast<double2*>(sre);
     __shared__ double2 tw[M];
-    __shared__ double2 twn[M];
+    __shared__ double2 twn[HALF ? M : 1];
     W += blockIdx.y * boff;
     for (int t = threadIdx.x; t < M; t += NTH) {
         tw[t] = twg[t];
-        twn[t] = twng[t];
+        if constexpr (HALF) twn[t] = twng[t];
     }
     __syncthreads();
     const int c = threadIdx.x % LPB;
@@ -411,6 +414,24 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
     // inverse-transformed, scaled by 1/(Nx Ny Nz)
     typedef double d2v __attribute__((ext_vector_type(2)));
     auto inv = [&](int sl, C2<T>* x) {
+        if constexpr (!HALF) {
+            // the full spectrum: this thread's first-stage values as they are
+            const unsigned own = lds_offset(&slot[sl][0]) + (unsigned)(tid * LPB + c) * 16u;
+            d2v a0, a1, a2, a3;
+            asm volatile(
+                "ds_read_b128 %0, %4\n\t"
+                "ds_read_b128 %1, %4 offset:%5\n\t"
+                "ds_read_b128 %2, %4 offset:%6\n\t"
+                "ds_read_b128 %3, %4 offset:%7\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3)
+                : "v"(own), "n"(Q * ROWB), "n"(2 * Q * ROWB), "n"(3 * Q * ROWB)
+                : "memory");
+            x[0] = mk<T>(a0.x, a0.y);
+            x[1] = mk<T>(a1.x, a1.y);
+            x[2] = mk<T>(a2.x, a2.y);
+            x[3] = mk<T>(a3.x, a3.y);
+        } else {
         const unsigned base = lds_offset(&slot[sl][0]);
         const unsigned own = base + (unsigned)(tid * LPB + c) * 16u;
         const unsigned mir = base + (unsigned)((tid == 0 ? Q : Q - tid) * LPB + c) * 16u;
@@ -442,12 +463,20 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
             const T ty = w.x * dif.y - w.y * dif.x;
             x[r] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
         }
+        }
         SyncLds::run();
         fft(std::integral_constant<int, 1>{}, x);
 #pragma unroll
         for (int s2 = 0; s2 < E; s2++) x[s2] = rmul<T>(scale, x[s2]);
     };
-    auto pmul = [](C2<T> a, C2<T> b) { return mk<T>(a.x * b.x, a.y * b.y); };
+    // the product of two fields: real and imaginary parts separately (two grid
+    // points of the packed half spectrum) or the full complex product
+    auto pmul = [](C2<T> a, C2<T> b) {
+        if constexpr (HALF)
+            return mk<T>(a.x * b.x, a.y * b.y);
+        else
+            return cmul<T>(a, b);
+    };
     // fields in the order w0 w1 u2 w2 u0 u1 (3 4 2 5 0 1), slots 0 1 2 0 1 2
     int t = blockIdx.x;
     if (t < ntile) {
@@ -511,6 +540,13 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
             for (int s2 = 0; s2 < E; s2++) x[s2] = o[d][s2];
             SyncLds::run();
             fft(std::integral_constant<int, -1>{}, x);
+            if constexpr (!HALF) {
+                if (valid)
+                    for_out<P>(tid, [&](int s2, int pos) {
+                        st_line<NT, T>(W + (cb + (unsigned)d * uF + (unsigned)pos * es), x[s2]);
+                    });
+                continue;
+            }
             // Z -> X (Z[M-k] from another thread through LDS)
             SyncLds::run();
             for_out<P>(tid, [&](int s2, int pos) {
@@ -1595,17 +1631,17 @@ int launch_cross_zh(sh_ns_s* s) {
 
 // kernel C of the half-spectrum layout with LDS-streamed inputs (FP64): a
 // persistent grid of the resident blocks looping over the column tiles
-template <typename T, class P, int LPB, bool NT = false, bool IL = false>
+template <typename T, class P, int LPB, bool NT = false, bool IL = false, bool HALF = true>
 int launch_cross_zd(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
-    const C2<T>* twn = (const C2<T>*)twiddles(s->nz, SH_F64);
+    const C2<T>* twn = HALF ? (const C2<T>*)twiddles(s->nz, SH_F64) : tw;
     if (!tw || !twn) {
         set_error("ns: twiddle table unavailable (first use inside a graph capture?)");
         return SH_ENOMEM;
     }
-    if (2 * P::N != s->nz) {
-        set_error("ns: half-spectrum z kernel of the wrong length");
+    if ((HALF ? 2 : 1) * P::N != s->nz) {
+        set_error("ns: fused z kernel of the wrong length");
         return SH_EINVAL;
     }
     if (6 * s->N * s->nboxes > 0xffffffffL) {  // 32-bit element offsets, as kernel C
@@ -1620,7 +1656,7 @@ int launch_cross_zd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL>, P::TPL * LPB, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF>, P::TPL * LPB, 0) !=
                 hipSuccess ||
             per <= 0)
             per = 1;
@@ -1632,7 +1668,7 @@ int launch_cross_zd(sh_ns_s* s) {
     dim3 grid((unsigned)g, (unsigned)s->nboxes), block(P::TPL * LPB);
     const double scale = 1.0 / ((double)s->nx * (double)s->ny * (double)s->nz);
     ProfScope ps("ns_cross_z", s->cst);
-    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL>), grid, block, 0, s->cst, (double2*)s->Wf + s->cx0,
+    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF>), grid, block, 0, s->cst, (double2*)s->Wf + s->cx0,
                        (const double2*)tw, (const double2*)twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale,
                        s->N, ntile);
     SH_LAUNCH_CHECK("k_ns_cross_zd");
@@ -1725,6 +1761,9 @@ int launch_fourier_xd(sh_ns_s* s) {
     X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4)                               \
     X(256, Q256, 4) X(512, P512, 4) X(512, W512, 2)
 
+// kernel C of the full spectrum with its inputs streamed into LDS
+// (k_ns_cross_zd with HALF = false, interleaved exchange): X(nz, plan, columns per block)
+#define SH_CROSS_F64_LDS_VARIANTS(X) X(128, W128, 8) X(256, W256, 8)
 // kernel C of the half-spectrum layout: X(nz, length-nz/2 plan, columns per block)
 #define SH_CROSSH_F64_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
@@ -1776,11 +1815,18 @@ const NsVariants& cross_variants(int n) {
 #define SH_V(n_, P_, C_)                           \
     m[n_].names.push_back(#P_ " columns=" #C_);    \
     m[n_].fns.push_back(&launch_cross_z<T, P_, C_>);
+#define SH_VF(n_, P_, C_)                                                                   \
+    m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange");          \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, false>);                     \
+    m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange nontemporal"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, false>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSS_F64_VARIANTS(SH_V)
+            SH_CROSS_F64_LDS_VARIANTS(SH_VF)
         } else {
             SH_CROSS_F32_VARIANTS(SH_V)
         }
+#undef SH_VF
 #undef SH_V
         return m;
     }();
