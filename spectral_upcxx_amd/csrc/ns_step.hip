@@ -344,13 +344,12 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
     return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)p;
 }
 
-template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true>
-__global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict__ W,
-                                                             const double2* __restrict__ twg,
-                                                             const double2* __restrict__ twng, int nx,
-                                                             int ncx, int ny, long F, double scale, long boff,
+template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(C2<T>* __restrict__ W,
+                                                             const C2<T>* __restrict__ twg,
+                                                             const C2<T>* __restrict__ twng, int nx,
+                                                             int ncx, int ny, long F, T scale, long boff,
                                                              int ntile) {
-    using T = double;
     constexpr int M = P::N;
     constexpr int R0 = P::radix(0);
     constexpr int Q = M / R0;
@@ -358,21 +357,25 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
     constexpr int E = P::E;
     constexpr int NTH = P::TPL * LPB;
     constexpr int NW = NTH / 64;
-    constexpr int RPI = 64 / LPB;            // rows of a slot one wave instruction fills
+    constexpr unsigned VB = sizeof(C2<T>);   // bytes per value: 16 (FP64) or 8 (FP32)
+    constexpr int VPL = 16 / VB;             // values one lane loads (16 bytes)
+    constexpr int LPR = LPB / VPL;           // lanes per slot row
+    constexpr int RPI = 64 / LPR;            // rows of a slot one wave instruction fills
     constexpr int IPW = M / (RPI * NW);      // slot loads per wave (and per lane) per field
-    constexpr unsigned ROWB = LPB * 16;      // bytes per slot row
+    constexpr unsigned ROWB = LPB * VB;      // bytes per slot row
     static_assert(R0 == 4 && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == 4 && P::nb(P::S - 1) == 1,
                   "the slot reads assume one radix-4 butterfly per thread in the first and last stage");
-    static_assert(NTH % 64 == 0 && 64 % LPB == 0 && M % (RPI * NW) == 0 && (R0 - 1) * Q * ROWB < 65536,
+    static_assert(NTH % 64 == 0 && LPB % VPL == 0 && 64 % LPR == 0 && M % (RPI * NW) == 0 &&
+                      (R0 - 1) * Q * ROWB < 65536,
                   "slot geometry");
-    __shared__ double2 slot[3][M * LPB];
+    __shared__ C2<T> slot[3][M * LPB];
     // the FFT exchange: split re / im arrays with padded rows, or (IL)
     // interleaved complex values, rows of LPB columns
     __shared__ __align__(16) T sre[IL ? 2 * M * LPB : M * CS];
     __shared__ T sim[IL ? 1 : M * CS];
-    double2* const sx = reinterpret_cast<double2*>(sre);
-    __shared__ double2 tw[M];
-    __shared__ double2 twn[HALF ? M : 1];
+    C2<T>* const sx = reinterpret_cast<C2<T>*>(sre);
+    __shared__ C2<T> tw[M];
+    __shared__ C2<T> twn[HALF ? M : 1];
     W += blockIdx.y * boff;
     for (int t = threadIdx.x; t < M; t += NTH) {
         tw[t] = twg[t];
@@ -393,76 +396,84 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(double2* __restrict
         else
             line_fft<P, SG, T, decltype(idx), SyncLds>(v, tid, sre, sim, idx, tw);
     };
-    // field f of tile t into slot sl: lane -> (column lane % LPB, row lane / LPB)
-    // of each RPI-row group; the LDS destination of a wave instruction is
-    // uniform (M0) and the lanes land 16 bytes apart
+    // field f of tile t into slot sl: lane -> (columns VPL (lane % LPR) and on,
+    // row lane / LPR) of each RPI-row group; the LDS destination of a wave
+    // instruction is uniform (M0) and the lanes land 16 bytes apart
     auto fetch = [&](int t, int f, int sl) {
         const int j = t / tiles_x;
-        const int i = (t % tiles_x) * LPB + lane % LPB;
+        const int i = (t % tiles_x) * LPB + (lane % LPR) * VPL;
         const unsigned cb = i < ncx ? (unsigned)i + (unsigned)j * (unsigned)nx : 0u;
 #pragma unroll
         for (int k = 0; k < IPW; k++) {
             const int row0 = (wave * IPW + k) * RPI;
-            const double2* src = W + (cb + (unsigned)f * uF + (unsigned)(row0 + lane / LPB) * es);
+            const C2<T>* src = W + (cb + (unsigned)f * uF + (unsigned)(row0 + lane / LPR) * es);
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                              (__attribute__((address_space(3))) void*)&slot[sl][row0 * LPB], 16,
                                              0, 0);
         }
     };
+    // this thread's four values of a slot, rows tid + r Q from LDS address a,
+    // and (read8) their mirrors, rows (Q - tid) + (3 - r) Q from address m, in
+    // one asm block that waits for them
+    typedef T v2 __attribute__((ext_vector_type(2)));
+#define SH_ZD_READ(INS)                                                                              \
+    asm volatile(INS " %0, %4\n\t" INS " %1, %4 offset:%5\n\t" INS " %2, %4 offset:%6\n\t" INS      \
+                     " %3, %4 offset:%7\n\t"                                                           \
+                     "s_waitcnt lgkmcnt(0)"                                                         \
+                 : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3])                              \
+                 : "v"(a), "n"(Q * ROWB), "n"(2 * Q * ROWB), "n"(3 * Q * ROWB)                      \
+                 : "memory")
+#define SH_ZD_READ8(INS)                                                                             \
+    asm volatile(INS " %0, %8\n\t" INS " %1, %8 offset:%10\n\t" INS " %2, %8 offset:%11\n\t" INS    \
+                     " %3, %8 offset:%12\n\t" INS " %4, %9 offset:%12\n\t" INS " %5, %9 offset:%11\n\t" \
+                 INS " %6, %9 offset:%10\n\t" INS " %7, %9\n\t"                                      \
+                     "s_waitcnt lgkmcnt(0)"                                                         \
+                 : "=&v"(o[0]), "=&v"(o[1]), "=&v"(o[2]), "=&v"(o[3]), "=&v"(mo[0]), "=&v"(mo[1]),   \
+                   "=&v"(mo[2]), "=&v"(mo[3])                                                       \
+                 : "v"(a), "v"(m), "n"(Q * ROWB), "n"(2 * Q * ROWB), "n"(3 * Q * ROWB)              \
+                 : "memory")
+    auto read4 = [](unsigned a, v2* o) {
+        if constexpr (VB == 16)
+            SH_ZD_READ("ds_read_b128");
+        else
+            SH_ZD_READ("ds_read_b64");
+    };
+    auto read8 = [](unsigned a, unsigned m, v2* o, v2* mo) {
+        if constexpr (VB == 16)
+            SH_ZD_READ8("ds_read_b128");
+        else
+            SH_ZD_READ8("ds_read_b64");
+    };
+#undef SH_ZD_READ
+#undef SH_ZD_READ8
     // X[k] and X[M-k] of this thread's first-stage positions k = tid + r Q
     // from slot sl (X[M] = 0, the Nyquist plane), packed into Z and
     // inverse-transformed, scaled by 1/(Nx Ny Nz)
-    typedef double d2v __attribute__((ext_vector_type(2)));
     auto inv = [&](int sl, C2<T>* x) {
+        const unsigned base = lds_offset(&slot[sl][0]);
+        const unsigned own = base + (unsigned)(tid * LPB + c) * VB;
+        v2 av[4];
         if constexpr (!HALF) {
             // the full spectrum: this thread's first-stage values as they are
-            const unsigned own = lds_offset(&slot[sl][0]) + (unsigned)(tid * LPB + c) * 16u;
-            d2v a0, a1, a2, a3;
-            asm volatile(
-                "ds_read_b128 %0, %4\n\t"
-                "ds_read_b128 %1, %4 offset:%5\n\t"
-                "ds_read_b128 %2, %4 offset:%6\n\t"
-                "ds_read_b128 %3, %4 offset:%7\n\t"
-                "s_waitcnt lgkmcnt(0)"
-                : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3)
-                : "v"(own), "n"(Q * ROWB), "n"(2 * Q * ROWB), "n"(3 * Q * ROWB)
-                : "memory");
-            x[0] = mk<T>(a0.x, a0.y);
-            x[1] = mk<T>(a1.x, a1.y);
-            x[2] = mk<T>(a2.x, a2.y);
-            x[3] = mk<T>(a3.x, a3.y);
-        } else {
-        const unsigned base = lds_offset(&slot[sl][0]);
-        const unsigned own = base + (unsigned)(tid * LPB + c) * 16u;
-        const unsigned mir = base + (unsigned)((tid == 0 ? Q : Q - tid) * LPB + c) * 16u;
-        d2v a0, a1, a2, a3, m0, m1, m2, m3;
-        asm volatile(
-            "ds_read_b128 %0, %8\n\t"
-            "ds_read_b128 %1, %8 offset:%10\n\t"
-            "ds_read_b128 %2, %8 offset:%11\n\t"
-            "ds_read_b128 %3, %8 offset:%12\n\t"
-            "ds_read_b128 %4, %9 offset:%12\n\t"
-            "ds_read_b128 %5, %9 offset:%11\n\t"
-            "ds_read_b128 %6, %9 offset:%10\n\t"
-            "ds_read_b128 %7, %9\n\t"
-            "s_waitcnt lgkmcnt(0)"
-            : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(m0), "=&v"(m1), "=&v"(m2), "=&v"(m3)
-            : "v"(own), "v"(mir), "n"(Q * ROWB), "n"(2 * Q * ROWB), "n"(3 * Q * ROWB)
-            : "memory");
-        const d2v av[4] = {a0, a1, a2, a3};
-        const d2v mv[4] = {m0, m1, m2, m3};
+            read4(own, av);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int pos = tid + r * Q;
-            const C2<T> a = mk<T>(av[r].x, av[r].y);
-            const C2<T> b = pos == 0 ? mk<T>(T(0), T(0)) : mk<T>(mv[r].x, mv[r].y);
-            const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // X[k] + conj X[M-k]
-            const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // X[k] - conj X[M-k]
-            const C2<T> w = twn[pos];                       // e^{-2 pi i k/nz}
-            const T tx = w.x * dif.x + w.y * dif.y;         // t = conj(w) dif
-            const T ty = w.x * dif.y - w.y * dif.x;
-            x[r] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
-        }
+            for (int r = 0; r < 4; r++) x[r] = mk<T>(av[r].x, av[r].y);
+        } else {
+            const unsigned mir = base + (unsigned)((tid == 0 ? Q : Q - tid) * LPB + c) * VB;
+            v2 mv[4];  // X[M - k] for r = 0..3
+            read8(own, mir, av, mv);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int pos = tid + r * Q;
+                const C2<T> a = mk<T>(av[r].x, av[r].y);
+                const C2<T> b = pos == 0 ? mk<T>(T(0), T(0)) : mk<T>(mv[r].x, mv[r].y);
+                const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // X[k] + conj X[M-k]
+                const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // X[k] - conj X[M-k]
+                const C2<T> w = twn[pos];                       // e^{-2 pi i k/nz}
+                const T tx = w.x * dif.x + w.y * dif.y;         // t = conj(w) dif
+                const T ty = w.x * dif.y - w.y * dif.x;
+                x[r] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
+            }
         }
         SyncLds::run();
         fft(std::integral_constant<int, 1>{}, x);
@@ -1633,9 +1644,12 @@ int launch_cross_zh(sh_ns_s* s) {
 // persistent grid of the resident blocks looping over the column tiles
 template <typename T, class P, int LPB, bool NT = false, bool IL = false, bool HALF = true>
 int launch_cross_zd(sh_ns_s* s) {
-    static_assert(sizeof(T) == 8, "FP64 only");
-    const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
-    const C2<T>* twn = HALF ? (const C2<T>*)twiddles(s->nz, SH_F64) : tw;
+    constexpr int prec = sizeof(T) == 8 ? SH_F64 : SH_F32;
+    // an FP32 lane loads two neighbouring columns: an odd column count would
+    // read one value past the last column of the grid
+    if (sizeof(T) == 4 && (s->cnx % 2) != 0) return launch_cross_zh<T, P, LPB>(s);
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, prec);
+    const C2<T>* twn = HALF ? (const C2<T>*)twiddles(s->nz, prec) : tw;
     if (!tw || !twn) {
         set_error("ns: twiddle table unavailable (first use inside a graph capture?)");
         return SH_ENOMEM;
@@ -1656,8 +1670,8 @@ int launch_cross_zd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF>, P::TPL * LPB, 0) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF, T>, P::TPL * LPB,
+                                                         0) != hipSuccess ||
             per <= 0)
             per = 1;
         resident = v * per;
@@ -1666,11 +1680,10 @@ int launch_cross_zd(sh_ns_s* s) {
     const int ntile = tiles * s->ny;
     const int g = ntile < resident ? ntile : resident;
     dim3 grid((unsigned)g, (unsigned)s->nboxes), block(P::TPL * LPB);
-    const double scale = 1.0 / ((double)s->nx * (double)s->ny * (double)s->nz);
+    const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
     ProfScope ps("ns_cross_z", s->cst);
-    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF>), grid, block, 0, s->cst, (double2*)s->Wf + s->cx0,
-                       (const double2*)tw, (const double2*)twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale,
-                       s->N, ntile);
+    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF, T>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0,
+                       tw, twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale, s->N, ntile);
     SH_LAUNCH_CHECK("k_ns_cross_zd");
     return SH_OK;
 }
@@ -1764,6 +1777,7 @@ int launch_fourier_xd(sh_ns_s* s) {
 // kernel C of the full spectrum with its inputs streamed into LDS
 // (k_ns_cross_zd with HALF = false, interleaved exchange): X(nz, plan, columns per block)
 #define SH_CROSS_F64_LDS_VARIANTS(X) X(128, W128, 8) X(256, W256, 8)
+#define SH_CROSS_F32_LDS_VARIANTS(X) X(256, W256, 16)
 // kernel C of the half-spectrum layout: X(nz, length-nz/2 plan, columns per block)
 #define SH_CROSSH_F64_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
@@ -1777,6 +1791,7 @@ int launch_fourier_xd(sh_ns_s* s) {
 #define SH_CROSSH_F64_LDS_VARIANTS(X) \
     X(64, Q32, 16) X(64, Q32, 8) X(128, W64, 16) X(128, W64, 8) X(256, W128, 16) X(256, W128, 8)
 #define SH_CROSSH_F32_NT_VARIANTS(X) X(256, Q128, 16)
+#define SH_CROSSH_F32_LDS_VARIANTS(X) X(256, W128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -1825,6 +1840,7 @@ const NsVariants& cross_variants(int n) {
             SH_CROSS_F64_LDS_VARIANTS(SH_VF)
         } else {
             SH_CROSS_F32_VARIANTS(SH_V)
+            SH_CROSS_F32_LDS_VARIANTS(SH_VF)
         }
 #undef SH_VF
 #undef SH_V
@@ -1859,6 +1875,7 @@ const NsVariants& crossh_variants(int n) {
         } else {
             SH_CROSSH_F32_VARIANTS(SH_V)
             SH_CROSSH_F32_NT_VARIANTS(SH_VT)
+            SH_CROSSH_F32_LDS_VARIANTS(SH_VD)
         }
 #undef SH_VD
 #undef SH_VT

@@ -4,7 +4,8 @@
     storage layouts: every compiled schedule of each fused role, the
     overlapped two-stream form and the autotuner's pick, each for the
     start-up + 3 steps at dt = 5e-4, against the C oracle at 1e-10 relative
-    L2 of the velocity (exec/ns.cpp:132-217);
+    L2 of the velocity (exec/ns.cpp:132-217); every FP32 schedule of the same
+    roles against it at 1e-5;
   * 512^3 FP32 fused step against the FP64 one from the same seed (20 steps
     at the stable dt 2.5e-4, both autotuned): velocity, shell spectrum and
     dissipation errors within stated bounds;
@@ -58,9 +59,9 @@ def ref256(hip):
     torch.cuda.empty_cache()
 
 
-def _run256(hip, V0, layout, overlap=0, tune=False):
+def _run256(hip, V0, layout, overlap=0, tune=False, prec=64):
     L = hip.lib()
-    ns = hip.NS((N256,) * 3, NU, DT, 64, fused=True)
+    ns = hip.NS((N256,) * 3, NU, DT, prec, fused=True)
     L.sh_ns_set_layout(ns.h, layout)
     L.sh_ns_set_overlap(ns.h, overlap)
     ns.set_initial(V0)
@@ -92,6 +93,24 @@ def test_ns_256_every_schedule_vs_oracle(hip, ref256):
             err = _rel(_run256(hip, V0, layout, overlap=1), Vr)
             print("256^3 overlapped layout=%d err=%.2e" % (layout, err), flush=True)
             assert err < 1e-10, (layout, err)
+    finally:
+        L.sh_tune_reset()
+
+
+def test_ns_256_fp32_every_schedule_vs_oracle(hip, ref256):
+    """The FP32 form's kernels at 256^3 (the bench's FP32 line): every
+    compiled FP32 schedule of each fused role against the FP64 oracle, at the
+    FP32 solver's 1e-5 relative L2."""
+    V0, Vr = ref256
+    L = hip.lib()
+    try:
+        for role, layout in NS_ROLES:
+            for v in range(_count(L, 32, N256, role)):
+                L.sh_tune_reset()
+                L.sh_tune_set_variant(32, N256, role, v)
+                err = _rel(_run256(hip, V0, layout, prec=32), Vr)
+                print("256^3 fp32 role=%d layout=%d variant=%d err=%.2e" % (role, layout, v, err), flush=True)
+                assert err < 1e-5, (role, layout, v, err)
     finally:
         L.sh_tune_reset()
 
