@@ -345,7 +345,8 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
 }
 
 template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double>
-__global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zd(C2<T>* __restrict__ W,
+__global__ __launch_bounds__(P::TPL* LPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_ns_cross_zd(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ twg,
                                                              const C2<T>* __restrict__ twng, int nx,
                                                              int ncx, int ny, long F, T scale, long boff,
