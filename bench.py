@@ -7,11 +7,14 @@ kernel schedule.  Inputs are resident in HBM when the timed region starts.
 
 `value` is the CHOSEN FORM: the start is real, so the solver stores and steps
 only the planes kz < nz/2 of the Hermitian spectrum (an R2C-equivalent
-storage; the same velocity to rounding), 384 N^3 algorithmic bytes per step.
-The PLAIN method -- the full complex spectrum, the reference's C2C arithmetic
-operation for operation, 768 N^3 fused bytes per step against the
-1056 N^3-byte contract of 9 3D FFTs + 12 array passes -- is reported beside
-it under "full_spectrum", with its own roofline and kernel table.
+storage; the same velocity to rounding), and the tuned kernel E keeps the
+merged AB2 history Y = (V1 + lap1) - K/2 N0 in place of V1 and N0: 42 field
+passes, 336 N^3 algorithmic bytes per step (384 N^3 with a kernel E that
+keeps V1 and N0).  The PLAIN method -- the full complex spectrum, the
+reference's C2C arithmetic operation for operation -- is reported beside it
+under "full_spectrum" (672 or 768 N^3 fused bytes per step against the
+1056 N^3-byte contract of 9 3D FFTs + 12 array passes), with its own roofline
+and kernel table.
 
 The reference scheme (explicit AB2 advection, Nyquist zeroing only) overflows
 at this dt between steps 60 and 70 (CPU oracle and GPU alike), so the default
@@ -50,6 +53,8 @@ MEASURED_TRAFFIC_256 = {
     "half": {"ns_cross_z": 1.208e9, "ns_fourier_x": 2.820e9, "ns_inv_x": 1.208e9,
              "ns_y_inv": 1.611e9, "ns_y_fwd": 0.805e9},
 }
+# kernel E with the merged AB2 history (keyed by layout); None until measured
+MEASURED_TRAFFIC_256_MERGED_E = {"full": None, "half": None}
 # algorithmic field passes of each kernel class (one complex element read or
 # written per stored grid point = one pass: 16 B in FP64, 8 B in FP32)
 PASSES = {
@@ -59,6 +64,8 @@ PASSES = {
     "ns_y_fwd": 3 + 3,        # y pass of 3 fields
     "ns_fourier_x": 9 + 12,   # read X, V1, N0; write V1, N0 and next step's 6 x-inverse fields
 }
+# kernel E with the merged AB2 history: read X and Y, write Y and the 6 fields
+PASSES_MERGED_E = 3 + 3 + 3 + 6
 NS_KERNELS = ("ns_inv_x", "ns_y_inv", "ns_cross_z", "ns_y_fwd", "ns_fourier_x")
 
 
@@ -106,6 +113,7 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
     L.sh_ns_get_layout(ns.h, ctypes.byref(half))
     if TUNE:
         L.sh_ns_autotune(ns.h, 3)
+    merged = ctypes.c_int(0)
     ns.steps(warmup)
     torch.cuda.synchronize()
     if profile:
@@ -122,6 +130,7 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
         ns.steps(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    L.sh_ns_get_history_form(ns.h, ctypes.byref(merged))
     kern = {}
     if profile:
         L.sh_prof_enable(0)
@@ -134,21 +143,25 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
     del V
     ns.close()
     torch.cuda.empty_cache()
-    return t1 - t0, kern, {"div_avg": div, "energy": e.value, "half": bool(half.value)}
+    return t1 - t0, kern, {"div_avg": div, "energy": e.value, "half": bool(half.value),
+                           "merged": bool(merged.value)}
 
 
-def kernel_table(kern, n, half, prec_bytes=16):
+def kernel_table(kern, n, half, merged=False, prec_bytes=16):
     """per-kernel average launch time, algorithmic bytes and GB/s; the dominant
     kernel's roofline object"""
     layout = "half" if half else "full"
     pts = float(n) ** 3 * (0.5 if half else 1.0)
+    passes = dict(PASSES)
+    if merged:
+        passes["ns_fourier_x"] = PASSES_MERGED_E
     kernels = {}
     dom, dom_t = None, -1.0
     for name, (ms, cnt) in kern.items():
         if cnt == 0:
             continue
         avg_ms = ms / cnt
-        byts = PASSES[name] * prec_bytes * pts
+        byts = passes[name] * prec_bytes * pts
         kernels[name] = {"avg_ms": round(avg_ms, 5), "launches": cnt, "algo_GB": round(byts / 1e9, 4),
                          "GBps": round(byts / (avg_ms * 1e-3) / 1e9, 1)}
         if ms > dom_t:
@@ -157,16 +170,19 @@ def kernel_table(kern, n, half, prec_bytes=16):
     if dom:
         k = kernels[dom]
         traffic = MEASURED_TRAFFIC_256[layout].get(dom) if n == 256 else None
+        if merged and dom == "ns_fourier_x":
+            traffic = MEASURED_TRAFFIC_256_MERGED_E[layout] if n == 256 else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(k["GBps"] / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "traffic_source": TRAFFIC_SOURCE if traffic else None,
-                "algo_bytes_per_launch": PASSES[dom] * prec_bytes * pts,
-                "algo_bytes_formula": "%d field passes x %d B x %s" % (PASSES[dom], prec_bytes,
+                "algo_bytes_per_launch": passes[dom] * prec_bytes * pts,
+                "algo_bytes_formula": "%d field passes x %d B x %s" % (passes[dom], prec_bytes,
                                                                        "N^3/2" if half else "N^3"),
+                "merged_history": bool(merged and dom == "ns_fourier_x"),
                 "avg_launch_ms": k["avg_ms"], "timing": "HIP events on the launch stream around every launch of 1 in %d steps of the timed region"
                 % PROF_EVERY}
-    step_passes = sum(PASSES[k] for k, (ms, cnt) in kern.items() if cnt > 0)
-    return kernels, roof, step_passes * prec_bytes * pts
+    step_passes = sum(passes[k] for k, (ms, cnt) in kern.items() if cnt > 0)
+    return kernels, roof, step_passes * prec_bytes * pts, step_passes
 
 
 def fft_bench(S, n, reps):
@@ -294,7 +310,7 @@ def main():
     value = a.steps / dt
     log("bench: %.2f steps/s (%.3f ms/step), final energy %r, div avg %.2e"
         % (value, 1e3 * dt / a.steps, st["energy"], st["div_avg"]))
-    kernels, roof, step_bytes = kernel_table(kern, n, st["half"])
+    kernels, roof, step_bytes, step_passes = kernel_table(kern, n, st["half"], st["merged"])
     state = {"final_energy": st["energy"], "final_div_avg": st["div_avg"],
              "finite": bool(st["energy"] == st["energy"] and abs(st["energy"]) < 1e30),
              "steps_done": a.warmup + a.steps,
@@ -305,13 +321,14 @@ def main():
     if not a.no_full:
         # the plain method: the same step on the full complex spectrum (C2C)
         dtf, kernf, stf = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=True, layout=0)
-        kf, rooff, bytesf = kernel_table(kernf, n, stf["half"])
+        kf, rooff, bytesf, passesf = kernel_table(kernf, n, stf["half"], stf["merged"])
         vf = a.steps / dtf
         full = {"steps_per_s": round(vf, 3), "ms_per_step": round(1e3 * dtf / a.steps, 4),
                 "layout": "full spectrum (C2C)", "roofline": rooff, "kernels": kf,
                 "step_algo_GB_fused": round(bytesf / 1e9, 3), "step_GBps_fused": round(bytesf * vf / 1e9, 1),
                 "step_contract_GB": round(1056.0 * n ** 3 / 1e9, 3),
-                "fused_bytes_formula": "768 N^3 B/step (48 field passes)",
+                "fused_bytes_formula": "%d N^3 B/step (%d field passes)" % (16 * passesf, passesf),
+                "merged_history": stf["merged"],
                 "contract_bytes_formula": "1056 N^3 B/step (9 3D FFTs + 12 array passes)",
                 "final_energy": stf["energy"]}
         log("bench: full-spectrum (plain C2C) %.2f steps/s" % vf)
@@ -323,8 +340,10 @@ def main():
         log("bench: dt=%g: %.2f steps/s, final energy %r" % (a.stable_dt, a.stable_steps / dts, sts["energy"]))
     step = {"chosen_form_algo_GB": round(step_bytes / 1e9, 3),
             "chosen_form_GBps": round(step_bytes * value / 1e9, 1),
-            "chosen_form_formula": "384 N^3 B/step (48 field passes over the N^3/2 stored points)"
-                                   if st["half"] else "768 N^3 B/step (48 field passes)"}
+            "chosen_form_formula": ("%d N^3 B/step (%d field passes over the N^3/2 stored points)"
+                                    % (8 * step_passes, step_passes)) if st["half"] else
+                                   ("%d N^3 B/step (%d field passes)" % (16 * step_passes, step_passes)),
+            "merged_history": st["merged"]}
     copy = None
     if not a.no_copy:
         log("bench: HBM copy ceiling")
@@ -392,8 +411,9 @@ def main():
         "config": {"workload": "ns_fractional_step_fp64_%d^3" % n, "grid": n, "nu": NU, "dt": DT,
                    "ic": "random solenoidal, E(k)~k^4 exp(-2(k/4)^2), u_rms=1, seed %d" % SEED,
                    "form": ("chosen form: half-spectrum storage of the real fields (planes kz < nz/2, "
-                            "R2C-equivalent), 384 N^3 B/step; the plain C2C method is full_spectrum")
-                   if st["half"] else "plain method: full complex spectrum (C2C), 768 N^3 B/step",
+                            "R2C-equivalent)%s, %d N^3 B/step; the plain C2C method is full_spectrum"
+                            % (", merged AB2 history in kernel E" if st["merged"] else "", 8 * step_passes))
+                   if st["half"] else "plain method: full complex spectrum (C2C), %d N^3 B/step" % (16 * step_passes),
                    "schedule": "fused (4 kernels/step)",
                    "layout": "half spectrum (kz < nz/2; real fields)" if st["half"] else "full spectrum"},
         "roofline": roof,

@@ -246,6 +246,12 @@ int sh_ns_set_nonlinear(sh_ns_t handle, int on);
 int sh_ns_set_layout(sh_ns_t handle, int mode);
 /* 1 if the solver currently steps the half-spectrum layout, else 0 */
 int sh_ns_get_layout(sh_ns_t handle, int* half);
+/* Replaces nothing in the reference: 1 if the fused state is in the merged
+ * AB2-history form that the last kernel E left (exec/ns.cpp:171-186 need V1
+ * and cross0 only through Y = (V1 + lap1) - K/2 cross0, so that kernel E keeps
+ * Y and the velocity's x-inverse, 15 field passes instead of 21), else 0.
+ * The getters, snapshots and schedule changes see (V1, cross0) either way. */
+int sh_ns_get_history_form(sh_ns_t handle, int* merged);
 /* Fused schedule only.  1: the y passes and the fused z kernel of a step run
  * as two independent halves of the x columns on the solver's stream and a
  * second (non-blocking, default-priority) stream of its own, joined before

@@ -426,6 +426,95 @@ __device__ __forceinline__ void line_fft_c(C2<T>* x, int tid, C2<T>* buf, const 
     }
 }
 
+// Per-thread stage twiddles held in registers: a thread's factors depend on
+// its index alone (k = j mod Ns), so a kernel that transforms many lines with
+// the same threads can load them once instead of reading the table in every
+// stage.  Layout: stages with Ns > 1 in order, butterfly b, r = 1..R-1.
+template <class P, int s>
+__host__ __device__ constexpr int rt_off() {
+    if constexpr (s <= 0)
+        return 0;
+    else
+        return rt_off<P, s - 1>() + (P::ns(s - 1) > 1 ? P::nb(s - 1) * (P::radix(s - 1) - 1) : 0);
+}
+template <class P>
+__host__ __device__ constexpr int rt_size() {
+    return rt_off<P, P::S>();
+}
+
+// this thread's factors from the natural table (the ones stage_bfly reads)
+template <class P, int s, typename T>
+__device__ __forceinline__ void rt_load(C2<T>* w, int tid, const C2<T>* __restrict__ tw) {
+    if constexpr (s < P::S) {
+        constexpr int R = P::radix(s), Ns = P::ns(s), NB = P::nb(s);
+        if constexpr (Ns > 1) {
+            constexpr int step = P::N / (Ns * R);
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int j = tid + b * P::TPL;
+                const int k = (P::full(s) || j < P::nbf(s)) ? j % Ns : 0;
+#pragma unroll
+                for (int r = 1; r < R; r++) w[rt_off<P, s>() + b * (R - 1) + r - 1] = tw[k * r * step];
+            }
+        }
+        rt_load<P, s + 1, T>(w, tid, tw);
+    }
+}
+
+template <class P, int s, int SIGN, typename T>
+__device__ __forceinline__ void stage_bfly_r(C2<T>* x, int tid, const C2<T>* w) {
+    constexpr int R = P::radix(s), Ns = P::ns(s), NB = P::nb(s);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = tid + b * P::TPL;
+        if (P::full(s) || j < P::nbf(s)) {
+            C2<T>* v = x + b * R;
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    C2<T> t = w[rt_off<P, s>() + b * (R - 1) + r - 1];
+                    if constexpr (SIGN > 0) t.y = -t.y;
+                    v[r] = cmul<T>(v[r], t);
+                }
+            }
+            Dft<T, R, SIGN>::run(v);
+        }
+    }
+}
+
+// line_fft_c with the stage twiddles from registers (rt_load)
+template <class P, int SIGN, typename T, class Idx, class Sync = SyncBlock>
+__device__ __forceinline__ void line_fft_cr(C2<T>* x, int tid, C2<T>* buf, const Idx& idx, const C2<T>* w) {
+    stage_bfly_r<P, 0, SIGN, T>(x, tid, w);
+    if constexpr (P::S > 1) {
+        lds_put_c<P, 0, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 1, T>(x, tid, buf, idx);
+        stage_bfly_r<P, 1, SIGN, T>(x, tid, w);
+    }
+    if constexpr (P::S > 2) {
+        Sync::run();
+        lds_put_c<P, 1, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 2, T>(x, tid, buf, idx);
+        stage_bfly_r<P, 2, SIGN, T>(x, tid, w);
+    }
+    if constexpr (P::S > 3) {
+        Sync::run();
+        lds_put_c<P, 2, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 3, T>(x, tid, buf, idx);
+        stage_bfly_r<P, 3, SIGN, T>(x, tid, w);
+    }
+    if constexpr (P::S > 4) {
+        Sync::run();
+        lds_put_c<P, 3, T>(x, tid, buf, idx);
+        Sync::run();
+        lds_get_c<P, 4, T>(x, tid, buf, idx);
+        stage_bfly_r<P, 4, SIGN, T>(x, tid, w);
+    }
+}
+
 // f(slot, pos) for every stage-0 input this thread owns
 template <class P, class F>
 __device__ __forceinline__ void for_in(int tid, F&& f) {

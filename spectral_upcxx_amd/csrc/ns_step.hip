@@ -344,7 +344,7 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
     return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)p;
 }
 
-template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double>
+template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double, int RT = 0>
 __global__ __launch_bounds__(P::TPL* LPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void k_ns_cross_zd(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ twg,
@@ -389,10 +389,22 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     const int tiles_x = (ncx + LPB - 1) / LPB;
     const unsigned es = (unsigned)nx * (unsigned)ny;
     const unsigned uF = (unsigned)F;
+    static_assert(RT == 0 || IL, "register twiddles: interleaved exchange");
+    // RT >= 1: this thread's packing factors e^{-2 pi i k/nz}, k = tid + r Q
+    // (the positions of its first-stage inputs and of its last-stage outputs);
+    // RT == 2: also its stage twiddles, loaded once for every tile
+    C2<T> twp[RT >= 1 ? 4 : 1], tws[RT == 2 ? rt_size<P>() : 1];
+    if constexpr (RT >= 1 && HALF) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) twp[r] = twng[tid + r * Q];
+    }
+    if constexpr (RT == 2) rt_load<P, 0, T>(tws, tid, twg);
     auto idx = [&](int pos) { return IL ? pos * LPB + c : pos * CS + c; };
     auto fft = [&](auto sign, C2<T>* v) {
         constexpr int SG = decltype(sign)::value;
-        if constexpr (IL)
+        if constexpr (RT == 2)
+            line_fft_cr<P, SG, T, decltype(idx), SyncLds>(v, tid, sx, idx, tws);
+        else if constexpr (IL)
             line_fft_c<P, SG, T, decltype(idx), SyncLds>(v, tid, sx, idx, tw);
         else
             line_fft<P, SG, T, decltype(idx), SyncLds>(v, tid, sre, sim, idx, tw);
@@ -470,7 +482,7 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
                 const C2<T> b = pos == 0 ? mk<T>(T(0), T(0)) : mk<T>(mv[r].x, mv[r].y);
                 const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // X[k] + conj X[M-k]
                 const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // X[k] - conj X[M-k]
-                const C2<T> w = twn[pos];                       // e^{-2 pi i k/nz}
+                const C2<T> w = RT >= 1 ? twp[r] : twn[pos];    // e^{-2 pi i k/nz}
                 const T tx = w.x * dif.x + w.y * dif.y;         // t = conj(w) dif
                 const T ty = w.x * dif.y - w.y * dif.x;
                 x[r] = mk<T>(sum.x - ty, sum.y + tx);           // Z = sum + i t
@@ -577,7 +589,7 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
                     const C2<T> b = IL ? sx[idx(mp)] : mk<T>(sre[idx(mp)], sim[idx(mp)]);
                     const C2<T> sum = mk<T>(a.x + b.x, a.y - b.y);  // Z[k] + conj Z[M-k]
                     const C2<T> dif = mk<T>(a.x - b.x, a.y + b.y);  // Z[k] - conj Z[M-k]
-                    const C2<T> wv = twn[pos];
+                    const C2<T> wv = RT >= 1 ? twp[s2] : twn[pos];  // last stage: s2 = r, pos = tid + r Q
                     const T tx = wv.x * dif.x - wv.y * dif.y;  // t = w dif
                     const T ty = wv.x * dif.y + wv.y * dif.x;
                     st_line<NT, T>(W + (cb + (unsigned)d * uF + (unsigned)pos * es),
@@ -1028,7 +1040,18 @@ __device__ __forceinline__ double* xd_exchange() {
 // per SIMD) instead of three.  G3 (with SHARE): the three product transforms,
 // and the emitted ones in two groups of three, run together stage by stage
 // (line_fft_xa3).
-template <class P, bool NT = true, bool SHARE = false, bool G3 = false>
+// YS (with SHARE and G3, Crank-Nicolson, single box): the merged AB2 history.
+// The update needs V1 and N0 only through Y = (V1 + lap1) - K/2 N0, so the N0
+// fields hold Y instead, V1 is neither read nor stored, and the kernel reads
+// 6 and writes 9 field passes instead of 9 and 12:
+//     S = Y + K/2 (3 N1)                  (= (V1 + lap1) + F)
+//     V = (S - grad G) / (1 + nu K/2 |k|^2),   G = lap^-1 div S
+//     Y' = (V + nu K/2 lap V) - K/2 N1    (the next step's Y)
+// The velocity is then held only as its unnormalised x-inverse in the first
+// three W fields (the next step's input); the host recovers V1 from them by
+// a forward x pass when it needs it.  The same values as the (V1, N0) form to
+// rounding (a different association of the same sums).
+template <class P, bool NT = true, bool SHARE = false, bool G3 = false, bool YS = false>
 __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
                                                       double2* __restrict__ N0, const double2* __restrict__ twg,
                                                       int ny, int nz, int nzp, long F,
@@ -1045,19 +1068,20 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                       P::nb(P::S - 1) == 1 && N % 64 == 0 && 3 * Q * 16 < 65536,
                   "one wave per line, one radix-4 butterfly per lane in the first and last stage");
     static_assert(!SHARE || sm_size<P>() <= N, "stage-major twiddles fit the table");
+    static_assert(!YS || (SHARE && G3), "the merged history runs the grouped, shared-slot kernel");
     // vector-memory operations a line issues after its products' loads (the
-    // next V1, the 6 + 6 field stores and, when SHARE, the next N0), capped at
-    // what vmcnt encodes
-    constexpr int AFTER_X = (SHARE ? 6 * PIECES : 3 * PIECES) + 12 * E;
+    // next V1, the 6 + 6 field stores and, when SHARE, the next N0; with YS
+    // the 3 + 6 stores and the next Y), capped at what vmcnt encodes
+    constexpr int AFTER_X = YS ? 3 * PIECES + 9 * E : (SHARE ? 6 * PIECES : 3 * PIECES) + 12 * E;
     constexpr int TOP_WAIT = SHARE ? (AFTER_X < 63 ? AFTER_X : 63) : 6 * PIECES;
     const StepCoef<T> cf = cfs[blockIdx.y];
     W += blockIdx.y * boff;
     V += blockIdx.y * boff;
     N0 += blockIdx.y * boff;
     divpart += (long)blockIdx.y * divstride;
-    __shared__ double2 xs[3][N];   // the products of the line
-    __shared__ double2 v1l[3][N];  // its V1
-    __shared__ double2 n0l[3][N];  // its N0
+    __shared__ double2 xs[3][N];                        // the products of the line
+    __shared__ double2 v1l[YS ? 1 : 3][YS ? 1 : N];     // its V1 (not read with YS)
+    __shared__ double2 n0l[3][N];                       // its N0 (YS: its Y)
     __shared__ double2 tw[N];
     T* sre = nullptr;  // the exchange of the transforms when it has LDS of its own
     if constexpr (!SHARE) sre = xd_exchange<LS>();
@@ -1101,10 +1125,10 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
     long line = blockIdx.x;
     if (line < nlines) {
         fetch(&xs[0][0], W, 3, line);
-        fetch(&v1l[0][0], V, 3, line);
+        if constexpr (!YS) fetch(&v1l[0][0], V, 3, line);
         fetch(&n0l[0][0], N0, 3, line);
         // the first line's products (TOP_WAIT may exceed what follows them here)
-        if constexpr (SHARE) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(6 * PIECES) : "memory");
+        if constexpr (SHARE) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((YS ? 3 : 6) * PIECES) : "memory");
     }
     for (; line < nlines; line += gridDim.x) {
         const long nxt = line + gridDim.x;
@@ -1147,16 +1171,66 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES) : "memory");
         else
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        C2<T> v1s[3][E], n0s[3][E], vn[3][E];
+        C2<T> v1s[YS ? 1 : 3][E], n0s[3][E], vn[3][E];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            read4(&v1l[d][0], v1s[d]);
+            if constexpr (!YS) read4(&v1l[d][0], v1s[d]);
             read4(&n0l[d][0], n0s[d]);
         }
         if (more) {
-            fetch(&v1l[0][0], V, 3, nxt);
+            if constexpr (!YS) fetch(&v1l[0][0], V, 3, nxt);
             if constexpr (!SHARE) fetch(&n0l[0][0], N0, 3, nxt);
         }
+        if constexpr (YS) {
+            for_out<P>(tid, [&](int s, int pos) {
+                const T k0 = freq<T>(pos, N);
+                const bool z = zline || pos == N / 2 ||
+                               (cf.dealias == 1 && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
+                const T sig =
+                    cf.dealias == 2 ? expfilter<T>(k0, N) * expfilter<T>(k1, ny) * expfilter<T>(k2, nz) : T(1);
+                C2<T> nn[3], S[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    nn[d] = z ? mk<T>(T(0), T(0)) : n1[d][s];
+                    if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
+                    // S = Y + K/2*(3.*cross1)  (= (V1 + lap1) + F)
+                    S[d] = cadd<T>(n0s[d][s], rmul<T>(cf.Kh, rmul<T>(T(3), nn[d])));
+                }
+                const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
+                // G = div(S); G.laplacian_inverse() (a=0, b=1)
+                C2<T> G = mk<T>(T(0), T(0));
+                G = cadd<T>(G, imul<T>(k0, S[0]));
+                G = cadd<T>(G, imul<T>(k1, S[1]));
+                G = cadd<T>(G, imul<T>(k2, S[2]));
+                T sum = 0;
+                sum += k0 * k0;
+                sum += k1 * k1;
+                sum += k2 * k2;
+                const T rg = sum != 0 ? T(1) / (T(0) - T(1) * sum) : T(1);
+                G.x *= rg;
+                G.y *= rg;
+                const double rv = 1.0 / (1.0 - (-cf.c1d) * (double)sum);  // 1 where sum == 0
+                const T kk3[3] = {k0, k1, k2};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    // V = Vhalf + lap1 = S - G.gradient(), then the CN divide (FP64 factor)
+                    const C2<T> vv = csub<T>(S[d], imul<T>(kk3[d], G));
+                    vn[d][s] = mk<T>(T((double)vv.x * rv), T((double)vv.y * rv));
+                }
+                C2<T> dv = mk<T>(T(0), T(0));
+                dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
+                dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
+                dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
+                dsum += dw * sqrt((double)dv.x * (double)dv.x + (double)dv.y * (double)dv.y);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    // the next step's Y = (V + lap V) - K/2 cross1
+                    const C2<T> yn = csub<T>(cadd<T>(vn[d][s], rmul<T>(cf.c1, rmul<T>(q, vn[d][s]))),
+                                             rmul<T>(cf.Kh, nn[d]));
+                    st_line<NT, T>(N0 + d * F + off + pos, yn);
+                }
+            });
+        } else
         for_out<P>(tid, [&](int s, int pos) {
             const T k0 = freq<T>(pos, N);
             const bool z = zline || pos == N / 2 ||
@@ -1276,6 +1350,27 @@ __global__ void k_z_to_c(const double2* __restrict__ a, float2* __restrict__ b, 
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         b[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
+// The merged AB2 history of kernel E (YS) and back, over the stored planes
+// (nzp) of the three fields: TOY: H = (V + nu K/2 lap V) - K/2 H (H = N0 in,
+// Y out); otherwise H = ((V + nu K/2 lap V) - H) / (K/2) (Y in, N0 out)
+template <bool TOY>
+__global__ void k_yform(const double2* __restrict__ V, double2* __restrict__ H, long F, int nx, int ny, int nz,
+                        int nzp, double c1, double Kh) {
+    const long n = (long)nx * ny * nzp;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < 3 * n; e += (long)gridDim.x * blockDim.x) {
+        const int d = (int)(e / n);
+        const long r = e % n;
+        const long jk = r / nx;
+        const double k0 = freq<double>((int)(r % nx), nx), k1 = freq<double>((int)(jk % ny), ny),
+                     k2 = freq<double>((int)(jk / ny), nz);
+        const double q = -(k0 * k0 + k1 * k1 + k2 * k2);
+        const long o = d * F + r;
+        const double2 v = V[o], h = H[o];
+        const double2 a = cadd<double>(v, rmul<double>(c1, rmul<double>(q, v)));  // V1 + lap1
+        H[o] = TOY ? csub<double>(a, rmul<double>(Kh, h)) : mk<double>((a.x - h.x) / Kh, (a.y - h.y) / Kh);
+    }
+}
+
 __global__ void k_c_to_z(const float2* __restrict__ a, double2* __restrict__ b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         b[i] = make_double2((double)a[i].x, (double)a[i].y);
@@ -1427,6 +1522,10 @@ struct sh_ns_s {
     int div_nb;  // partials written by the last kernel E
     bool have_div;
     bool w_primed;  // W holds the x-inverse of V1 and curl V1 for the next step
+    // the merged AB2 history (kernel E with YS): N0f holds Y = (V1 + lap1) -
+    // K/2 N0, V1f is stale, and the first three W fields hold the unnormalised
+    // x-inverse of V1 (W is primed); ns_canonical returns to (V1, N0)
+    bool yform;
     double div_avg;
     const void* forcing;  // body force (3 fields per box, precision prec) or null
     int cur_box;          // box whose start-up is running
@@ -1519,6 +1618,47 @@ int add_forcing64(sh_ns_s* s, double2* cr) {
 }
 
 // exec/ns.cpp:132-153
+// ---------------------------------------------------------------- merged AB2 history
+// The merged history serves one FP64 box with the Crank-Nicolson scheme and no
+// body force (kernel E with YS); everything else keeps (V1, N0).
+bool y_ok(const sh_ns_s* s) {
+    return s->fused && s->prec == SH_F64 && s->nboxes == 1 && !s->forcing && s->scheme == 0;
+}
+
+double y_c1(const sh_ns_s* s) { return s->nus[0] * s->K / 2; }
+
+// V1 of a solver in the merged form into dst (3 fields at stride N): the
+// forward x pass of W's first three fields (their unnormalised x-inverse),
+// scaled by 1/nx, Nyquist lines zeroed as every forward() leaves them
+int v1_into(sh_ns_s* s, double2* dst) {
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const size_t cnt = (size_t)s->nx * s->ny * nzp;
+    for (int d = 0; d < 3; d++)
+        SH_HIP(hipMemcpyAsync(dst + d * s->N, (const double2*)s->Wf + d * s->N, sizeof(double2) * cnt,
+                              hipMemcpyDeviceToDevice, s->st));
+    Epi<double> ep{1.0 / s->nx, 3, s->ny, s->ny / 2, s->nz / 2};
+    return pass_contig<double>(dst, s->nx, (long)s->ny * nzp, s->nx, 3, s->N, -1, ep, s->st, -1);
+}
+
+template <bool TOY>
+int launch_yform(sh_ns_s* s) {
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    hipLaunchKernelGGL(k_yform<TOY>, dim3(egrid(3L * s->nx * s->ny * nzp)), dim3(256), 0, s->st,
+                       (const double2*)s->V1f, (double2*)s->N0f, s->N, s->nx, s->ny, s->nz, nzp, y_c1(s),
+                       s->K / 2);
+    SH_LAUNCH_CHECK("k_yform");
+    return SH_OK;
+}
+
+// back to (V1, N0): V1 from W, N0 = ((V1 + lap1) - Y) / (K/2)
+int ns_canonical(sh_ns_s* s) {
+    if (!s->yform) return SH_OK;
+    RC(v1_into(s, (double2*)s->V1f));
+    RC(launch_yform<false>(s));
+    s->yform = false;
+    return SH_OK;
+}
+
 int startup64(sh_ns_s* s) {
     const long V = 3 * s->N;
     RC(cross(s, s->V0, s->om0, s->cr1));
@@ -1643,7 +1783,7 @@ int launch_cross_zh(sh_ns_s* s) {
 
 // kernel C of the half-spectrum layout with LDS-streamed inputs (FP64): a
 // persistent grid of the resident blocks looping over the column tiles
-template <typename T, class P, int LPB, bool NT = false, bool IL = false, bool HALF = true>
+template <typename T, class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, int RT = 0>
 int launch_cross_zd(sh_ns_s* s) {
     constexpr int prec = sizeof(T) == 8 ? SH_F64 : SH_F32;
     // an FP32 lane loads two neighbouring columns: an odd column count would
@@ -1671,7 +1811,7 @@ int launch_cross_zd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF, T>, P::TPL * LPB,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF, T, RT>, P::TPL * LPB,
                                                          0) != hipSuccess ||
             per <= 0)
             per = 1;
@@ -1683,7 +1823,7 @@ int launch_cross_zd(sh_ns_s* s) {
     dim3 grid((unsigned)g, (unsigned)s->nboxes), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
     ProfScope ps("ns_cross_z", s->cst);
-    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF, T>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0,
+    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF, T, RT>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0,
                        tw, twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale, s->N, ntile);
     SH_LAUNCH_CHECK("k_ns_cross_zd");
     return SH_OK;
@@ -1750,6 +1890,45 @@ int launch_fourier_xd(sh_ns_s* s) {
     return SH_OK;
 }
 
+// kernel E with the merged AB2 history (YS): converts a (V1, N0) state first;
+// where the merged form does not apply, the (V1, N0) kernel of the same plan
+template <typename T, class P, bool NT = true>
+int launch_fourier_xy(sh_ns_s* s) {
+    static_assert(sizeof(T) == 8, "FP64 only");
+    if (!y_ok(s)) return launch_fourier_xd<T, P, NT, true, true>(s);
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
+    if (!tw) return SH_ENOMEM;
+    if (!s->yform) RC(launch_yform<true>(s));
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long nlines = (long)s->ny * nzp;
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, v = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true>, 64, 0) !=
+                hipSuccess ||
+            per <= 0)
+            per = 1;
+        resident = v * per;
+    }
+    const unsigned nb = (unsigned)(nlines < resident ? nlines : resident);
+    if ((int)nb > s->divblocks) {
+        set_error("ns: divergence partial buffer too small");
+        return SH_EINVAL;
+    }
+    ProfScope ps("ns_fourier_x", s->st);
+    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true>), dim3(nb, 1u), dim3(64), 0, s->st,
+                       (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz, nzp,
+                       s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
+    SH_LAUNCH_CHECK("k_ns_fourier_xd");
+    s->div_nb = (int)nb;
+    s->w_primed = true;
+    s->yform = true;
+    return SH_OK;
+}
+
 // ---------------------------------------------------------------- tunable fused schedules
 // X(n, plan, columns or lines per block); the first entry of a length is its
 // default.  Kernel C holds up to seven fields per thread and kernel E three
@@ -1791,6 +1970,9 @@ int launch_fourier_xd(sh_ns_s* s) {
 // nontemporal stores: X(nz, length-nz/2 plan, columns per block)
 #define SH_CROSSH_F64_LDS_VARIANTS(X) \
     X(64, Q32, 16) X(64, Q32, 8) X(128, W64, 16) X(128, W64, 8) X(256, W128, 16) X(256, W128, 8)
+// the same (interleaved exchange) with the packing factors (RT = 1) or also
+// the stage twiddles (RT = 2) held in registers: X(nz, plan, columns per block)
+#define SH_CROSSH_F64_RT_VARIANTS(X) X(128, W64, 8) X(256, W128, 8) X(256, W128, 16)
 #define SH_CROSSH_F32_NT_VARIANTS(X) X(256, Q128, 16)
 #define SH_CROSSH_F32_LDS_VARIANTS(X) X(256, W128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
@@ -1809,6 +1991,9 @@ int launch_fourier_xd(sh_ns_s* s) {
 #define SH_FOURIER_F64_LDS_VARIANTS(X)                                                        \
     X(256, W256, true, false, false) X(256, W256, false, false, false) X(256, W256, true, true, false) \
     X(256, W256, false, true, false) X(256, W256, true, true, true)
+// the same grouped, shared-slot kernel with the merged AB2 history (YS):
+// X(n, plan, nontemporal stores)
+#define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true) X(256, W256, false)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -1822,6 +2007,7 @@ using NsLaunch = int (*)(sh_ns_s*);
 struct NsVariants {
     std::vector<const char*> names;
     std::vector<NsLaunch> fns;
+    std::vector<NsLaunch> merged;  // the entries of fns that keep the merged AB2 history
 };
 
 template <typename T>
@@ -1836,13 +2022,18 @@ const NsVariants& cross_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, false>);                     \
     m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange nontemporal"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, false>);
+#define SH_VFR(n_, P_, C_)                                                                             \
+    m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, false, 2>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSS_F64_VARIANTS(SH_V)
             SH_CROSS_F64_LDS_VARIANTS(SH_VF)
+            SH_CROSS_F64_LDS_VARIANTS(SH_VFR)
         } else {
             SH_CROSS_F32_VARIANTS(SH_V)
             SH_CROSS_F32_LDS_VARIANTS(SH_VF)
         }
+#undef SH_VFR
 #undef SH_VF
 #undef SH_V
         return m;
@@ -1869,15 +2060,24 @@ const NsVariants& crossh_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true>);                           \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true>);
+#define SH_VR(n_, P_, C_)                                                                                   \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange packing factors in registers"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 1>);                                    \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers");    \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 2>);                                    \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSSH_F64_VARIANTS(SH_V)
             SH_CROSSH_F64_NT_VARIANTS(SH_VT)
             SH_CROSSH_F64_LDS_VARIANTS(SH_VD)
+            SH_CROSSH_F64_RT_VARIANTS(SH_VR)
         } else {
             SH_CROSSH_F32_VARIANTS(SH_V)
             SH_CROSSH_F32_NT_VARIANTS(SH_VT)
             SH_CROSSH_F32_LDS_VARIANTS(SH_VD)
         }
+#undef SH_VR
 #undef SH_VD
 #undef SH_VT
 #undef SH_V
@@ -1914,18 +2114,24 @@ const NsVariants& fourier_variants(int n) {
     m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_ ", shared exchange=" #SH_ \
                           ", grouped=" #G3_);                                                          \
     m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_, SH_, G3_>);
+#define SH_VY(n_, P_, NT_)                                                                                    \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, nontemporal=" #NT_);       \
+    m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_>);                                                      \
+    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F64_BATCHED_VARIANTS(SH_VB)
             SH_FOURIER_F64_NT_VARIANTS(SH_VT)
             SH_FOURIER_F64_LDS_VARIANTS(SH_VS)
+            SH_FOURIER_F64_Y_VARIANTS(SH_VY)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F32_BATCHED_VARIANTS(SH_VB)
             SH_FOURIER_F32_NT_VARIANTS(SH_VT)
         }
+#undef SH_VY
 #undef SH_VS
 #undef SH_VT
 #undef SH_V
@@ -1971,6 +2177,18 @@ int dispatch_fourier_x(sh_ns_s* s) {
     return run_variant<T>(fourier_variants<T>(s->nx), s, s->nx, ROLE_FOURIER);
 }
 
+// whether this step's kernel E keeps the merged AB2 history
+bool fourier_merged(sh_ns_s* s) {
+    if (!y_ok(s)) return false;
+    const NsVariants& vl = fourier_variants<double>(s->nx);
+    if (vl.fns.empty()) return false;
+    int v = tune_choice(SH_F64, s->nx, ROLE_FOURIER);
+    if (v < 0 || v >= (int)vl.fns.size()) v = 0;
+    for (NsLaunch f : vl.merged)
+        if (f == vl.fns[v]) return true;
+    return false;
+}
+
 // kernels B, C, D over the columns [x0, x0 + ncx) on stream st
 template <typename T>
 int step_bcd(sh_ns_s* s, hipStream_t st, int x0, int ncx) {
@@ -2004,6 +2222,9 @@ int step_fused_front(sh_ns_s* s) {
         set_error("ns: the fused schedule always includes the nonlinear term");
         return SH_EINVAL;
     }
+    // a merged-history state goes back to (V1, N0) before a kernel E that
+    // needs them (while W still holds the x-inverse of V1)
+    if (s->yform && !fourier_merged(s)) RC(ns_canonical(s));
     // kernel A runs once after a (re)start; afterwards kernel E emits its output
     if (!s->w_primed) {
         RC(dispatch_inv_x<T>(s));
@@ -2099,7 +2320,7 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     SH_HIP(hipMemcpyAsync(sv, s->V1f, 3 * fb, hipMemcpyDeviceToDevice, s->st));
     SH_HIP(hipMemcpyAsync(sv + 3 * fb, s->N0f, 3 * fb, hipMemcpyDeviceToDevice, s->st));
     SH_HIP(hipMemcpyAsync(sv + 6 * fb, s->Wf, 6 * fb, hipMemcpyDeviceToDevice, s->st));
-    const bool primed = s->w_primed;
+    const bool primed = s->w_primed, yform = s->yform;
     hipEvent_t e0, e1;
     SH_HIP(hipEventCreate(&e0));
     SH_HIP(hipEventCreate(&e1));
@@ -2140,6 +2361,7 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     SH_HIP(hipStreamSynchronize(s->st));
     pool_free(save);
     s->w_primed = primed;
+    s->yform = yform;
     s->have_div = false;
     return rc;
 }
@@ -2179,6 +2401,7 @@ int expand_half(sh_ns_s* s, void* f, int count) {
 }
 
 int expand_state(sh_ns_s* s) {
+    RC(ns_canonical(s));
     const int nf = 3 * s->nboxes;
     if (s->prec == SH_F64) {
         RC(expand_half<double>(s, s->V1f, nf));
@@ -2239,6 +2462,7 @@ int decide_layout(sh_ns_s* s, int b) {
 // copy the FP64 start-up result (V1, cross0) into box b of the fused state
 int install_box(sh_ns_s* s, int b) {
     const long N = s->N;
+    s->yform = false;
     for (int d = 0; d < 3; d++) {
         const long off = ((long)d * s->nboxes + b) * N;
         if (s->prec == SH_F64) {
@@ -2402,6 +2626,10 @@ int sh_ns_set_initial_box(sh_ns_t s, int box, const void* V0) {
 int sh_ns_get_velocity_box(sh_ns_t s, int box, void* out) {
     if (!s || !out || box < 0 || box >= s->nboxes || !s->fused) return SH_EINVAL;
     const long N = s->N;
+    if (s->yform) {  // one box: V1 from W
+        RC(v1_into(s, (double2*)out));
+        return s->half ? expand_half<double>(s, out, 3) : SH_OK;
+    }
     for (int d = 0; d < 3; d++) {
         const long off = ((long)d * s->nboxes + box) * N;
         if (s->prec == SH_F64) {
@@ -2481,6 +2709,7 @@ int sh_ns_snapshot(sh_ns_t s, int ratio, void* out) {
         return SH_EINVAL;
     }
     RC(ensure_snap(s));
+    RC(ns_canonical(s));
     return s->prec == SH_F64 ? snapshot_t<double>(s, ratio, (float*)out) : snapshot_t<float>(s, ratio, (float*)out);
 }
 
@@ -2492,6 +2721,7 @@ int sh_ns_snapshot_force(sh_ns_t s, int ratio, void* out) {
         return SH_EINVAL;
     }
     RC(ensure_snap(s));
+    RC(ns_canonical(s));
     return s->prec == SH_F64 ? snapshot_force_t<double>(s, ratio, (float*)out)
                              : snapshot_force_t<float>(s, ratio, (float*)out);
 }
@@ -2537,6 +2767,7 @@ int sh_ns_set_scheme(sh_ns_t s, int scheme) {
         set_error("sh_ns_set_scheme: scheme must be 0 (exec/ns.cpp) or 1 (exec/ns2.cpp)");
         return SH_EINVAL;
     }
+    if (s->fused) RC(ns_canonical(s));  // Y holds the old scheme's viscous factor
     s->scheme = scheme;
     return s->fused ? upload_coefs(s) : SH_OK;
 }
@@ -2609,6 +2840,8 @@ int sh_ns_get_velocity(sh_ns_t s, void* out) {
     const long n3 = 3 * s->N;
     if (!s->fused) {
         SH_HIP(hipMemcpyAsync(out, s->V1, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
+    } else if (s->yform) {
+        RC(v1_into(s, (double2*)out));
     } else if (s->prec == SH_F64) {
         SH_HIP(hipMemcpyAsync(out, s->V1f, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
     } else {
@@ -2649,6 +2882,12 @@ int sh_ns_set_overlap(sh_ns_t s, int on) {
 int sh_ns_get_layout(sh_ns_t s, int* half) {
     if (!s || !half) return SH_EINVAL;
     *half = s->half ? 1 : 0;
+    return SH_OK;
+}
+
+int sh_ns_get_history_form(sh_ns_t s, int* merged) {
+    if (!s || !merged) return SH_EINVAL;
+    *merged = s->yform ? 1 : 0;
     return SH_OK;
 }
 
@@ -2699,7 +2938,10 @@ int sh_ns_state_bytes(sh_ns_t s, size_t* bytes) {
 
 namespace sh {
 // the fused solver's current velocity (3 Fourier fields per box, precision prec)
-void* ns_fused_velocity(sh_ns_t s) { return s && s->fused ? s->V1f : nullptr; }
+void* ns_fused_velocity(sh_ns_t s) {
+    if (!s || !s->fused || ns_canonical(s) != SH_OK) return nullptr;
+    return s->V1f;
+}
 }  // namespace sh
 
 namespace sh {
