@@ -53,8 +53,8 @@ MEASURED_TRAFFIC_256 = {
     "half": {"ns_cross_z": 1.208e9, "ns_fourier_x": 2.820e9, "ns_inv_x": 1.208e9,
              "ns_y_inv": 1.611e9, "ns_y_fwd": 0.805e9},
 }
-# kernel E with the merged AB2 history (keyed by layout); None until measured
-MEASURED_TRAFFIC_256_MERGED_E = {"full": None, "half": None}
+# kernel E with the merged AB2 history (keyed by layout; the full layout not measured)
+MEASURED_TRAFFIC_256_MERGED_E = {"full": None, "half": 2.014e9}
 # algorithmic field passes of each kernel class (one complex element read or
 # written per stored grid point = one pass: 16 B in FP64, 8 B in FP32)
 PASSES = {

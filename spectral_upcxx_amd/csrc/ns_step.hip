@@ -1891,8 +1891,10 @@ int launch_fourier_xd(sh_ns_s* s) {
 }
 
 // kernel E with the merged AB2 history (YS): converts a (V1, N0) state first;
-// where the merged form does not apply, the (V1, N0) kernel of the same plan
-template <typename T, class P, bool NT = true>
+// where the merged form does not apply, the (V1, N0) kernel of the same plan.
+// CAP > 0 limits the persistent grid to CAP blocks per compute unit (the
+// kernel's LDS admits five one-wave blocks, one more than a CU has SIMDs)
+template <typename T, class P, bool NT = true, int CAP = 0>
 int launch_fourier_xy(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     if (!y_ok(s)) return launch_fourier_xd<T, P, NT, true, true>(s);
@@ -1911,6 +1913,7 @@ int launch_fourier_xy(sh_ns_s* s) {
                 hipSuccess ||
             per <= 0)
             per = 1;
+        if (CAP > 0 && per > CAP) per = CAP;
         resident = v * per;
     }
     const unsigned nb = (unsigned)(nlines < resident ? nlines : resident);
@@ -1992,8 +1995,8 @@ int launch_fourier_xy(sh_ns_s* s) {
     X(256, W256, true, false, false) X(256, W256, false, false, false) X(256, W256, true, true, false) \
     X(256, W256, false, true, false) X(256, W256, true, true, true)
 // the same grouped, shared-slot kernel with the merged AB2 history (YS):
-// X(n, plan, nontemporal stores)
-#define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true) X(256, W256, false)
+// X(n, plan, nontemporal stores, blocks per CU (0: as many as fit))
+#define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0) X(256, W256, true, 4)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -2114,10 +2117,11 @@ const NsVariants& fourier_variants(int n) {
     m[n_].names.push_back(#P_ " LDS-streamed persistent lines, nontemporal=" #NT_ ", shared exchange=" #SH_ \
                           ", grouped=" #G3_);                                                          \
     m[n_].fns.push_back(&launch_fourier_xd<T, P_, NT_, SH_, G3_>);
-#define SH_VY(n_, P_, NT_)                                                                                    \
-    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, nontemporal=" #NT_);       \
-    m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_>);                                                      \
-    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_>);
+#define SH_VY(n_, P_, NT_, CAP_)                                                                              \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, nontemporal=" #NT_          \
+                          ", blocks per CU=" #CAP_);                                                            \
+    m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);                                                 \
+    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
