@@ -31,6 +31,7 @@ int mlp_infer_packed(sh_mlp_t h, const float2* X, long ld, long n, float2* Y, lo
 int mlp_infer_packed_planar(sh_mlp_t h, const float2* X, long ld, long n, float* Y, long ldy, int clip, float smag,
                             hipStream_t st);
 void* ns_fused_velocity(sh_ns_t s);
+void ns_keep_velocity(sh_ns_t s);
 int ns_step_phase(sh_ns_t s, int phase);
 int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx);
 int mlp_infer_planar_packed(sh_mlp_t h, const float* X, long ld, long n, float2* Y, long ldy, hipStream_t st);
@@ -900,6 +901,8 @@ int sh_les_create(sh_les_t* handle, int n, double nu, double K, sh_mlp_t model, 
     // the solver steps the half-spectrum layout once its start is real; the
     // closure reads its velocity in place in either layout
     int rc = sh_ns_create(&h->ns, n, n, n, nu, K, SH_F32, 1, stream);
+    // the closure reads V1 in place every step: no merged AB2 history
+    if (!rc) ns_keep_velocity(h->ns);
     void* p;
     const long sizes[6] = {3, 3, 5, 3, 3, 3};
     float2** dst[6] = {&h->F, &h->F1, &h->Z, &h->T, &h->Vc, &h->H};

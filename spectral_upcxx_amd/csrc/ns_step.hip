@@ -664,7 +664,10 @@ __global__ __launch_bounds__(256) void k_herm_dev(const double2* __restrict__ f,
 // BL: the line's V and N0 stores are issued after the whole update, so no
 // slot's V1/N0 loads wait behind an earlier slot's stores (vector-memory
 // counters count loads and stores together)
-template <typename T, class P, int LPB, bool EMIT, bool BL = false, bool NT = false>
+// YS: the merged AB2 history (see k_ns_fourier_xd): N0 holds Y = (V1 + lap1)
+// - K/2 N0, V is neither read nor written, and the new V survives only as the
+// emitted x-inverse (so YS needs EMIT)
+template <typename T, class P, int LPB, bool EMIT, bool BL = false, bool NT = false, bool YS = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
@@ -676,6 +679,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
                                                               long boff, int divstride) {
     static_assert(!EMIT || (P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1)),
                   "emitting the next inverse pass needs a symmetric plan");
+    static_assert(!YS || EMIT, "the merged history keeps V only as the emitted x-inverse");
     // per-box coefficients (each box of an ensemble has its own viscosity)
     const StepCoef<T> cf = cfs[blockIdx.y];
     W += blockIdx.y * boff;
@@ -741,17 +745,24 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             for (int d = 0; d < 3; d++) {
                 nn[d] = z ? mk<T>(T(0), T(0)) : forced ? cadd<T>(n1[d][s], frc[d * F + off + pos]) : n1[d][s];
                 if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
-                v1[d] = V[d * F + off + pos];
-                const C2<T> n0 = N0[d * F + off + pos];
-                // F = K/2*(3.*cross1 - cross0)
-                f[d] = rmul<T>(cf.Kh, csub<T>(rmul<T>(T(3), nn[d]), n0));
+                if constexpr (YS) {
+                    // S = Y + K/2*(3.*cross1)  (= (V1 + lap1) + F)
+                    S[d] = cadd<T>(N0[d * F + off + pos], rmul<T>(cf.Kh, rmul<T>(T(3), nn[d])));
+                } else {
+                    v1[d] = V[d * F + off + pos];
+                    const C2<T> n0 = N0[d * F + off + pos];
+                    // F = K/2*(3.*cross1 - cross0)
+                    f[d] = rmul<T>(cf.Kh, csub<T>(rmul<T>(T(3), nn[d]), n0));
+                }
             }
             // lap1 = nu*K/2*V1.laplacian() = c1 * (-(k.k) V1)
             const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
+            if constexpr (!YS) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                L[d] = rmul<T>(cf.c1, rmul<T>(q, v1[d]));
-                S[d] = cadd<T>(cadd<T>(v1[d], L[d]), f[d]);  // (V1 + lap1) + F
+                for (int d = 0; d < 3; d++) {
+                    L[d] = rmul<T>(cf.c1, rmul<T>(q, v1[d]));
+                    S[d] = cadd<T>(cadd<T>(v1[d], L[d]), f[d]);  // (V1 + lap1) + F
+                }
             }
             // G = div(S); G.laplacian_inverse() (a=0, b=1)
             C2<T> G = mk<T>(T(0), T(0));
@@ -771,9 +782,14 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             const T kk3[3] = {k0, k1, k2};
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1 (exec/ns2.cpp:185: V = Vhalf)
-                C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
-                C2<T> vv = cf.be ? vh : cadd<T>(vh, L[d]);
+                C2<T> vv;
+                if constexpr (YS) {
+                    vv = csub<T>(S[d], imul<T>(kk3[d], G));  // Vhalf + lap1 = S - G.gradient()
+                } else {
+                    // Vhalf = V1 + F - G.gradient();  V = Vhalf + lap1 (exec/ns2.cpp:185: V = Vhalf)
+                    C2<T> vh = csub<T>(cadd<T>(v1[d], f[d]), imul<T>(kk3[d], G));
+                    vv = cf.be ? vh : cadd<T>(vh, L[d]);
+                }
                 // V[d].laplacian_inverse(1, -nu*K/2): divide by 1 + (nu K/2)|k|^2 (FP64 factor)
                 vv = mk<T>(T((double)vv.x * rv), T((double)vv.y * rv));
                 vn[d][s] = vv;
@@ -785,11 +801,15 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             dsum += dw * sqrt((double)dv.x * (double)dv.x + (double)dv.y * (double)dv.y);  // |div V| (no overflow here)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
+                // YS: the next step's Y = (V + lap V) - K/2 cross1 in place of N0
+                const C2<T> keep = YS ? csub<T>(cadd<T>(vn[d][s], rmul<T>(cf.c1, rmul<T>(q, vn[d][s]))),
+                                                rmul<T>(cf.Kh, nn[d]))
+                                      : nn[d];
                 if constexpr (BL) {
-                    nna[d][s] = nn[d];
+                    nna[d][s] = keep;
                 } else {
-                    st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
-                    st_line<NT, T>(N0 + d * F + off + pos, nn[d]);
+                    if constexpr (!YS) st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
+                    st_line<NT, T>(N0 + d * F + off + pos, keep);
                 }
             }
         });
@@ -797,7 +817,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             for_out<P>(tid, [&](int s, int pos) {
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
+                    if constexpr (!YS) st_line<NT, T>(V + d * F + off + pos, vn[d][s]);
                     st_line<NT, T>(N0 + d * F + off + pos, nna[d][s]);
                 }
             });
@@ -1353,21 +1373,20 @@ __global__ void k_z_to_c(const double2* __restrict__ a, float2* __restrict__ b, 
 // The merged AB2 history of kernel E (YS) and back, over the stored planes
 // (nzp) of the three fields: TOY: H = (V + nu K/2 lap V) - K/2 H (H = N0 in,
 // Y out); otherwise H = ((V + nu K/2 lap V) - H) / (K/2) (Y in, N0 out)
-template <bool TOY>
-__global__ void k_yform(const double2* __restrict__ V, double2* __restrict__ H, long F, int nx, int ny, int nz,
-                        int nzp, double c1, double Kh) {
+template <typename T, bool TOY>
+__global__ void k_yform(const C2<T>* __restrict__ V, C2<T>* __restrict__ H, long F, int nx, int ny, int nz,
+                        int nzp, T c1, T Kh) {
     const long n = (long)nx * ny * nzp;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < 3 * n; e += (long)gridDim.x * blockDim.x) {
         const int d = (int)(e / n);
         const long r = e % n;
         const long jk = r / nx;
-        const double k0 = freq<double>((int)(r % nx), nx), k1 = freq<double>((int)(jk % ny), ny),
-                     k2 = freq<double>((int)(jk / ny), nz);
-        const double q = -(k0 * k0 + k1 * k1 + k2 * k2);
+        const T k0 = freq<T>((int)(r % nx), nx), k1 = freq<T>((int)(jk % ny), ny), k2 = freq<T>((int)(jk / ny), nz);
+        const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
         const long o = d * F + r;
-        const double2 v = V[o], h = H[o];
-        const double2 a = cadd<double>(v, rmul<double>(c1, rmul<double>(q, v)));  // V1 + lap1
-        H[o] = TOY ? csub<double>(a, rmul<double>(Kh, h)) : mk<double>((a.x - h.x) / Kh, (a.y - h.y) / Kh);
+        const C2<T> v = V[o], h = H[o];
+        const C2<T> a = cadd<T>(v, rmul<T>(c1, rmul<T>(q, v)));  // V1 + lap1
+        H[o] = TOY ? csub<T>(a, rmul<T>(Kh, h)) : mk<T>((a.x - h.x) / Kh, (a.y - h.y) / Kh);
     }
 }
 
@@ -1526,6 +1545,7 @@ struct sh_ns_s {
     // K/2 N0, V1f is stale, and the first three W fields hold the unnormalised
     // x-inverse of V1 (W is primed); ns_canonical returns to (V1, N0)
     bool yform;
+    bool keep_v1;  // the merged history is off (a caller reads V1 in place)
     double div_avg;
     const void* forcing;  // body force (3 fields per box, precision prec) or null
     int cur_box;          // box whose start-up is running
@@ -1622,40 +1642,70 @@ int add_forcing64(sh_ns_s* s, double2* cr) {
 // The merged history serves one FP64 box with the Crank-Nicolson scheme and no
 // body force (kernel E with YS); everything else keeps (V1, N0).
 bool y_ok(const sh_ns_s* s) {
-    return s->fused && s->prec == SH_F64 && s->nboxes == 1 && !s->forcing && s->scheme == 0;
+    return s->fused && s->nboxes == 1 && !s->forcing && s->scheme == 0 && !s->keep_v1;
 }
 
-double y_c1(const sh_ns_s* s) { return s->nus[0] * s->K / 2; }
+bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
 
 // V1 of a solver in the merged form into dst (3 fields at stride N): the
 // forward x pass of W's first three fields (their unnormalised x-inverse),
 // scaled by 1/nx, Nyquist lines zeroed as every forward() leaves them
-int v1_into(sh_ns_s* s, double2* dst) {
+template <typename T>
+int v1_into(sh_ns_s* s, C2<T>* dst) {
     const int nzp = s->half ? s->nz / 2 : s->nz;
     const size_t cnt = (size_t)s->nx * s->ny * nzp;
     for (int d = 0; d < 3; d++)
-        SH_HIP(hipMemcpyAsync(dst + d * s->N, (const double2*)s->Wf + d * s->N, sizeof(double2) * cnt,
+        SH_HIP(hipMemcpyAsync(dst + d * s->N, (const C2<T>*)s->Wf + d * s->N, sizeof(C2<T>) * cnt,
                               hipMemcpyDeviceToDevice, s->st));
-    Epi<double> ep{1.0 / s->nx, 3, s->ny, s->ny / 2, s->nz / 2};
-    return pass_contig<double>(dst, s->nx, (long)s->ny * nzp, s->nx, 3, s->N, -1, ep, s->st, -1);
+    Epi<T> ep{T(1.0 / s->nx), 3, s->ny, s->ny / 2, s->nz / 2};
+    return pass_contig<T>(dst, s->nx, (long)s->ny * nzp, s->nx, 3, s->N, -1, ep, s->st, -1);
 }
 
-template <bool TOY>
+// TOY: N0 -> Y; otherwise Y -> N0 (V1 current), with the step's coefficients
+template <typename T, bool TOY>
 int launch_yform(sh_ns_s* s) {
     const int nzp = s->half ? s->nz / 2 : s->nz;
-    hipLaunchKernelGGL(k_yform<TOY>, dim3(egrid(3L * s->nx * s->ny * nzp)), dim3(256), 0, s->st,
-                       (const double2*)s->V1f, (double2*)s->N0f, s->N, s->nx, s->ny, s->nz, nzp, y_c1(s),
-                       s->K / 2);
+    hipLaunchKernelGGL((k_yform<T, TOY>), dim3(egrid(3L * s->nx * s->ny * nzp)), dim3(256), 0, s->st,
+                       (const C2<T>*)s->V1f, (C2<T>*)s->N0f, s->N, s->nx, s->ny, s->nz, nzp,
+                       T(s->nus[0] * s->K / 2), T(s->K / 2));
     SH_LAUNCH_CHECK("k_yform");
     return SH_OK;
 }
 
-// back to (V1, N0): V1 from W, N0 = ((V1 + lap1) - Y) / (K/2)
+// back to (V1, N0): V1 from W, N0 = ((V1 + lap1) - Y) / (K/2).  Never inside
+// a graph capture: a replay would convert again a state already converted.
 int ns_canonical(sh_ns_s* s) {
     if (!s->yform) return SH_OK;
-    RC(v1_into(s, (double2*)s->V1f));
-    RC(launch_yform<false>(s));
+    if (capturing(s->st)) {
+        set_error("ns: the solver state would change form inside a graph capture; run one step "
+                  "with the captured schedule before capturing");
+        return SH_EINVAL;
+    }
+    if (s->prec == SH_F64) {
+        RC(v1_into<double>(s, (double2*)s->V1f));
+        RC((launch_yform<double, false>(s)));
+    } else {
+        RC(v1_into<float>(s, (float2*)s->V1f));
+        RC((launch_yform<float, false>(s)));
+    }
     s->yform = false;
+    return SH_OK;
+}
+
+int ensure_snap(sh_ns_s* s);
+
+// V1 of a merged-history state as complex128 into out (3 fields at stride
+// N), leaving the state as it is (FP32: through the snapshot scratch)
+int v1_into_z(sh_ns_s* s, double2* out) {
+    if (s->prec == SH_F64) return v1_into<double>(s, out);
+    RC(ensure_snap(s));
+    RC(v1_into<float>(s, (float2*)s->snap));
+    const long n3 = 3 * s->N;
+    hipLaunchKernelGGL(k_c_to_z, dim3(egrid(n3)), dim3(256), 0, s->st, (const float2*)s->snap, out, n3);
+    SH_LAUNCH_CHECK("k_c_to_z");
     return SH_OK;
 }
 
@@ -1831,8 +1881,12 @@ int launch_cross_zd(sh_ns_s* s) {
 
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
-template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false>
+template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false, bool YS = false>
 int launch_fourier_x(sh_ns_s* s) {
+    if (!YS && s->yform) {
+        set_error("ns: kernel E of the (V1, N0) form on a merged-history state");
+        return SH_EINVAL;
+    }
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
     const int nzp = s->half ? s->nz / 2 : s->nz;
@@ -1843,14 +1897,25 @@ int launch_fourier_x(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT>), dim3(nb, (unsigned)s->nboxes),
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT, YS>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
                        (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, nzp, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
     s->w_primed = EMIT;
+    if (YS) s->yform = true;
     return SH_OK;
+}
+
+// the register-loaded kernel E with the merged AB2 history: converts a
+// (V1, N0) state first (never inside a graph capture); where the merged form
+// does not apply, the (V1, N0) kernel of the same schedule
+template <typename T, class P, int LPB, bool BL = false, bool NT = false>
+int launch_fourier_xm(sh_ns_s* s) {
+    if (!y_ok(s) || (!s->yform && capturing(s->st))) return launch_fourier_x<T, P, LPB, true, BL, NT>(s);
+    if (!s->yform) RC((launch_yform<T, true>(s)));
+    return launch_fourier_x<T, P, LPB, true, BL, NT, true>(s);
 }
 
 // kernel E with LDS-streamed inputs: a persistent grid of one-wave blocks, as
@@ -1860,6 +1925,10 @@ template <typename T, class P, bool NT = true, bool SHARE = false, bool G3 = fal
 int launch_fourier_xd(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     if (s->forcing) return launch_fourier_x<T, P, 1, true, false, NT>(s);
+    if (s->yform) {
+        set_error("ns: kernel E of the (V1, N0) form on a merged-history state");
+        return SH_EINVAL;
+    }
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
     if (!tw) return SH_ENOMEM;
     const int nzp = s->half ? s->nz / 2 : s->nz;
@@ -1897,10 +1966,11 @@ int launch_fourier_xd(sh_ns_s* s) {
 template <typename T, class P, bool NT = true, int CAP = 0>
 int launch_fourier_xy(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
-    if (!y_ok(s)) return launch_fourier_xd<T, P, NT, true, true>(s);
+    // a (V1, N0) state is converted only outside a graph capture
+    if (!y_ok(s) || (!s->yform && capturing(s->st))) return launch_fourier_xd<T, P, NT, true, true>(s);
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
     if (!tw) return SH_ENOMEM;
-    if (!s->yform) RC(launch_yform<true>(s));
+    if (!s->yform) RC((launch_yform<T, true>(s)));
     const int nzp = s->half ? s->nz / 2 : s->nz;
     const long nlines = (long)s->ny * nzp;
     static int resident = 0;
@@ -1997,6 +2067,11 @@ int launch_fourier_xy(sh_ns_s* s) {
 // the same grouped, shared-slot kernel with the merged AB2 history (YS):
 // X(n, plan, nontemporal stores, blocks per CU (0: as many as fit))
 #define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0) X(256, W256, true, 4)
+// the register-loaded kernel E with the merged history: X(n, plan, lines per
+// block), plain and nontemporal stores
+#define SH_FOURIER_F64_YR_VARIANTS(X) X(64, W64, 16) X(96, Q96, 10) X(128, W128, 8) X(512, W512, 2) X(512, P512, 4)
+#define SH_FOURIER_F32_YR_VARIANTS(X) \
+    X(64, W64, 16) X(64, P64, 32) X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4) X(512, P512, 4) X(512, W512, 2)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -2122,6 +2197,13 @@ const NsVariants& fourier_variants(int n) {
                           ", blocks per CU=" #CAP_);                                                            \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);                                                 \
     m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);
+#define SH_VM(n_, P_, L_)                                                                     \
+    m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
+    m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
+    m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_>);                                     \
+    m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history nontemporal stores");         \
+    m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true>);                           \
+    m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_, false, true>);
         if constexpr (sizeof(T) == 8) {
             SH_FOURIER_F64_VARIANTS(SH_V)
             SH_FOURIER_F64_NOEMIT_VARIANTS(SH_VN)
@@ -2129,12 +2211,15 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_NT_VARIANTS(SH_VT)
             SH_FOURIER_F64_LDS_VARIANTS(SH_VS)
             SH_FOURIER_F64_Y_VARIANTS(SH_VY)
+            SH_FOURIER_F64_YR_VARIANTS(SH_VM)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F32_BATCHED_VARIANTS(SH_VB)
             SH_FOURIER_F32_NT_VARIANTS(SH_VT)
+            SH_FOURIER_F32_YR_VARIANTS(SH_VM)
         }
+#undef SH_VM
 #undef SH_VY
 #undef SH_VS
 #undef SH_VT
@@ -2184,9 +2269,9 @@ int dispatch_fourier_x(sh_ns_s* s) {
 // whether this step's kernel E keeps the merged AB2 history
 bool fourier_merged(sh_ns_s* s) {
     if (!y_ok(s)) return false;
-    const NsVariants& vl = fourier_variants<double>(s->nx);
+    const NsVariants& vl = s->prec == SH_F64 ? fourier_variants<double>(s->nx) : fourier_variants<float>(s->nx);
     if (vl.fns.empty()) return false;
-    int v = tune_choice(SH_F64, s->nx, ROLE_FOURIER);
+    int v = tune_choice(s->prec, s->nx, ROLE_FOURIER);
     if (v < 0 || v >= (int)vl.fns.size()) v = 0;
     for (NsLaunch f : vl.merged)
         if (f == vl.fns[v]) return true;
@@ -2631,7 +2716,7 @@ int sh_ns_get_velocity_box(sh_ns_t s, int box, void* out) {
     if (!s || !out || box < 0 || box >= s->nboxes || !s->fused) return SH_EINVAL;
     const long N = s->N;
     if (s->yform) {  // one box: V1 from W
-        RC(v1_into(s, (double2*)out));
+        RC(v1_into_z(s, (double2*)out));
         return s->half ? expand_half<double>(s, out, 3) : SH_OK;
     }
     for (int d = 0; d < 3; d++) {
@@ -2660,7 +2745,10 @@ int snapshot_t(sh_ns_s* s, int r, float* out) {
     // all boxes at once: one copy, one 3-pass inverse transform, one filter launch
     C2<T>* snap = (C2<T>*)s->snap;
     const long nf = 3L * s->nboxes;
-    SH_HIP(hipMemcpyAsync(snap, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
+    if (s->yform)  // one box, V1 from W (the state stays as it is)
+        RC(v1_into<T>(s, snap));
+    else
+        SH_HIP(hipMemcpyAsync(snap, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
     if (s->half) RC(expand_half<T>(s, snap, (int)nf));
     RC(fft3d<T>(snap, (int)nf, N, s->nx, s->ny, s->nz, +1, SH_NORMALIZE, s->st));
     unsigned gx = egrid(nc);
@@ -2687,7 +2775,10 @@ int snapshot_force_t(sh_ns_s* s, int r, float* out) {
     const long nf = 3L * s->nboxes;
     C2<T>* u = (C2<T>*)s->snap;
     C2<T>* w = u + nf * N;
-    SH_HIP(hipMemcpyAsync(u, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
+    if (s->yform)
+        RC(v1_into<T>(s, u));
+    else
+        SH_HIP(hipMemcpyAsync(u, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
     if (s->half) RC(expand_half<T>(s, u, (int)nf));
     hipLaunchKernelGGL(k_curl_boxes<T>, dim3(egrid(N), (unsigned)s->nboxes), dim3(256), 0, s->st, u, w, N,
                        N * s->nboxes, s->nx, s->ny, s->nz);
@@ -2713,7 +2804,6 @@ int sh_ns_snapshot(sh_ns_t s, int ratio, void* out) {
         return SH_EINVAL;
     }
     RC(ensure_snap(s));
-    RC(ns_canonical(s));
     return s->prec == SH_F64 ? snapshot_t<double>(s, ratio, (float*)out) : snapshot_t<float>(s, ratio, (float*)out);
 }
 
@@ -2725,7 +2815,6 @@ int sh_ns_snapshot_force(sh_ns_t s, int ratio, void* out) {
         return SH_EINVAL;
     }
     RC(ensure_snap(s));
-    RC(ns_canonical(s));
     return s->prec == SH_F64 ? snapshot_force_t<double>(s, ratio, (float*)out)
                              : snapshot_force_t<float>(s, ratio, (float*)out);
 }
@@ -2845,7 +2934,7 @@ int sh_ns_get_velocity(sh_ns_t s, void* out) {
     if (!s->fused) {
         SH_HIP(hipMemcpyAsync(out, s->V1, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
     } else if (s->yform) {
-        RC(v1_into(s, (double2*)out));
+        RC(v1_into_z(s, (double2*)out));
     } else if (s->prec == SH_F64) {
         SH_HIP(hipMemcpyAsync(out, s->V1f, sizeof(double2) * n3, hipMemcpyDeviceToDevice, s->st));
     } else {
@@ -2945,6 +3034,13 @@ namespace sh {
 void* ns_fused_velocity(sh_ns_t s) {
     if (!s || !s->fused || ns_canonical(s) != SH_OK) return nullptr;
     return s->V1f;
+}
+
+// V1 is read in place by the caller from now on: the state keeps (V1, N0)
+void ns_keep_velocity(sh_ns_t s) {
+    if (!s) return;
+    s->keep_v1 = true;
+    (void)ns_canonical(s);
 }
 }  // namespace sh
 

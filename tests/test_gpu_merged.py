@@ -22,21 +22,21 @@ def _rel(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
-def _variants(L):
-    """(merged, plain): E schedules at 256 with and without the merged history
-    (plain: the same LDS-streamed kernel keeping V1 and N0)."""
+def _variants(L, prec=64, n=N, plain_tag="grouped=true"):
+    """(merged, plain): E schedules with and without the merged history (at
+    256 FP64, plain: the same LDS-streamed kernel keeping V1 and N0)."""
     c = ctypes.c_int()
-    L.sh_tune_variant_count(64, N, ROLE_FOURIER, ctypes.byref(c))
+    L.sh_tune_variant_count(prec, n, ROLE_FOURIER, ctypes.byref(c))
     merged, plain = [], []
     buf = ctypes.create_string_buffer(8192)
     for v in range(c.value):
         L.sh_tune_reset()
-        L.sh_tune_set_variant(64, N, ROLE_FOURIER, v)
+        L.sh_tune_set_variant(prec, n, ROLE_FOURIER, v)
         L.sh_tune_report(buf, 8192)
         name = buf.value.decode()
         if "merged AB2 history" in name:
             merged.append(v)
-        elif "LDS-streamed" in name and "grouped=true" in name:
+        elif plain_tag in name and "no-emit" not in name:
             plain.append(v)
     L.sh_tune_reset()
     assert merged and plain
@@ -61,9 +61,9 @@ def _solver(hip, V0, layout=1):
     return ns
 
 
-def _use(L, v):
+def _use(L, v, prec=64, n=N):
     L.sh_tune_reset()
-    L.sh_tune_set_variant(64, N, ROLE_FOURIER, v)
+    L.sh_tune_set_variant(prec, n, ROLE_FOURIER, v)
 
 
 def test_merged_getter_does_not_perturb(hip, v0):
@@ -163,3 +163,98 @@ def test_merged_layout_snapshot_and_scheme(hip, v0):
         assert np.isfinite(e_snap)
     finally:
         L.sh_tune_reset()
+
+
+def test_merged_fp32_register_kernel(hip, v0):
+    """The register-loaded kernel E with the merged history in FP32 at 256^3:
+    the getter leaves the run unperturbed, and the velocity agrees with the
+    (V1, N0) kernel to FP32 rounding."""
+    L = hip.lib()
+    merged, plain = _variants(L, 32, N, "lines=")
+    out = {}
+    try:
+        for tag, v in (("merged", merged[0]), ("plain", plain[0])):
+            _use(L, v, 32)
+            a = hip.NS((N,) * 3, NU, DT, 32, fused=True)
+            a.set_initial(v0)
+            a.startup()
+            a.steps(3)
+            mid = a.velocity()
+            a.steps(3)
+            out[tag] = (mid, a.velocity())
+            f = ctypes.c_int()
+            L.sh_ns_get_history_form(a.h, ctypes.byref(f))
+            assert f.value == (1 if tag == "merged" else 0)
+            a.close()
+        b = hip.NS((N,) * 3, NU, DT, 32, fused=True)
+        _use(L, merged[0], 32)
+        b.set_initial(v0)
+        b.startup()
+        b.steps(6)
+        assert bool((b.velocity() == out["merged"][1]).all())
+        b.close()
+        e3 = _rel(out["merged"][0], out["plain"][0])
+        e6 = _rel(out["merged"][1], out["plain"][1])
+        print("FP32 merged vs (V1, N0): 3 steps %.2e, 6 steps %.2e" % (e3, e6), flush=True)
+        assert e3 < 1e-6 and e6 < 1e-6
+    finally:
+        L.sh_tune_reset()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_merged_graph_capture(hip, prec):
+    """Graphs of merged-history steps replay bit for bit like eager steps; a
+    capture that starts from a (V1, N0) state runs the (V1, N0) kernel (no
+    state conversion is recorded) and also replays bit for bit."""
+    import torch
+    from oracle import pyoracle as O
+    L = hip.lib()
+    n = 64
+    merged, plain = _variants(L, prec, n, "lines=")
+    V0 = torch.from_numpy(O.solenoidal((n,) * 3, 41, 3.0, 1.0)).cuda()
+    torch.cuda.synchronize()
+    st = ctypes.c_void_p()
+    L.sh_stream_create(ctypes.byref(st))
+    try:
+        for warm_variant in (merged[0], plain[0]):
+            hs = []
+            for _ in range(2):
+                h = ctypes.c_void_p()
+                L.sh_ns_create(ctypes.byref(h), n, n, n, 0.01, 1e-3, prec, 1, st)
+                L.sh_ns_set_initial(h, hip.ptr(V0))
+                L.sh_ns_startup(h)
+                hs.append(h)
+            _use(L, warm_variant, prec, n)
+            for h in hs:
+                L.sh_ns_steps(h, 1)
+            _use(L, merged[0], prec, n)
+            L.sh_stream_synchronize(st)
+            assert L.sh_graph_begin(st) == 0
+            assert L.sh_ns_steps(hs[0], 4) == 0
+            g = ctypes.c_void_p()
+            assert L.sh_graph_end(st, ctypes.byref(g)) == 0
+            for rep in range(3):
+                L.sh_graph_launch(g, st)
+                L.sh_ns_steps(hs[1], 4)
+                L.sh_stream_synchronize(st)
+                a = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+                b = torch.empty_like(a)
+                L.sh_ns_get_velocity(hs[0], hip.ptr(a))
+                L.sh_ns_get_velocity(hs[1], hip.ptr(b))
+                L.sh_stream_synchronize(st)
+                fa, fb = ctypes.c_int(), ctypes.c_int()
+                L.sh_ns_get_history_form(hs[0], ctypes.byref(fa))
+                L.sh_ns_get_history_form(hs[1], ctypes.byref(fb))
+                print("fp%d warm=%d rep=%d forms %d %d" % (prec, warm_variant, rep, fa.value, fb.value), flush=True)
+                if warm_variant == merged[0]:
+                    assert bool((a == b).all()), rep
+                else:
+                    # the graph holds the (V1, N0) kernel, the eager twin converts
+                    assert fa.value == 0 and fb.value == 1
+                    assert _rel(a, b) < (1e-13 if prec == 64 else 1e-6), rep
+            L.sh_graph_destroy(g)
+            for h in hs:
+                L.sh_ns_destroy(h)
+    finally:
+        L.sh_tune_reset()
+        L.sh_stream_destroy(st)
