@@ -8,11 +8,12 @@ kernel schedule.  Inputs are resident in HBM when the timed region starts.
 `value` is the CHOSEN FORM: the start is real, so the solver stores and steps
 only the planes kz < nz/2 of the Hermitian spectrum (an R2C-equivalent
 storage; the same velocity to rounding), and the tuned kernel E keeps the
-merged AB2 history Y = (V1 + lap1) - K/2 N0 in place of V1 and N0: 42 field
-passes, 336 N^3 algorithmic bytes per step (384 N^3 with a kernel E that
-keeps V1 and N0).  The PLAIN method -- the full complex spectrum, the
+merged AB2 history Y = (V1 + lap1) - K/2 N0 in place of V1 and N0 and emits
+five fields from which the next y pass forms the vorticity: 40 field passes,
+320 N^3 algorithmic bytes per step (42 / 336 N^3 with six emitted fields,
+48 / 384 N^3 with a kernel E that keeps V1 and N0).  The PLAIN method -- the full complex spectrum, the
 reference's C2C arithmetic operation for operation -- is reported beside it
-under "full_spectrum" (672 or 768 N^3 fused bytes per step against the
+under "full_spectrum" (640 to 768 N^3 fused bytes per step against the
 1056 N^3-byte contract of 9 3D FFTs + 12 array passes), with its own roofline
 and kernel table.
 
@@ -53,8 +54,9 @@ MEASURED_TRAFFIC_256 = {
     "half": {"ns_cross_z": 1.208e9, "ns_fourier_x": 2.820e9, "ns_inv_x": 1.208e9,
              "ns_y_inv": 1.611e9, "ns_y_fwd": 0.805e9},
 }
-# kernel E with the merged AB2 history (keyed by layout; the full layout not measured)
-MEASURED_TRAFFIC_256_MERGED_E = {"full": None, "half": 2.014e9}
+# kernel E with the merged AB2 history, keyed by (layout, form): form 1 emits
+# six fields, form 2 five (the full layout not measured)
+MEASURED_TRAFFIC_256_MERGED_E = {("full", 1): None, ("half", 1): 2.014e9, ("full", 2): None, ("half", 2): 1.880e9}
 # algorithmic field passes of each kernel class (one complex element read or
 # written per stored grid point = one pass: 16 B in FP64, 8 B in FP32)
 PASSES = {
@@ -65,7 +67,10 @@ PASSES = {
     "ns_fourier_x": 9 + 12,   # read X, V1, N0; write V1, N0 and next step's 6 x-inverse fields
 }
 # kernel E with the merged AB2 history: read X and Y, write Y and the 6 fields
-PASSES_MERGED_E = 3 + 3 + 3 + 6
+# (form 1) or 5 fields (form 2: V and d/dx Vy, d/dx Vz); with form 2 the y
+# inverse reads those 5 and writes V and the vorticity (6)
+PASSES_MERGED_E = {1: 3 + 3 + 3 + 6, 2: 3 + 3 + 3 + 5}
+PASSES_Y_CURL = 5 + 6
 NS_KERNELS = ("ns_inv_x", "ns_y_inv", "ns_cross_z", "ns_y_fwd", "ns_fourier_x")
 
 
@@ -113,7 +118,7 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
     L.sh_ns_get_layout(ns.h, ctypes.byref(half))
     if TUNE:
         L.sh_ns_autotune(ns.h, 3)
-    merged = ctypes.c_int(0)
+    form = ctypes.c_int(0)
     ns.steps(warmup)
     torch.cuda.synchronize()
     if profile:
@@ -130,7 +135,7 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
         ns.steps(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    L.sh_ns_get_history_form(ns.h, ctypes.byref(merged))
+    L.sh_ns_get_history_form(ns.h, ctypes.byref(form))
     kern = {}
     if profile:
         L.sh_prof_enable(0)
@@ -144,17 +149,19 @@ def run_ns(S, n, precision, warmup, steps, profile, dt=DT, layout=1):
     ns.close()
     torch.cuda.empty_cache()
     return t1 - t0, kern, {"div_avg": div, "energy": e.value, "half": bool(half.value),
-                           "merged": bool(merged.value)}
+                           "form": form.value}
 
 
-def kernel_table(kern, n, half, merged=False, prec_bytes=16):
+def kernel_table(kern, n, half, form=0, prec_bytes=16):
     """per-kernel average launch time, algorithmic bytes and GB/s; the dominant
-    kernel's roofline object"""
+    kernel's roofline object (form: the solver's history form, 0 / 1 / 2)"""
     layout = "half" if half else "full"
     pts = float(n) ** 3 * (0.5 if half else 1.0)
     passes = dict(PASSES)
-    if merged:
-        passes["ns_fourier_x"] = PASSES_MERGED_E
+    if form:
+        passes["ns_fourier_x"] = PASSES_MERGED_E[form]
+    if form == 2:
+        passes["ns_y_inv"] = PASSES_Y_CURL
     kernels = {}
     dom, dom_t = None, -1.0
     for name, (ms, cnt) in kern.items():
@@ -170,15 +177,15 @@ def kernel_table(kern, n, half, merged=False, prec_bytes=16):
     if dom:
         k = kernels[dom]
         traffic = MEASURED_TRAFFIC_256[layout].get(dom) if n == 256 else None
-        if merged and dom == "ns_fourier_x":
-            traffic = MEASURED_TRAFFIC_256_MERGED_E[layout] if n == 256 else None
+        if form and dom == "ns_fourier_x":
+            traffic = MEASURED_TRAFFIC_256_MERGED_E[(layout, form)] if n == 256 else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(k["GBps"] / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "traffic_source": TRAFFIC_SOURCE if traffic else None,
                 "algo_bytes_per_launch": passes[dom] * prec_bytes * pts,
                 "algo_bytes_formula": "%d field passes x %d B x %s" % (passes[dom], prec_bytes,
                                                                        "N^3/2" if half else "N^3"),
-                "merged_history": bool(merged and dom == "ns_fourier_x"),
+                "history_form": form if dom == "ns_fourier_x" else 0,
                 "avg_launch_ms": k["avg_ms"], "timing": "HIP events on the launch stream around every launch of 1 in %d steps of the timed region"
                 % PROF_EVERY}
     step_passes = sum(passes[k] for k, (ms, cnt) in kern.items() if cnt > 0)
@@ -310,7 +317,7 @@ def main():
     value = a.steps / dt
     log("bench: %.2f steps/s (%.3f ms/step), final energy %r, div avg %.2e"
         % (value, 1e3 * dt / a.steps, st["energy"], st["div_avg"]))
-    kernels, roof, step_bytes, step_passes = kernel_table(kern, n, st["half"], st["merged"])
+    kernels, roof, step_bytes, step_passes = kernel_table(kern, n, st["half"], st["form"])
     state = {"final_energy": st["energy"], "final_div_avg": st["div_avg"],
              "finite": bool(st["energy"] == st["energy"] and abs(st["energy"]) < 1e30),
              "steps_done": a.warmup + a.steps,
@@ -321,14 +328,14 @@ def main():
     if not a.no_full:
         # the plain method: the same step on the full complex spectrum (C2C)
         dtf, kernf, stf = run_ns(S, n, S.SH_F64, a.warmup, a.steps, profile=True, layout=0)
-        kf, rooff, bytesf, passesf = kernel_table(kernf, n, stf["half"], stf["merged"])
+        kf, rooff, bytesf, passesf = kernel_table(kernf, n, stf["half"], stf["form"])
         vf = a.steps / dtf
         full = {"steps_per_s": round(vf, 3), "ms_per_step": round(1e3 * dtf / a.steps, 4),
                 "layout": "full spectrum (C2C)", "roofline": rooff, "kernels": kf,
                 "step_algo_GB_fused": round(bytesf / 1e9, 3), "step_GBps_fused": round(bytesf * vf / 1e9, 1),
                 "step_contract_GB": round(1056.0 * n ** 3 / 1e9, 3),
                 "fused_bytes_formula": "%d N^3 B/step (%d field passes)" % (16 * passesf, passesf),
-                "merged_history": stf["merged"],
+                "history_form": stf["form"],
                 "contract_bytes_formula": "1056 N^3 B/step (9 3D FFTs + 12 array passes)",
                 "final_energy": stf["energy"]}
         log("bench: full-spectrum (plain C2C) %.2f steps/s" % vf)
@@ -343,7 +350,7 @@ def main():
             "chosen_form_formula": ("%d N^3 B/step (%d field passes over the N^3/2 stored points)"
                                     % (8 * step_passes, step_passes)) if st["half"] else
                                    ("%d N^3 B/step (%d field passes)" % (16 * step_passes, step_passes)),
-            "merged_history": st["merged"]}
+            "history_form": st["form"]}
     copy = None
     if not a.no_copy:
         log("bench: HBM copy ceiling")
@@ -412,7 +419,8 @@ def main():
                    "ic": "random solenoidal, E(k)~k^4 exp(-2(k/4)^2), u_rms=1, seed %d" % SEED,
                    "form": ("chosen form: half-spectrum storage of the real fields (planes kz < nz/2, "
                             "R2C-equivalent)%s, %d N^3 B/step; the plain C2C method is full_spectrum"
-                            % (", merged AB2 history in kernel E" if st["merged"] else "", 8 * step_passes))
+                            % ({0: "", 1: ", merged AB2 history in kernel E",
+                                2: ", merged AB2 history in kernel E, five emitted fields"}[st["form"]], 8 * step_passes))
                    if st["half"] else "plain method: full complex spectrum (C2C), %d N^3 B/step" % (16 * step_passes),
                    "schedule": "fused (4 kernels/step)",
                    "layout": "half spectrum (kz < nz/2; real fields)" if st["half"] else "full spectrum"},

@@ -246,12 +246,15 @@ int sh_ns_set_nonlinear(sh_ns_t handle, int on);
 int sh_ns_set_layout(sh_ns_t handle, int mode);
 /* 1 if the solver currently steps the half-spectrum layout, else 0 */
 int sh_ns_get_layout(sh_ns_t handle, int* half);
-/* Replaces nothing in the reference: 1 if the fused state is in the merged
- * AB2-history form that the last kernel E left (exec/ns.cpp:171-186 need V1
- * and cross0 only through Y = (V1 + lap1) - K/2 cross0, so that kernel E keeps
- * Y and the velocity's x-inverse, 15 field passes instead of 21), else 0.
- * The getters, snapshots and schedule changes see (V1, cross0) either way. */
-int sh_ns_get_history_form(sh_ns_t handle, int* merged);
+/* Replaces nothing in the reference: the form of the fused state that the
+ * last kernel E left.  0: V1 and cross0.  1: the merged AB2 history
+ * (exec/ns.cpp:171-186 need V1 and cross0 only through Y = (V1 + lap1) -
+ * K/2 cross0, so kernel E keeps Y and the velocity's x-inverse: 15 field
+ * passes instead of 21).  2: the same with five emitted fields (the x-inverses
+ * of V and of d/dx Vy, d/dx Vz; the next y pass forms the vorticity from them:
+ * kernel E 14 passes, the y inverse 11 instead of 12).  The getters,
+ * snapshots and schedule changes see (V1, cross0) in every form. */
+int sh_ns_get_history_form(sh_ns_t handle, int* form);
 /* Fused schedule only.  1: the y passes and the fused z kernel of a step run
  * as two independent halves of the x columns on the solver's stream and a
  * second (non-blocking, default-priority) stream of its own, joined before

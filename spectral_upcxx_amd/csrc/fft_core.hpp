@@ -734,8 +734,10 @@ using X512 = FftPlan<512, 32, 16, 2, 16>;
     X(512, W512, 2)
 
 // pass roles, so y and z lines (different strides) are tuned separately;
-// ROLE_CROSS_H is the fused z kernel of the real-field (half-spectrum) layout
-enum { ROLE_X = 0, ROLE_Y = 1, ROLE_Z = 2, ROLE_CROSS = 3, ROLE_FOURIER = 4, ROLE_CROSS_H = 5, NROLES = 6 };
+// ROLE_CROSS_H is the fused z kernel of the real-field (half-spectrum) layout;
+// ROLE_YCURL the solver's y inverse that forms the vorticity on load
+enum { ROLE_X = 0, ROLE_Y = 1, ROLE_Z = 2, ROLE_CROSS = 3, ROLE_FOURIER = 4, ROLE_CROSS_H = 5, ROLE_YCURL = 6,
+       NROLES = 7 };
 
 // chosen variant for (precision, length, role); 0 when untuned
 int tune_choice(int precision, int n, int role);

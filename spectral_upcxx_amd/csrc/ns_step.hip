@@ -1071,7 +1071,10 @@ __device__ __forceinline__ double* xd_exchange() {
 // three W fields (the next step's input); the host recovers V1 from them by
 // a forward x pass when it needs it.  The same values as the (V1, N0) form to
 // rounding (a different association of the same sums).
-template <class P, bool NT = true, bool SHARE = false, bool G3 = false, bool YS = false>
+// E5 (with YS): the emitted fields are the x-inverses of V and of i kx Vy,
+// i kx Vz -- five instead of six; the y pass forms the vorticity from them
+// on load (k_ns_y_curl), since omega's remaining terms need only ky and kz.
+template <class P, bool NT = true, bool SHARE = false, bool G3 = false, bool YS = false, bool E5 = false>
 __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
                                                       double2* __restrict__ N0, const double2* __restrict__ twg,
                                                       int ny, int nz, int nzp, long F,
@@ -1089,10 +1092,11 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                   "one wave per line, one radix-4 butterfly per lane in the first and last stage");
     static_assert(!SHARE || sm_size<P>() <= N, "stage-major twiddles fit the table");
     static_assert(!YS || (SHARE && G3), "the merged history runs the grouped, shared-slot kernel");
+    static_assert(!E5 || YS, "five emitted fields with the merged history only");
     // vector-memory operations a line issues after its products' loads (the
     // next V1, the 6 + 6 field stores and, when SHARE, the next N0; with YS
-    // the 3 + 6 stores and the next Y), capped at what vmcnt encodes
-    constexpr int AFTER_X = YS ? 3 * PIECES + 9 * E : (SHARE ? 6 * PIECES : 3 * PIECES) + 12 * E;
+    // the 3 + 6 (E5: 3 + 5) stores and the next Y), capped at what vmcnt encodes
+    constexpr int AFTER_X = YS ? 3 * PIECES + (E5 ? 8 : 9) * E : (SHARE ? 6 * PIECES : 3 * PIECES) + 12 * E;
     constexpr int TOP_WAIT = SHARE ? (AFTER_X < 63 ? AFTER_X : 63) : 6 * PIECES;
     const StepCoef<T> cf = cfs[blockIdx.y];
     W += blockIdx.y * boff;
@@ -1320,6 +1324,10 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                     if (h == 0) {
 #pragma unroll
                         for (int d = 0; d < 3; d++) x3[d][s] = vn[d][s];
+                    } else if constexpr (E5) {
+                        x3[0][s] = imul<T>(k0, vn[1][s]);  // d/dx Vy
+                        x3[1][s] = imul<T>(k0, vn[2][s]);  // d/dx Vz
+                        x3[2][s] = mk<T>(T(0), T(0));      // (transformed, not stored)
                     } else {
                         x3[0][s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
                         x3[1][s] = csub<T>(imul<T>(k2, vn[0][s]), imul<T>(k0, vn[2][s]));
@@ -1328,7 +1336,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                 });
                 line_fft_xa3<P, 1>(x3, tid, lds_offset(&n0l[0][0]), tw);  // in the N0 slots, read above
 #pragma unroll
-                for (int d = 0; d < 3; d++)
+                for (int d = 0; d < ((E5 && h == 1) ? 2 : 3); d++)
                     for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + (3 * h + d) * F + off + pos, x3[d][s]); });
             }
         }
@@ -1363,6 +1371,71 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, 64);
     if (tid == 0) divpart[blockIdx.x] = dsum;
+}
+
+// B of a step whose kernel E emitted five fields (E5): the y inverse of V
+// (fields 0..2) and of omega = i k x V, formed on load from V and from the
+// x-inverses of i kx Vy and i kx Vz (fields 3, 4), since after the x pass
+// omega's other terms need only ky (the line position) and kz (the plane):
+//     omega_x = i (ky Vz - kz Vy),  omega_y = i kz Vx - d/dx Vz,
+//     omega_z = d/dx Vy - i ky Vx   -> fields 3, 4, 5.
+// A block holds its columns of all five inputs in registers and transforms
+// the six outputs in turn: 5 field reads and 6 writes instead of 6 and 6.
+// In place: each block reads its own tile of every input before writing.
+// Its barriers order LDS only (SyncLds): a __syncthreads fence would wait for
+// the previous output's stores before every transform.  UNR: the six outputs
+// unrolled (more registers, the compiler interleaves them).  RT: the stage
+// twiddles held in registers (loaded once) and interleaved-complex exchanges.
+template <typename T, class P, int LPB, bool NT, bool UNR, bool RT = false>
+__global__ __launch_bounds__(P::TPL* LPB) void k_ns_y_curl(C2<T>* __restrict__ W, const C2<T>* __restrict__ tw,
+                                                           int ncols, int nouter, long odist, long es, long F,
+                                                           int ny, int nz) {
+    constexpr int CS = LPB + 1;
+    __shared__ __align__(16) T sre[RT ? 2 * P::N * LPB : P::N * CS];
+    __shared__ T sim[RT ? 1 : P::N * CS];
+    C2<T>* const sx = reinterpret_cast<C2<T>*>(sre);
+    const int c = threadIdx.x % LPB;
+    const int tid = threadIdx.x / LPB;
+    const int tiles = (ncols + LPB - 1) / LPB;
+    const int o = blockIdx.x / tiles;
+    const int col = (blockIdx.x % tiles) * LPB + c;
+    const bool valid = col < ncols && o < nouter;
+    C2<T>* base = W + (valid ? (col + (long)o * odist) : 0);
+    const T kz = freq<T>(o, nz);
+    // 32-bit element offsets within the five (six) fields
+    const unsigned uF = (unsigned)F, ues = (unsigned)es;
+    C2<T> v[5][P::E], x[P::E];
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+        for_in<P>(tid, [&](int s, int pos) { v[f][s] = base[(unsigned)f * uF + (unsigned)pos * ues]; });
+    auto idx = [&](int pos) { return RT ? pos * LPB + c : pos * CS + c; };
+    C2<T> tws[RT ? rt_size<P>() : 1];
+    if constexpr (RT) rt_load<P, 0, T>(tws, tid, tw);
+    // one output at a time; not unrolled, the inputs stay the only live state
+#pragma unroll(UNR ? 6 : 1)
+    for (int g = 0; g < 6; g++) {
+        const int f = g < 3 ? g + 3 : g - 3;  // the vorticity first
+        for_in<P>(tid, [&](int s, int pos) {
+            const T ky = freq<T>(pos, ny);
+            if (f == 3)
+                x[s] = csub<T>(imul<T>(ky, v[2][s]), imul<T>(kz, v[1][s]));
+            else if (f == 4)
+                x[s] = csub<T>(imul<T>(kz, v[0][s]), v[4][s]);
+            else if (f == 5)
+                x[s] = csub<T>(v[3][s], imul<T>(ky, v[0][s]));
+            else
+                x[s] = f == 0 ? v[0][s] : f == 1 ? v[1][s] : v[2][s];
+        });
+        if (g > 0) SyncLds::run();
+        if constexpr (RT)
+            line_fft_cr<P, 1, T, decltype(idx), SyncLds>(x, tid, sx, idx, tws);
+        else
+            line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+        if (valid)
+            for_out<P>(tid, [&](int s, int pos) {
+                st_line<NT, T>(base + ((unsigned)f * uF + (unsigned)pos * ues), x[s]);
+            });
+    }
 }
 
 // ---------------------------------------------------------------- conversion kernels
@@ -1546,6 +1619,9 @@ struct sh_ns_s {
     // x-inverse of V1 (W is primed); ns_canonical returns to (V1, N0)
     bool yform;
     bool keep_v1;  // the merged history is off (a caller reads V1 in place)
+    // W holds the five-field emission (V and i kx Vy, i kx Vz x-inverted): the
+    // next y inverse forms the vorticity on load (k_ns_y_curl)
+    bool w5;
     double div_avg;
     const void* forcing;  // body force (3 fields per box, precision prec) or null
     int cur_box;          // box whose start-up is running
@@ -1782,6 +1858,27 @@ int launch_inv_x(sh_ns_s* s) {
     hipLaunchKernelGGL((k_ns_inv_x<T, P, LPB>), grid, block, 0, s->st, (const C2<T>*)s->V1f,
                        (C2<T>*)s->Wf, tw, s->ny, s->nz, nzp, s->N * s->nboxes, s->N);
     SH_LAUNCH_CHECK("k_ns_inv_x");
+    s->w5 = false;
+    return SH_OK;
+}
+
+// the y inverse that forms the vorticity on load (k_ns_y_curl) over the
+// columns [cx0, cx0 + cnx) on stream cst
+template <typename T, class P, int LPB, bool NT = true, bool UNR = false, bool RT = false>
+int launch_y_curl(sh_ns_s* s) {
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
+    if (!tw) return SH_ENOMEM;
+    if (P::N != s->ny || s->nboxes != 1) {
+        set_error("ns: the vorticity-forming y pass needs one box and a plan of length ny");
+        return SH_EINVAL;
+    }
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long tiles = (s->cnx + LPB - 1) / LPB;
+    ProfScope ps("ns_y_inv", s->cst);
+    hipLaunchKernelGGL((k_ns_y_curl<T, P, LPB, NT, UNR, RT>), dim3((unsigned)(tiles * nzp)), dim3(P::TPL * LPB), 0, s->cst,
+                       (C2<T>*)s->Wf + s->cx0, tw, s->cnx, nzp, (long)s->nx * s->ny, (long)s->nx, s->N, s->ny,
+                       s->nz);
+    SH_LAUNCH_CHECK("k_ns_y_curl");
     return SH_OK;
 }
 
@@ -1887,6 +1984,11 @@ int launch_fourier_x(sh_ns_s* s) {
         set_error("ns: kernel E of the (V1, N0) form on a merged-history state");
         return SH_EINVAL;
     }
+    if (s->w5 && capturing(s->st)) {
+        set_error("ns: the emitted form of kernel E would change inside a graph capture; run one step "
+                  "with the captured schedule before capturing");
+        return SH_EINVAL;
+    }
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
     const int nzp = s->half ? s->nz / 2 : s->nz;
@@ -1904,6 +2006,7 @@ int launch_fourier_x(sh_ns_s* s) {
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
     s->w_primed = EMIT;
+    s->w5 = false;
     if (YS) s->yform = true;
     return SH_OK;
 }
@@ -1927,6 +2030,11 @@ int launch_fourier_xd(sh_ns_s* s) {
     if (s->forcing) return launch_fourier_x<T, P, 1, true, false, NT>(s);
     if (s->yform) {
         set_error("ns: kernel E of the (V1, N0) form on a merged-history state");
+        return SH_EINVAL;
+    }
+    if (s->w5 && capturing(s->st)) {
+        set_error("ns: the emitted form of kernel E would change inside a graph capture; run one step "
+                  "with the captured schedule before capturing");
         return SH_EINVAL;
     }
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
@@ -1956,6 +2064,7 @@ int launch_fourier_xd(sh_ns_s* s) {
     SH_LAUNCH_CHECK("k_ns_fourier_xd");
     s->div_nb = (int)nb;
     s->w_primed = true;
+    s->w5 = false;
     return SH_OK;
 }
 
@@ -1963,11 +2072,14 @@ int launch_fourier_xd(sh_ns_s* s) {
 // where the merged form does not apply, the (V1, N0) kernel of the same plan.
 // CAP > 0 limits the persistent grid to CAP blocks per compute unit (the
 // kernel's LDS admits five one-wave blocks, one more than a CU has SIMDs)
-template <typename T, class P, bool NT = true, int CAP = 0>
+template <typename T, class P, bool NT = true, int CAP = 0, bool E5 = false>
 int launch_fourier_xy(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     // a (V1, N0) state is converted only outside a graph capture
-    if (!y_ok(s) || (!s->yform && capturing(s->st))) return launch_fourier_xd<T, P, NT, true, true>(s);
+    const bool cap = capturing(s->st);
+    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_xd<T, P, NT, true, true>(s);
+    // inside a capture the emitted form stays what the captured y pass expects
+    const bool e5 = cap ? s->w5 : E5;
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
     if (!tw) return SH_ENOMEM;
     if (!s->yform) RC((launch_yform<T, true>(s)));
@@ -1979,8 +2091,8 @@ int launch_fourier_xy(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true>, 64, 0) !=
-                hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true, E5>, 64,
+                                                         0) != hipSuccess ||
             per <= 0)
             per = 1;
         if (CAP > 0 && per > CAP) per = CAP;
@@ -1992,13 +2104,19 @@ int launch_fourier_xy(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true>), dim3(nb, 1u), dim3(64), 0, s->st,
-                       (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz, nzp,
-                       s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
+    if (e5)
+        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, true>), dim3(nb, 1u), dim3(64), 0, s->st,
+                           (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz,
+                           nzp, s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
+    else
+        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, false>), dim3(nb, 1u), dim3(64), 0, s->st,
+                           (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz,
+                           nzp, s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_xd");
     s->div_nb = (int)nb;
     s->w_primed = true;
     s->yform = true;
+    s->w5 = e5;
     return SH_OK;
 }
 
@@ -2067,6 +2185,14 @@ int launch_fourier_xy(sh_ns_s* s) {
 // the same grouped, shared-slot kernel with the merged AB2 history (YS):
 // X(n, plan, nontemporal stores, blocks per CU (0: as many as fit))
 #define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0) X(256, W256, true, 4)
+// the same emitting five fields (E5; the y pass forms the vorticity)
+#define SH_FOURIER_F64_Y5_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
+// the vorticity-forming y pass: X(ny, plan, columns per block, nontemporal
+// stores, outputs unrolled, twiddles in registers); the first is the default
+// (the not-unrolled forms ran 490-540 us at 256^3 against 314-357 us, and were removed)
+#define SH_YCURL_F64_VARIANTS(X)                                                                      \
+    X(256, Q256, 8, true, true, true) X(256, Q256, 8, true, true, false) X(256, W256, 8, true, true, false) \
+    X(256, W256, 8, true, true, true)
 // the register-loaded kernel E with the merged history: X(n, plan, lines per
 // block), plain and nontemporal stores
 #define SH_FOURIER_F64_YR_VARIANTS(X) X(64, W64, 16) X(96, Q96, 10) X(128, W128, 8) X(512, W512, 2) X(512, P512, 4)
@@ -2197,6 +2323,11 @@ const NsVariants& fourier_variants(int n) {
                           ", blocks per CU=" #CAP_);                                                            \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);                                                 \
     m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);
+#define SH_VY5(n_, P_, NT_, CAP_)                                                                             \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "      \
+                          "nontemporal=" #NT_);                                                                 \
+    m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);                                           \
+    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -2211,6 +2342,7 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_NT_VARIANTS(SH_VT)
             SH_FOURIER_F64_LDS_VARIANTS(SH_VS)
             SH_FOURIER_F64_Y_VARIANTS(SH_VY)
+            SH_FOURIER_F64_Y5_VARIANTS(SH_VY5)
             SH_FOURIER_F64_YR_VARIANTS(SH_VM)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
@@ -2220,6 +2352,7 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F32_YR_VARIANTS(SH_VM)
         }
 #undef SH_VM
+#undef SH_VY5
 #undef SH_VY
 #undef SH_VS
 #undef SH_VT
@@ -2266,6 +2399,30 @@ int dispatch_fourier_x(sh_ns_s* s) {
     return run_variant<T>(fourier_variants<T>(s->nx), s, s->nx, ROLE_FOURIER);
 }
 
+template <typename T>
+const NsVariants& ycurl_variants(int n) {
+    static const std::map<int, NsVariants> tab = [] {
+        std::map<int, NsVariants> m;
+#define SH_V(n_, P_, C_, NT_, U_, RT_)                                                              \
+    m[n_].names.push_back(#P_ " columns=" #C_ " vorticity on load, nontemporal=" #NT_ ", unrolled=" #U_   \
+                          ", twiddles in registers=" #RT_);                                              \
+    m[n_].fns.push_back(&launch_y_curl<T, P_, C_, NT_, U_, RT_>);
+        if constexpr (sizeof(T) == 8) {
+            SH_YCURL_F64_VARIANTS(SH_V)
+        }
+#undef SH_V
+        return m;
+    }();
+    static const NsVariants none;
+    auto it = tab.find(n);
+    return it == tab.end() ? none : it->second;
+}
+
+template <typename T>
+int dispatch_y_curl(sh_ns_s* s) {
+    return run_variant<T>(ycurl_variants<T>(s->ny), s, s->ny, ROLE_YCURL);
+}
+
 // whether this step's kernel E keeps the merged AB2 history
 bool fourier_merged(sh_ns_s* s) {
     if (!y_ok(s)) return false;
@@ -2288,7 +2445,9 @@ int step_bcd(sh_ns_s* s, hipStream_t st, int x0, int ncx) {
     s->cst = st;
     s->cx0 = x0;
     s->cnx = ncx;
-    {
+    if (s->w5) {
+        RC(dispatch_y_curl<T>(s));  // its own profiling scope, "ns_y_inv"
+    } else {
         ProfScope ps("ns_y_inv", st);
         RC(pass_strided<T>(w, s->ny, ncx, 1, nzp, sxy, s->nx, 6 * s->nboxes, s->N, +1, plain, st, ROLE_Y));
     }
@@ -2364,6 +2523,9 @@ namespace sh {
 int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx);
 
 int ns_variant_count(int precision, int n, int role) {
+    if (role == ROLE_YCURL)
+        return precision == SH_F64 ? (int)ycurl_variants<double>(n).fns.size()
+                                   : (int)ycurl_variants<float>(n).fns.size();
     if (role == ROLE_CROSS_H)
         return precision == SH_F64 ? (int)crossh_variants<double>(n).fns.size()
                                    : (int)crossh_variants<float>(n).fns.size();
@@ -2376,7 +2538,8 @@ int ns_variant_count(int precision, int n, int role) {
 
 const char* ns_variant_name(int precision, int n, int role, int v) {
     const NsVariants& vl =
-        role == ROLE_CROSS_H ? (precision == SH_F64 ? crossh_variants<double>(n) : crossh_variants<float>(n))
+        role == ROLE_YCURL   ? (precision == SH_F64 ? ycurl_variants<double>(n) : ycurl_variants<float>(n))
+        : role == ROLE_CROSS_H ? (precision == SH_F64 ? crossh_variants<double>(n) : crossh_variants<float>(n))
         : role == ROLE_CROSS ? (precision == SH_F64 ? cross_variants<double>(n) : cross_variants<float>(n))
                              : (precision == SH_F64 ? fourier_variants<double>(n) : fourier_variants<float>(n));
     return v >= 0 && v < (int)vl.names.size() ? vl.names[v] : "default";
@@ -2409,14 +2572,16 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     SH_HIP(hipMemcpyAsync(sv, s->V1f, 3 * fb, hipMemcpyDeviceToDevice, s->st));
     SH_HIP(hipMemcpyAsync(sv + 3 * fb, s->N0f, 3 * fb, hipMemcpyDeviceToDevice, s->st));
     SH_HIP(hipMemcpyAsync(sv + 6 * fb, s->Wf, 6 * fb, hipMemcpyDeviceToDevice, s->st));
-    const bool primed = s->w_primed, yform = s->yform;
+    const bool primed = s->w_primed, yform = s->yform, w5 = s->w5;
     hipEvent_t e0, e1;
     SH_HIP(hipEventCreate(&e0));
     SH_HIP(hipEventCreate(&e1));
     int rc = SH_OK;
-    const int roles[3] = {ROLE_Y, s->half ? ROLE_CROSS_H : ROLE_CROSS, ROLE_FOURIER};
-    const int lens[3] = {s->ny, s->nz, s->nx};
-    for (int k = 0; k < 3 && !rc; k++) {
+    // the vorticity-forming y pass last: it runs only after a kernel E that
+    // emits five fields, so it is timed with the chosen E
+    const int roles[4] = {ROLE_Y, s->half ? ROLE_CROSS_H : ROLE_CROSS, ROLE_FOURIER, ROLE_YCURL};
+    const int lens[4] = {s->ny, s->nz, s->nx, s->ny};
+    for (int k = 0; k < 4 && !rc; k++) {
         const int role = roles[k], n = lens[k];
         const int nv = role == ROLE_Y ? pass_variant_count(s->prec, n, role) : ns_variant_count(s->prec, n, role);
         if (nv <= 1) continue;
@@ -2451,6 +2616,7 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     pool_free(save);
     s->w_primed = primed;
     s->yform = yform;
+    s->w5 = w5;
     s->have_div = false;
     return rc;
 }
@@ -2552,6 +2718,7 @@ int decide_layout(sh_ns_s* s, int b) {
 int install_box(sh_ns_s* s, int b) {
     const long N = s->N;
     s->yform = false;
+    s->w5 = false;
     for (int d = 0; d < 3; d++) {
         const long off = ((long)d * s->nboxes + b) * N;
         if (s->prec == SH_F64) {
@@ -2978,9 +3145,9 @@ int sh_ns_get_layout(sh_ns_t s, int* half) {
     return SH_OK;
 }
 
-int sh_ns_get_history_form(sh_ns_t s, int* merged) {
-    if (!s || !merged) return SH_EINVAL;
-    *merged = s->yform ? 1 : 0;
+int sh_ns_get_history_form(sh_ns_t s, int* form) {
+    if (!s || !form) return SH_EINVAL;
+    *form = s->yform ? (s->w5 ? 2 : 1) : 0;
     return SH_OK;
 }
 

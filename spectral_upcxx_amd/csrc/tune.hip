@@ -37,7 +37,7 @@ void tune_set(int precision, int n, int role, int variant, double ms) {
 }
 
 static const char* role_name(int role) {
-    static const char* names[NROLES] = {"x", "y", "z", "ns_cross_z", "ns_fourier_x", "ns_cross_zh"};
+    static const char* names[NROLES] = {"x", "y", "z", "ns_cross_z", "ns_fourier_x", "ns_cross_zh", "ns_y_curl"};
     return role >= 0 && role < NROLES ? names[role] : "?";
 }
 

@@ -184,7 +184,7 @@ def test_merged_fp32_register_kernel(hip, v0):
             out[tag] = (mid, a.velocity())
             f = ctypes.c_int()
             L.sh_ns_get_history_form(a.h, ctypes.byref(f))
-            assert f.value == (1 if tag == "merged" else 0)
+            assert f.value == (1 if tag == "merged" else 0)  # the register kernel emits six fields
             a.close()
         b = hip.NS((N,) * 3, NU, DT, 32, fused=True)
         _use(L, merged[0], 32)
@@ -258,3 +258,44 @@ def test_merged_graph_capture(hip, prec):
     finally:
         L.sh_tune_reset()
         L.sh_stream_destroy(st)
+
+
+def test_merged_five_fields_every_y_curl(hip, v0):
+    """Kernel E emitting five fields (V and d/dx Vy, d/dx Vz) with every
+    schedule of the y pass that forms the vorticity from them, both layouts,
+    against the (V1, N0) kernel E with the six-field y pass."""
+    L = hip.lib()
+    merged, plain = _variants(L)
+    five = []
+    buf = ctypes.create_string_buffer(8192)
+    for v in merged:
+        _use(L, v)
+        L.sh_tune_report(buf, 8192)
+        if "five emitted fields" in buf.value.decode():
+            five.append(v)
+    assert five
+    c = ctypes.c_int()
+    L.sh_tune_variant_count(64, N, 6, ctypes.byref(c))
+    assert c.value > 0
+    try:
+        for layout in (1, 0):
+            _use(L, plain[0])
+            ref = _solver(hip, v0, layout)
+            ref.steps(5)
+            vr = ref.velocity()
+            ref.close()
+            for e in five:
+                for y in range(c.value):
+                    _use(L, e)
+                    L.sh_tune_set_variant(64, N, 6, y)
+                    ns = _solver(hip, v0, layout)
+                    ns.steps(5)
+                    form = ctypes.c_int()
+                    L.sh_ns_get_history_form(ns.h, ctypes.byref(form))
+                    assert form.value == 2
+                    err = _rel(ns.velocity(), vr)
+                    ns.close()
+                    print("layout=%d E=%d y_curl=%d vs (V1, N0): %.2e" % (layout, e, y, err), flush=True)
+                    assert err < 1e-13, (layout, e, y, err)
+    finally:
+        L.sh_tune_reset()
