@@ -666,8 +666,10 @@ __global__ __launch_bounds__(256) void k_herm_dev(const double2* __restrict__ f,
 // counters count loads and stores together)
 // YS: the merged AB2 history (see k_ns_fourier_xd): N0 holds Y = (V1 + lap1)
 // - K/2 N0, V is neither read nor written, and the new V survives only as the
-// emitted x-inverse (so YS needs EMIT)
-template <typename T, class P, int LPB, bool EMIT, bool BL = false, bool NT = false, bool YS = false>
+// emitted x-inverse (so YS needs EMIT).  E5 (with YS): five emitted fields,
+// the x-inverses of V and of i kx Vy, i kx Vz (see k_ns_y_curl)
+template <typename T, class P, int LPB, bool EMIT, bool BL = false, bool NT = false, bool YS = false,
+          bool E5 = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
@@ -680,6 +682,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     static_assert(!EMIT || (P::radix(0) == P::radix(P::S - 1) && P::nb(0) == P::nb(P::S - 1)),
                   "emitting the next inverse pass needs a symmetric plan");
     static_assert(!YS || EMIT, "the merged history keeps V only as the emitted x-inverse");
+    static_assert(!E5 || YS, "five emitted fields with the merged history only");
     // per-box coefficients (each box of an ensemble has its own viscosity)
     const StepCoef<T> cf = cfs[blockIdx.y];
     W += blockIdx.y * boff;
@@ -823,13 +826,16 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
             });
     }
     if constexpr (EMIT) {
-        // next step's kernel A: inverse x of V (g = 0..2) and of i k x V (g = 3..5)
+        // next step's kernel A: inverse x of V (g = 0..2) and of i k x V (g = 3..5);
+        // E5: of i kx Vy and i kx Vz (g = 3, 4)
 #pragma unroll
-        for (int g = 0; g < 6; g++) {
+        for (int g = 0; g < (E5 ? 5 : 6); g++) {
             for_out<P>(tid, [&](int s, int pos) {
                 const T k0 = freq<T>(pos, N);
                 if (g < 3)
                     x[s] = vn[g][s];
+                else if (E5)
+                    x[s] = imul<T>(k0, vn[g - 2][s]);
                 else if (g == 3)
                     x[s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
                 else if (g == 4)
@@ -1978,13 +1984,14 @@ int launch_cross_zd(sh_ns_s* s) {
 
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
-template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false, bool YS = false>
+template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false, bool YS = false,
+          bool E5 = false>
 int launch_fourier_x(sh_ns_s* s) {
     if (!YS && s->yform) {
         set_error("ns: kernel E of the (V1, N0) form on a merged-history state");
         return SH_EINVAL;
     }
-    if (s->w5 && capturing(s->st)) {
+    if (s->w5 != E5 && capturing(s->st)) {
         set_error("ns: the emitted form of kernel E would change inside a graph capture; run one step "
                   "with the captured schedule before capturing");
         return SH_EINVAL;
@@ -1999,14 +2006,14 @@ int launch_fourier_x(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT, YS>), dim3(nb, (unsigned)s->nboxes),
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT, YS, E5>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
                        (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, nzp, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_x");
     s->div_nb = (int)nb;
     s->w_primed = EMIT;
-    s->w5 = false;
+    s->w5 = E5;
     if (YS) s->yform = true;
     return SH_OK;
 }
@@ -2014,10 +2021,17 @@ int launch_fourier_x(sh_ns_s* s) {
 // the register-loaded kernel E with the merged AB2 history: converts a
 // (V1, N0) state first (never inside a graph capture); where the merged form
 // does not apply, the (V1, N0) kernel of the same schedule
-template <typename T, class P, int LPB, bool BL = false, bool NT = false>
+// a vorticity-forming y pass is compiled for this solver's ny (a five-field
+// kernel E needs one for the next step)
+bool y_curl_ok(sh_ns_s* s);
+
+template <typename T, class P, int LPB, bool BL = false, bool NT = false, bool E5 = false>
 int launch_fourier_xm(sh_ns_s* s) {
-    if (!y_ok(s) || (!s->yform && capturing(s->st))) return launch_fourier_x<T, P, LPB, true, BL, NT>(s);
+    const bool cap = capturing(s->st);
+    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_x<T, P, LPB, true, BL, NT>(s);
     if (!s->yform) RC((launch_yform<T, true>(s)));
+    // inside a capture the emitted form stays what the captured y pass expects
+    if (cap ? s->w5 : (E5 && y_curl_ok(s))) return launch_fourier_x<T, P, LPB, true, BL, NT, true, true>(s);
     return launch_fourier_x<T, P, LPB, true, BL, NT, true>(s);
 }
 
@@ -2079,7 +2093,7 @@ int launch_fourier_xy(sh_ns_s* s) {
     const bool cap = capturing(s->st);
     if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_xd<T, P, NT, true, true>(s);
     // inside a capture the emitted form stays what the captured y pass expects
-    const bool e5 = cap ? s->w5 : E5;
+    const bool e5 = cap ? s->w5 : (E5 && y_curl_ok(s));
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
     if (!tw) return SH_ENOMEM;
     if (!s->yform) RC((launch_yform<T, true>(s)));
@@ -2192,12 +2206,19 @@ int launch_fourier_xy(sh_ns_s* s) {
 // (the not-unrolled forms ran 490-540 us at 256^3 against 314-357 us, and were removed)
 #define SH_YCURL_F64_VARIANTS(X)                                                                      \
     X(256, Q256, 8, true, true, true) X(256, Q256, 8, true, true, false) X(256, W256, 8, true, true, false) \
-    X(256, W256, 8, true, true, true)
+    X(256, W256, 8, true, true, true) X(512, W512, 4, true, true, true) X(512, W512, 4, true, true, false)  \
+    X(512, P512, 4, true, true, true)
+#define SH_YCURL_F32_VARIANTS(X)                                                                      \
+    X(256, Q256, 8, true, true, true) X(256, W256, 8, true, true, true) X(256, Q256, 16, true, true, true) \
+    X(512, W512, 4, true, true, true) X(512, P512, 4, true, true, true) X(512, P512, 8, true, true, true)
 // the register-loaded kernel E with the merged history: X(n, plan, lines per
 // block), plain and nontemporal stores
 #define SH_FOURIER_F64_YR_VARIANTS(X) X(64, W64, 16) X(96, Q96, 10) X(128, W128, 8) X(512, W512, 2) X(512, P512, 4)
 #define SH_FOURIER_F32_YR_VARIANTS(X) \
     X(64, W64, 16) X(64, P64, 32) X(96, Q96, 10) X(128, Q128, 16) X(256, Q256, 8) X(256, W256, 4) X(512, P512, 4) X(512, W512, 2)
+// the register-loaded merged kernel E emitting five fields: X(n, plan, lines per block)
+#define SH_FOURIER_F64_YR5_VARIANTS(X) X(512, W512, 2) X(512, P512, 4)
+#define SH_FOURIER_F32_YR5_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4) X(512, P512, 4) X(512, W512, 2)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -2328,6 +2349,10 @@ const NsVariants& fourier_variants(int n) {
                           "nontemporal=" #NT_);                                                                 \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);                                           \
     m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);
+#define SH_VM5(n_, P_, L_)                                                                      \
+    m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, five emitted fields, nontemporal stores"); \
+    m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);                                \
+    m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -2344,13 +2369,16 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_Y_VARIANTS(SH_VY)
             SH_FOURIER_F64_Y5_VARIANTS(SH_VY5)
             SH_FOURIER_F64_YR_VARIANTS(SH_VM)
+            SH_FOURIER_F64_YR5_VARIANTS(SH_VM5)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
             SH_FOURIER_F32_BATCHED_VARIANTS(SH_VB)
             SH_FOURIER_F32_NT_VARIANTS(SH_VT)
             SH_FOURIER_F32_YR_VARIANTS(SH_VM)
+            SH_FOURIER_F32_YR5_VARIANTS(SH_VM5)
         }
+#undef SH_VM5
 #undef SH_VM
 #undef SH_VY5
 #undef SH_VY
@@ -2409,6 +2437,8 @@ const NsVariants& ycurl_variants(int n) {
     m[n_].fns.push_back(&launch_y_curl<T, P_, C_, NT_, U_, RT_>);
         if constexpr (sizeof(T) == 8) {
             SH_YCURL_F64_VARIANTS(SH_V)
+        } else {
+            SH_YCURL_F32_VARIANTS(SH_V)
         }
 #undef SH_V
         return m;
@@ -2421,6 +2451,11 @@ const NsVariants& ycurl_variants(int n) {
 template <typename T>
 int dispatch_y_curl(sh_ns_s* s) {
     return run_variant<T>(ycurl_variants<T>(s->ny), s, s->ny, ROLE_YCURL);
+}
+
+bool y_curl_ok(sh_ns_s* s) {
+    return s->nboxes == 1 && !(s->prec == SH_F64 ? ycurl_variants<double>(s->ny) : ycurl_variants<float>(s->ny))
+                                 .fns.empty();
 }
 
 // whether this step's kernel E keeps the merged AB2 history

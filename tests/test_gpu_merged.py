@@ -260,42 +260,60 @@ def test_merged_graph_capture(hip, prec):
         L.sh_stream_destroy(st)
 
 
-def test_merged_five_fields_every_y_curl(hip, v0):
+@pytest.mark.parametrize("prec,n", [(64, 256), (32, 256), (64, 512), (32, 512)])
+def test_merged_five_fields_every_y_curl(hip, prec, n):
     """Kernel E emitting five fields (V and d/dx Vy, d/dx Vz) with every
     schedule of the y pass that forms the vorticity from them, both layouts,
-    against the (V1, N0) kernel E with the six-field y pass."""
+    against the (V1, N0) kernel E with the six-field y pass (the streamed
+    kernel E at 256^3 FP64, the register-loaded one otherwise)."""
+    import torch
     L = hip.lib()
-    merged, plain = _variants(L)
+    merged, plain = _variants(L, prec, n, "grouped=true" if (prec, n) == (64, 256) else "lines=")
     five = []
     buf = ctypes.create_string_buffer(8192)
     for v in merged:
-        _use(L, v)
+        _use(L, v, prec, n)
         L.sh_tune_report(buf, 8192)
         if "five emitted fields" in buf.value.decode():
             five.append(v)
     assert five
     c = ctypes.c_int()
-    L.sh_tune_variant_count(64, N, 6, ctypes.byref(c))
+    L.sh_tune_variant_count(prec, n, 6, ctypes.byref(c))
     assert c.value > 0
+    V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    L.sh_solenoidal_ic_z(hip.ptr(V0), n, n, n, 12, 4.0, 1.0, hip.stream())
+    tol = 1e-13 if prec == 64 else 1e-6
+    steps = 5 if n == 256 else 3
+
+    def run(layout):
+        ns = hip.NS((n,) * 3, NU, DT, prec, fused=True)
+        L.sh_ns_set_layout(ns.h, layout)
+        ns.set_initial(V0)
+        ns.startup()
+        ns.steps(steps)
+        form = ctypes.c_int()
+        L.sh_ns_get_history_form(ns.h, ctypes.byref(form))
+        v = ns.velocity()
+        ns.close()
+        return v, form.value
+
     try:
         for layout in (1, 0):
-            _use(L, plain[0])
-            ref = _solver(hip, v0, layout)
-            ref.steps(5)
-            vr = ref.velocity()
-            ref.close()
+            _use(L, plain[0], prec, n)
+            vr, f0 = run(layout)
+            assert f0 == 0
             for e in five:
                 for y in range(c.value):
-                    _use(L, e)
-                    L.sh_tune_set_variant(64, N, 6, y)
-                    ns = _solver(hip, v0, layout)
-                    ns.steps(5)
-                    form = ctypes.c_int()
-                    L.sh_ns_get_history_form(ns.h, ctypes.byref(form))
-                    assert form.value == 2
-                    err = _rel(ns.velocity(), vr)
-                    ns.close()
-                    print("layout=%d E=%d y_curl=%d vs (V1, N0): %.2e" % (layout, e, y, err), flush=True)
-                    assert err < 1e-13, (layout, e, y, err)
+                    _use(L, e, prec, n)
+                    L.sh_tune_set_variant(prec, n, 6, y)
+                    v, form = run(layout)
+                    err = _rel(v, vr)
+                    print("fp%d n=%d layout=%d E=%d y_curl=%d vs (V1, N0): %.2e" % (prec, n, layout, e, y, err),
+                          flush=True)
+                    assert form == 2
+                    assert err < tol, (layout, e, y, err)
     finally:
         L.sh_tune_reset()
+        del V0
+        torch.cuda.empty_cache()
+        L.sh_pool_trim()
