@@ -1080,7 +1080,10 @@ __device__ __forceinline__ double* xd_exchange() {
 // E5 (with YS): the emitted fields are the x-inverses of V and of i kx Vy,
 // i kx Vz -- five instead of six; the y pass forms the vorticity from them
 // on load (k_ns_y_curl), since omega's remaining terms need only ky and kz.
-template <class P, bool NT = true, bool SHARE = false, bool G3 = false, bool YS = false, bool E5 = false>
+// REV: the lines in descending order (the top planes first, which the
+// preceding y pass wrote last and the last-level cache may still hold)
+template <class P, bool NT = true, bool SHARE = false, bool G3 = false, bool YS = false, bool E5 = false,
+          bool REV = false>
 __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
                                                       double2* __restrict__ N0, const double2* __restrict__ twg,
                                                       int ny, int nz, int nzp, long F,
@@ -1152,22 +1155,25 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
         x[3] = mk<T>(v3.x, v3.y);
     };
     double dsum = 0;
+    // the line a block's k-th trip handles (the trips themselves ascend)
+    auto at = [&](long t) { return REV ? nlines - 1 - t : t; };
     long line = blockIdx.x;
     if (line < nlines) {
-        fetch(&xs[0][0], W, 3, line);
-        if constexpr (!YS) fetch(&v1l[0][0], V, 3, line);
-        fetch(&n0l[0][0], N0, 3, line);
+        fetch(&xs[0][0], W, 3, at(line));
+        if constexpr (!YS) fetch(&v1l[0][0], V, 3, at(line));
+        fetch(&n0l[0][0], N0, 3, at(line));
         // the first line's products (TOP_WAIT may exceed what follows them here)
         if constexpr (SHARE) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((YS ? 3 : 6) * PIECES) : "memory");
     }
     for (; line < nlines; line += gridDim.x) {
         const long nxt = line + gridDim.x;
         const bool more = nxt < nlines;
-        const int jj = (int)(line % ny), kk = (int)(line / ny);
+        const long pl = at(line);
+        const int jj = (int)(pl % ny), kk = (int)(pl / ny);
         const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
         const bool zline = (jj == ny / 2) || (kk == nz / 2);
         const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
-        const long off = line * N;
+        const long off = pl * N;
         // products of this line landed: issued after them are this line's V1 and
         // N0 and, from the second line on, the previous line's stores (and, when
         // SHARE, only V1 ahead of the stores and N0 after them)
@@ -1194,7 +1200,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
                 for (int s2 = 0; s2 < E; s2++) n1[d][s2] = x[s2];
             }
         }
-        if (more) fetch(&xs[0][0], W, 3, nxt);
+        if (more) fetch(&xs[0][0], W, 3, at(nxt));
         // V1 and N0 of this line: issued after them, the previous line's stores
         // and the next line's products (none of either on a block's only line)
         if (more)
@@ -1208,8 +1214,8 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             read4(&n0l[d][0], n0s[d]);
         }
         if (more) {
-            if constexpr (!YS) fetch(&v1l[0][0], V, 3, nxt);
-            if constexpr (!SHARE) fetch(&n0l[0][0], N0, 3, nxt);
+            if constexpr (!YS) fetch(&v1l[0][0], V, 3, at(nxt));
+            if constexpr (!SHARE) fetch(&n0l[0][0], N0, 3, at(nxt));
         }
         if constexpr (YS) {
             for_out<P>(tid, [&](int s, int pos) {
@@ -1368,7 +1374,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + g * F + off + pos, x[s]); });
         }
         if constexpr (SHARE) {
-            if (more) fetch(&n0l[0][0], N0, 3, nxt);
+            if (more) fetch(&n0l[0][0], N0, 3, at(nxt));
         } else {
             SyncLds::run();
         }
@@ -2086,7 +2092,7 @@ int launch_fourier_xd(sh_ns_s* s) {
 // where the merged form does not apply, the (V1, N0) kernel of the same plan.
 // CAP > 0 limits the persistent grid to CAP blocks per compute unit (the
 // kernel's LDS admits five one-wave blocks, one more than a CU has SIMDs)
-template <typename T, class P, bool NT = true, int CAP = 0, bool E5 = false>
+template <typename T, class P, bool NT = true, int CAP = 0, bool E5 = false, bool REV = false>
 int launch_fourier_xy(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     // a (V1, N0) state is converted only outside a graph capture
@@ -2105,7 +2111,7 @@ int launch_fourier_xy(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true, E5>, 64,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true, E5, REV>, 64,
                                                          0) != hipSuccess ||
             per <= 0)
             per = 1;
@@ -2119,11 +2125,11 @@ int launch_fourier_xy(sh_ns_s* s) {
     }
     ProfScope ps("ns_fourier_x", s->st);
     if (e5)
-        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, true>), dim3(nb, 1u), dim3(64), 0, s->st,
+        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, true, REV>), dim3(nb, 1u), dim3(64), 0, s->st,
                            (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz,
                            nzp, s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     else
-        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, false>), dim3(nb, 1u), dim3(64), 0, s->st,
+        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, false, REV>), dim3(nb, 1u), dim3(64), 0, s->st,
                            (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz,
                            nzp, s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_xd");
@@ -2201,13 +2207,15 @@ int launch_fourier_xy(sh_ns_s* s) {
 #define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0) X(256, W256, true, 4)
 // the same emitting five fields (E5; the y pass forms the vorticity)
 #define SH_FOURIER_F64_Y5_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
+// the same traversing the lines top plane first
+#define SH_FOURIER_F64_Y5R_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
 // the vorticity-forming y pass: X(ny, plan, columns per block, nontemporal
 // stores, outputs unrolled, twiddles in registers); the first is the default
 // (the not-unrolled forms ran 490-540 us at 256^3 against 314-357 us, and were removed)
 #define SH_YCURL_F64_VARIANTS(X)                                                                      \
     X(256, Q256, 8, true, true, true) X(256, Q256, 8, true, true, false) X(256, W256, 8, true, true, false) \
-    X(256, W256, 8, true, true, true) X(512, W512, 4, true, true, true) X(512, W512, 4, true, true, false)  \
-    X(512, P512, 4, true, true, true)
+    X(256, W256, 8, true, true, true) X(256, Q256, 8, false, true, true)                                   \
+    X(512, W512, 4, true, true, true) X(512, W512, 4, true, true, false) X(512, P512, 4, true, true, true)
 #define SH_YCURL_F32_VARIANTS(X)                                                                      \
     X(256, Q256, 8, true, true, true) X(256, W256, 8, true, true, true) X(256, Q256, 16, true, true, true) \
     X(512, W512, 4, true, true, true) X(512, P512, 4, true, true, true) X(512, P512, 8, true, true, true)
@@ -2349,6 +2357,11 @@ const NsVariants& fourier_variants(int n) {
                           "nontemporal=" #NT_);                                                                 \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);                                           \
     m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);
+#define SH_VY5R(n_, P_, NT_, CAP_)                                                                            \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "      \
+                          "nontemporal=" #NT_ ", top plane first");                                            \
+    m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);                                     \
+    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);
 #define SH_VM5(n_, P_, L_)                                                                      \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, five emitted fields, nontemporal stores"); \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);                                \
@@ -2368,6 +2381,7 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_LDS_VARIANTS(SH_VS)
             SH_FOURIER_F64_Y_VARIANTS(SH_VY)
             SH_FOURIER_F64_Y5_VARIANTS(SH_VY5)
+            SH_FOURIER_F64_Y5R_VARIANTS(SH_VY5R)
             SH_FOURIER_F64_YR_VARIANTS(SH_VM)
             SH_FOURIER_F64_YR5_VARIANTS(SH_VM5)
         } else {
@@ -2381,6 +2395,7 @@ const NsVariants& fourier_variants(int n) {
 #undef SH_VM5
 #undef SH_VM
 #undef SH_VY5
+#undef SH_VY5R
 #undef SH_VY
 #undef SH_VS
 #undef SH_VT
@@ -2614,10 +2629,14 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     int rc = SH_OK;
     // the vorticity-forming y pass last: it runs only after a kernel E that
     // emits five fields, so it is timed with the chosen E
+    // Two rounds over the roles: a kernel's stores (plain or nontemporal) and
+    // its traversal order decide how much of its output the next kernel finds
+    // in the last-level cache, so each role is timed in whole steps and chosen
+    // again once the others have their picks.
     const int roles[4] = {ROLE_Y, s->half ? ROLE_CROSS_H : ROLE_CROSS, ROLE_FOURIER, ROLE_YCURL};
     const int lens[4] = {s->ny, s->nz, s->nx, s->ny};
-    for (int k = 0; k < 4 && !rc; k++) {
-        const int role = roles[k], n = lens[k];
+    for (int k = 0; k < 8 && !rc; k++) {
+        const int role = roles[k % 4], n = lens[k % 4];
         const int nv = role == ROLE_Y ? pass_variant_count(s->prec, n, role) : ns_variant_count(s->prec, n, role);
         if (nv <= 1) continue;
         int best = 0;
