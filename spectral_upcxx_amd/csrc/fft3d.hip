@@ -59,7 +59,9 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_contig(C2<T>* __restrict__ 
     for_out<P>(tid, [&](int slot, int pos) { base[pos] = epilogue<T, P>(x[slot], pos, zline, ep); });
 }
 
-template <typename T, class P, int LPB, int SIGN, bool NT = false>
+// NT: nontemporal stores; LNT: nontemporal loads (each value is read once, so
+// the loads need not displace in the last-level cache what was just written)
+template <typename T, class P, int LPB, int SIGN, bool NT = false, bool LNT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_fft_strided(C2<T>* __restrict__ data,
                                                              const C2<T>* __restrict__ tw,
                                                              int ncols, long cdist, int nouter,
@@ -76,7 +78,15 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_strided(C2<T>* __restrict__
     const bool valid = col < ncols && o < nouter;
     C2<T>* base = data + (long)blockIdx.y * fstride + (valid ? (col * cdist + (long)o * odist) : 0);
     C2<T> x[P::E];
-    for_in<P>(tid, [&](int slot, int pos) { x[slot] = base[pos * es]; });
+    for_in<P>(tid, [&](int slot, int pos) {
+        if constexpr (LNT) {
+            typedef T t2 __attribute__((ext_vector_type(2)));
+            const t2 w = __builtin_nontemporal_load(reinterpret_cast<const t2*>(base + pos * es));
+            x[slot] = mk<T>(w.x, w.y);
+        } else {
+            x[slot] = base[pos * es];
+        }
+    });
     auto idx = [&](int pos) { return pos * CS + c; };
     line_fft<P, SIGN, T>(x, tid, sre, sim, idx, tw);
     if (!valid) return;
@@ -105,7 +115,7 @@ static int launch_contig(C2<T>* data, long nlines, long ldist, int count, long f
     return SH_OK;
 }
 
-template <typename T, class P, int LPB, bool NT = false>
+template <typename T, class P, int LPB, bool NT = false, bool LNT = false>
 static int launch_strided(C2<T>* data, int ncols, long cdist, int nouter, long odist, long es,
                           int count, long fstride, int dir, const Epi<T>& ep, hipStream_t st) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
@@ -117,10 +127,10 @@ static int launch_strided(C2<T>* data, int ncols, long cdist, int nouter, long o
     dim3 grid((unsigned)(tiles * nouter), (unsigned)count);
     dim3 block(P::TPL * LPB);
     if (dir < 0)
-        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, -1, NT>), grid, block, 0, st, data, tw, ncols, cdist,
+        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, -1, NT, LNT>), grid, block, 0, st, data, tw, ncols, cdist,
                            nouter, odist, es, fstride, ep);
     else
-        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, 1, NT>), grid, block, 0, st, data, tw, ncols, cdist,
+        hipLaunchKernelGGL((k_fft_strided<T, P, LPB, 1, NT, LNT>), grid, block, 0, st, data, tw, ncols, cdist,
                            nouter, odist, es, fstride, ep);
     SH_LAUNCH_CHECK("k_fft_strided");
     return SH_OK;
@@ -169,6 +179,11 @@ const VariantList<StridedLaunch<T>>& strided_variants(int n) {
     m[n_].fns.push_back(&launch_strided<T, P_, C_, true>);
         SH_STRIDED_NT_VARIANTS(SH_VT)
 #undef SH_VT
+#define SH_VL(n_, P_, C_, NT_)                                                                    \
+    m[n_].names.push_back(#P_ " columns=" #C_ " nontemporal loads, nontemporal stores=" #NT_);    \
+    m[n_].fns.push_back(&launch_strided<T, P_, C_, NT_, true>);
+        SH_STRIDED_LNT_VARIANTS(SH_VL)
+#undef SH_VL
         return m;
     }();
     static const VariantList<StridedLaunch<T>> none;

@@ -698,6 +698,9 @@ using X512 = FftPlan<512, 32, 16, 2, 16>;
 // strided pass: X(n, plan, columns per block)
 // the same with nontemporal stores (X(n, plan, columns per block))
 #define SH_STRIDED_NT_VARIANTS(X) X(256, Q256, 16) X(256, Q256, 8) X(256, W256, 16) X(512, X512, 16) X(512, P512, 8)
+// with nontemporal loads: X(n, plan, columns per block, nontemporal stores)
+#define SH_STRIDED_LNT_VARIANTS(X) \
+    X(256, Q256, 16, true) X(256, Q256, 16, false) X(256, Q256, 8, false) X(512, X512, 16, true) X(512, X512, 16, false)
 #define SH_STRIDED_VARIANTS(X) \
     X(64, P64, 16)             \
     X(64, P64, 32)             \

@@ -344,7 +344,8 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
     return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)p;
 }
 
-template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double, int RT = 0>
+template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double, int RT = 0,
+          int LAUX = 0>
 __global__ __launch_bounds__(P::TPL* LPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void k_ns_cross_zd(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ twg,
@@ -422,7 +423,7 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
             const C2<T>* src = W + (cb + (unsigned)f * uF + (unsigned)(row0 + lane / LPR) * es);
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                              (__attribute__((address_space(3))) void*)&slot[sl][row0 * LPB], 16,
-                                             0, 0);
+                                             0, LAUX);
         }
     };
     // this thread's four values of a slot, rows tid + r Q from LDS address a,
@@ -1082,8 +1083,11 @@ __device__ __forceinline__ double* xd_exchange() {
 // on load (k_ns_y_curl), since omega's remaining terms need only ky and kz.
 // REV: the lines in descending order (the top planes first, which the
 // preceding y pass wrote last and the last-level cache may still hold)
+// LAUX: cache-policy bits of the loads into LDS (2: nontemporal, so the
+// inputs, read once, do not displace in the last-level cache what the
+// kernel writes for the next one)
 template <class P, bool NT = true, bool SHARE = false, bool G3 = false, bool YS = false, bool E5 = false,
-          bool REV = false>
+          bool REV = false, int LAUX = 0>
 __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, double2* __restrict__ V,
                                                       double2* __restrict__ N0, const double2* __restrict__ twg,
                                                       int ny, int nz, int nzp, long F,
@@ -1133,7 +1137,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
             for (int k = 0; k < PIECES; k++)
                 __builtin_amdgcn_global_load_lds(
                     (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * 64 + tid),
-                    (__attribute__((address_space(3))) void*)(slots + f * N + k * 64), 16, 0, 0);
+                    (__attribute__((address_space(3))) void*)(slots + f * N + k * 64), 16, 0, LAUX);
     };
     typedef double d2v __attribute__((ext_vector_type(2)));
     // this lane's first-stage positions tid + r Q of a landed field
@@ -1398,7 +1402,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
 // the previous output's stores before every transform.  UNR: the six outputs
 // unrolled (more registers, the compiler interleaves them).  RT: the stage
 // twiddles held in registers (loaded once) and interleaved-complex exchanges.
-template <typename T, class P, int LPB, bool NT, bool UNR, bool RT = false>
+template <typename T, class P, int LPB, bool NT, bool UNR, bool RT = false, bool LNT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_y_curl(C2<T>* __restrict__ W, const C2<T>* __restrict__ tw,
                                                            int ncols, int nouter, long odist, long es, long F,
                                                            int ny, int nz) {
@@ -1417,9 +1421,18 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_y_curl(C2<T>* __restrict__ W
     // 32-bit element offsets within the five (six) fields
     const unsigned uF = (unsigned)F, ues = (unsigned)es;
     C2<T> v[5][P::E], x[P::E];
+    typedef T t2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int f = 0; f < 5; f++)
-        for_in<P>(tid, [&](int s, int pos) { v[f][s] = base[(unsigned)f * uF + (unsigned)pos * ues]; });
+        for_in<P>(tid, [&](int s, int pos) {
+            const C2<T>* a = base + ((unsigned)f * uF + (unsigned)pos * ues);
+            if constexpr (LNT) {  // nontemporal loads (read once)
+                const t2 w = __builtin_nontemporal_load(reinterpret_cast<const t2*>(a));
+                v[f][s] = mk<T>(w.x, w.y);
+            } else {
+                v[f][s] = *a;
+            }
+        });
     auto idx = [&](int pos) { return RT ? pos * LPB + c : pos * CS + c; };
     C2<T> tws[RT ? rt_size<P>() : 1];
     if constexpr (RT) rt_load<P, 0, T>(tws, tid, tw);
@@ -1876,7 +1889,7 @@ int launch_inv_x(sh_ns_s* s) {
 
 // the y inverse that forms the vorticity on load (k_ns_y_curl) over the
 // columns [cx0, cx0 + cnx) on stream cst
-template <typename T, class P, int LPB, bool NT = true, bool UNR = false, bool RT = false>
+template <typename T, class P, int LPB, bool NT = true, bool UNR = false, bool RT = false, bool LNT = false>
 int launch_y_curl(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
@@ -1887,7 +1900,8 @@ int launch_y_curl(sh_ns_s* s) {
     const int nzp = s->half ? s->nz / 2 : s->nz;
     const long tiles = (s->cnx + LPB - 1) / LPB;
     ProfScope ps("ns_y_inv", s->cst);
-    hipLaunchKernelGGL((k_ns_y_curl<T, P, LPB, NT, UNR, RT>), dim3((unsigned)(tiles * nzp)), dim3(P::TPL * LPB), 0, s->cst,
+    hipLaunchKernelGGL((k_ns_y_curl<T, P, LPB, NT, UNR, RT, LNT>), dim3((unsigned)(tiles * nzp)), dim3(P::TPL * LPB), 0,
+                       s->cst,
                        (C2<T>*)s->Wf + s->cx0, tw, s->cnx, nzp, (long)s->nx * s->ny, (long)s->nx, s->N, s->ny,
                        s->nz);
     SH_LAUNCH_CHECK("k_ns_y_curl");
@@ -1942,7 +1956,8 @@ int launch_cross_zh(sh_ns_s* s) {
 
 // kernel C of the half-spectrum layout with LDS-streamed inputs (FP64): a
 // persistent grid of the resident blocks looping over the column tiles
-template <typename T, class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, int RT = 0>
+template <typename T, class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, int RT = 0,
+          int LAUX = 0>
 int launch_cross_zd(sh_ns_s* s) {
     constexpr int prec = sizeof(T) == 8 ? SH_F64 : SH_F32;
     // an FP32 lane loads two neighbouring columns: an odd column count would
@@ -1970,7 +1985,8 @@ int launch_cross_zd(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF, T, RT>, P::TPL * LPB,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_cross_zd<P, LPB, NT, IL, HALF, T, RT, LAUX>,
+                                                         P::TPL * LPB,
                                                          0) != hipSuccess ||
             per <= 0)
             per = 1;
@@ -1982,7 +1998,7 @@ int launch_cross_zd(sh_ns_s* s) {
     dim3 grid((unsigned)g, (unsigned)s->nboxes), block(P::TPL * LPB);
     const T scale = T(1.0 / ((double)s->nx * (double)s->ny * (double)s->nz));
     ProfScope ps("ns_cross_z", s->cst);
-    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF, T, RT>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0,
+    hipLaunchKernelGGL((k_ns_cross_zd<P, LPB, NT, IL, HALF, T, RT, LAUX>), grid, block, 0, s->cst, (C2<T>*)s->Wf + s->cx0,
                        tw, twn, s->nx, s->cnx, s->ny, s->N * s->nboxes, scale, s->N, ntile);
     SH_LAUNCH_CHECK("k_ns_cross_zd");
     return SH_OK;
@@ -2092,7 +2108,7 @@ int launch_fourier_xd(sh_ns_s* s) {
 // where the merged form does not apply, the (V1, N0) kernel of the same plan.
 // CAP > 0 limits the persistent grid to CAP blocks per compute unit (the
 // kernel's LDS admits five one-wave blocks, one more than a CU has SIMDs)
-template <typename T, class P, bool NT = true, int CAP = 0, bool E5 = false, bool REV = false>
+template <typename T, class P, bool NT = true, int CAP = 0, bool E5 = false, bool REV = false, int LAUX = 0>
 int launch_fourier_xy(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     // a (V1, N0) state is converted only outside a graph capture
@@ -2111,8 +2127,8 @@ int launch_fourier_xy(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true, E5, REV>, 64,
-                                                         0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xd<P, NT, true, true, true, E5, REV, LAUX>,
+                                                         64, 0) != hipSuccess ||
             per <= 0)
             per = 1;
         if (CAP > 0 && per > CAP) per = CAP;
@@ -2125,11 +2141,11 @@ int launch_fourier_xy(sh_ns_s* s) {
     }
     ProfScope ps("ns_fourier_x", s->st);
     if (e5)
-        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, true, REV>), dim3(nb, 1u), dim3(64), 0, s->st,
+        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, true, REV, LAUX>), dim3(nb, 1u), dim3(64), 0, s->st,
                            (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz,
                            nzp, s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     else
-        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, false, REV>), dim3(nb, 1u), dim3(64), 0, s->st,
+        hipLaunchKernelGGL((k_ns_fourier_xd<P, NT, true, true, true, false, REV, LAUX>), dim3(nb, 1u), dim3(64), 0, s->st,
                            (double2*)s->Wf, (double2*)s->V1f, (double2*)s->N0f, (const double2*)tw, s->ny, s->nz,
                            nzp, s->N, (const StepCoef<double>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
     SH_LAUNCH_CHECK("k_ns_fourier_xd");
@@ -2186,6 +2202,8 @@ int launch_fourier_xy(sh_ns_s* s) {
 #define SH_CROSSH_F64_RT_VARIANTS(X) X(128, W64, 8) X(256, W128, 8) X(256, W128, 16)
 #define SH_CROSSH_F32_NT_VARIANTS(X) X(256, Q128, 16)
 #define SH_CROSSH_F32_LDS_VARIANTS(X) X(256, W128, 16)
+// FP32 kernel C streamed into LDS with nontemporal loads: X(nz, plan, columns per block)
+#define SH_CROSSH_F32_LNT_VARIANTS(X) X(256, W128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -2207,8 +2225,9 @@ int launch_fourier_xy(sh_ns_s* s) {
 #define SH_FOURIER_F64_Y_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0) X(256, W256, true, 4)
 // the same emitting five fields (E5; the y pass forms the vorticity)
 #define SH_FOURIER_F64_Y5_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
-// the same traversing the lines top plane first
+// the same traversing the lines top plane first (and with nontemporal loads)
 #define SH_FOURIER_F64_Y5R_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
+#define SH_FOURIER_F64_Y5RL_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
 // the vorticity-forming y pass: X(ny, plan, columns per block, nontemporal
 // stores, outputs unrolled, twiddles in registers); the first is the default
 // (the not-unrolled forms ran 490-540 us at 256^3 against 314-357 us, and were removed)
@@ -2216,6 +2235,7 @@ int launch_fourier_xy(sh_ns_s* s) {
     X(256, Q256, 8, true, true, true) X(256, Q256, 8, true, true, false) X(256, W256, 8, true, true, false) \
     X(256, W256, 8, true, true, true) X(256, Q256, 8, false, true, true)                                   \
     X(512, W512, 4, true, true, true) X(512, W512, 4, true, true, false) X(512, P512, 4, true, true, true)
+#define SH_YCURL_F64_LNT_VARIANTS(X) X(256, Q256, 8, true, true, true) X(256, Q256, 8, false, true, true)
 #define SH_YCURL_F32_VARIANTS(X)                                                                      \
     X(256, Q256, 8, true, true, true) X(256, W256, 8, true, true, true) X(256, Q256, 16, true, true, true) \
     X(512, W512, 4, true, true, true) X(512, P512, 4, true, true, true) X(512, P512, 8, true, true, true)
@@ -2257,7 +2277,11 @@ const NsVariants& cross_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, false>);
 #define SH_VFR(n_, P_, C_)                                                                             \
     m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers"); \
-    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, false, 2>);
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, false, 2>);                              \
+    m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, false, 2, 2>);                           \
+    m[n_].names.push_back(#P_ " columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, false, 2, 2>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSS_F64_VARIANTS(SH_V)
             SH_CROSS_F64_LDS_VARIANTS(SH_VF)
@@ -2293,13 +2317,22 @@ const NsVariants& crossh_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true>);                           \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true>);
+#define SH_VDL(n_, P_, C_)                                                                                  \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange nontemporal loads");           \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 0, 2>);                                  \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 0, 2>);
 #define SH_VR(n_, P_, C_)                                                                                   \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange packing factors in registers"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 1>);                                    \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers");    \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 2>);                                    \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal"); \
-    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2>);
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2>);                                     \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 2, 2>);                                 \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2, 2>);
         if constexpr (sizeof(T) == 8) {
             SH_CROSSH_F64_VARIANTS(SH_V)
             SH_CROSSH_F64_NT_VARIANTS(SH_VT)
@@ -2309,8 +2342,10 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F32_VARIANTS(SH_V)
             SH_CROSSH_F32_NT_VARIANTS(SH_VT)
             SH_CROSSH_F32_LDS_VARIANTS(SH_VD)
+            SH_CROSSH_F32_LNT_VARIANTS(SH_VDL)
         }
 #undef SH_VR
+#undef SH_VDL
 #undef SH_VD
 #undef SH_VT
 #undef SH_V
@@ -2362,6 +2397,11 @@ const NsVariants& fourier_variants(int n) {
                           "nontemporal=" #NT_ ", top plane first");                                            \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);                                     \
     m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);
+#define SH_VY5RL(n_, P_, NT_, CAP_)                                                                           \
+    m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "      \
+                          "nontemporal=" #NT_ ", top plane first, nontemporal loads");                         \
+    m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true, 2>);                                  \
+    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true, 2>);
 #define SH_VM5(n_, P_, L_)                                                                      \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, five emitted fields, nontemporal stores"); \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);                                \
@@ -2382,6 +2422,7 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_Y_VARIANTS(SH_VY)
             SH_FOURIER_F64_Y5_VARIANTS(SH_VY5)
             SH_FOURIER_F64_Y5R_VARIANTS(SH_VY5R)
+            SH_FOURIER_F64_Y5RL_VARIANTS(SH_VY5RL)
             SH_FOURIER_F64_YR_VARIANTS(SH_VM)
             SH_FOURIER_F64_YR5_VARIANTS(SH_VM5)
         } else {
@@ -2396,6 +2437,7 @@ const NsVariants& fourier_variants(int n) {
 #undef SH_VM
 #undef SH_VY5
 #undef SH_VY5R
+#undef SH_VY5RL
 #undef SH_VY
 #undef SH_VS
 #undef SH_VT
@@ -2450,11 +2492,17 @@ const NsVariants& ycurl_variants(int n) {
     m[n_].names.push_back(#P_ " columns=" #C_ " vorticity on load, nontemporal=" #NT_ ", unrolled=" #U_   \
                           ", twiddles in registers=" #RT_);                                              \
     m[n_].fns.push_back(&launch_y_curl<T, P_, C_, NT_, U_, RT_>);
+#define SH_VL(n_, P_, C_, NT_, U_, RT_)                                                             \
+    m[n_].names.push_back(#P_ " columns=" #C_ " vorticity on load, nontemporal=" #NT_ ", unrolled=" #U_   \
+                          ", twiddles in registers=" #RT_ ", nontemporal loads");                        \
+    m[n_].fns.push_back(&launch_y_curl<T, P_, C_, NT_, U_, RT_, true>);
         if constexpr (sizeof(T) == 8) {
             SH_YCURL_F64_VARIANTS(SH_V)
+            SH_YCURL_F64_LNT_VARIANTS(SH_VL)
         } else {
             SH_YCURL_F32_VARIANTS(SH_V)
         }
+#undef SH_VL
 #undef SH_V
         return m;
     }();

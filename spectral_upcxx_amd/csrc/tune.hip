@@ -43,26 +43,24 @@ static const char* role_name(int role) {
 
 int ns_variant_count(int precision, int n, int role);
 const char* ns_variant_name(int precision, int n, int role, int v);
-
-// time one forward+inverse pair of a single pass role on buf (reps pairs)
 template <typename T>
-static int time_pass(C2<T>* buf, int nx, int ny, int nz, int role, int reps, hipStream_t st,
-                     hipEvent_t e0, hipEvent_t e1, double* ms) {
-    Epi<T> plain{T(1), 0, 1, -1, -1};
-    const long sxy = (long)nx * ny;
-    auto run = [&](int dir) -> int {
-        if (role == ROLE_X) return pass_contig<T>(buf, nx, (long)ny * nz, nx, 1, 0, dir, plain, st, ROLE_X);
-        if (role == ROLE_Y) return pass_strided<T>(buf, ny, nx, 1, nz, sxy, nx, 1, 0, dir, plain, st, ROLE_Y);
-        return pass_strided<T>(buf, nz, nx, 1, ny, nx, sxy, 1, 0, dir, plain, st, ROLE_Z);
+int fft3d(void* data, int count, long fstride, int nx, int ny, int nz, int dir, int flags, void* stream);
+
+// time `reps` forward + inverse 3D transforms of buf (after one warm pair):
+// the passes run in sequence, as in use, so a schedule's effect on the next
+// pass (what it leaves in the caches) counts
+template <typename T>
+static int time_fft3d(C2<T>* buf, int nx, int ny, int nz, int reps, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                      double* ms) {
+    auto pair = [&]() -> int {
+        int rc = fft3d<T>(buf, 1, 0, nx, ny, nz, -1, 0, st);
+        if (!rc) rc = fft3d<T>(buf, 1, 0, nx, ny, nz, 1, 0, st);
+        return rc;
     };
-    int rc = run(-1);
-    if (!rc) rc = run(1);
+    int rc = pair();
     if (rc) return rc;
     SH_HIP(hipEventRecord(e0, st));
-    for (int r = 0; r < reps && !rc; r++) {
-        rc = run(-1);
-        if (!rc) rc = run(1);
-    }
+    for (int r = 0; r < reps && !rc; r++) rc = pair();
     if (rc) return rc;
     SH_HIP(hipEventRecord(e1, st));
     SH_HIP(hipEventSynchronize(e1));
@@ -84,7 +82,10 @@ static int autotune_fft3d(int nx, int ny, int nz, int reps, hipStream_t st) {
     SH_HIP(hipEventCreate(&e0));
     SH_HIP(hipEventCreate(&e1));
     const int lens[3] = {nx, ny, nz};
-    for (int role = ROLE_X; role <= ROLE_Z && !rc; role++) {
+    // two rounds over the pass roles, each schedule timed in whole transforms
+    // with the other roles at their current picks
+    for (int k = 0; k < 6 && !rc; k++) {
+        const int role = ROLE_X + k % 3;
         const int len = lens[role];
         const int nv = pass_variant_count(prec, len, role);
         if (nv <= 1) continue;
@@ -93,7 +94,7 @@ static int autotune_fft3d(int nx, int ny, int nz, int reps, hipStream_t st) {
         for (int v = 0; v < nv && !rc; v++) {
             tune_set(prec, len, role, v, 0);
             double ms = 0;
-            rc = time_pass<T>(buf, nx, ny, nz, role, reps, st, e0, e1, &ms);
+            rc = time_fft3d<T>(buf, nx, ny, nz, reps, st, e0, e1, &ms);
             if (!rc && ms < best_ms) {
                 best_ms = ms;
                 best = v;
