@@ -668,9 +668,10 @@ __global__ __launch_bounds__(256) void k_herm_dev(const double2* __restrict__ f,
 // YS: the merged AB2 history (see k_ns_fourier_xd): N0 holds Y = (V1 + lap1)
 // - K/2 N0, V is neither read nor written, and the new V survives only as the
 // emitted x-inverse (so YS needs EMIT).  E5 (with YS): five emitted fields,
-// the x-inverses of V and of i kx Vy, i kx Vz (see k_ns_y_curl)
+// the x-inverses of V and of i kx Vy, i kx Vz (see k_ns_y_curl).  LNT:
+// nontemporal loads (every input is read once)
 template <typename T, class P, int LPB, bool EMIT, bool BL = false, bool NT = false, bool YS = false,
-          bool E5 = false>
+          bool E5 = false, bool LNT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict__ W,
                                                               C2<T>* __restrict__ V,
                                                               C2<T>* __restrict__ N0,
@@ -711,7 +712,16 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
     const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
     const long off = lc * N;
     auto idx = [&](int pos) { return l * LS + pos + (pos >> 4); };
-    auto load = [&](int f, C2<T>* x) { for_in<P>(tid, [&](int s, int pos) { x[s] = W[f * F + off + pos]; }); };
+    auto ld = [](const C2<T>* a) {
+        if constexpr (LNT) {
+            typedef T t2 __attribute__((ext_vector_type(2)));
+            const t2 w = __builtin_nontemporal_load(reinterpret_cast<const t2*>(a));
+            return mk<T>(w.x, w.y);
+        } else {
+            return *a;
+        }
+    };
+    auto load = [&](int f, C2<T>* x) { for_in<P>(tid, [&](int s, int pos) { x[s] = ld(W + f * F + off + pos); }); };
     // forward x transforms of the three products; the next field's loads are
     // issued before the current transform so they overlap its LDS phases
     C2<T> n1[3][E], x[E], pf[E];
@@ -751,10 +761,10 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_fourier_x(C2<T>* __restrict_
                 if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
                 if constexpr (YS) {
                     // S = Y + K/2*(3.*cross1)  (= (V1 + lap1) + F)
-                    S[d] = cadd<T>(N0[d * F + off + pos], rmul<T>(cf.Kh, rmul<T>(T(3), nn[d])));
+                    S[d] = cadd<T>(ld(N0 + d * F + off + pos), rmul<T>(cf.Kh, rmul<T>(T(3), nn[d])));
                 } else {
-                    v1[d] = V[d * F + off + pos];
-                    const C2<T> n0 = N0[d * F + off + pos];
+                    v1[d] = ld(V + d * F + off + pos);
+                    const C2<T> n0 = ld(N0 + d * F + off + pos);
                     // F = K/2*(3.*cross1 - cross0)
                     f[d] = rmul<T>(cf.Kh, csub<T>(rmul<T>(T(3), nn[d]), n0));
                 }
@@ -2007,7 +2017,7 @@ int launch_cross_zd(sh_ns_s* s) {
 // EMIT = false leaves kernel A to the next step (w_primed false), which moves
 // that work off this kernel into the next step's first phase
 template <typename T, class P, int LPB, bool EMIT = true, bool BL = false, bool NT = false, bool YS = false,
-          bool E5 = false>
+          bool E5 = false, bool LNT = false>
 int launch_fourier_x(sh_ns_s* s) {
     if (!YS && s->yform) {
         set_error("ns: kernel E of the (V1, N0) form on a merged-history state");
@@ -2028,7 +2038,7 @@ int launch_fourier_x(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT, YS, E5>), dim3(nb, (unsigned)s->nboxes),
+    hipLaunchKernelGGL((k_ns_fourier_x<T, P, LPB, EMIT, BL, NT, YS, E5, LNT>), dim3(nb, (unsigned)s->nboxes),
                        dim3(P::TPL * LPB), 0, s->st, (C2<T>*)s->Wf, (C2<T>*)s->V1f,
                        (C2<T>*)s->N0f, (const C2<T>*)s->forcing, tw, s->ny, s->nz, nzp, s->N * s->nboxes,
                        (const StepCoef<T>*)s->coefs, s->divpart, s->N, s->divblocks + 1);
@@ -2047,14 +2057,15 @@ int launch_fourier_x(sh_ns_s* s) {
 // kernel E needs one for the next step)
 bool y_curl_ok(sh_ns_s* s);
 
-template <typename T, class P, int LPB, bool BL = false, bool NT = false, bool E5 = false>
+template <typename T, class P, int LPB, bool BL = false, bool NT = false, bool E5 = false, bool LNT = false>
 int launch_fourier_xm(sh_ns_s* s) {
     const bool cap = capturing(s->st);
-    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_x<T, P, LPB, true, BL, NT>(s);
+    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_x<T, P, LPB, true, BL, NT, false, false, LNT>(s);
     if (!s->yform) RC((launch_yform<T, true>(s)));
     // inside a capture the emitted form stays what the captured y pass expects
-    if (cap ? s->w5 : (E5 && y_curl_ok(s))) return launch_fourier_x<T, P, LPB, true, BL, NT, true, true>(s);
-    return launch_fourier_x<T, P, LPB, true, BL, NT, true>(s);
+    if (cap ? s->w5 : (E5 && y_curl_ok(s)))
+        return launch_fourier_x<T, P, LPB, true, BL, NT, true, true, LNT>(s);
+    return launch_fourier_x<T, P, LPB, true, BL, NT, true, false, LNT>(s);
 }
 
 // kernel E with LDS-streamed inputs: a persistent grid of one-wave blocks, as
@@ -2199,11 +2210,11 @@ int launch_fourier_xy(sh_ns_s* s) {
     X(64, Q32, 16) X(64, Q32, 8) X(128, W64, 16) X(128, W64, 8) X(256, W128, 16) X(256, W128, 8)
 // the same (interleaved exchange) with the packing factors (RT = 1) or also
 // the stage twiddles (RT = 2) held in registers: X(nz, plan, columns per block)
-#define SH_CROSSH_F64_RT_VARIANTS(X) X(128, W64, 8) X(256, W128, 8) X(256, W128, 16)
+#define SH_CROSSH_F64_RT_VARIANTS(X) X(128, W64, 8) X(256, W128, 8) X(256, W128, 16) X(512, W256, 8)
 #define SH_CROSSH_F32_NT_VARIANTS(X) X(256, Q128, 16)
 #define SH_CROSSH_F32_LDS_VARIANTS(X) X(256, W128, 16)
 // FP32 kernel C streamed into LDS with nontemporal loads: X(nz, plan, columns per block)
-#define SH_CROSSH_F32_LNT_VARIANTS(X) X(256, W128, 16)
+#define SH_CROSSH_F32_LNT_VARIANTS(X) X(256, W128, 16) X(512, W256, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -2247,6 +2258,9 @@ int launch_fourier_xy(sh_ns_s* s) {
 // the register-loaded merged kernel E emitting five fields: X(n, plan, lines per block)
 #define SH_FOURIER_F64_YR5_VARIANTS(X) X(512, W512, 2) X(512, P512, 4)
 #define SH_FOURIER_F32_YR5_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4) X(512, P512, 4) X(512, W512, 2)
+// the register-loaded merged kernel E with nontemporal loads: X(n, plan, lines per block)
+#define SH_FOURIER_F64_YRL_VARIANTS(X) X(512, W512, 2) X(512, P512, 4)
+#define SH_FOURIER_F32_YRL_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4) X(512, P512, 4) X(512, W512, 2)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
@@ -2406,6 +2420,13 @@ const NsVariants& fourier_variants(int n) {
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, five emitted fields, nontemporal stores"); \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);                                \
     m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);
+#define SH_VML(n_, P_, L_)                                                                                  \
+    m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, nontemporal loads");                        \
+    m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, false, false, true>);                           \
+    m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_, false, false, false, true>);                        \
+    m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, nontemporal loads and stores");             \
+    m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, false, true>);                            \
+    m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_, false, true, false, true>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -2425,6 +2446,7 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_Y5RL_VARIANTS(SH_VY5RL)
             SH_FOURIER_F64_YR_VARIANTS(SH_VM)
             SH_FOURIER_F64_YR5_VARIANTS(SH_VM5)
+            SH_FOURIER_F64_YRL_VARIANTS(SH_VML)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
@@ -2432,7 +2454,9 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F32_NT_VARIANTS(SH_VT)
             SH_FOURIER_F32_YR_VARIANTS(SH_VM)
             SH_FOURIER_F32_YR5_VARIANTS(SH_VM5)
+            SH_FOURIER_F32_YRL_VARIANTS(SH_VML)
         }
+#undef SH_VML
 #undef SH_VM5
 #undef SH_VM
 #undef SH_VY5

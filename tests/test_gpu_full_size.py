@@ -187,3 +187,39 @@ def test_ensemble_64_96_graph_vs_single_and_oracle(hip):
     finally:
         ens.close()
         L.sh_tune_reset()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ns_512_cross_schedules_agree(hip, prec):
+    """Every half-spectrum kernel C schedule at 512^3 (the LDS-streamed ones
+    with packed lines of 256 included) gives the default schedule's velocity
+    after the start-up and 2 steps: 1e-12 in FP64, 1e-6 in FP32."""
+    import torch
+    L = hip.lib()
+    n = 512
+    V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    L.sh_solenoidal_ic_z(hip.ptr(V0), n, n, n, 5, 4.0, 1.0, hip.stream())
+    tol = 1e-12 if prec == 64 else 1e-6
+    ref = None
+    try:
+        for v in range(_count(L, prec, n, ROLE_CROSS_H)):
+            L.sh_tune_reset()
+            L.sh_tune_set_variant(prec, n, ROLE_CROSS_H, v)
+            ns = hip.NS((n,) * 3, NU, 2.5e-4, prec, fused=True)
+            ns.set_initial(V0)
+            ns.startup()
+            ns.steps(2)
+            V = ns.velocity()
+            ns.close()
+            if ref is None:
+                ref = V
+                continue
+            err = _rel(V, ref)
+            print("512^3 fp%d cross_zh variant=%d vs default: %.2e" % (prec, v, err), flush=True)
+            assert err < tol, (v, err)
+            del V
+    finally:
+        L.sh_tune_reset()
+        del V0, ref
+        torch.cuda.empty_cache()
+        L.sh_pool_trim()
