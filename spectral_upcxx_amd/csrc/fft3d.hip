@@ -33,7 +33,8 @@ struct ContigLds {
     static constexpr int LS = N + N / 16 + 1;
 };
 
-template <typename T, class P, int LPB, int SIGN>
+// NT: nontemporal stores; LNT: nontemporal loads
+template <typename T, class P, int LPB, int SIGN, bool NT = false, bool LNT = false>
 __global__ __launch_bounds__(P::TPL* LPB) void k_fft_contig(C2<T>* __restrict__ data,
                                                             const C2<T>* __restrict__ tw,
                                                             long nlines, long ldist, long fstride,
@@ -47,7 +48,15 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_contig(C2<T>* __restrict__ 
     const bool valid = line < nlines;
     C2<T>* base = data + (long)blockIdx.y * fstride + (valid ? line : 0) * ldist;
     C2<T> x[P::E];
-    for_in<P>(tid, [&](int slot, int pos) { x[slot] = base[pos]; });
+    for_in<P>(tid, [&](int slot, int pos) {
+        if constexpr (LNT) {
+            typedef T t2 __attribute__((ext_vector_type(2)));
+            const t2 w = __builtin_nontemporal_load(reinterpret_cast<const t2*>(base + pos));
+            x[slot] = mk<T>(w.x, w.y);
+        } else {
+            x[slot] = base[pos];
+        }
+    });
     auto idx = [&](int pos) { return l * LS + pos + (pos >> 4); };
     line_fft<P, SIGN, T>(x, tid, sre, sim, idx, tw);
     if (!valid) return;
@@ -56,7 +65,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_contig(C2<T>* __restrict__ 
         const long a = line % ep.a_n, b = line / ep.a_n;
         zline = (a == ep.za) || (b == ep.zb);
     }
-    for_out<P>(tid, [&](int slot, int pos) { base[pos] = epilogue<T, P>(x[slot], pos, zline, ep); });
+    for_out<P>(tid, [&](int slot, int pos) { st_line<NT, T>(base + pos, epilogue<T, P>(x[slot], pos, zline, ep)); });
 }
 
 // NT: nontemporal stores; LNT: nontemporal loads (each value is read once, so
@@ -95,7 +104,7 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_fft_strided(C2<T>* __restrict__
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename T, class P, int LPB>
+template <typename T, class P, int LPB, bool NT = false, bool LNT = false>
 static int launch_contig(C2<T>* data, long nlines, long ldist, int count, long fstride, int dir,
                          const Epi<T>& ep, hipStream_t st) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
@@ -106,10 +115,10 @@ static int launch_contig(C2<T>* data, long nlines, long ldist, int count, long f
     dim3 grid((unsigned)((nlines + LPB - 1) / LPB), (unsigned)count);
     dim3 block(P::TPL * LPB);
     if (dir < 0)
-        hipLaunchKernelGGL((k_fft_contig<T, P, LPB, -1>), grid, block, 0, st, data, tw, nlines, ldist,
+        hipLaunchKernelGGL((k_fft_contig<T, P, LPB, -1, NT, LNT>), grid, block, 0, st, data, tw, nlines, ldist,
                            fstride, ep);
     else
-        hipLaunchKernelGGL((k_fft_contig<T, P, LPB, 1>), grid, block, 0, st, data, tw, nlines, ldist,
+        hipLaunchKernelGGL((k_fft_contig<T, P, LPB, 1, NT, LNT>), grid, block, 0, st, data, tw, nlines, ldist,
                            fstride, ep);
     SH_LAUNCH_CHECK("k_fft_contig");
     return SH_OK;
@@ -158,6 +167,11 @@ const VariantList<ContigLaunch<T>>& contig_variants(int n) {
     m[n_].fns.push_back(&launch_contig<T, P_, L_>);
         SH_CONTIG_VARIANTS(SH_V)
 #undef SH_V
+#define SH_VH(n_, P_, L_, NT_, LNT_)                                                                 \
+    m[n_].names.push_back(#P_ " lines=" #L_ " nontemporal stores=" #NT_ ", nontemporal loads=" #LNT_); \
+    m[n_].fns.push_back(&launch_contig<T, P_, L_, NT_, LNT_>);
+        SH_CONTIG_HINT_VARIANTS(SH_VH)
+#undef SH_VH
         return m;
     }();
     static const VariantList<ContigLaunch<T>> none;

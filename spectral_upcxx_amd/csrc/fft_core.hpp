@@ -736,6 +736,12 @@ using X512 = FftPlan<512, 32, 16, 2, 16>;
     X(512, X512, 4)           \
     X(512, W512, 2)
 
+// contiguous pass with cache hints: X(n, plan, lines per block, nontemporal
+// stores, nontemporal loads)
+#define SH_CONTIG_HINT_VARIANTS(X)                                                                   \
+    X(256, Q256, 8, true, false) X(256, Q256, 8, false, true) X(256, Q256, 8, true, true)             \
+    X(512, P512, 2, true, false) X(512, P512, 2, false, true) X(512, P512, 2, true, true)
+
 // pass roles, so y and z lines (different strides) are tuned separately;
 // ROLE_CROSS_H is the fused z kernel of the real-field (half-spectrum) layout;
 // ROLE_YCURL the solver's y inverse that forms the vorticity on load
