@@ -23,6 +23,7 @@
 //    rounding of a different (equally exact) operation order.
 //
 // State (Fourier space): V1, N0 (= cross0), and six work fields W.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -1412,10 +1413,23 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
 // the previous output's stores before every transform.  UNR: the six outputs
 // unrolled (more registers, the compiler interleaves them).  RT: the stage
 // twiddles held in registers (loaded once) and interleaved-complex exchanges.
-template <typename T, class P, int LPB, bool NT, bool UNR, bool RT = false, bool LNT = false>
-__global__ __launch_bounds__(P::TPL* LPB) void k_ns_y_curl(C2<T>* __restrict__ W, const C2<T>* __restrict__ tw,
-                                                           int ncols, int nouter, long odist, long es, long F,
-                                                           int ny, int nz) {
+// ORD: the inputs loaded in stages, in the order the outputs consume them, so
+// that the loads of later fields are in flight while the first outputs are
+// transformed; 1: at most three input fields live,
+//     load Vx, dVz, dVy -> Vx, omega_y (dVz dead) -> load Vy -> omega_z (Vx,
+//     dVy dead) -> load Vz -> Vy, omega_x, Vz;
+// 2: at most two,
+//     load Vx, dVz -> Vx, omega_y -> load dVy -> omega_z -> load Vy, Vz ->
+//     Vy, omega_x, Vz;
+// 3: as 2, each load issued once the output that frees its registers is
+// formed, before that output's transform (in flight during it).
+// OCC: the waves per SIMD the register allocation must allow (two blocks of
+// the fewer registers per compute unit, one loading while the other computes).
+template <typename T, class P, int LPB, bool NT, bool UNR, bool RT = false, bool LNT = false, int ORD = 0,
+          int OCC = 1>
+__global__ __launch_bounds__(P::TPL* LPB) __attribute__((amdgpu_waves_per_eu(OCC))) void k_ns_y_curl(
+    C2<T>* __restrict__ W, const C2<T>* __restrict__ tw, int ncols, int nouter, long odist, long es, long F, int ny,
+    int nz) {
     constexpr int CS = LPB + 1;
     __shared__ __align__(16) T sre[RT ? 2 * P::N * LPB : P::N * CS];
     __shared__ T sim[RT ? 1 : P::N * CS];
@@ -1430,22 +1444,77 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_y_curl(C2<T>* __restrict__ W
     const T kz = freq<T>(o, nz);
     // 32-bit element offsets within the five (six) fields
     const unsigned uF = (unsigned)F, ues = (unsigned)es;
-    C2<T> v[5][P::E], x[P::E];
+    C2<T> x[P::E];
     typedef T t2 __attribute__((ext_vector_type(2)));
-#pragma unroll
-    for (int f = 0; f < 5; f++)
+    // this thread's stage-0 values of input field f
+    auto ld = [&](C2<T>* v, int f) {
         for_in<P>(tid, [&](int s, int pos) {
             const C2<T>* a = base + ((unsigned)f * uF + (unsigned)pos * ues);
             if constexpr (LNT) {  // nontemporal loads (read once)
                 const t2 w = __builtin_nontemporal_load(reinterpret_cast<const t2*>(a));
-                v[f][s] = mk<T>(w.x, w.y);
+                v[s] = mk<T>(w.x, w.y);
             } else {
-                v[f][s] = *a;
+                v[s] = *a;
             }
         });
+    };
     auto idx = [&](int pos) { return RT ? pos * LPB + c : pos * CS + c; };
     C2<T> tws[RT ? rt_size<P>() : 1];
     if constexpr (RT) rt_load<P, 0, T>(tws, tid, tw);
+    if constexpr (ORD > 0) {
+        // transform x and store it as output field f (a barrier before each
+        // transform but the first: the exchange buffer is reused)
+        auto emit = [&](int f, bool first) {
+            if (!first) SyncLds::run();
+            if constexpr (RT)
+                line_fft_cr<P, 1, T, decltype(idx), SyncLds>(x, tid, sx, idx, tws);
+            else
+                line_fft<P, 1, T, decltype(idx), SyncLds>(x, tid, sre, sim, idx, tw);
+            if (valid)
+                for_out<P>(tid, [&](int s, int pos) {
+                    st_line<NT, T>(base + ((unsigned)f * uF + (unsigned)pos * ues), x[s]);
+                });
+        };
+        // ORD 1 -- a: Vx; b: d/dx Vz, then Vy; e: d/dx Vy, then Vz
+        // ORD 2 -- a: Vx, then Vy; b: d/dx Vz, then d/dx Vy, then Vz
+        C2<T> a[P::E], b[P::E], e[ORD == 1 ? P::E : 1];
+        C2<T>* const dvy = ORD == 1 ? e : b;  // d/dx Vy
+        C2<T>* const vy = ORD == 1 ? b : a;
+        C2<T>* const vz = ORD == 1 ? e : b;
+        ld(a, 0);
+        ld(b, 4);
+        if constexpr (ORD == 1) ld(e, 3);
+#pragma unroll
+        for (int s = 0; s < P::E; s++) x[s] = a[s];
+        emit(0, true);
+        constexpr bool EARLY = ORD == 3;
+        for_in<P>(tid, [&](int s, int pos) { x[s] = csub<T>(imul<T>(kz, a[s]), b[s]); });  // omega_y
+        if constexpr (EARLY) ld(dvy, 3);
+        emit(4, false);
+        if constexpr (!EARLY) ld(ORD == 1 ? b : dvy, ORD == 1 ? 1 : 3);
+        for_in<P>(tid, [&](int s, int pos) { x[s] = csub<T>(dvy[s], imul<T>(freq<T>(pos, ny), a[s])); });  // omega_z
+        if constexpr (EARLY) {
+            ld(vy, 1);
+            ld(vz, 2);
+        }
+        emit(5, false);
+        if constexpr (ORD == 2) ld(vy, 1);
+        if constexpr (!EARLY) ld(vz, 2);
+#pragma unroll
+        for (int s = 0; s < P::E; s++) x[s] = vy[s];
+        emit(1, false);
+        for_in<P>(tid, [&](int s, int pos) {  // omega_x
+            x[s] = csub<T>(imul<T>(freq<T>(pos, ny), vz[s]), imul<T>(kz, vy[s]));
+        });
+        emit(3, false);
+#pragma unroll
+        for (int s = 0; s < P::E; s++) x[s] = vz[s];
+        emit(2, false);
+        return;
+    }
+    C2<T> v[5][P::E];
+#pragma unroll
+    for (int f = 0; f < 5; f++) ld(v[f], f);
     // one output at a time; not unrolled, the inputs stay the only live state
 #pragma unroll(UNR ? 6 : 1)
     for (int g = 0; g < 6; g++) {
@@ -1899,7 +1968,8 @@ int launch_inv_x(sh_ns_s* s) {
 
 // the y inverse that forms the vorticity on load (k_ns_y_curl) over the
 // columns [cx0, cx0 + cnx) on stream cst
-template <typename T, class P, int LPB, bool NT = true, bool UNR = false, bool RT = false, bool LNT = false>
+template <typename T, class P, int LPB, bool NT = true, bool UNR = false, bool RT = false, bool LNT = false,
+          int ORD = 0, int OCC = 1>
 int launch_y_curl(sh_ns_s* s) {
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, sizeof(T) == 8 ? SH_F64 : SH_F32);
     if (!tw) return SH_ENOMEM;
@@ -1910,7 +1980,8 @@ int launch_y_curl(sh_ns_s* s) {
     const int nzp = s->half ? s->nz / 2 : s->nz;
     const long tiles = (s->cnx + LPB - 1) / LPB;
     ProfScope ps("ns_y_inv", s->cst);
-    hipLaunchKernelGGL((k_ns_y_curl<T, P, LPB, NT, UNR, RT, LNT>), dim3((unsigned)(tiles * nzp)), dim3(P::TPL * LPB), 0,
+    hipLaunchKernelGGL((k_ns_y_curl<T, P, LPB, NT, UNR, RT, LNT, ORD, OCC>), dim3((unsigned)(tiles * nzp)),
+                       dim3(P::TPL * LPB), 0,
                        s->cst,
                        (C2<T>*)s->Wf + s->cx0, tw, s->cnx, nzp, (long)s->nx * s->ny, (long)s->nx, s->N, s->ny,
                        s->nz);
@@ -2239,14 +2310,27 @@ int launch_fourier_xy(sh_ns_s* s) {
 // the same traversing the lines top plane first (and with nontemporal loads)
 #define SH_FOURIER_F64_Y5R_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
 #define SH_FOURIER_F64_Y5RL_VARIANTS(X) X(256, W256, true, 0) X(256, W256, false, 0)
-// the vorticity-forming y pass: X(ny, plan, columns per block, nontemporal
-// stores, outputs unrolled, twiddles in registers); the first is the default
-// (the not-unrolled forms ran 490-540 us at 256^3 against 314-357 us, and were removed)
+// the vorticity-forming y pass with all five inputs loaded at once: X(ny,
+// plan, columns per block, nontemporal stores, outputs unrolled, twiddles in
+// registers); the first is the default at 512 (the not-unrolled forms ran
+// 490-540 us at 256^3 against 314-357 us, and were removed)
 #define SH_YCURL_F64_VARIANTS(X)                                                                      \
-    X(256, Q256, 8, true, true, true) X(256, Q256, 8, true, true, false) X(256, W256, 8, true, true, false) \
-    X(256, W256, 8, true, true, true) X(256, Q256, 8, false, true, true)                                   \
+    X(256, Q256, 8, true, true, true) X(256, Q256, 8, true, true, false) X(256, W256, 8, true, true, true) \
+    X(256, Q256, 8, false, true, true)                                                                     \
     X(512, W512, 4, true, true, true) X(512, W512, 4, true, true, false) X(512, P512, 4, true, true, true)
 #define SH_YCURL_F64_LNT_VARIANTS(X) X(256, Q256, 8, true, true, true) X(256, Q256, 8, false, true, true)
+// the staged-load forms: X(ny, plan, columns per block, nontemporal stores,
+// twiddles in registers, input fields live at most (2e: loads issued early),
+// waves per SIMD); the first is the default at 256.  At 256^3 FP64 (Q256, 8
+// columns, two blocks per compute unit) B took 266 us with two fields live
+// against 288 us for the unrolled form; in FP32 and at 512 every form ran
+// within 1% of the unrolled one (memory-bound), so none is compiled there.
+#define SH_ORD_3 1
+#define SH_ORD_2 2
+#define SH_ORD_2e 3
+#define SH_YCURL_F64_ORD_VARIANTS(X)                                                                  \
+    X(256, Q256, 8, true, true, 2, 2) X(256, Q256, 8, false, true, 2, 2) X(256, Q256, 8, true, true, 2e, 2) \
+    X(256, Q256, 8, true, true, 3, 2)
 #define SH_YCURL_F32_VARIANTS(X)                                                                      \
     X(256, Q256, 8, true, true, true) X(256, W256, 8, true, true, true) X(256, Q256, 16, true, true, true) \
     X(512, W512, 4, true, true, true) X(512, P512, 4, true, true, true) X(512, P512, 8, true, true, true)
@@ -2520,12 +2604,18 @@ const NsVariants& ycurl_variants(int n) {
     m[n_].names.push_back(#P_ " columns=" #C_ " vorticity on load, nontemporal=" #NT_ ", unrolled=" #U_   \
                           ", twiddles in registers=" #RT_ ", nontemporal loads");                        \
     m[n_].fns.push_back(&launch_y_curl<T, P_, C_, NT_, U_, RT_, true>);
+#define SH_VO(n_, P_, C_, NT_, RT_, LIVE_, OCC_)                                                         \
+    m[n_].names.push_back(#P_ " columns=" #C_ " vorticity on load, staged loads (" #LIVE_ " fields live), nontemporal=" #NT_ \
+                          ", twiddles in registers=" #RT_ ", waves per SIMD=" #OCC_);                    \
+    m[n_].fns.push_back(&launch_y_curl<T, P_, C_, NT_, true, RT_, false, SH_ORD_##LIVE_, OCC_>);
         if constexpr (sizeof(T) == 8) {
+            SH_YCURL_F64_ORD_VARIANTS(SH_VO)
             SH_YCURL_F64_VARIANTS(SH_V)
             SH_YCURL_F64_LNT_VARIANTS(SH_VL)
         } else {
             SH_YCURL_F32_VARIANTS(SH_V)
         }
+#undef SH_VO
 #undef SH_VL
 #undef SH_V
         return m;
@@ -2707,22 +2797,40 @@ int ns_autotune_with(sh_ns_t s, int reps, int (*step_cb)(void*), void* ctx) {
     // again once the others have their picks.
     const int roles[4] = {ROLE_Y, s->half ? ROLE_CROSS_H : ROLE_CROSS, ROLE_FOURIER, ROLE_YCURL};
     const int lens[4] = {s->ny, s->nz, s->nx, s->ny};
+    {
+        // warm steps first (their state is restored with the rest): the
+        // device's clocks settle before the schedules are timed
+        double ms = 0;
+        rc = time_steps(s, 8 * reps, e0, e1, &ms, step_cb, ctx);
+    }
     for (int k = 0; k < 8 && !rc; k++) {
         const int role = roles[k % 4], n = lens[k % 4];
         const int nv = role == ROLE_Y ? pass_variant_count(s->prec, n, role) : ns_variant_count(s->prec, n, role);
         if (nv <= 1) continue;
-        int best = 0;
-        double best_ms = INFINITY;
-        for (int v = 0; v < nv && !rc; v++) {
+        std::vector<double> t(nv, INFINITY);
+        auto time_variant = [&](int v) {
             tune_set(s->prec, n, role, v, 0);
             double ms = 0;
             rc = time_steps(s, reps, e0, e1, &ms, step_cb, ctx);
-            if (!rc && ms < best_ms) {
-                best_ms = ms;
-                best = v;
-            }
-        }
-        if (!rc) tune_set(s->prec, n, role, best, best_ms);
+            if (!rc) t[v] = std::min(t[v], ms);
+        };
+        // two sweeps, then a playoff: the three fastest timed alternately three
+        // more times, the least time of each counting (schedules within a
+        // percent of each other are otherwise told apart by the noise of one
+        // timing)
+        for (int pass = 0; pass < 2; pass++)
+            for (int v = 0; v < nv && !rc; v++) time_variant(v);
+        std::vector<int> top(nv);
+        for (int v = 0; v < nv; v++) top[v] = v;
+        std::sort(top.begin(), top.end(), [&](int a, int b) { return t[a] < t[b]; });
+        top.resize(std::min(nv, 3));
+        for (int r = 0; r < 3 && !rc; r++)
+            for (int v : top)
+                if (!rc) time_variant(v);
+        int best = top[0];
+        for (int v : top)
+            if (t[v] < t[best]) best = v;
+        if (!rc) tune_set(s->prec, n, role, best, t[best]);
     }
     // then the serial or the overlapped (two column halves on two streams) B-D
     if (!rc) {
