@@ -1400,6 +1400,284 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xd(double2* __restrict__ W, d
     if (tid == 0) divpart[blockIdx.x] = dsum;
 }
 
+// ---- FP32 kernel E with its inputs streamed into LDS
+// The exchanges of line_fft_xa3 with 8-byte complex values (ds_write_b64 /
+// ds_read_b64 at the same swizzled index), in inline asm for the same reason:
+// the compiler then inserts no wait for the loads into LDS in flight.
+typedef float xf_f2 __attribute__((ext_vector_type(2)));
+
+template <class P, int s>
+__device__ __forceinline__ void xf_put3(const C2<float> (&x)[3][4], int tid, unsigned buf) {
+    constexpr int R = P::radix(s), Ns = P::ns(s);
+    const int base = (tid / Ns) * Ns * R + (tid % Ns);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned a = buf + 8u * (unsigned)SwzIdx()(base + r * Ns);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const xf_f2 v = {x[d][r].x, x[d][r].y};
+            asm volatile("ds_write_b64 %0, %1 offset:%2" ::"v"(a), "v"(v), "n"(d * 8 * P::N) : "memory");
+        }
+    }
+}
+
+// the values tid + r N/4 of each line after stage s - 1 and this lane's three
+// twiddles of stage s, in one asm block that waits for them
+template <class P, int s>
+__device__ __forceinline__ void xf_get3(C2<float> (&x)[3][4], C2<float>* w, int tid, unsigned buf,
+                                        unsigned twb) {
+    constexpr int Q = P::N / 4;
+    constexpr int D = 8 * Q;
+    constexpr int Ns = P::ns(s);
+    static_assert(Q % 64 == 0 && 2 * 8 * P::N + 3 * D < 65536 && 16 * Ns < 65536, "constant offsets");
+    const unsigned a0 = buf + 8u * (unsigned)SwzIdx()(tid);
+    const unsigned a1 = a0 + 8u * P::N, a2 = a1 + 8u * P::N;
+    const unsigned t = twb + 8u * (unsigned)(sm_off<P, s>() + tid % Ns);
+    xf_f2 v[3][4], w1, w2, w3;
+    asm volatile(
+        "ds_read_b64 %0, %15\n\t"
+        "ds_read_b64 %1, %15 offset:%19\n\t"
+        "ds_read_b64 %2, %15 offset:%20\n\t"
+        "ds_read_b64 %3, %15 offset:%21\n\t"
+        "ds_read_b64 %4, %16\n\t"
+        "ds_read_b64 %5, %16 offset:%19\n\t"
+        "ds_read_b64 %6, %16 offset:%20\n\t"
+        "ds_read_b64 %7, %16 offset:%21\n\t"
+        "ds_read_b64 %8, %17\n\t"
+        "ds_read_b64 %9, %17 offset:%19\n\t"
+        "ds_read_b64 %10, %17 offset:%20\n\t"
+        "ds_read_b64 %11, %17 offset:%21\n\t"
+        "ds_read_b64 %12, %18\n\t"
+        "ds_read_b64 %13, %18 offset:%22\n\t"
+        "ds_read_b64 %14, %18 offset:%23\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v[0][0]), "=&v"(v[0][1]), "=&v"(v[0][2]), "=&v"(v[0][3]), "=&v"(v[1][0]), "=&v"(v[1][1]),
+          "=&v"(v[1][2]), "=&v"(v[1][3]), "=&v"(v[2][0]), "=&v"(v[2][1]), "=&v"(v[2][2]), "=&v"(v[2][3]),
+          "=&v"(w1), "=&v"(w2), "=&v"(w3)
+        : "v"(a0), "v"(a1), "v"(a2), "v"(t), "n"(D), "n"(2 * D), "n"(3 * D), "n"(8 * Ns), "n"(16 * Ns)
+        : "memory");
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) x[d][r] = mk<float>(v[d][r].x, v[d][r].y);
+    w[0] = mk<float>(w1.x, w1.y);
+    w[1] = mk<float>(w2.x, w2.y);
+    w[2] = mk<float>(w3.x, w3.y);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void xf_bfly(C2<float>* x, const C2<float>* w) {
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+        C2<float> t = w[r - 1];
+        if constexpr (SIGN > 0) t.y = -t.y;
+        x[r] = cmul<float>(x[r], t);
+    }
+    Dft<float, 4, SIGN>::run(x);
+}
+
+// three 256-point lines of one wave, stage by stage (line_fft_xa3 in FP32)
+template <class P, int SIGN>
+__device__ __forceinline__ void line_fft_xf3(C2<float> (&x)[3][4], int tid, unsigned buf,
+                                             const C2<float>* __restrict__ tws) {
+    static_assert(P::TPL == 64 && P::S == 4 && P::radix(0) == 4 && P::radix(1) == 4 && P::radix(2) == 4 &&
+                      P::radix(3) == 4 && P::N == 256 && P::ns(0) == 1,
+                  "four radix-4 stages, one butterfly per lane");
+    const unsigned twb = lds_offset(tws);
+    C2<float> w[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) stage_bfly<P, 0, SIGN, float>(x[d], tid, tws);  // no twiddles
+    xf_put3<P, 0>(x, tid, buf);
+    xf_get3<P, 1>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xf_bfly<SIGN>(x[d], w);
+    xf_put3<P, 1>(x, tid, buf);
+    xf_get3<P, 2>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xf_bfly<SIGN>(x[d], w);
+    xf_put3<P, 2>(x, tid, buf);
+    xf_get3<P, 3>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xf_bfly<SIGN>(x[d], w);
+}
+
+// Kernel E of the merged AB2 history in FP32, its inputs streamed into LDS:
+// the arithmetic of k_ns_fourier_x with YS (and E5) in the structure of
+// k_ns_fourier_xd with SHARE, G3 and YS -- one wave per x line, a persistent
+// grid of one-wave blocks; a line's products and its Y arrive in LDS by
+// direct global->LDS loads issued a line ahead (16 bytes, two values, per
+// lane), the product transforms exchange in the product slots, the emitted
+// inverse transforms in the Y slots, and every LDS access in the loop is
+// inline asm that waits for itself.  Single box, no body force.
+template <class P, bool NT, bool E5, bool REV, int LAUX>
+__global__ __launch_bounds__(64) void k_ns_fourier_xs(
+    float2* __restrict__ W, float2* __restrict__ N0,
+                                                      const float2* __restrict__ twg, int ny, int nz, int nzp,
+                                                      long F, const StepCoef<float>* __restrict__ cfs,
+                                                      double* __restrict__ divpart) {
+    using T = float;
+    constexpr int N = P::N;
+    constexpr int E = P::E;
+    constexpr int Q = N / 4;
+    constexpr int PIECES = N / 128;  // 128-value pieces of a line (one load into LDS each)
+    static_assert(P::TPL == 64 && E == 4 && N % 128 == 0 && sm_size<P>() <= N,
+                  "one wave per line, one radix-4 butterfly per lane; stage-major twiddles fit the table");
+    // vector-memory operations a line issues after its products' loads: the
+    // 3 + 6 (E5: 3 + 5) field stores and the next line's Y
+    constexpr int AFTER_X = 3 * PIECES + (E5 ? 8 : 9) * E;
+    constexpr int TOP_WAIT = AFTER_X < 63 ? AFTER_X : 63;
+    const StepCoef<T> cf = cfs[0];
+    __shared__ float2 xs[3][N];   // the products of the line
+    __shared__ float2 n0l[3][N];  // its Y
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x;
+    sm_fill<P, 1, T>(tw, twg, tid, 64);  // stage-major
+    __syncthreads();
+    const long nlines = (long)ny * nzp;
+    auto fetch = [&](float2* slots, const float2* src, long line) {
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int k = 0; k < PIECES; k++)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * 128 + 2 * tid),
+                    (__attribute__((address_space(3))) void*)(slots + f * N + k * 128), 16, 0, LAUX);
+    };
+    // this lane's first-stage positions tid + r Q of a landed field
+    auto read4 = [&](const float2* slot, C2<T>* x) {
+        const unsigned a = lds_offset(slot + tid);
+        xf_f2 v0, v1, v2, v3;
+        asm volatile(
+            "ds_read_b64 %0, %4\n\t"
+            "ds_read_b64 %1, %4 offset:%5\n\t"
+            "ds_read_b64 %2, %4 offset:%6\n\t"
+            "ds_read_b64 %3, %4 offset:%7\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+            : "v"(a), "n"(Q * 8), "n"(2 * Q * 8), "n"(3 * Q * 8)
+            : "memory");
+        x[0] = mk<T>(v0.x, v0.y);
+        x[1] = mk<T>(v1.x, v1.y);
+        x[2] = mk<T>(v2.x, v2.y);
+        x[3] = mk<T>(v3.x, v3.y);
+    };
+    double dsum = 0;
+    // the line a block's k-th trip handles (the trips themselves ascend)
+    auto at = [&](long t) { return REV ? nlines - 1 - t : t; };
+    long line = blockIdx.x;
+    if (line < nlines) {
+        fetch(&xs[0][0], W, at(line));
+        fetch(&n0l[0][0], N0, at(line));
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES) : "memory");  // the first line's products
+    }
+    for (; line < nlines; line += gridDim.x) {
+        const long nxt = line + gridDim.x;
+        const bool more = nxt < nlines;
+        const long pl = at(line);
+        const int jj = (int)(pl % ny), kk = (int)(pl / ny);
+        const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
+        const bool zline = (jj == ny / 2) || (kk == nz / 2);
+        const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
+        const long off = pl * N;
+        // products of this line landed: issued after them are this line's Y
+        // and, from the second line on, the previous line's stores
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(TOP_WAIT) : "memory");
+        C2<T> n1[3][E];
+#pragma unroll
+        for (int d = 0; d < 3; d++) read4(&xs[d][0], n1[d]);
+        // the exchanges reuse the product slots; their last reads have
+        // completed before the next products are loaded there
+        line_fft_xf3<P, -1>(n1, tid, lds_offset(&xs[0][0]), tw);
+        if (more) fetch(&xs[0][0], W, at(nxt));
+        // Y of this line: issued after it, the next line's products (none on a
+        // block's last line)
+        if (more)
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        C2<T> n0s[3][E], vn[3][E];
+#pragma unroll
+        for (int d = 0; d < 3; d++) read4(&n0l[d][0], n0s[d]);
+        for_out<P>(tid, [&](int s, int pos) {
+            const T k0 = freq<T>(pos, N);
+            const bool z = zline || pos == N / 2 ||
+                           (cf.dealias == 1 && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
+            const T sig = cf.dealias == 2 ? expfilter<T>(k0, N) * expfilter<T>(k1, ny) * expfilter<T>(k2, nz) : T(1);
+            C2<T> nn[3], S[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                nn[d] = z ? mk<T>(T(0), T(0)) : n1[d][s];
+                if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
+                // S = Y + K/2*(3.*cross1)  (= (V1 + lap1) + F)
+                S[d] = cadd<T>(n0s[d][s], rmul<T>(cf.Kh, rmul<T>(T(3), nn[d])));
+            }
+            const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
+            // G = div(S); G.laplacian_inverse() (a=0, b=1)
+            C2<T> G = mk<T>(T(0), T(0));
+            G = cadd<T>(G, imul<T>(k0, S[0]));
+            G = cadd<T>(G, imul<T>(k1, S[1]));
+            G = cadd<T>(G, imul<T>(k2, S[2]));
+            T sum = 0;
+            sum += k0 * k0;
+            sum += k1 * k1;
+            sum += k2 * k2;
+            const T rg = sum != 0 ? T(1) / (T(0) - T(1) * sum) : T(1);
+            G.x *= rg;
+            G.y *= rg;
+            const double rv = 1.0 / (1.0 - (-cf.c1d) * (double)sum);  // 1 where sum == 0
+            const T kk3[3] = {k0, k1, k2};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // V = Vhalf + lap1 = S - G.gradient(), then the CN divide (FP64 factor)
+                const C2<T> vv = csub<T>(S[d], imul<T>(kk3[d], G));
+                vn[d][s] = mk<T>(T((double)vv.x * rv), T((double)vv.y * rv));
+            }
+            C2<T> dv = mk<T>(T(0), T(0));
+            dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
+            dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
+            dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
+            dsum += dw * sqrt((double)dv.x * (double)dv.x + (double)dv.y * (double)dv.y);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // the next step's Y = (V + lap V) - K/2 cross1
+                const C2<T> yn = csub<T>(cadd<T>(vn[d][s], rmul<T>(cf.c1, rmul<T>(q, vn[d][s]))),
+                                         rmul<T>(cf.Kh, nn[d]));
+                st_line<NT, T>(N0 + d * F + off + pos, yn);
+            }
+        });
+        // the next step's kernel A: inverse x of V (group 0) and of i k x V
+        // (group 1; E5: of i kx Vy, i kx Vz and a line of zeros, not stored)
+        C2<T> x3[3][E];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            for_out<P>(tid, [&](int s, int pos) {
+                const T k0 = freq<T>(pos, N);
+                if (h == 0) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) x3[d][s] = vn[d][s];
+                } else if constexpr (E5) {
+                    x3[0][s] = imul<T>(k0, vn[1][s]);  // d/dx Vy
+                    x3[1][s] = imul<T>(k0, vn[2][s]);  // d/dx Vz
+                    x3[2][s] = mk<T>(T(0), T(0));
+                } else {
+                    x3[0][s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
+                    x3[1][s] = csub<T>(imul<T>(k2, vn[0][s]), imul<T>(k0, vn[2][s]));
+                    x3[2][s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
+                }
+            });
+            line_fft_xf3<P, 1>(x3, tid, lds_offset(&n0l[0][0]), tw);  // in the Y slots, read above
+#pragma unroll
+            for (int d = 0; d < ((E5 && h == 1) ? 2 : 3); d++)
+                for_out<P>(tid, [&](int s, int pos) { st_line<NT, T>(W + (3 * h + d) * F + off + pos, x3[d][s]); });
+        }
+        if (more) fetch(&n0l[0][0], N0, at(nxt));
+    }
+    // this block's sum of |div V| over its lines (FP64; lanes in a fixed order)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, 64);
+    if (tid == 0) divpart[blockIdx.x] = dsum;
+}
+
 // B of a step whose kernel E emitted five fields (E5): the y inverse of V
 // (fields 0..2) and of omega = i k x V, formed on load from V and from the
 // x-inverses of i kx Vy and i kx Vz (fields 3, 4), since after the x pass
@@ -2238,6 +2516,58 @@ int launch_fourier_xy(sh_ns_s* s) {
     return SH_OK;
 }
 
+// FP32 kernel E with LDS-streamed inputs and the merged AB2 history
+// (k_ns_fourier_xs): converts a (V1, N0) state first; where the merged form
+// does not apply, the register-loaded (V1, N0) kernel of the same plan
+template <typename T, class P, bool NT = false, bool E5 = false, bool REV = false, int LAUX = 0>
+int launch_fourier_xs(sh_ns_s* s) {
+    static_assert(sizeof(T) == 4, "FP32 only");
+    // a (V1, N0) state is converted only outside a graph capture
+    const bool cap = capturing(s->st);
+    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_x<T, P, 1, true, false, NT>(s);
+    // inside a capture the emitted form stays what the captured y pass expects
+    const bool e5 = cap ? s->w5 : (E5 && y_curl_ok(s));
+    const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F32);
+    if (!tw) return SH_ENOMEM;
+    if (!s->yform) RC((launch_yform<T, true>(s)));
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long nlines = (long)s->ny * nzp;
+    static int resident[2] = {0, 0};  // the grid of each emitted form's kernel
+    int& res = resident[e5 ? 1 : 0];
+    if (res == 0) {
+        int dev = 0, v = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        const hipError_t e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                      &per, k_ns_fourier_xs<P, NT, true, REV, LAUX>, 64, 0)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                      &per, k_ns_fourier_xs<P, NT, false, REV, LAUX>, 64, 0);
+        if (e != hipSuccess || per <= 0) per = 1;
+        res = v * per;
+    }
+    const unsigned nb = (unsigned)(nlines < res ? nlines : res);
+    if ((int)nb > s->divblocks) {
+        set_error("ns: divergence partial buffer too small");
+        return SH_EINVAL;
+    }
+    ProfScope ps("ns_fourier_x", s->st);
+    if (e5)
+        hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                           (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                           (const StepCoef<float>*)s->coefs, s->divpart);
+    else
+        hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                           (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                           (const StepCoef<float>*)s->coefs, s->divpart);
+    SH_LAUNCH_CHECK("k_ns_fourier_xs");
+    s->div_nb = (int)nb;
+    s->w_primed = true;
+    s->yform = true;
+    s->w5 = e5;
+    return SH_OK;
+}
+
 // ---------------------------------------------------------------- tunable fused schedules
 // X(n, plan, columns or lines per block); the first entry of a length is its
 // default.  Kernel C holds up to seven fields per thread and kernel E three
@@ -2346,6 +2676,12 @@ int launch_fourier_xy(sh_ns_s* s) {
 #define SH_FOURIER_F64_YRL_VARIANTS(X) X(512, W512, 2) X(512, P512, 4)
 #define SH_FOURIER_F32_YRL_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4) X(512, P512, 4) X(512, W512, 2)
 #define SH_FOURIER_F32_NT_VARIANTS(X) X(256, Q256, 8) X(256, W256, 4)
+// the FP32 LDS-streamed kernel E with the merged history (k_ns_fourier_xs):
+// X(n, plan, nontemporal stores, five emitted fields, top plane first,
+// cache-policy bits of the loads into LDS)
+#define SH_FOURIER_F32_S_VARIANTS(X)                                                                    \
+    X(256, W256, true, false, true, 2) X(256, W256, false, false, true, 2) X(256, W256, true, false, false, 2) \
+    X(256, W256, true, false, true, 0) X(256, W256, false, true, true, 2) X(256, W256, true, true, true, 2)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
@@ -2511,6 +2847,13 @@ const NsVariants& fourier_variants(int n) {
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, nontemporal loads and stores");             \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, false, true>);                            \
     m[n_].merged.push_back(&launch_fourier_xm<T, P_, L_, false, true, false, true>);
+#define SH_VXS(n_, P_, NT_, E5_, REV_, LAUX_)                                                            \
+    m[n_].names.push_back(E5_ ? #P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "  \
+                                "nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_      \
+                              : #P_ " LDS-streamed persistent lines, merged AB2 history, six emitted fields, "   \
+                                "nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_);     \
+    m[n_].fns.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);                                  \
+    m[n_].merged.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -2539,7 +2882,9 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F32_YR_VARIANTS(SH_VM)
             SH_FOURIER_F32_YR5_VARIANTS(SH_VM5)
             SH_FOURIER_F32_YRL_VARIANTS(SH_VML)
+            SH_FOURIER_F32_S_VARIANTS(SH_VXS)
         }
+#undef SH_VXS
 #undef SH_VML
 #undef SH_VM5
 #undef SH_VM
