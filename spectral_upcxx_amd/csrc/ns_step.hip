@@ -1501,6 +1501,103 @@ __device__ __forceinline__ void line_fft_xf3(C2<float> (&x)[3][4], int tid, unsi
     for (int d = 0; d < 3; d++) xf_bfly<SIGN>(x[d], w);
 }
 
+// The same for the 512-point plan P512 (three radix-8 stages, one butterfly
+// per lane): each line's eight reads in an asm block of their own that waits
+// for them, the stage's seven twiddles in another.
+template <class P, int s>
+__device__ __forceinline__ void xf8_put3(const C2<float> (&x)[3][8], int tid, unsigned buf) {
+    constexpr int R = P::radix(s), Ns = P::ns(s);
+    static_assert(R == 8 && 2 * 8 * P::N < 65536, "radix-8 stages, constant offsets");
+    const int base = (tid / Ns) * Ns * R + (tid % Ns);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned a = buf + 8u * (unsigned)SwzIdx()(base + r * Ns);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const xf_f2 v = {x[d][r].x, x[d][r].y};
+            asm volatile("ds_write_b64 %0, %1 offset:%2" ::"v"(a), "v"(v), "n"(d * 8 * P::N) : "memory");
+        }
+    }
+}
+
+template <class P, int s>
+__device__ __forceinline__ void xf8_get3(C2<float> (&x)[3][8], C2<float>* w, int tid, unsigned buf,
+                                         unsigned twb) {
+    constexpr int Q = P::N / 8;
+    constexpr int D = 8 * Q;
+    constexpr int Ns = P::ns(s);
+    static_assert(Q % 64 == 0 && 2 * 8 * P::N + 7 * D < 65536 && 48 * Ns < 65536, "constant offsets");
+    const unsigned a0 = buf + 8u * (unsigned)SwzIdx()(tid);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const unsigned a = a0 + (unsigned)d * 8u * P::N;
+        xf_f2 v[8];
+        asm volatile(
+            "ds_read_b64 %0, %8\n\t"
+            "ds_read_b64 %1, %8 offset:%9\n\t"
+            "ds_read_b64 %2, %8 offset:%10\n\t"
+            "ds_read_b64 %3, %8 offset:%11\n\t"
+            "ds_read_b64 %4, %8 offset:%12\n\t"
+            "ds_read_b64 %5, %8 offset:%13\n\t"
+            "ds_read_b64 %6, %8 offset:%14\n\t"
+            "ds_read_b64 %7, %8 offset:%15\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
+              "=&v"(v[7])
+            : "v"(a), "n"(D), "n"(2 * D), "n"(3 * D), "n"(4 * D), "n"(5 * D), "n"(6 * D), "n"(7 * D)
+            : "memory");
+#pragma unroll
+        for (int r = 0; r < 8; r++) x[d][r] = mk<float>(v[r].x, v[r].y);
+    }
+    const unsigned t = twb + 8u * (unsigned)(sm_off<P, s>() + tid % Ns);
+    xf_f2 u[7];
+    asm volatile(
+        "ds_read_b64 %0, %7\n\t"
+        "ds_read_b64 %1, %7 offset:%8\n\t"
+        "ds_read_b64 %2, %7 offset:%9\n\t"
+        "ds_read_b64 %3, %7 offset:%10\n\t"
+        "ds_read_b64 %4, %7 offset:%11\n\t"
+        "ds_read_b64 %5, %7 offset:%12\n\t"
+        "ds_read_b64 %6, %7 offset:%13\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(u[0]), "=&v"(u[1]), "=&v"(u[2]), "=&v"(u[3]), "=&v"(u[4]), "=&v"(u[5]), "=&v"(u[6])
+        : "v"(t), "n"(8 * Ns), "n"(16 * Ns), "n"(24 * Ns), "n"(32 * Ns), "n"(40 * Ns), "n"(48 * Ns)
+        : "memory");
+#pragma unroll
+    for (int r = 0; r < 7; r++) w[r] = mk<float>(u[r].x, u[r].y);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void xf8_bfly(C2<float>* x, const C2<float>* w) {
+#pragma unroll
+    for (int r = 1; r < 8; r++) {
+        C2<float> t = w[r - 1];
+        if constexpr (SIGN > 0) t.y = -t.y;
+        x[r] = cmul<float>(x[r], t);
+    }
+    Dft<float, 8, SIGN>::run(x);
+}
+
+template <class P, int SIGN>
+__device__ __forceinline__ void line_fft_xf3(C2<float> (&x)[3][8], int tid, unsigned buf,
+                                             const C2<float>* __restrict__ tws) {
+    static_assert(P::TPL == 64 && P::S == 3 && P::radix(0) == 8 && P::radix(1) == 8 && P::radix(2) == 8 &&
+                      P::N == 512 && P::ns(0) == 1,
+                  "three radix-8 stages, one butterfly per lane");
+    const unsigned twb = lds_offset(tws);
+    C2<float> w[7];
+#pragma unroll
+    for (int d = 0; d < 3; d++) stage_bfly<P, 0, SIGN, float>(x[d], tid, tws);  // no twiddles
+    xf8_put3<P, 0>(x, tid, buf);
+    xf8_get3<P, 1>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xf8_bfly<SIGN>(x[d], w);
+    xf8_put3<P, 1>(x, tid, buf);
+    xf8_get3<P, 2>(x, w, tid, buf, twb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xf8_bfly<SIGN>(x[d], w);
+}
+
 // Kernel E of the merged AB2 history in FP32, its inputs streamed into LDS:
 // the arithmetic of k_ns_fourier_x with YS (and E5) in the structure of
 // k_ns_fourier_xd with SHARE, G3 and YS -- one wave per x line, a persistent
@@ -1518,10 +1615,10 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xs(
     using T = float;
     constexpr int N = P::N;
     constexpr int E = P::E;
-    constexpr int Q = N / 4;
+    constexpr int Q = N / E;
     constexpr int PIECES = N / 128;  // 128-value pieces of a line (one load into LDS each)
-    static_assert(P::TPL == 64 && E == 4 && N % 128 == 0 && sm_size<P>() <= N,
-                  "one wave per line, one radix-4 butterfly per lane; stage-major twiddles fit the table");
+    static_assert(P::TPL == 64 && (E == 4 || E == 8) && P::radix(0) == E && N % 128 == 0 && sm_size<P>() <= N,
+                  "one wave per line, one radix-4 or radix-8 butterfly per lane; stage-major twiddles fit the table");
     // vector-memory operations a line issues after its products' loads: the
     // 3 + 6 (E5: 3 + 5) field stores and the next line's Y
     constexpr int AFTER_X = 3 * PIECES + (E5 ? 8 : 9) * E;
@@ -1543,23 +1640,27 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xs(
                     (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * 128 + 2 * tid),
                     (__attribute__((address_space(3))) void*)(slots + f * N + k * 128), 16, 0, LAUX);
     };
-    // this lane's first-stage positions tid + r Q of a landed field
+    // this lane's first-stage positions tid + r Q of a landed field (in
+    // groups of four reads, each asm block waiting for its own)
     auto read4 = [&](const float2* slot, C2<T>* x) {
-        const unsigned a = lds_offset(slot + tid);
-        xf_f2 v0, v1, v2, v3;
-        asm volatile(
-            "ds_read_b64 %0, %4\n\t"
-            "ds_read_b64 %1, %4 offset:%5\n\t"
-            "ds_read_b64 %2, %4 offset:%6\n\t"
-            "ds_read_b64 %3, %4 offset:%7\n\t"
-            "s_waitcnt lgkmcnt(0)"
-            : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
-            : "v"(a), "n"(Q * 8), "n"(2 * Q * 8), "n"(3 * Q * 8)
-            : "memory");
-        x[0] = mk<T>(v0.x, v0.y);
-        x[1] = mk<T>(v1.x, v1.y);
-        x[2] = mk<T>(v2.x, v2.y);
-        x[3] = mk<T>(v3.x, v3.y);
+#pragma unroll
+        for (int g = 0; g < E / 4; g++) {
+            const unsigned a = lds_offset(slot + tid + 4 * g * Q);
+            xf_f2 v0, v1, v2, v3;
+            asm volatile(
+                "ds_read_b64 %0, %4\n\t"
+                "ds_read_b64 %1, %4 offset:%5\n\t"
+                "ds_read_b64 %2, %4 offset:%6\n\t"
+                "ds_read_b64 %3, %4 offset:%7\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+                : "v"(a), "n"(Q * 8), "n"(2 * Q * 8), "n"(3 * Q * 8)
+                : "memory");
+            x[4 * g + 0] = mk<T>(v0.x, v0.y);
+            x[4 * g + 1] = mk<T>(v1.x, v1.y);
+            x[4 * g + 2] = mk<T>(v2.x, v2.y);
+            x[4 * g + 3] = mk<T>(v3.x, v3.y);
+        }
     };
     double dsum = 0;
     // the line a block's k-th trip handles (the trips themselves ascend)
@@ -2681,7 +2782,9 @@ int launch_fourier_xs(sh_ns_s* s) {
 // cache-policy bits of the loads into LDS)
 #define SH_FOURIER_F32_S_VARIANTS(X)                                                                    \
     X(256, W256, true, false, true, 2) X(256, W256, false, false, true, 2) X(256, W256, true, false, false, 2) \
-    X(256, W256, true, false, true, 0) X(256, W256, false, true, true, 2) X(256, W256, true, true, true, 2)
+    X(256, W256, true, false, true, 0) X(256, W256, false, true, true, 2) X(256, W256, true, true, true, 2) \
+    X(512, P512, true, false, true, 2) X(512, P512, false, false, true, 2) X(512, P512, true, true, true, 2)   \
+    X(512, P512, false, true, true, 2)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound

@@ -223,3 +223,55 @@ def test_ns_512_cross_schedules_agree(hip, prec):
         del V0, ref
         torch.cuda.empty_cache()
         L.sh_pool_trim()
+
+
+def test_ns_512_fp32_streamed_fourier_agree(hip):
+    """The FP32 kernel E with LDS-streamed inputs at 512^3 (512-point lines,
+    three radix-8 stages per wave), six and five emitted fields, gives the
+    default schedule's velocity after the start-up and 3 steps (1e-6), and
+    its history form is the merged one."""
+    import torch
+    L = hip.lib()
+    n, prec = 512, 32
+    names = []
+    buf = ctypes.create_string_buffer(8192)
+    for v in range(_count(L, prec, n, ROLE_FOURIER)):
+        L.sh_tune_reset()
+        L.sh_tune_set_variant(prec, n, ROLE_FOURIER, v)
+        L.sh_tune_report(buf, 8192)
+        if "LDS-streamed" in buf.value.decode():
+            names.append((v, "five emitted fields" in buf.value.decode()))
+    assert len(names) >= 2
+    V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    L.sh_solenoidal_ic_z(hip.ptr(V0), n, n, n, 6, 4.0, 1.0, hip.stream())
+
+    def run(v):
+        L.sh_tune_reset()
+        if v is not None:
+            L.sh_tune_set_variant(prec, n, ROLE_FOURIER, v)
+        ns = hip.NS((n,) * 3, NU, 2.5e-4, prec, fused=True)
+        ns.set_initial(V0)
+        ns.startup()
+        ns.steps(3)
+        form = ctypes.c_int()
+        L.sh_ns_get_history_form(ns.h, ctypes.byref(form))
+        V = ns.velocity()
+        ns.close()
+        return V, form.value
+
+    try:
+        ref, f0 = run(None)
+        assert f0 == 0
+        for v, five in names:
+            V, form = run(v)
+            err = _rel(V, ref)
+            print("512^3 fp32 streamed E variant=%d five=%s form=%d vs default: %.2e" % (v, five, form, err),
+                  flush=True)
+            assert form == (2 if five else 1), (v, form)
+            assert err < 1e-6, (v, err)
+            del V
+    finally:
+        L.sh_tune_reset()
+        del V0, ref
+        torch.cuda.empty_cache()
+        L.sh_pool_trim()
