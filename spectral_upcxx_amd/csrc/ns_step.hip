@@ -1779,6 +1779,301 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xs(
     if (tid == 0) divpart[blockIdx.x] = dsum;
 }
 
+// ---- FP32 kernel E with wavefront exchanges
+// A 256-point line held by one wave as 4 values per lane, transformed with no
+// LDS exchange: radix-4 stages whose data movement is a swap of two lane-index
+// bits with the two register-index bits, done with cross-lane instructions
+// (lane bits 5, 4: v_permlane32_swap / v_permlane16_swap; bits 3, 2: DPP row
+// shifts by 8 and 4 writing only the lanes whose bit is set or clear, by bank
+// mask; bits 1, 0: DPP quad permutations and a select).
+//
+// xw_swap1<B>: a and b are one dword of the registers whose register-index bit
+// is 0 and 1; afterwards a holds, on lanes with bit B set, b of the lane 2^B
+// below, and b, on lanes with bit B clear, a of the lane 2^B above.
+template <int B>
+__device__ __forceinline__ void xw_swap1(float& a, float& b, int lane) {
+    const unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+    unsigned na, nb;
+    if constexpr (B == 5) {
+        const auto r = __builtin_amdgcn_permlane32_swap(ua, ub, false, false);
+        na = r[0];
+        nb = r[1];
+    } else if constexpr (B == 4) {
+        const auto r = __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
+        na = r[0];
+        nb = r[1];
+    } else if constexpr (B == 3) {
+        na = __builtin_amdgcn_update_dpp(ua, ub, 0x118, 0xF, 0xC, false);  // row_shr:8, lanes 8-15 of a row
+        nb = __builtin_amdgcn_update_dpp(ub, ua, 0x108, 0xF, 0x3, false);  // row_shl:8, lanes 0-7
+    } else if constexpr (B == 2) {
+        na = __builtin_amdgcn_update_dpp(ua, ub, 0x114, 0xF, 0xA, false);  // row_shr:4, lanes 4-7, 12-15
+        nb = __builtin_amdgcn_update_dpp(ub, ua, 0x104, 0xF, 0x5, false);  // row_shl:4, lanes 0-3, 8-11
+    } else {
+        constexpr int QP = B == 1 ? 0x4E : 0xB1;  // quad_perm [2,3,0,1] (lane ^ 2), [1,0,3,2] (lane ^ 1)
+        const unsigned pa = __builtin_amdgcn_mov_dpp(ua, QP, 0xF, 0xF, false);
+        const unsigned pb = __builtin_amdgcn_mov_dpp(ub, QP, 0xF, 0xF, false);
+        const bool hi = (lane >> B) & 1;
+        na = hi ? pb : ua;
+        nb = hi ? ub : pa;
+    }
+    a = __uint_as_float(na);
+    b = __uint_as_float(nb);
+}
+
+// lane bits (H, L) <-> register-index bits (1, 0) of x[0..3]: afterwards
+// lane bits (H, L) hold what the register index was, and the register index
+// what lane bits (H, L) were
+template <int H, int L>
+__device__ __forceinline__ void xw_swap2(C2<float>* x, int lane) {
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        xw_swap1<H>(x[p].x, x[p + 2].x, lane);
+        xw_swap1<H>(x[p].y, x[p + 2].y, lane);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p += 2) {
+        xw_swap1<L>(x[p].x, x[p + 1].x, lane);
+        xw_swap1<L>(x[p].y, x[p + 1].y, lane);
+    }
+}
+
+__device__ __forceinline__ void xw_twiddle(C2<float>* x, const C2<float>* w) {
+#pragma unroll
+    for (int r = 1; r < 4; r++) x[r] = cmul<float>(x[r], w[r - 1]);
+}
+
+// the base-4 digit reversal of a lane index (three digits)
+__device__ __forceinline__ int xw_rev(int l) { return (l >> 4) + 4 * ((l >> 2) & 3) + 16 * (l & 3); }
+
+// this lane's twiddles: forward stages 1-3 (tf), inverse stages 1-3 (ti);
+// twg[k] = exp(-2 pi i k / 256)
+struct XwTw {
+    C2<float> tf[3][3], ti[3][3];
+};
+__device__ __forceinline__ void xw_tw_load(XwTw& t, const float2* __restrict__ twg, int l) {
+    const int dr = xw_rev(l);
+    const int k01 = (l >> 4) + 4 * ((l >> 2) & 3);
+#pragma unroll
+    for (int j = 1; j < 4; j++) {
+        const int ef[3] = {j * l, 4 * j * (l & 15), 16 * j * (l & 3)};
+        const int ei[3] = {j * dr, 4 * j * k01, 16 * j * (l >> 4)};
+#pragma unroll
+        for (int st = 0; st < 3; st++) {
+            const float2 a = twg[ef[st] & 255], b = twg[ei[st] & 255];
+            t.tf[st][j - 1] = mk<float>(a.x, a.y);
+            t.ti[st][j - 1] = mk<float>(b.x, -b.y);
+        }
+    }
+}
+
+// forward: lane l, register r hold x[l + 64 r] on entry and X[xw_rev(l) + 64 r]
+// on exit (radix-4 decimation in frequency; the register index carries the
+// digit being transformed)
+__device__ __forceinline__ void xw_fwd(C2<float>* x, const XwTw& t, int l) {
+    Dft<float, 4, -1>::run(x);
+    xw_twiddle(x, t.tf[0]);
+    xw_swap2<5, 4>(x, l);
+    Dft<float, 4, -1>::run(x);
+    xw_twiddle(x, t.tf[1]);
+    xw_swap2<3, 2>(x, l);
+    Dft<float, 4, -1>::run(x);
+    xw_twiddle(x, t.tf[2]);
+    xw_swap2<1, 0>(x, l);
+    Dft<float, 4, -1>::run(x);
+}
+
+// inverse (unnormalised): X[xw_rev(l) + 64 r] on entry, x[l + 64 r] on exit
+__device__ __forceinline__ void xw_inv(C2<float>* x, const XwTw& t, int l) {
+    Dft<float, 4, 1>::run(x);
+    xw_twiddle(x, t.ti[0]);
+    xw_swap2<1, 0>(x, l);
+    Dft<float, 4, 1>::run(x);
+    xw_twiddle(x, t.ti[1]);
+    xw_swap2<3, 2>(x, l);
+    Dft<float, 4, 1>::run(x);
+    xw_twiddle(x, t.ti[2]);
+    xw_swap2<5, 4>(x, l);
+    Dft<float, 4, 1>::run(x);
+}
+
+// Kernel E of the merged AB2 history in FP32 for 256-point lines with the
+// wavefront-exchange transforms: k_ns_fourier_xs's loop (one wave per x line,
+// persistent one-wave blocks, products and Y loaded into LDS a line ahead)
+// with no LDS exchange, so each slot is free for the next line's loads once
+// read into registers.  The update runs in the forward transform's output
+// order: the lane's frequencies are xw_rev(l) + 64 r, where Y is read and
+// written; the emitted inverse transforms return to natural order.
+template <bool NT, bool E5, bool REV, int LAUX>
+__global__ __launch_bounds__(64) void k_ns_fourier_xw(float2* __restrict__ W, float2* __restrict__ N0,
+                                                      const float2* __restrict__ twg, int ny, int nz, int nzp,
+                                                      long F, const StepCoef<float>* __restrict__ cfs,
+                                                      double* __restrict__ divpart) {
+    using T = float;
+    constexpr int N = 256, E = 4, Q = 64;
+    constexpr int PIECES = N / 128;  // 128-value pieces of a line (one 16-byte-per-lane load into LDS each)
+    constexpr int STORES = (E5 ? 8 : 9) * E;  // a line's Y and emitted-field stores
+    static_assert(3 * PIECES + STORES < 64, "vmcnt range");
+    const StepCoef<T> cf = cfs[0];
+    __shared__ float2 xs[3][N];   // the products of the line
+    __shared__ float2 n0l[3][N];  // its Y
+    const int l = threadIdx.x;
+    XwTw tws;
+    xw_tw_load(tws, twg, l);
+    const int dr = xw_rev(l);
+    const long nlines = (long)ny * nzp;
+    auto fetch = [&](float2* slots, const float2* src, long line) {
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int k = 0; k < PIECES; k++)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * 128 + 2 * l),
+                    (__attribute__((address_space(3))) void*)(slots + f * N + k * 128), 16, 0, LAUX);
+    };
+    // positions p0 + 64 r (r = 0..3) of a landed field, one asm block that waits for them
+    auto read4 = [&](const float2* slot, int p0, C2<T>* x) {
+        const unsigned a = lds_offset(slot + p0);
+        xf_f2 v0, v1, v2, v3;
+        asm volatile(
+            "ds_read_b64 %0, %4\n\t"
+            "ds_read_b64 %1, %4 offset:%5\n\t"
+            "ds_read_b64 %2, %4 offset:%6\n\t"
+            "ds_read_b64 %3, %4 offset:%7\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+            : "v"(a), "n"(Q * 8), "n"(2 * Q * 8), "n"(3 * Q * 8)
+            : "memory");
+        x[0] = mk<T>(v0.x, v0.y);
+        x[1] = mk<T>(v1.x, v1.y);
+        x[2] = mk<T>(v2.x, v2.y);
+        x[3] = mk<T>(v3.x, v3.y);
+    };
+    double dsum = 0;
+    auto at = [&](long t) { return REV ? nlines - 1 - t : t; };
+    long line = blockIdx.x;
+    if (line < nlines) {
+        fetch(&xs[0][0], W, at(line));
+        fetch(&n0l[0][0], N0, at(line));
+    }
+    bool first = true;
+    for (; line < nlines; line += gridDim.x) {
+        const long nxt = line + gridDim.x;
+        const bool more = nxt < nlines;
+        const long pl = at(line);
+        const int jj = (int)(pl % ny), kk = (int)(pl / ny);
+        const T k1 = freq<T>(jj, ny), k2 = freq<T>(kk, nz);
+        const bool zline = (jj == ny / 2) || (kk == nz / 2);
+        const double dw = (nzp < nz && kk > 0) ? 2.0 : 1.0;
+        const long off = pl * N;
+        // this line's products landed: issued after them are its Y and, from
+        // the second line on, the previous line's stores
+        if (first)
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES + STORES) : "memory");
+        C2<T> n1[3][E];
+#pragma unroll
+        for (int d = 0; d < 3; d++) read4(&xs[d][0], l, n1[d]);
+        if (more) fetch(&xs[0][0], W, at(nxt));
+#pragma unroll
+        for (int d = 0; d < 3; d++) xw_fwd(n1[d], tws, l);
+        // this line's Y landed: issued after it are the previous line's stores
+        // (not on a block's first line) and the next line's products (not on its last)
+        if (first) {
+            if (more)
+                asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES) : "memory");
+            else
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        } else {
+            if (more)
+                asm volatile("s_waitcnt vmcnt(%0)" ::"n"(3 * PIECES + STORES) : "memory");
+            else
+                asm volatile("s_waitcnt vmcnt(%0)" ::"n"(STORES) : "memory");
+        }
+        first = false;
+        C2<T> n0s[3][E], vn[3][E];
+#pragma unroll
+        for (int d = 0; d < 3; d++) read4(&n0l[d][0], dr, n0s[d]);
+        if (more) fetch(&n0l[0][0], N0, at(nxt));
+#pragma unroll
+        for (int s = 0; s < E; s++) {
+            const int pos = dr + Q * s;
+            const T k0 = freq<T>(pos, N);
+            const bool z = zline || pos == N / 2 ||
+                           (cf.dealias == 1 && (3 * fabs(k0) > N || 3 * fabs(k1) > ny || 3 * fabs(k2) > nz));
+            const T sig = cf.dealias == 2 ? expfilter<T>(k0, N) * expfilter<T>(k1, ny) * expfilter<T>(k2, nz) : T(1);
+            C2<T> nn[3], S[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                nn[d] = z ? mk<T>(T(0), T(0)) : n1[d][s];
+                if (cf.dealias == 2) nn[d] = rmul<T>(sig, nn[d]);
+                // S = Y + K/2*(3.*cross1)  (= (V1 + lap1) + F)
+                S[d] = cadd<T>(n0s[d][s], rmul<T>(cf.Kh, rmul<T>(T(3), nn[d])));
+            }
+            const T q = -(k0 * k0 + k1 * k1 + k2 * k2);
+            // G = div(S); G.laplacian_inverse() (a=0, b=1)
+            C2<T> G = mk<T>(T(0), T(0));
+            G = cadd<T>(G, imul<T>(k0, S[0]));
+            G = cadd<T>(G, imul<T>(k1, S[1]));
+            G = cadd<T>(G, imul<T>(k2, S[2]));
+            T sum = 0;
+            sum += k0 * k0;
+            sum += k1 * k1;
+            sum += k2 * k2;
+            const T rg = sum != 0 ? T(1) / (T(0) - T(1) * sum) : T(1);
+            G.x *= rg;
+            G.y *= rg;
+            const double rv = 1.0 / (1.0 - (-cf.c1d) * (double)sum);  // 1 where sum == 0
+            const T kk3[3] = {k0, k1, k2};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // V = Vhalf + lap1 = S - G.gradient(), then the CN divide (FP64 factor)
+                const C2<T> vv = csub<T>(S[d], imul<T>(kk3[d], G));
+                vn[d][s] = mk<T>(T((double)vv.x * rv), T((double)vv.y * rv));
+            }
+            C2<T> dv = mk<T>(T(0), T(0));
+            dv = cadd<T>(dv, imul<T>(k0, vn[0][s]));
+            dv = cadd<T>(dv, imul<T>(k1, vn[1][s]));
+            dv = cadd<T>(dv, imul<T>(k2, vn[2][s]));
+            dsum += dw * sqrt((double)dv.x * (double)dv.x + (double)dv.y * (double)dv.y);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // the next step's Y = (V + lap V) - K/2 cross1
+                const C2<T> yn = csub<T>(cadd<T>(vn[d][s], rmul<T>(cf.c1, rmul<T>(q, vn[d][s]))),
+                                         rmul<T>(cf.Kh, nn[d]));
+                st_line<NT, T>(N0 + d * F + off + pos, yn);
+            }
+        }
+        // the next step's kernel A: inverse x of V and of i k x V (E5: of
+        // i kx Vy and i kx Vz), stored in natural order
+#pragma unroll
+        for (int g = 0; g < (E5 ? 5 : 6); g++) {
+            C2<T> x[E];
+#pragma unroll
+            for (int s = 0; s < E; s++) {
+                const T k0 = freq<T>(dr + Q * s, N);
+                if (g < 3)
+                    x[s] = vn[g][s];
+                else if (E5)
+                    x[s] = imul<T>(k0, vn[g - 2][s]);  // d/dx Vy, d/dx Vz
+                else if (g == 3)
+                    x[s] = csub<T>(imul<T>(k1, vn[2][s]), imul<T>(k2, vn[1][s]));
+                else if (g == 4)
+                    x[s] = csub<T>(imul<T>(k2, vn[0][s]), imul<T>(k0, vn[2][s]));
+                else
+                    x[s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
+            }
+            xw_inv(x, tws, l);
+#pragma unroll
+            for (int s = 0; s < E; s++) st_line<NT, T>(W + g * F + off + l + Q * s, x[s]);
+        }
+    }
+    // this block's sum of |div V| over its lines (FP64; lanes in a fixed order)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, 64);
+    if (l == 0) divpart[blockIdx.x] = dsum;
+}
+
 // B of a step whose kernel E emitted five fields (E5): the y inverse of V
 // (fields 0..2) and of omega = i k x V, formed on load from V and from the
 // x-inverses of i kx Vy and i kx Vz (fields 3, 4), since after the x pass
@@ -2618,10 +2913,12 @@ int launch_fourier_xy(sh_ns_s* s) {
 }
 
 // FP32 kernel E with LDS-streamed inputs and the merged AB2 history
-// (k_ns_fourier_xs): converts a (V1, N0) state first; where the merged form
+// (k_ns_fourier_xs; WX: k_ns_fourier_xw, the wavefront-exchange transforms,
+// 256-point lines): converts a (V1, N0) state first; where the merged form
 // does not apply, the register-loaded (V1, N0) kernel of the same plan
-template <typename T, class P, bool NT = false, bool E5 = false, bool REV = false, int LAUX = 0>
+template <typename T, class P, bool NT = false, bool E5 = false, bool REV = false, int LAUX = 0, bool WX = false>
 int launch_fourier_xs(sh_ns_s* s) {
+    static_assert(!WX || P::N == 256, "wavefront-exchange transforms of 256-point lines");
     static_assert(sizeof(T) == 4, "FP32 only");
     // a (V1, N0) state is converted only outside a graph capture
     const bool cap = capturing(s->st);
@@ -2640,10 +2937,14 @@ int launch_fourier_xs(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        const hipError_t e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                      &per, k_ns_fourier_xs<P, NT, true, REV, LAUX>, 64, 0)
-                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                      &per, k_ns_fourier_xs<P, NT, false, REV, LAUX>, 64, 0);
+        hipError_t e;
+        if constexpr (WX)
+            e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xw<NT, true, REV, LAUX>, 64, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xw<NT, false, REV, LAUX>, 64, 0);
+        else
+            e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xs<P, NT, true, REV, LAUX>, 64, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xs<P, NT, false, REV, LAUX>, 64,
+                                                                  0);
         if (e != hipSuccess || per <= 0) per = 1;
         res = v * per;
     }
@@ -2653,15 +2954,27 @@ int launch_fourier_xs(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    if (e5)
-        hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
-                           (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
-                           (const StepCoef<float>*)s->coefs, s->divpart);
-    else
-        hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
-                           (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
-                           (const StepCoef<float>*)s->coefs, s->divpart);
-    SH_LAUNCH_CHECK("k_ns_fourier_xs");
+    if constexpr (WX) {
+        if (e5)
+            hipLaunchKernelGGL((k_ns_fourier_xw<NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                               (const StepCoef<float>*)s->coefs, s->divpart);
+        else
+            hipLaunchKernelGGL((k_ns_fourier_xw<NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                               (const StepCoef<float>*)s->coefs, s->divpart);
+        SH_LAUNCH_CHECK("k_ns_fourier_xw");
+    } else {
+        if (e5)
+            hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                               (const StepCoef<float>*)s->coefs, s->divpart);
+        else
+            hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                               (const StepCoef<float>*)s->coefs, s->divpart);
+        SH_LAUNCH_CHECK("k_ns_fourier_xs");
+    }
     s->div_nb = (int)nb;
     s->w_primed = true;
     s->yform = true;
@@ -2785,6 +3098,11 @@ int launch_fourier_xs(sh_ns_s* s) {
     X(256, W256, true, false, true, 0) X(256, W256, false, true, true, 2) X(256, W256, true, true, true, 2) \
     X(512, P512, true, false, true, 2) X(512, P512, false, false, true, 2) X(512, P512, true, true, true, 2)   \
     X(512, P512, false, true, true, 2)
+// the FP32 kernel E with the wavefront-exchange transforms (k_ns_fourier_xw):
+// X(n, nontemporal stores, five emitted fields, top plane first, load cache bits)
+#define SH_FOURIER_F32_W_VARIANTS(X)                                                                    \
+    X(256, true, false, true, 2) X(256, false, false, true, 2) X(256, false, true, true, 2)              \
+    X(256, true, true, true, 2)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
@@ -2957,6 +3275,13 @@ const NsVariants& fourier_variants(int n) {
                                 "nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_);     \
     m[n_].fns.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);                                  \
     m[n_].merged.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);
+#define SH_VXW(n_, NT_, E5_, REV_, LAUX_)                                                                   \
+    m[n_].names.push_back(E5_ ? "W256 LDS-streamed persistent lines, wavefront exchanges, merged AB2 history, five "   \
+                                "emitted fields, nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_ \
+                              : "W256 LDS-streamed persistent lines, wavefront exchanges, merged AB2 history, six "    \
+                                "emitted fields, nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_); \
+    m[n_].fns.push_back(&launch_fourier_xs<T, W256, NT_, E5_, REV_, LAUX_, true>);                            \
+    m[n_].merged.push_back(&launch_fourier_xs<T, W256, NT_, E5_, REV_, LAUX_, true>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -2986,7 +3311,9 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F32_YR5_VARIANTS(SH_VM5)
             SH_FOURIER_F32_YRL_VARIANTS(SH_VML)
             SH_FOURIER_F32_S_VARIANTS(SH_VXS)
+            SH_FOURIER_F32_W_VARIANTS(SH_VXW)
         }
+#undef SH_VXW
 #undef SH_VXS
 #undef SH_VML
 #undef SH_VM5
