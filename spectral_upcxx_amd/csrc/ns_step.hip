@@ -1791,158 +1791,245 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xs(
 // is 0 and 1; afterwards a holds, on lanes with bit B set, b of the lane 2^B
 // below, and b, on lanes with bit B clear, a of the lane 2^B above.
 template <int B>
-__device__ __forceinline__ void xw_swap1(float& a, float& b, int lane) {
-    const unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
-    unsigned na, nb;
+__device__ __forceinline__ void xw_swap1(unsigned& a, unsigned& b, int lane) {
+    const unsigned ua = a, ub = b;
     if constexpr (B == 5) {
         const auto r = __builtin_amdgcn_permlane32_swap(ua, ub, false, false);
-        na = r[0];
-        nb = r[1];
+        a = r[0];
+        b = r[1];
     } else if constexpr (B == 4) {
         const auto r = __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
-        na = r[0];
-        nb = r[1];
+        a = r[0];
+        b = r[1];
     } else if constexpr (B == 3) {
-        na = __builtin_amdgcn_update_dpp(ua, ub, 0x118, 0xF, 0xC, false);  // row_shr:8, lanes 8-15 of a row
-        nb = __builtin_amdgcn_update_dpp(ub, ua, 0x108, 0xF, 0x3, false);  // row_shl:8, lanes 0-7
+        a = __builtin_amdgcn_update_dpp(ua, ub, 0x118, 0xF, 0xC, false);  // row_shr:8, lanes 8-15 of a row
+        b = __builtin_amdgcn_update_dpp(ub, ua, 0x108, 0xF, 0x3, false);  // row_shl:8, lanes 0-7
     } else if constexpr (B == 2) {
-        na = __builtin_amdgcn_update_dpp(ua, ub, 0x114, 0xF, 0xA, false);  // row_shr:4, lanes 4-7, 12-15
-        nb = __builtin_amdgcn_update_dpp(ub, ua, 0x104, 0xF, 0x5, false);  // row_shl:4, lanes 0-3, 8-11
+        a = __builtin_amdgcn_update_dpp(ua, ub, 0x114, 0xF, 0xA, false);  // row_shr:4, lanes 4-7, 12-15
+        b = __builtin_amdgcn_update_dpp(ub, ua, 0x104, 0xF, 0x5, false);  // row_shl:4, lanes 0-3, 8-11
     } else {
         constexpr int QP = B == 1 ? 0x4E : 0xB1;  // quad_perm [2,3,0,1] (lane ^ 2), [1,0,3,2] (lane ^ 1)
         const unsigned pa = __builtin_amdgcn_mov_dpp(ua, QP, 0xF, 0xF, false);
         const unsigned pb = __builtin_amdgcn_mov_dpp(ub, QP, 0xF, 0xF, false);
         const bool hi = (lane >> B) & 1;
-        na = hi ? pb : ua;
-        nb = hi ? ub : pa;
+        a = hi ? pb : ua;
+        b = hi ? ub : pa;
     }
-    a = __uint_as_float(na);
-    b = __uint_as_float(nb);
+}
+
+// the same for one real part or imaginary part (bit-exact moves)
+template <int B>
+__device__ __forceinline__ void xw_swapv(float& a, float& b, int lane) {
+    unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+    xw_swap1<B>(ua, ub, lane);
+    a = __uint_as_float(ua);
+    b = __uint_as_float(ub);
+}
+template <int B>
+__device__ __forceinline__ void xw_swapv(double& a, double& b, int lane) {
+    unsigned al = (unsigned)__double2loint(a), ah = (unsigned)__double2hiint(a);
+    unsigned bl = (unsigned)__double2loint(b), bh = (unsigned)__double2hiint(b);
+    xw_swap1<B>(al, bl, lane);
+    xw_swap1<B>(ah, bh, lane);
+    a = __hiloint2double((int)ah, (int)al);
+    b = __hiloint2double((int)bh, (int)bl);
 }
 
 // lane bits (H, L) <-> register-index bits (1, 0) of x[0..3]: afterwards
 // lane bits (H, L) hold what the register index was, and the register index
 // what lane bits (H, L) were
-template <int H, int L>
-__device__ __forceinline__ void xw_swap2(C2<float>* x, int lane) {
+template <int H, int L, typename T>
+__device__ __forceinline__ void xw_swap2(C2<T>* x, int lane) {
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-        xw_swap1<H>(x[p].x, x[p + 2].x, lane);
-        xw_swap1<H>(x[p].y, x[p + 2].y, lane);
+        xw_swapv<H>(x[p].x, x[p + 2].x, lane);
+        xw_swapv<H>(x[p].y, x[p + 2].y, lane);
     }
 #pragma unroll
     for (int p = 0; p < 4; p += 2) {
-        xw_swap1<L>(x[p].x, x[p + 1].x, lane);
-        xw_swap1<L>(x[p].y, x[p + 1].y, lane);
+        xw_swapv<L>(x[p].x, x[p + 1].x, lane);
+        xw_swapv<L>(x[p].y, x[p + 1].y, lane);
     }
 }
 
-__device__ __forceinline__ void xw_twiddle(C2<float>* x, const C2<float>* w) {
+template <typename T>
+__device__ __forceinline__ void xw_twiddle(C2<T>* x, const C2<T>* w) {
 #pragma unroll
-    for (int r = 1; r < 4; r++) x[r] = cmul<float>(x[r], w[r - 1]);
+    for (int r = 1; r < 4; r++) x[r] = cmul<T>(x[r], w[r - 1]);
 }
 
 // the base-4 digit reversal of a lane index (three digits)
 __device__ __forceinline__ int xw_rev(int l) { return (l >> 4) + 4 * ((l >> 2) & 3) + 16 * (l & 3); }
 
-// this lane's twiddles: forward stages 1-3 (tf), inverse stages 1-3 (ti);
-// twg[k] = exp(-2 pi i k / 256)
+// the table entries (of exp(-2 pi i k / 256)) of this lane's twiddles j = 1..3
+// of forward stage st (0..2) and (conjugated) of inverse stage st
+__device__ __forceinline__ int xw_fidx(int st, int j, int l) {
+    return (st == 0 ? j * l : st == 1 ? 4 * j * (l & 15) : 16 * j * (l & 3)) & 255;
+}
+__device__ __forceinline__ int xw_iidx(int st, int j, int l) {
+    return (st == 0 ? j * xw_rev(l) : st == 1 ? 4 * j * ((l >> 4) + 4 * ((l >> 2) & 3)) : 16 * j * (l >> 4)) & 255;
+}
+
+// FP32: this lane's twiddles held in registers (loaded once per kernel)
 struct XwTw {
     C2<float> tf[3][3], ti[3][3];
-};
-__device__ __forceinline__ void xw_tw_load(XwTw& t, const float2* __restrict__ twg, int l) {
-    const int dr = xw_rev(l);
-    const int k01 = (l >> 4) + 4 * ((l >> 2) & 3);
+    __device__ __forceinline__ void load(const float2* __restrict__ twg, int l) {
 #pragma unroll
-    for (int j = 1; j < 4; j++) {
-        const int ef[3] = {j * l, 4 * j * (l & 15), 16 * j * (l & 3)};
-        const int ei[3] = {j * dr, 4 * j * k01, 16 * j * (l >> 4)};
+        for (int st = 0; st < 3; st++)
 #pragma unroll
-        for (int st = 0; st < 3; st++) {
-            const float2 a = twg[ef[st] & 255], b = twg[ei[st] & 255];
-            t.tf[st][j - 1] = mk<float>(a.x, a.y);
-            t.ti[st][j - 1] = mk<float>(b.x, -b.y);
-        }
+            for (int j = 1; j < 4; j++) {
+                const float2 a = twg[xw_fidx(st, j, l)], b = twg[xw_iidx(st, j, l)];
+                tf[st][j - 1] = mk<float>(a.x, a.y);
+                ti[st][j - 1] = mk<float>(b.x, -b.y);
+            }
     }
-}
+    __device__ __forceinline__ void fwd(int st, C2<float>* w, int) const {
+#pragma unroll
+        for (int j = 0; j < 3; j++) w[j] = tf[st][j];
+    }
+    __device__ __forceinline__ void inv(int st, C2<float>* w, int) const {
+#pragma unroll
+        for (int j = 0; j < 3; j++) w[j] = ti[st][j];
+    }
+};
+
+// FP64: this lane's twiddles read at each stage from the natural table in
+// LDS (inline asm that waits for itself)
+struct XwTwLds {
+    unsigned base;  // LDS address of the table
+    __device__ __forceinline__ void get(int st, C2<double>* w, int l, bool inverse) const {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        d2 a, b, c;
+        const unsigned i1 = base + 16u * (unsigned)(inverse ? xw_iidx(st, 1, l) : xw_fidx(st, 1, l));
+        const unsigned i2 = base + 16u * (unsigned)(inverse ? xw_iidx(st, 2, l) : xw_fidx(st, 2, l));
+        const unsigned i3 = base + 16u * (unsigned)(inverse ? xw_iidx(st, 3, l) : xw_fidx(st, 3, l));
+        asm volatile(
+            "ds_read_b128 %0, %3\n\t"
+            "ds_read_b128 %1, %4\n\t"
+            "ds_read_b128 %2, %5\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(a), "=&v"(b), "=&v"(c)
+            : "v"(i1), "v"(i2), "v"(i3)
+            : "memory");
+        const double sg = inverse ? -1.0 : 1.0;
+        w[0] = mk<double>(a.x, sg * a.y);
+        w[1] = mk<double>(b.x, sg * b.y);
+        w[2] = mk<double>(c.x, sg * c.y);
+    }
+    __device__ __forceinline__ void fwd(int st, C2<double>* w, int l) const { get(st, w, l, false); }
+    __device__ __forceinline__ void inv(int st, C2<double>* w, int l) const { get(st, w, l, true); }
+};
 
 // forward: lane l, register r hold x[l + 64 r] on entry and X[xw_rev(l) + 64 r]
 // on exit (radix-4 decimation in frequency; the register index carries the
 // digit being transformed)
-__device__ __forceinline__ void xw_fwd(C2<float>* x, const XwTw& t, int l) {
-    Dft<float, 4, -1>::run(x);
-    xw_twiddle(x, t.tf[0]);
-    xw_swap2<5, 4>(x, l);
-    Dft<float, 4, -1>::run(x);
-    xw_twiddle(x, t.tf[1]);
-    xw_swap2<3, 2>(x, l);
-    Dft<float, 4, -1>::run(x);
-    xw_twiddle(x, t.tf[2]);
-    xw_swap2<1, 0>(x, l);
-    Dft<float, 4, -1>::run(x);
+template <typename T, class TW>
+__device__ __forceinline__ void xw_fwd(C2<T>* x, const TW& t, int l) {
+    C2<T> w[3];
+    Dft<T, 4, -1>::run(x);
+    t.fwd(0, w, l);
+    xw_twiddle<T>(x, w);
+    xw_swap2<5, 4, T>(x, l);
+    Dft<T, 4, -1>::run(x);
+    t.fwd(1, w, l);
+    xw_twiddle<T>(x, w);
+    xw_swap2<3, 2, T>(x, l);
+    Dft<T, 4, -1>::run(x);
+    t.fwd(2, w, l);
+    xw_twiddle<T>(x, w);
+    xw_swap2<1, 0, T>(x, l);
+    Dft<T, 4, -1>::run(x);
 }
 
 // inverse (unnormalised): X[xw_rev(l) + 64 r] on entry, x[l + 64 r] on exit
-__device__ __forceinline__ void xw_inv(C2<float>* x, const XwTw& t, int l) {
-    Dft<float, 4, 1>::run(x);
-    xw_twiddle(x, t.ti[0]);
-    xw_swap2<1, 0>(x, l);
-    Dft<float, 4, 1>::run(x);
-    xw_twiddle(x, t.ti[1]);
-    xw_swap2<3, 2>(x, l);
-    Dft<float, 4, 1>::run(x);
-    xw_twiddle(x, t.ti[2]);
-    xw_swap2<5, 4>(x, l);
-    Dft<float, 4, 1>::run(x);
+template <typename T, class TW>
+__device__ __forceinline__ void xw_inv(C2<T>* x, const TW& t, int l) {
+    C2<T> w[3];
+    Dft<T, 4, 1>::run(x);
+    t.inv(0, w, l);
+    xw_twiddle<T>(x, w);
+    xw_swap2<1, 0, T>(x, l);
+    Dft<T, 4, 1>::run(x);
+    t.inv(1, w, l);
+    xw_twiddle<T>(x, w);
+    xw_swap2<3, 2, T>(x, l);
+    Dft<T, 4, 1>::run(x);
+    t.inv(2, w, l);
+    xw_twiddle<T>(x, w);
+    xw_swap2<5, 4, T>(x, l);
+    Dft<T, 4, 1>::run(x);
 }
 
-// Kernel E of the merged AB2 history in FP32 for 256-point lines with the
+// Kernel E of the merged AB2 history for 256-point lines with the
 // wavefront-exchange transforms: k_ns_fourier_xs's loop (one wave per x line,
-// persistent one-wave blocks, products and Y loaded into LDS a line ahead)
-// with no LDS exchange, so each slot is free for the next line's loads once
-// read into registers.  The update runs in the forward transform's output
-// order: the lane's frequencies are xw_rev(l) + 64 r, where Y is read and
-// written; the emitted inverse transforms return to natural order.
-template <bool NT, bool E5, bool REV, int LAUX>
-__global__ __launch_bounds__(64) void k_ns_fourier_xw(float2* __restrict__ W, float2* __restrict__ N0,
-                                                      const float2* __restrict__ twg, int ny, int nz, int nzp,
-                                                      long F, const StepCoef<float>* __restrict__ cfs,
+// persistent one-wave blocks, products and Y loaded into LDS a line ahead,
+// 16 bytes per lane per load) with no LDS exchange, so each slot is free for
+// the next line's loads once read into registers.  The update runs in the
+// forward transform's output order: the lane's frequencies are xw_rev(l) +
+// 64 r, where Y is read and written; the emitted inverse transforms return
+// to natural order.  Twiddles: FP32 in registers, FP64 from a table in LDS.
+template <typename T, bool NT, bool E5, bool REV, int LAUX>
+__global__ __launch_bounds__(64) void k_ns_fourier_xw(C2<T>* __restrict__ W, C2<T>* __restrict__ N0,
+                                                      const C2<T>* __restrict__ twg, int ny, int nz, int nzp,
+                                                      long F, const StepCoef<T>* __restrict__ cfs,
                                                       double* __restrict__ divpart) {
-    using T = float;
     constexpr int N = 256, E = 4, Q = 64;
-    constexpr int PIECES = N / 128;  // 128-value pieces of a line (one 16-byte-per-lane load into LDS each)
+    constexpr unsigned VB = sizeof(C2<T>);  // bytes per value
+    constexpr int VPL = 16 / VB;            // values one lane loads into LDS at a time
+    constexpr int PIECE = 64 * VPL;         // values of one load instruction
+    constexpr int PIECES = N / PIECE;
     constexpr int STORES = (E5 ? 8 : 9) * E;  // a line's Y and emitted-field stores
     static_assert(3 * PIECES + STORES < 64, "vmcnt range");
     const StepCoef<T> cf = cfs[0];
-    __shared__ float2 xs[3][N];   // the products of the line
-    __shared__ float2 n0l[3][N];  // its Y
+    __shared__ C2<T> xs[3][N];   // the products of the line
+    __shared__ C2<T> n0l[3][N];  // its Y
+    __shared__ C2<T> twt[VB == 16 ? N : 1];
     const int l = threadIdx.x;
-    XwTw tws;
-    xw_tw_load(tws, twg, l);
+    typedef std::conditional_t<VB == 16, XwTwLds, XwTw> TW;
+    TW tws;
+    if constexpr (VB == 16) {
+        for (int t = l; t < N; t += 64) twt[t] = twg[t];
+        tws.base = lds_offset(&twt[0]);
+        __syncthreads();
+    } else {
+        tws.load(twg, l);
+    }
     const int dr = xw_rev(l);
     const long nlines = (long)ny * nzp;
-    auto fetch = [&](float2* slots, const float2* src, long line) {
+    auto fetch = [&](C2<T>* slots, const C2<T>* src, long line) {
 #pragma unroll
         for (int f = 0; f < 3; f++)
 #pragma unroll
             for (int k = 0; k < PIECES; k++)
                 __builtin_amdgcn_global_load_lds(
-                    (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * 128 + 2 * l),
-                    (__attribute__((address_space(3))) void*)(slots + f * N + k * 128), 16, 0, LAUX);
+                    (const __attribute__((address_space(1))) void*)(src + f * F + line * N + k * PIECE + VPL * l),
+                    (__attribute__((address_space(3))) void*)(slots + f * N + k * PIECE), 16, 0, LAUX);
     };
     // positions p0 + 64 r (r = 0..3) of a landed field, one asm block that waits for them
-    auto read4 = [&](const float2* slot, int p0, C2<T>* x) {
+    typedef T t2 __attribute__((ext_vector_type(2)));
+    auto read4 = [&](const C2<T>* slot, int p0, C2<T>* x) {
         const unsigned a = lds_offset(slot + p0);
-        xf_f2 v0, v1, v2, v3;
-        asm volatile(
-            "ds_read_b64 %0, %4\n\t"
-            "ds_read_b64 %1, %4 offset:%5\n\t"
-            "ds_read_b64 %2, %4 offset:%6\n\t"
-            "ds_read_b64 %3, %4 offset:%7\n\t"
-            "s_waitcnt lgkmcnt(0)"
-            : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
-            : "v"(a), "n"(Q * 8), "n"(2 * Q * 8), "n"(3 * Q * 8)
-            : "memory");
+        t2 v0, v1, v2, v3;
+        if constexpr (VB == 16)
+            asm volatile(
+                "ds_read_b128 %0, %4\n\t"
+                "ds_read_b128 %1, %4 offset:%5\n\t"
+                "ds_read_b128 %2, %4 offset:%6\n\t"
+                "ds_read_b128 %3, %4 offset:%7\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+                : "v"(a), "n"(Q * VB), "n"(2 * Q * VB), "n"(3 * Q * VB)
+                : "memory");
+        else
+            asm volatile(
+                "ds_read_b64 %0, %4\n\t"
+                "ds_read_b64 %1, %4 offset:%5\n\t"
+                "ds_read_b64 %2, %4 offset:%6\n\t"
+                "ds_read_b64 %3, %4 offset:%7\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+                : "v"(a), "n"(Q * VB), "n"(2 * Q * VB), "n"(3 * Q * VB)
+                : "memory");
         x[0] = mk<T>(v0.x, v0.y);
         x[1] = mk<T>(v1.x, v1.y);
         x[2] = mk<T>(v2.x, v2.y);
@@ -1976,7 +2063,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xw(float2* __restrict__ W, fl
         for (int d = 0; d < 3; d++) read4(&xs[d][0], l, n1[d]);
         if (more) fetch(&xs[0][0], W, at(nxt));
 #pragma unroll
-        for (int d = 0; d < 3; d++) xw_fwd(n1[d], tws, l);
+        for (int d = 0; d < 3; d++) xw_fwd<T>(n1[d], tws, l);
         // this line's Y landed: issued after it are the previous line's stores
         // (not on a block's first line) and the next line's products (not on its last)
         if (first) {
@@ -2063,7 +2150,7 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xw(float2* __restrict__ W, fl
                 else
                     x[s] = csub<T>(imul<T>(k0, vn[1][s]), imul<T>(k1, vn[0][s]));
             }
-            xw_inv(x, tws, l);
+            xw_inv<T>(x, tws, l);
 #pragma unroll
             for (int s = 0; s < E; s++) st_line<NT, T>(W + g * F + off + l + Q * s, x[s]);
         }
@@ -2913,12 +3000,10 @@ int launch_fourier_xy(sh_ns_s* s) {
 }
 
 // FP32 kernel E with LDS-streamed inputs and the merged AB2 history
-// (k_ns_fourier_xs; WX: k_ns_fourier_xw, the wavefront-exchange transforms,
-// 256-point lines): converts a (V1, N0) state first; where the merged form
+// (k_ns_fourier_xs): converts a (V1, N0) state first; where the merged form
 // does not apply, the register-loaded (V1, N0) kernel of the same plan
-template <typename T, class P, bool NT = false, bool E5 = false, bool REV = false, int LAUX = 0, bool WX = false>
+template <typename T, class P, bool NT = false, bool E5 = false, bool REV = false, int LAUX = 0>
 int launch_fourier_xs(sh_ns_s* s) {
-    static_assert(!WX || P::N == 256, "wavefront-exchange transforms of 256-point lines");
     static_assert(sizeof(T) == 4, "FP32 only");
     // a (V1, N0) state is converted only outside a graph capture
     const bool cap = capturing(s->st);
@@ -2937,14 +3022,10 @@ int launch_fourier_xs(sh_ns_s* s) {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
             v = 256;
-        hipError_t e;
-        if constexpr (WX)
-            e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xw<NT, true, REV, LAUX>, 64, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xw<NT, false, REV, LAUX>, 64, 0);
-        else
-            e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xs<P, NT, true, REV, LAUX>, 64, 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xs<P, NT, false, REV, LAUX>, 64,
-                                                                  0);
+        const hipError_t e = e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                      &per, k_ns_fourier_xs<P, NT, true, REV, LAUX>, 64, 0)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                      &per, k_ns_fourier_xs<P, NT, false, REV, LAUX>, 64, 0);
         if (e != hipSuccess || per <= 0) per = 1;
         res = v * per;
     }
@@ -2954,27 +3035,72 @@ int launch_fourier_xs(sh_ns_s* s) {
         return SH_EINVAL;
     }
     ProfScope ps("ns_fourier_x", s->st);
-    if constexpr (WX) {
-        if (e5)
-            hipLaunchKernelGGL((k_ns_fourier_xw<NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
-                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
-                               (const StepCoef<float>*)s->coefs, s->divpart);
+    if (e5)
+        hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                           (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                           (const StepCoef<float>*)s->coefs, s->divpart);
+    else
+        hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
+                           (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
+                           (const StepCoef<float>*)s->coefs, s->divpart);
+    SH_LAUNCH_CHECK("k_ns_fourier_xs");
+    s->div_nb = (int)nb;
+    s->w_primed = true;
+    s->yform = true;
+    s->w5 = e5;
+    return SH_OK;
+}
+
+// kernel E with the wavefront-exchange transforms and the merged AB2 history
+// (k_ns_fourier_xw, 256-point lines): converts a (V1, N0) state first; where
+// the merged form does not apply, the (V1, N0) kernel of the same length
+template <typename T, bool NT = false, bool E5 = false, bool REV = false, int LAUX = 0>
+int launch_fourier_xw(sh_ns_s* s) {
+    // a (V1, N0) state is converted only outside a graph capture
+    const bool cap = capturing(s->st);
+    if (!y_ok(s) || (!s->yform && cap)) {
+        if constexpr (sizeof(T) == 8)
+            return launch_fourier_xd<T, W256, NT, true, true>(s);
         else
-            hipLaunchKernelGGL((k_ns_fourier_xw<NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
-                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
-                               (const StepCoef<float>*)s->coefs, s->divpart);
-        SH_LAUNCH_CHECK("k_ns_fourier_xw");
-    } else {
-        if (e5)
-            hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
-                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
-                               (const StepCoef<float>*)s->coefs, s->divpart);
-        else
-            hipLaunchKernelGGL((k_ns_fourier_xs<P, NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st,
-                               (float2*)s->Wf, (float2*)s->N0f, (const float2*)tw, s->ny, s->nz, nzp, s->N,
-                               (const StepCoef<float>*)s->coefs, s->divpart);
-        SH_LAUNCH_CHECK("k_ns_fourier_xs");
+            return launch_fourier_x<T, W256, 1, true, false, NT>(s);
     }
+    // inside a capture the emitted form stays what the captured y pass expects
+    const bool e5 = cap ? s->w5 : (E5 && y_curl_ok(s));
+    const C2<T>* tw = (const C2<T>*)twiddles(256, sizeof(T) == 8 ? SH_F64 : SH_F32);
+    if (!tw) return SH_ENOMEM;
+    if (s->nx != 256) {
+        set_error("ns: the wavefront-exchange kernel E needs nx = 256");
+        return SH_EINVAL;
+    }
+    if (!s->yform) RC((launch_yform<T, true>(s)));
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const long nlines = (long)s->ny * nzp;
+    static int resident[2] = {0, 0};  // the grid of each emitted form's kernel
+    int& res = resident[e5 ? 1 : 0];
+    if (res == 0) {
+        int dev = 0, v = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        const hipError_t e =
+            e5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xw<T, NT, true, REV, LAUX>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ns_fourier_xw<T, NT, false, REV, LAUX>, 64, 0);
+        if (e != hipSuccess || per <= 0) per = 1;
+        res = v * per;
+    }
+    const unsigned nb = (unsigned)(nlines < res ? nlines : res);
+    if ((int)nb > s->divblocks) {
+        set_error("ns: divergence partial buffer too small");
+        return SH_EINVAL;
+    }
+    ProfScope ps("ns_fourier_x", s->st);
+    if (e5)
+        hipLaunchKernelGGL((k_ns_fourier_xw<T, NT, true, REV, LAUX>), dim3(nb), dim3(64), 0, s->st, (C2<T>*)s->Wf,
+                           (C2<T>*)s->N0f, tw, s->ny, s->nz, nzp, s->N, (const StepCoef<T>*)s->coefs, s->divpart);
+    else
+        hipLaunchKernelGGL((k_ns_fourier_xw<T, NT, false, REV, LAUX>), dim3(nb), dim3(64), 0, s->st, (C2<T>*)s->Wf,
+                           (C2<T>*)s->N0f, tw, s->ny, s->nz, nzp, s->N, (const StepCoef<T>*)s->coefs, s->divpart);
+    SH_LAUNCH_CHECK("k_ns_fourier_xw");
     s->div_nb = (int)nb;
     s->w_primed = true;
     s->yform = true;
@@ -3098,11 +3224,14 @@ int launch_fourier_xs(sh_ns_s* s) {
     X(256, W256, true, false, true, 0) X(256, W256, false, true, true, 2) X(256, W256, true, true, true, 2) \
     X(512, P512, true, false, true, 2) X(512, P512, false, false, true, 2) X(512, P512, true, true, true, 2)   \
     X(512, P512, false, true, true, 2)
-// the FP32 kernel E with the wavefront-exchange transforms (k_ns_fourier_xw):
-// X(n, nontemporal stores, five emitted fields, top plane first, load cache bits)
+// kernel E with the wavefront-exchange transforms (k_ns_fourier_xw): X(n,
+// nontemporal stores, five emitted fields, top plane first, load cache bits)
 #define SH_FOURIER_F32_W_VARIANTS(X)                                                                    \
     X(256, true, false, true, 2) X(256, false, false, true, 2) X(256, false, true, true, 2)              \
     X(256, true, true, true, 2)
+// (FP64: 331-336 us at 256^3 like the LDS-exchange kernel; the swaps of
+// four-dword values cost what the LDS round trips did)
+#define SH_FOURIER_F64_W_VARIANTS(X) X(256, false, true, true, 2)
 
 // kernel E without the next step's x-inverse (kernel A runs each step): for
 // small grids, where A can overlap other work and E alone is latency-bound
@@ -3280,8 +3409,8 @@ const NsVariants& fourier_variants(int n) {
                                 "emitted fields, nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_ \
                               : "W256 LDS-streamed persistent lines, wavefront exchanges, merged AB2 history, six "    \
                                 "emitted fields, nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_); \
-    m[n_].fns.push_back(&launch_fourier_xs<T, W256, NT_, E5_, REV_, LAUX_, true>);                            \
-    m[n_].merged.push_back(&launch_fourier_xs<T, W256, NT_, E5_, REV_, LAUX_, true>);
+    m[n_].fns.push_back(&launch_fourier_xw<T, NT_, E5_, REV_, LAUX_>);                                       \
+    m[n_].merged.push_back(&launch_fourier_xw<T, NT_, E5_, REV_, LAUX_>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -3302,6 +3431,7 @@ const NsVariants& fourier_variants(int n) {
             SH_FOURIER_F64_YR_VARIANTS(SH_VM)
             SH_FOURIER_F64_YR5_VARIANTS(SH_VM5)
             SH_FOURIER_F64_YRL_VARIANTS(SH_VML)
+            SH_FOURIER_F64_W_VARIANTS(SH_VXW)
         } else {
             SH_FOURIER_F32_VARIANTS(SH_V)
             SH_FOURIER_F32_NOEMIT_VARIANTS(SH_VN)
