@@ -317,3 +317,73 @@ def test_merged_five_fields_every_y_curl(hip, prec, n):
         del V0
         torch.cuda.empty_cache()
         L.sh_pool_trim()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_streamed_and_wavefront_kernels_graph_capture(hip, prec):
+    """At 256^3, steps captured in a graph with the LDS-streamed kernel E and
+    with the wavefront-exchange one (five emitted fields; the vorticity y
+    pass after them) replay bit for bit like the same steps run eagerly on a
+    twin solver, replay after replay."""
+    import torch
+    L = hip.lib()
+    n = N
+    picks = []
+    buf = ctypes.create_string_buffer(8192)
+    c = ctypes.c_int()
+    L.sh_tune_variant_count(prec, n, ROLE_FOURIER, ctypes.byref(c))
+    for v in range(c.value):
+        _use(L, v, prec, n)
+        L.sh_tune_report(buf, 8192)
+        name = buf.value.decode()
+        if "LDS-streamed" in name and "five emitted fields" in name:
+            picks.append((v, "wavefront" in name))
+    L.sh_tune_reset()
+    assert any(w for _, w in picks) and any(not w for _, w in picks)
+    chosen = [next(v for v, w in picks if w), next(v for v, w in picks if not w)]
+    V0 = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+    L.sh_solenoidal_ic_z(hip.ptr(V0), n, n, n, 13, 4.0, 1.0, hip.stream())
+    torch.cuda.synchronize()
+    st = ctypes.c_void_p()
+    L.sh_stream_create(ctypes.byref(st))
+    try:
+        for v in chosen:
+            _use(L, v, prec, n)
+            hs = []
+            for _ in range(2):
+                h = ctypes.c_void_p()
+                assert L.sh_ns_create(ctypes.byref(h), n, n, n, NU, DT, prec, 1, st) == 0
+                L.sh_ns_set_initial(h, hip.ptr(V0))
+                L.sh_ns_startup(h)
+                L.sh_ns_steps(h, 1)  # the merged form and its emitted fields are set up eagerly
+                hs.append(h)
+            L.sh_stream_synchronize(st)
+            assert L.sh_graph_begin(st) == 0
+            assert L.sh_ns_steps(hs[0], 2) == 0
+            g = ctypes.c_void_p()
+            assert L.sh_graph_end(st, ctypes.byref(g)) == 0
+            for rep in range(3):
+                L.sh_graph_launch(g, st)
+                L.sh_ns_steps(hs[1], 2)
+                L.sh_stream_synchronize(st)
+                a = torch.empty((3, n, n, n), dtype=torch.complex128, device="cuda")
+                b = torch.empty_like(a)
+                L.sh_ns_get_velocity(hs[0], hip.ptr(a))
+                L.sh_ns_get_velocity(hs[1], hip.ptr(b))
+                L.sh_stream_synchronize(st)
+                fa = ctypes.c_int()
+                L.sh_ns_get_history_form(hs[0], ctypes.byref(fa))
+                print("fp%d variant=%d rep=%d form=%d equal=%s" % (prec, v, rep, fa.value, bool((a == b).all())),
+                      flush=True)
+                assert fa.value == 2
+                assert bool((a == b).all()), (v, rep)
+                del a, b
+            L.sh_graph_destroy(g)
+            for h in hs:
+                L.sh_ns_destroy(h)
+    finally:
+        L.sh_tune_reset()
+        L.sh_stream_destroy(st)
+        del V0
+        torch.cuda.empty_cache()
+        L.sh_pool_trim()
