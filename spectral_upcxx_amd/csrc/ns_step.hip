@@ -3156,6 +3156,9 @@ int launch_fourier_xw(sh_ns_s* s) {
 #define SH_CROSSH_F32_LDS_VARIANTS(X) X(256, W128, 16)
 // FP32 kernel C streamed into LDS with nontemporal loads: X(nz, plan, columns per block)
 #define SH_CROSSH_F32_LNT_VARIANTS(X) X(256, W128, 16) X(512, W256, 16)
+// FP32 with the stage twiddles and packing factors in registers, nontemporal
+// loads (at 256^3: 127-129 us against 139-141; 8 columns per block ran 211-256)
+#define SH_CROSSH_F32_RTL_VARIANTS(X) X(256, W128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -3309,6 +3312,11 @@ const NsVariants& crossh_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 0, 2>);                                  \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange nontemporal loads and stores"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 0, 2>);
+#define SH_VRL(n_, P_, C_)                                                                                  \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 2, 2>);                                 \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2, 2>);
 #define SH_VR(n_, P_, C_)                                                                                   \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange packing factors in registers"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 1>);                                    \
@@ -3330,7 +3338,9 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F32_NT_VARIANTS(SH_VT)
             SH_CROSSH_F32_LDS_VARIANTS(SH_VD)
             SH_CROSSH_F32_LNT_VARIANTS(SH_VDL)
+            SH_CROSSH_F32_RTL_VARIANTS(SH_VRL)
         }
+#undef SH_VRL
 #undef SH_VR
 #undef SH_VDL
 #undef SH_VD
