@@ -3196,11 +3196,14 @@ int launch_fourier_xw(sh_ns_s* s) {
 // twiddles in registers, input fields live at most (2e: loads issued early),
 // waves per SIMD); the first is the default at 256.  At 256^3 FP64 (Q256, 8
 // columns, two blocks per compute unit) B took 266 us with two fields live
-// against 288 us for the unrolled form; in FP32 and at 512 every form ran
-// within 1% of the unrolled one (memory-bound), so none is compiled there.
+// against 288 us for the unrolled form; at 512 every form ran within 1% of
+// the unrolled one (memory-bound), so none is compiled there.
 #define SH_ORD_3 1
 #define SH_ORD_2 2
 #define SH_ORD_2e 3
+// (FP32 at 256^3: B 157 -> 145 us with loads issued early, the next kernel C
+// 130 -> 137 us, the step within noise; 8-column forms ran 192-206 us)
+#define SH_YCURL_F32_ORD_VARIANTS(X) X(256, Q256, 16, true, true, 2e, 2) X(256, Q256, 16, true, true, 2, 2)
 #define SH_YCURL_F64_ORD_VARIANTS(X)                                                                  \
     X(256, Q256, 8, true, true, 2, 2) X(256, Q256, 8, false, true, 2, 2) X(256, Q256, 8, true, true, 2e, 2) \
     X(256, Q256, 8, true, true, 3, 2)
@@ -3532,6 +3535,7 @@ const NsVariants& ycurl_variants(int n) {
             SH_YCURL_F64_LNT_VARIANTS(SH_VL)
         } else {
             SH_YCURL_F32_VARIANTS(SH_V)
+            SH_YCURL_F32_ORD_VARIANTS(SH_VO)
         }
 #undef SH_VO
 #undef SH_VL
