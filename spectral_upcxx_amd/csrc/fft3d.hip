@@ -198,6 +198,13 @@ const VariantList<StridedLaunch<T>>& strided_variants(int n) {
     m[n_].fns.push_back(&launch_strided<T, P_, C_, NT_, true>);
         SH_STRIDED_LNT_VARIANTS(SH_VL)
 #undef SH_VL
+#define SH_VF(n_, P_, C_, NT_, LNT_)                                                                  \
+    m[n_].names.push_back(#P_ " columns=" #C_ " nontemporal stores=" #NT_ ", nontemporal loads=" #LNT_); \
+    m[n_].fns.push_back(&launch_strided<T, P_, C_, NT_, LNT_>);
+        if constexpr (sizeof(T) == 4) {
+            SH_STRIDED_F32_VARIANTS(SH_VF)
+        }
+#undef SH_VF
         return m;
     }();
     static const VariantList<StridedLaunch<T>> none;
@@ -271,12 +278,11 @@ int pass_variant_count(int precision, int n, int role) {
 
 const char* pass_variant_name(int precision, int n, int role, int v) {
     if (role == ROLE_X) {
-        const auto& vl = contig_variants<double>(n);
-        return v < (int)vl.names.size() ? vl.names[v] : "default";
+        const auto& names = precision == SH_F64 ? contig_variants<double>(n).names : contig_variants<float>(n).names;
+        return v >= 0 && v < (int)names.size() ? names[v] : "default";
     }
-    const auto& vl = strided_variants<double>(n);
-    (void)precision;
-    return v < (int)vl.names.size() ? vl.names[v] : "default";
+    const auto& names = precision == SH_F64 ? strided_variants<double>(n).names : strided_variants<float>(n).names;
+    return v >= 0 && v < (int)names.size() ? names[v] : "default";
 }
 
 template <typename T>

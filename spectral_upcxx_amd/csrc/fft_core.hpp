@@ -699,6 +699,9 @@ using X512 = FftPlan<512, 32, 16, 2, 16>;
 // the same with nontemporal stores (X(n, plan, columns per block))
 #define SH_STRIDED_NT_VARIANTS(X) X(256, Q256, 16) X(256, Q256, 8) X(256, W256, 16) X(512, X512, 16) X(512, P512, 8)
 // with nontemporal loads: X(n, plan, columns per block, nontemporal stores)
+// FP32 only (an FP64 tile of 32 columns of 512 would not fit in LDS):
+// X(n, plan, columns per block, nontemporal stores, nontemporal loads)
+#define SH_STRIDED_F32_VARIANTS(X) X(512, X512, 32, true, true) X(512, X512, 32, false, true) X(512, X512, 32, true, false)
 #define SH_STRIDED_LNT_VARIANTS(X) \
     X(256, Q256, 16, true) X(256, Q256, 16, false) X(256, Q256, 8, false) X(512, X512, 16, true) X(512, X512, 16, false)
 #define SH_STRIDED_VARIANTS(X) \
