@@ -255,6 +255,15 @@ int sh_ns_get_layout(sh_ns_t handle, int* half);
  * kernel E 14 passes, the y inverse 11 instead of 12).  The getters,
  * snapshots and schedule changes see (V1, cross0) in every form. */
 int sh_ns_get_history_form(sh_ns_t handle, int* form);
+
+/* Replaces nothing in the reference (graph capture is this implementation's):
+ * puts the fused state in the form the current schedule steps -- the
+ * x-inverse of V1 that the first kernel of a step reads, and the merged AB2
+ * history when the chosen kernel E keeps it -- without advancing time, so
+ * that a graph captured next records the merged kernels (a capture never
+ * converts the state: a replay would convert it again).  The velocity is
+ * unchanged to rounding.  Not inside a capture. */
+int sh_ns_prepare(sh_ns_t handle);
 /* Fused schedule only.  1: the y passes and the fused z kernel of a step run
  * as two independent halves of the x columns on the solver's stream and a
  * second (non-blocking, default-priority) stream of its own, joined before

@@ -2309,12 +2309,17 @@ __global__ void k_z_to_c(const double2* __restrict__ a, float2* __restrict__ b, 
         b[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
 // The merged AB2 history of kernel E (YS) and back, over the stored planes
-// (nzp) of the three fields: TOY: H = (V + nu K/2 lap V) - K/2 H (H = N0 in,
-// Y out); otherwise H = ((V + nu K/2 lap V) - H) / (K/2) (Y in, N0 out)
+// (nzp) of the three fields of box blockIdx.y (field stride F, box offset
+// boff), with that box's coefficients: TOY: H = (V + nu K/2 lap V) - K/2 H
+// (H = N0 in, Y out); otherwise H = ((V + nu K/2 lap V) - H) / (K/2) (Y in,
+// N0 out)
 template <typename T, bool TOY>
-__global__ void k_yform(const C2<T>* __restrict__ V, C2<T>* __restrict__ H, long F, int nx, int ny, int nz,
-                        int nzp, T c1, T Kh) {
+__global__ void k_yform(const C2<T>* __restrict__ V, C2<T>* __restrict__ H, long F, long boff, int nx, int ny,
+                        int nz, int nzp, const StepCoef<T>* __restrict__ cfs) {
     const long n = (long)nx * ny * nzp;
+    const T c1 = cfs[blockIdx.y].c1, Kh = cfs[blockIdx.y].Kh;
+    V += blockIdx.y * boff;
+    H += blockIdx.y * boff;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < 3 * n; e += (long)gridDim.x * blockDim.x) {
         const int d = (int)(e / n);
         const long r = e % n;
@@ -2583,7 +2588,7 @@ int add_forcing64(sh_ns_s* s, double2* cr) {
 // The merged history serves one FP64 box with the Crank-Nicolson scheme and no
 // body force (kernel E with YS); everything else keeps (V1, N0).
 bool y_ok(const sh_ns_s* s) {
-    return s->fused && s->nboxes == 1 && !s->forcing && s->scheme == 0 && !s->keep_v1;
+    return s->fused && !s->forcing && s->scheme == 0 && !s->keep_v1;
 }
 
 bool capturing(hipStream_t st) {
@@ -2591,16 +2596,30 @@ bool capturing(hipStream_t st) {
     return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
-// V1 of a solver in the merged form into dst (3 fields at stride N): the
-// forward x pass of W's first three fields (their unnormalised x-inverse),
-// scaled by 1/nx, Nyquist lines zeroed as every forward() leaves them
+// V1 of a solver in the merged form into dst (3 fields of every box, at
+// stride N in the state's order: field d of box b in slot d * nboxes + b):
+// the forward x pass of W's first three fields (their unnormalised
+// x-inverse), scaled by 1/nx, Nyquist lines zeroed as every forward() leaves them
 template <typename T>
 int v1_into(sh_ns_s* s, C2<T>* dst) {
     const int nzp = s->half ? s->nz / 2 : s->nz;
     const size_t cnt = (size_t)s->nx * s->ny * nzp;
-    for (int d = 0; d < 3; d++)
-        SH_HIP(hipMemcpyAsync(dst + d * s->N, (const C2<T>*)s->Wf + d * s->N, sizeof(C2<T>) * cnt,
+    const int slots = 3 * s->nboxes;
+    for (int q = 0; q < slots; q++)
+        SH_HIP(hipMemcpyAsync(dst + q * s->N, (const C2<T>*)s->Wf + q * s->N, sizeof(C2<T>) * cnt,
                               hipMemcpyDeviceToDevice, s->st));
+    Epi<T> ep{T(1.0 / s->nx), 3, s->ny, s->ny / 2, s->nz / 2};
+    return pass_contig<T>(dst, s->nx, (long)s->ny * nzp, s->nx, slots, s->N, -1, ep, s->st, -1);
+}
+
+// the same for box b only, into dst (3 fields at stride N)
+template <typename T>
+int v1_into_box(sh_ns_s* s, int b, C2<T>* dst) {
+    const int nzp = s->half ? s->nz / 2 : s->nz;
+    const size_t cnt = (size_t)s->nx * s->ny * nzp;
+    for (int d = 0; d < 3; d++)
+        SH_HIP(hipMemcpyAsync(dst + d * s->N, (const C2<T>*)s->Wf + ((long)d * s->nboxes + b) * s->N,
+                              sizeof(C2<T>) * cnt, hipMemcpyDeviceToDevice, s->st));
     Epi<T> ep{T(1.0 / s->nx), 3, s->ny, s->ny / 2, s->nz / 2};
     return pass_contig<T>(dst, s->nx, (long)s->ny * nzp, s->nx, 3, s->N, -1, ep, s->st, -1);
 }
@@ -2609,9 +2628,9 @@ int v1_into(sh_ns_s* s, C2<T>* dst) {
 template <typename T, bool TOY>
 int launch_yform(sh_ns_s* s) {
     const int nzp = s->half ? s->nz / 2 : s->nz;
-    hipLaunchKernelGGL((k_yform<T, TOY>), dim3(egrid(3L * s->nx * s->ny * nzp)), dim3(256), 0, s->st,
-                       (const C2<T>*)s->V1f, (C2<T>*)s->N0f, s->N, s->nx, s->ny, s->nz, nzp,
-                       T(s->nus[0] * s->K / 2), T(s->K / 2));
+    hipLaunchKernelGGL((k_yform<T, TOY>), dim3(egrid(3L * s->nx * s->ny * nzp), (unsigned)s->nboxes), dim3(256), 0,
+                       s->st, (const C2<T>*)s->V1f, (C2<T>*)s->N0f, s->N * s->nboxes, s->N, s->nx, s->ny, s->nz,
+                       nzp, (const StepCoef<T>*)s->coefs);
     SH_LAUNCH_CHECK("k_yform");
     return SH_OK;
 }
@@ -2638,12 +2657,12 @@ int ns_canonical(sh_ns_s* s) {
 
 int ensure_snap(sh_ns_s* s);
 
-// V1 of a merged-history state as complex128 into out (3 fields at stride
-// N), leaving the state as it is (FP32: through the snapshot scratch)
-int v1_into_z(sh_ns_s* s, double2* out) {
-    if (s->prec == SH_F64) return v1_into<double>(s, out);
+// V1 of box b of a merged-history state as complex128 into out (3 fields at
+// stride N), leaving the state as it is (FP32: through the snapshot scratch)
+int v1_into_z(sh_ns_s* s, double2* out, int b = 0) {
+    if (s->prec == SH_F64) return v1_into_box<double>(s, b, out);
     RC(ensure_snap(s));
-    RC(v1_into<float>(s, (float2*)s->snap));
+    RC(v1_into_box<float>(s, b, (float2*)s->snap));
     const long n3 = 3 * s->N;
     hipLaunchKernelGGL(k_c_to_z, dim3(egrid(n3)), dim3(256), 0, s->st, (const float2*)s->snap, out, n3);
     SH_LAUNCH_CHECK("k_c_to_z");
@@ -2956,7 +2975,7 @@ int launch_fourier_xy(sh_ns_s* s) {
     static_assert(sizeof(T) == 8, "FP64 only");
     // a (V1, N0) state is converted only outside a graph capture
     const bool cap = capturing(s->st);
-    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_xd<T, P, NT, true, true>(s);
+    if (!y_ok(s) || s->nboxes != 1 || (!s->yform && cap)) return launch_fourier_xd<T, P, NT, true, true>(s);
     // inside a capture the emitted form stays what the captured y pass expects
     const bool e5 = cap ? s->w5 : (E5 && y_curl_ok(s));
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F64);
@@ -3007,7 +3026,7 @@ int launch_fourier_xs(sh_ns_s* s) {
     static_assert(sizeof(T) == 4, "FP32 only");
     // a (V1, N0) state is converted only outside a graph capture
     const bool cap = capturing(s->st);
-    if (!y_ok(s) || (!s->yform && cap)) return launch_fourier_x<T, P, 1, true, false, NT>(s);
+    if (!y_ok(s) || s->nboxes != 1 || (!s->yform && cap)) return launch_fourier_x<T, P, 1, true, false, NT>(s);
     // inside a capture the emitted form stays what the captured y pass expects
     const bool e5 = cap ? s->w5 : (E5 && y_curl_ok(s));
     const C2<T>* tw = (const C2<T>*)twiddles(P::N, SH_F32);
@@ -3058,7 +3077,7 @@ template <typename T, bool NT = false, bool E5 = false, bool REV = false, int LA
 int launch_fourier_xw(sh_ns_s* s) {
     // a (V1, N0) state is converted only outside a graph capture
     const bool cap = capturing(s->st);
-    if (!y_ok(s) || (!s->yform && cap)) {
+    if (!y_ok(s) || s->nboxes != 1 || (!s->yform && cap)) {
         if constexpr (sizeof(T) == 8)
             return launch_fourier_xd<T, W256, NT, true, true>(s);
         else
@@ -3253,7 +3272,8 @@ using NsLaunch = int (*)(sh_ns_s*);
 struct NsVariants {
     std::vector<const char*> names;
     std::vector<NsLaunch> fns;
-    std::vector<NsLaunch> merged;  // the entries of fns that keep the merged AB2 history
+    std::vector<NsLaunch> merged;   // the entries of fns that keep the merged AB2 history
+    std::vector<NsLaunch> merged1;  // those that keep it for a single box only
 };
 
 template <typename T>
@@ -3386,22 +3406,22 @@ const NsVariants& fourier_variants(int n) {
     m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, nontemporal=" #NT_          \
                           ", blocks per CU=" #CAP_);                                                            \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);                                                 \
-    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);
+    m[n_].merged1.push_back(&launch_fourier_xy<T, P_, NT_, CAP_>);
 #define SH_VY5(n_, P_, NT_, CAP_)                                                                             \
     m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "      \
                           "nontemporal=" #NT_);                                                                 \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);                                           \
-    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);
+    m[n_].merged1.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true>);
 #define SH_VY5R(n_, P_, NT_, CAP_)                                                                            \
     m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "      \
                           "nontemporal=" #NT_ ", top plane first");                                            \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);                                     \
-    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);
+    m[n_].merged1.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true>);
 #define SH_VY5RL(n_, P_, NT_, CAP_)                                                                           \
     m[n_].names.push_back(#P_ " LDS-streamed persistent lines, merged AB2 history, five emitted fields, "      \
                           "nontemporal=" #NT_ ", top plane first, nontemporal loads");                         \
     m[n_].fns.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true, 2>);                                  \
-    m[n_].merged.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true, 2>);
+    m[n_].merged1.push_back(&launch_fourier_xy<T, P_, NT_, CAP_, true, true, 2>);
 #define SH_VM5(n_, P_, L_)                                                                      \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history, five emitted fields, nontemporal stores"); \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_, false, true, true>);                                \
@@ -3419,14 +3439,14 @@ const NsVariants& fourier_variants(int n) {
                               : #P_ " LDS-streamed persistent lines, merged AB2 history, six emitted fields, "   \
                                 "nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_);     \
     m[n_].fns.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);                                  \
-    m[n_].merged.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);
+    m[n_].merged1.push_back(&launch_fourier_xs<T, P_, NT_, E5_, REV_, LAUX_>);
 #define SH_VXW(n_, NT_, E5_, REV_, LAUX_)                                                                   \
     m[n_].names.push_back(E5_ ? "W256 LDS-streamed persistent lines, wavefront exchanges, merged AB2 history, five "   \
                                 "emitted fields, nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_ \
                               : "W256 LDS-streamed persistent lines, wavefront exchanges, merged AB2 history, six "    \
                                 "emitted fields, nontemporal=" #NT_ ", top plane first=" #REV_ ", load cache bits=" #LAUX_); \
     m[n_].fns.push_back(&launch_fourier_xw<T, NT_, E5_, REV_, LAUX_>);                                       \
-    m[n_].merged.push_back(&launch_fourier_xw<T, NT_, E5_, REV_, LAUX_>);
+    m[n_].merged1.push_back(&launch_fourier_xw<T, NT_, E5_, REV_, LAUX_>);
 #define SH_VM(n_, P_, L_)                                                                     \
     m[n_].names.push_back(#P_ " lines=" #L_ " merged AB2 history");                            \
     m[n_].fns.push_back(&launch_fourier_xm<T, P_, L_>);                                        \
@@ -3566,6 +3586,9 @@ bool fourier_merged(sh_ns_s* s) {
     if (v < 0 || v >= (int)vl.fns.size()) v = 0;
     for (NsLaunch f : vl.merged)
         if (f == vl.fns[v]) return true;
+    if (s->nboxes == 1)
+        for (NsLaunch f : vl.merged1)
+            if (f == vl.fns[v]) return true;
     return false;
 }
 
@@ -3874,7 +3897,8 @@ int decide_layout(sh_ns_s* s, int b) {
 // copy the FP64 start-up result (V1, cross0) into box b of the fused state
 int install_box(sh_ns_s* s, int b) {
     const long N = s->N;
-    s->yform = false;
+    // the other boxes go back to (V1, N0) with the state's form
+    RC(ns_canonical(s));
     s->w5 = false;
     for (int d = 0; d < 3; d++) {
         const long off = ((long)d * s->nboxes + b) * N;
@@ -4022,6 +4046,8 @@ int sh_ns_box_count(sh_ns_t s, int* count) {
 // start-up (exec/ns.cpp:132-153) with its own viscosity
 int sh_ns_set_initial_box(sh_ns_t s, int box, const void* V0) {
     if (!s || !V0 || box < 0 || box >= s->nboxes) return SH_EINVAL;
+    // the other boxes go back to (V1, N0) while W still holds their x-inverse
+    RC(ns_canonical(s));
     SH_HIP(hipMemcpyAsync(s->V0, V0, sizeof(double2) * 3 * s->N, hipMemcpyDeviceToDevice, s->st));
     RC(curl(s, s->V0, s->om0));
     const double nu0 = s->nu;
@@ -4039,8 +4065,8 @@ int sh_ns_set_initial_box(sh_ns_t s, int box, const void* V0) {
 int sh_ns_get_velocity_box(sh_ns_t s, int box, void* out) {
     if (!s || !out || box < 0 || box >= s->nboxes || !s->fused) return SH_EINVAL;
     const long N = s->N;
-    if (s->yform) {  // one box: V1 from W
-        RC(v1_into_z(s, (double2*)out));
+    if (s->yform) {  // V1 of the box from W
+        RC(v1_into_z(s, (double2*)out, box));
         return s->half ? expand_half<double>(s, out, 3) : SH_OK;
     }
     for (int d = 0; d < 3; d++) {
@@ -4069,7 +4095,7 @@ int snapshot_t(sh_ns_s* s, int r, float* out) {
     // all boxes at once: one copy, one 3-pass inverse transform, one filter launch
     C2<T>* snap = (C2<T>*)s->snap;
     const long nf = 3L * s->nboxes;
-    if (s->yform)  // one box, V1 from W (the state stays as it is)
+    if (s->yform)  // V1 of every box from W (the state stays as it is)
         RC(v1_into<T>(s, snap));
     else
         SH_HIP(hipMemcpyAsync(snap, s->V1f, sizeof(C2<T>) * N * nf, hipMemcpyDeviceToDevice, s->st));
@@ -4305,6 +4331,28 @@ int sh_ns_get_layout(sh_ns_t s, int* half) {
 int sh_ns_get_history_form(sh_ns_t s, int* form) {
     if (!s || !form) return SH_EINVAL;
     *form = s->yform ? (s->w5 ? 2 : 1) : 0;
+    return SH_OK;
+}
+
+int sh_ns_prepare(sh_ns_t s) {
+    if (!s) return SH_EINVAL;
+    if (!s->fused) return SH_OK;
+    if (capturing(s->st)) {
+        set_error("sh_ns_prepare: not inside a graph capture");
+        return SH_EINVAL;
+    }
+    const bool f64 = s->prec == SH_F64;
+    const bool merged = fourier_merged(s);
+    if (s->yform && !merged) RC(ns_canonical(s));
+    if (!s->w_primed) {  // kernel A, as a step's first kernel would run it
+        RC(f64 ? dispatch_inv_x<double>(s) : dispatch_inv_x<float>(s));
+        s->w_primed = true;
+    }
+    if (!s->yform && merged) {
+        const int rc = f64 ? launch_yform<double, true>(s) : launch_yform<float, true>(s);
+        if (rc) return rc;
+        s->yform = true;
+    }
     return SH_OK;
 }
 

@@ -146,7 +146,12 @@ class Ensemble:
             self.L.sh_ns_snapshot(h, int(ratio), ptr(b))
 
     def capture(self, nsteps, every, ratio, snaps):
-        """One graph: nsteps steps of every group with a snapshot after each `every`."""
+        """One graph: nsteps steps of every group with a snapshot after each `every`.
+        Each handle's state is first put in the form its tuned schedule steps
+        (sh_ns_prepare), so the graph records the merged-history kernels."""
+        for h in self.handles:
+            if self.L.sh_ns_prepare(h) != 0:
+                raise RuntimeError(self.L.sh_last_error().decode())
         self.L.sh_graph_begin(self.stream)
         try:
             for k in range(nsteps // every):

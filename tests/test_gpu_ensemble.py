@@ -165,3 +165,96 @@ def test_ensemble_run_and_snapshot_file(hip, tmp_path):
     ks = [b["k0"] for g in header["arrays"] for b in g["boxes"]]
     nus = [b["nu"] for g in header["arrays"] for b in g["boxes"]]
     assert all(2 <= k <= 6 for k in ks) and all(1 / 1000 <= v <= 1 / 300 for v in nus)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ensemble_merged_history(hip, prec):
+    """Ensembles keep the merged AB2 history in the register-loaded kernel E
+    (box = blockIdx.y, each box with its own coefficients): each box matches
+    the single solver on the same schedule and the oracle; installing one
+    box's start converts the others back to (V1, N0) without changing them;
+    sh_ns_prepare converts without advancing time, and a graph captured
+    after it holds the merged kernel and replays like eager steps."""
+    import torch
+    L = hip.lib()
+    n, K = 64, 1e-3
+    nus = [0.01, 0.003, 0.006]
+    ics = [O.solenoidal((n, n, n), 50 + b, 2.0 + b, 1.0) for b in range(3)]
+    buf = ctypes.create_string_buffer(8192)
+    c = ctypes.c_int()
+    L.sh_tune_variant_count(prec, n, 4, ctypes.byref(c))
+    merged = []
+    for v in range(c.value):
+        L.sh_tune_reset()
+        L.sh_tune_set_variant(prec, n, 4, v)
+        L.sh_tune_report(buf, 8192)
+        if "merged AB2 history" in buf.value.decode() and "LDS-streamed" not in buf.value.decode():
+            merged.append(v)
+    assert merged
+    tol = 1e-14 if prec == 64 else 1e-6
+
+    def form(h):
+        f = ctypes.c_int()
+        L.sh_ns_get_history_form(h, ctypes.byref(f))
+        return f.value
+
+    try:
+        L.sh_tune_reset()
+        L.sh_tune_set_variant(prec, n, 4, merged[0])
+        h = _make_ensemble(hip, n, nus, K, prec, ics)
+        L.sh_ns_steps(h, 6)
+        assert form(h) == 1
+        vs = [_box_velocity(hip, h, b, n) for b in range(3)]
+        for b in range(3):
+            single = hip.NS((n, n, n), nus[b], K, prec, fused=True)
+            single.set_initial(_dev(ics[b]))
+            single.startup()
+            single.steps(6)
+            err = rel_l2(vs[b], _host(single.velocity()))
+            print("fp%d box %d vs single: %.2e" % (prec, b, err), flush=True)
+            assert err < tol, b
+            single.close()
+            if prec == 64:
+                ref = O.NS((n, n, n), nus[b], K)
+                ref.set_initial(ics[b])
+                ref.startup()
+                ref.steps(6)
+                assert rel_l2(vs[b], ref.velocity()) < 1e-10, b
+        # a new start for box 1: the others back to (V1, N0), unchanged
+        L.sh_ns_set_initial_box(h, 1, hip.ptr(_dev(ics[0])))
+        assert form(h) == 0
+        for b in (0, 2):
+            assert rel_l2(_box_velocity(hip, h, b, n), vs[b]) < (1e-13 if prec == 64 else 1e-6), b
+        # prepare: merged without a step, the velocities unchanged to rounding
+        before = [_box_velocity(hip, h, b, n) for b in range(3)]
+        assert L.sh_ns_prepare(h) == 0
+        assert form(h) == 1
+        for b in range(3):
+            assert rel_l2(_box_velocity(hip, h, b, n), before[b]) < (1e-13 if prec == 64 else 1e-6), b
+        # a graph captured after prepare replays like an eager twin
+        st = ctypes.c_void_p()
+        L.sh_stream_create(ctypes.byref(st))
+        twins = [_make_ensemble(hip, n, nus, K, prec, ics, st) for _ in range(2)]
+        for t in twins:
+            assert L.sh_ns_prepare(t) == 0
+        L.sh_stream_synchronize(st)
+        assert L.sh_graph_begin(st) == 0
+        assert L.sh_ns_steps(twins[0], 3) == 0
+        g = ctypes.c_void_p()
+        assert L.sh_graph_end(st, ctypes.byref(g)) == 0
+        for rep in range(2):
+            L.sh_graph_launch(g, st)
+            L.sh_ns_steps(twins[1], 3)
+            L.sh_stream_synchronize(st)
+            assert form(twins[0]) == 1 and form(twins[1]) == 1
+            for b in range(3):
+                a = _box_velocity(hip, twins[0], b, n)
+                e = _box_velocity(hip, twins[1], b, n)
+                assert np.array_equal(a, e), (rep, b)
+        L.sh_graph_destroy(g)
+        for t in twins:
+            L.sh_ns_destroy(t)
+        L.sh_stream_destroy(st)
+        L.sh_ns_destroy(h)
+    finally:
+        L.sh_tune_reset()
