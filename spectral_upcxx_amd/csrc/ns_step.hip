@@ -318,6 +318,176 @@ __global__ __launch_bounds__(P::TPL* LPB) void k_ns_cross_zh(C2<T>* __restrict__
     }
 }
 
+// ---- cross-lane exchanges
+// A line's values spread over a wave's lanes and registers, lane l and
+// register r holding position l + 64 r (or, for a line over 32 lanes, l + 32
+// r): a radix-4 stage's data movement is then a swap of two lane-index bits
+// with the two register-index bits, done with cross-lane instructions (lane
+// bits 5, 4: v_permlane32_swap / v_permlane16_swap; bits 3, 2: DPP row shifts
+// by 8 and 4 writing only the lanes whose bit is set or clear, by bank mask;
+// bits 1, 0: DPP quad permutations and a select), with no LDS traffic and no
+// barrier.
+//
+// xw_swap1<B>: a and b are one dword of the registers whose register-index bit
+// is 0 and 1; afterwards a holds, on lanes with bit B set, b of the lane 2^B
+// below, and b, on lanes with bit B clear, a of the lane 2^B above.
+template <int B>
+__device__ __forceinline__ void xw_swap1(unsigned& a, unsigned& b, int lane) {
+    const unsigned ua = a, ub = b;
+    if constexpr (B == 5) {
+        const auto r = __builtin_amdgcn_permlane32_swap(ua, ub, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (B == 4) {
+        const auto r = __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (B == 3) {
+        a = __builtin_amdgcn_update_dpp(ua, ub, 0x118, 0xF, 0xC, false);  // row_shr:8, lanes 8-15 of a row
+        b = __builtin_amdgcn_update_dpp(ub, ua, 0x108, 0xF, 0x3, false);  // row_shl:8, lanes 0-7
+    } else if constexpr (B == 2) {
+        a = __builtin_amdgcn_update_dpp(ua, ub, 0x114, 0xF, 0xA, false);  // row_shr:4, lanes 4-7, 12-15
+        b = __builtin_amdgcn_update_dpp(ub, ua, 0x104, 0xF, 0x5, false);  // row_shl:4, lanes 0-3, 8-11
+    } else {
+        constexpr int QP = B == 1 ? 0x4E : 0xB1;  // quad_perm [2,3,0,1] (lane ^ 2), [1,0,3,2] (lane ^ 1)
+        const unsigned pa = __builtin_amdgcn_mov_dpp(ua, QP, 0xF, 0xF, false);
+        const unsigned pb = __builtin_amdgcn_mov_dpp(ub, QP, 0xF, 0xF, false);
+        const bool hi = (lane >> B) & 1;
+        a = hi ? pb : ua;
+        b = hi ? ub : pa;
+    }
+}
+
+// the same for one real part or imaginary part (bit-exact moves)
+template <int B>
+__device__ __forceinline__ void xw_swapv(float& a, float& b, int lane) {
+    unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
+    xw_swap1<B>(ua, ub, lane);
+    a = __uint_as_float(ua);
+    b = __uint_as_float(ub);
+}
+template <int B>
+__device__ __forceinline__ void xw_swapv(double& a, double& b, int lane) {
+    unsigned al = (unsigned)__double2loint(a), ah = (unsigned)__double2hiint(a);
+    unsigned bl = (unsigned)__double2loint(b), bh = (unsigned)__double2hiint(b);
+    xw_swap1<B>(al, bl, lane);
+    xw_swap1<B>(ah, bh, lane);
+    a = __hiloint2double((int)ah, (int)al);
+    b = __hiloint2double((int)bh, (int)bl);
+}
+
+// lane bits (H, L) <-> register-index bits (1, 0) of x[0..3]: afterwards
+// lane bits (H, L) hold what the register index was, and the register index
+// what lane bits (H, L) were
+template <int H, int L, typename T>
+__device__ __forceinline__ void xw_swap2(C2<T>* x, int lane) {
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        xw_swapv<H>(x[p].x, x[p + 2].x, lane);
+        xw_swapv<H>(x[p].y, x[p + 2].y, lane);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p += 2) {
+        xw_swapv<L>(x[p].x, x[p + 1].x, lane);
+        xw_swapv<L>(x[p].y, x[p + 1].y, lane);
+    }
+}
+
+// ---- 128-point transforms of kernel C with one LDS exchange
+// A block of four waves transforms eight z lines (one per column c = lane % 8)
+// of 128 points; thread (wave w, lane l) is line position t = 8 w + l / 8 and
+// register r holds position t + 32 r, the layout the slot reads and the
+// product stores use.  Of the transform's four stages (radix 4, 4, 4, 2 in
+// decimation in frequency), only the first moves data between waves: one
+// exchange through LDS (a transpose of the wave index with the register
+// index, contiguous 1 KiB per wave instruction) and two barriers.  The
+// second stage's data movement is the swap of register bits with lane bits 5,
+// 4 (t bits 2, 1), the third's that of register bit 0 with lane bit 3 (t bit
+// 0).
+//   zw_inv: natural input (X[t + 32 r]), sign +1, output x[n] at
+//     n = w + 4 ((l >> 4) & 3) + 16 ((l >> 3) & 1) + 32 (r >> 1) + 64 (r & 1);
+//   zw_fwd: the inverse process (input in that order, sign -1, natural output,
+//     unnormalised): pointwise products of zw_inv outputs need no reordering.
+// Twiddles (e^{-2 pi i k / 128}, conjugated for zw_inv) in registers: a[j] at
+// k = j t, b[j] at k = 4 j (t % 8), c[j] at k = 16 j (t % 2).
+template <typename T>
+struct ZwTw {
+    C2<T> a[3], b[3], c[3];
+    __device__ __forceinline__ void load(const C2<T>* __restrict__ twg, int t) {
+#pragma unroll
+        for (int j = 1; j < 4; j++) {
+            a[j - 1] = twg[(j * t) & 127];
+            b[j - 1] = twg[(4 * j * (t & 7)) & 127];
+            c[j - 1] = twg[(16 * j * (t & 1)) & 127];
+        }
+    }
+};
+
+// x[1..3] times w[0..2] (CONJ: times their conjugates)
+template <bool CONJ, typename T>
+__device__ __forceinline__ void zw_twiddle(C2<T>* x, const C2<T>* w) {
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+        const C2<T> a = x[r], b = w[r - 1];
+        x[r] = CONJ ? mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y) : cmul<T>(a, b);
+    }
+}
+
+// the exchange: register r of (wave w, lane l) to entry 256 r + 64 w + l, then
+// register r from entry 256 w + 64 r + l (the caller has synchronised the
+// previous users of sx)
+template <typename T>
+__device__ __forceinline__ void zw_exchange(C2<T>* x, C2<T>* sx, int wave, int lane) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) sx[256 * r + 64 * wave + lane] = x[r];
+    SyncLds::run();
+#pragma unroll
+    for (int r = 0; r < 4; r++) x[r] = sx[256 * wave + 64 * r + lane];
+}
+
+// register bit 0 <-> lane bit 3, then the radix-2 butterflies over register bit 0
+template <typename T>
+__device__ __forceinline__ void zw_swap3(C2<T>* x, int lane) {
+#pragma unroll
+    for (int p = 0; p < 4; p += 2) {
+        xw_swapv<3>(x[p].x, x[p + 1].x, lane);
+        xw_swapv<3>(x[p].y, x[p + 1].y, lane);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void zw_radix2(C2<T>* x) {
+#pragma unroll
+    for (int p = 0; p < 4; p += 2) Dft<T, 2, 1>::run(x + p);
+}
+
+template <typename T>
+__device__ __forceinline__ void zw_inv(C2<T>* x, const ZwTw<T>& t, C2<T>* sx, int wave, int lane) {
+    Dft<T, 4, 1>::run(x);
+    zw_twiddle<true, T>(x, t.a);
+    zw_exchange<T>(x, sx, wave, lane);
+    Dft<T, 4, 1>::run(x);
+    zw_twiddle<true, T>(x, t.b);
+    xw_swap2<5, 4, T>(x, lane);
+    Dft<T, 4, 1>::run(x);
+    zw_twiddle<true, T>(x, t.c);
+    zw_swap3<T>(x, lane);
+    zw_radix2<T>(x);
+}
+
+template <typename T>
+__device__ __forceinline__ void zw_fwd(C2<T>* x, const ZwTw<T>& t, C2<T>* sx, int wave, int lane) {
+    zw_radix2<T>(x);
+    zw_swap3<T>(x, lane);
+    zw_twiddle<false, T>(x, t.c);
+    Dft<T, 4, -1>::run(x);
+    xw_swap2<5, 4, T>(x, lane);
+    zw_twiddle<false, T>(x, t.b);
+    Dft<T, 4, -1>::run(x);
+    zw_exchange<T>(x, sx, wave, lane);
+    zw_twiddle<false, T>(x, t.a);
+    Dft<T, 4, -1>::run(x);
+}
+
 // C of the half-spectrum layout with its inputs streamed into LDS (FP64).
 // The same arithmetic as k_ns_cross_zh; what changes is how the six input
 // fields arrive.  Each block loops over column tiles (a persistent grid of
@@ -392,19 +562,29 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     const unsigned es = (unsigned)nx * (unsigned)ny;
     const unsigned uF = (unsigned)F;
     static_assert(RT == 0 || IL, "register twiddles: interleaved exchange");
+    static_assert(RT != 3 || (M == 128 && P::TPL == 32 && E == 4 && LPB == 8),
+                  "the one-exchange transforms: 128-point lines, eight per block of four waves");
     // RT >= 1: this thread's packing factors e^{-2 pi i k/nz}, k = tid + r Q
     // (the positions of its first-stage inputs and of its last-stage outputs);
-    // RT == 2: also its stage twiddles, loaded once for every tile
+    // RT == 2: also its stage twiddles, loaded once for every tile; RT == 3:
+    // the transforms with one LDS exchange (zw_inv, zw_fwd) and their twiddles
     C2<T> twp[RT >= 1 ? 4 : 1], tws[RT == 2 ? rt_size<P>() : 1];
+    ZwTw<T> zt;
     if constexpr (RT >= 1 && HALF) {
 #pragma unroll
         for (int r = 0; r < 4; r++) twp[r] = twng[tid + r * Q];
     }
     if constexpr (RT == 2) rt_load<P, 0, T>(tws, tid, twg);
+    if constexpr (RT == 3) zt.load(twg, tid);
     auto idx = [&](int pos) { return IL ? pos * LPB + c : pos * CS + c; };
     auto fft = [&](auto sign, C2<T>* v) {
         constexpr int SG = decltype(sign)::value;
-        if constexpr (RT == 2)
+        if constexpr (RT == 3) {
+            if constexpr (SG > 0)
+                zw_inv<T>(v, zt, sx, wave, lane);
+            else
+                zw_fwd<T>(v, zt, sx, wave, lane);
+        } else if constexpr (RT == 2)
             line_fft_cr<P, SG, T, decltype(idx), SyncLds>(v, tid, sx, idx, tws);
         else if constexpr (IL)
             line_fft_c<P, SG, T, decltype(idx), SyncLds>(v, tid, sx, idx, tw);
@@ -1780,77 +1960,9 @@ __global__ __launch_bounds__(64) void k_ns_fourier_xs(
 }
 
 // ---- FP32 kernel E with wavefront exchanges
-// A 256-point line held by one wave as 4 values per lane, transformed with no
-// LDS exchange: radix-4 stages whose data movement is a swap of two lane-index
-// bits with the two register-index bits, done with cross-lane instructions
-// (lane bits 5, 4: v_permlane32_swap / v_permlane16_swap; bits 3, 2: DPP row
-// shifts by 8 and 4 writing only the lanes whose bit is set or clear, by bank
-// mask; bits 1, 0: DPP quad permutations and a select).
-//
-// xw_swap1<B>: a and b are one dword of the registers whose register-index bit
-// is 0 and 1; afterwards a holds, on lanes with bit B set, b of the lane 2^B
-// below, and b, on lanes with bit B clear, a of the lane 2^B above.
-template <int B>
-__device__ __forceinline__ void xw_swap1(unsigned& a, unsigned& b, int lane) {
-    const unsigned ua = a, ub = b;
-    if constexpr (B == 5) {
-        const auto r = __builtin_amdgcn_permlane32_swap(ua, ub, false, false);
-        a = r[0];
-        b = r[1];
-    } else if constexpr (B == 4) {
-        const auto r = __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
-        a = r[0];
-        b = r[1];
-    } else if constexpr (B == 3) {
-        a = __builtin_amdgcn_update_dpp(ua, ub, 0x118, 0xF, 0xC, false);  // row_shr:8, lanes 8-15 of a row
-        b = __builtin_amdgcn_update_dpp(ub, ua, 0x108, 0xF, 0x3, false);  // row_shl:8, lanes 0-7
-    } else if constexpr (B == 2) {
-        a = __builtin_amdgcn_update_dpp(ua, ub, 0x114, 0xF, 0xA, false);  // row_shr:4, lanes 4-7, 12-15
-        b = __builtin_amdgcn_update_dpp(ub, ua, 0x104, 0xF, 0x5, false);  // row_shl:4, lanes 0-3, 8-11
-    } else {
-        constexpr int QP = B == 1 ? 0x4E : 0xB1;  // quad_perm [2,3,0,1] (lane ^ 2), [1,0,3,2] (lane ^ 1)
-        const unsigned pa = __builtin_amdgcn_mov_dpp(ua, QP, 0xF, 0xF, false);
-        const unsigned pb = __builtin_amdgcn_mov_dpp(ub, QP, 0xF, 0xF, false);
-        const bool hi = (lane >> B) & 1;
-        a = hi ? pb : ua;
-        b = hi ? ub : pa;
-    }
-}
-
-// the same for one real part or imaginary part (bit-exact moves)
-template <int B>
-__device__ __forceinline__ void xw_swapv(float& a, float& b, int lane) {
-    unsigned ua = __float_as_uint(a), ub = __float_as_uint(b);
-    xw_swap1<B>(ua, ub, lane);
-    a = __uint_as_float(ua);
-    b = __uint_as_float(ub);
-}
-template <int B>
-__device__ __forceinline__ void xw_swapv(double& a, double& b, int lane) {
-    unsigned al = (unsigned)__double2loint(a), ah = (unsigned)__double2hiint(a);
-    unsigned bl = (unsigned)__double2loint(b), bh = (unsigned)__double2hiint(b);
-    xw_swap1<B>(al, bl, lane);
-    xw_swap1<B>(ah, bh, lane);
-    a = __hiloint2double((int)ah, (int)al);
-    b = __hiloint2double((int)bh, (int)bl);
-}
-
-// lane bits (H, L) <-> register-index bits (1, 0) of x[0..3]: afterwards
-// lane bits (H, L) hold what the register index was, and the register index
-// what lane bits (H, L) were
-template <int H, int L, typename T>
-__device__ __forceinline__ void xw_swap2(C2<T>* x, int lane) {
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-        xw_swapv<H>(x[p].x, x[p + 2].x, lane);
-        xw_swapv<H>(x[p].y, x[p + 2].y, lane);
-    }
-#pragma unroll
-    for (int p = 0; p < 4; p += 2) {
-        xw_swapv<L>(x[p].x, x[p + 1].x, lane);
-        xw_swapv<L>(x[p].y, x[p + 1].y, lane);
-    }
-}
+// A 256-point line held by one wave as 4 values per lane (lane l, register r:
+// position l + 64 r), transformed with no LDS exchange: four radix-4 stages
+// with the cross-lane swaps above between them.
 
 template <typename T>
 __device__ __forceinline__ void xw_twiddle(C2<T>* x, const C2<T>* w) {
@@ -3178,6 +3290,12 @@ int launch_fourier_xw(sh_ns_s* s) {
 // FP32 with the stage twiddles and packing factors in registers, nontemporal
 // loads (at 256^3: 127-129 us against 139-141; 8 columns per block ran 211-256)
 #define SH_CROSSH_F32_RTL_VARIANTS(X) X(256, W128, 16)
+// FP64 with the transforms of one LDS exchange (zw_inv, zw_fwd), nontemporal
+// loads, plain and nontemporal stores: X(nz, plan, columns per block) (at 256^3:
+// 221-222 us against 240; FP32 needs 16 columns per block for whole 128-byte
+// rows, which this layout of eight lines per four waves does not give: 212 us
+// against 138 with 8)
+#define SH_CROSSH_F64_ZW_VARIANTS(X) X(256, W128, 8)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -3340,6 +3458,11 @@ const NsVariants& crossh_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 2, 2>);                                 \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads and stores"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2, 2>);
+#define SH_VZW(n_, P_, C_)                                                                                  \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms nontemporal loads");     \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 3, 2>);                                 \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 3, 2>);
 #define SH_VR(n_, P_, C_)                                                                                   \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange packing factors in registers"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 1>);                                    \
@@ -3356,6 +3479,7 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F64_NT_VARIANTS(SH_VT)
             SH_CROSSH_F64_LDS_VARIANTS(SH_VD)
             SH_CROSSH_F64_RT_VARIANTS(SH_VR)
+            SH_CROSSH_F64_ZW_VARIANTS(SH_VZW)
         } else {
             SH_CROSSH_F32_VARIANTS(SH_V)
             SH_CROSSH_F32_NT_VARIANTS(SH_VT)
@@ -3363,6 +3487,7 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F32_LNT_VARIANTS(SH_VDL)
             SH_CROSSH_F32_RTL_VARIANTS(SH_VRL)
         }
+#undef SH_VZW
 #undef SH_VRL
 #undef SH_VR
 #undef SH_VDL
