@@ -51,8 +51,13 @@ def test_fft_variants_vs_oracle(hip, n, prec):
                 t = _dev(x if prec == 64 else x.astype(np.complex64))
                 hip.fft3d(t, -1, 0)
                 err = rel_l2(_host(t).astype(np.complex128), ref)
-                print("fp%d n=%d role=%d variant=%d err=%.2e" % (prec, n, role, v, err), flush=True)
+                # the same schedule's inverse: back to x (normalised)
+                hip.fft3d(t, 1, hip.SH_NORMALIZE)
+                back = rel_l2(_host(t).astype(np.complex128), x)
+                print("fp%d n=%d role=%d variant=%d err=%.2e round trip %.2e" % (prec, n, role, v, err, back),
+                      flush=True)
                 assert err < tol, (role, v, err)
+                assert back < tol, (role, v, back)
     finally:
         L.sh_tune_reset()
 
