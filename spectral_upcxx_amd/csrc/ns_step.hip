@@ -488,6 +488,75 @@ __device__ __forceinline__ void zw_fwd(C2<T>* x, const ZwTw<T>& t, C2<T>* sx, in
     Dft<T, 4, -1>::run(x);
 }
 
+// The same with eight values per thread, for sixteen 128-point lines (one per
+// column c = lane % 16) on four waves: thread (wave w, lane l) is line position
+// t = 4 w + l / 16 and register r holds position t + 16 r.  Radix 8 over the
+// registers; one LDS exchange of register bits 1, 0 with the wave index
+// (register bit 2 stays); radix 4 over register bits 1, 0 (positions 4 w);
+// the swap of register bits 1, 0 with lane bits 5, 4 (positions l / 16);
+// radix 4.
+//   zw8_inv: natural input, sign +1, output x[n] at
+//     n = w + 4 (r >> 2) + 8 ((l >> 4) & 3) + 32 (r & 3);
+//   zw8_fwd: the inverse process (that order in, sign -1, natural out).
+// Twiddles (e^{-2 pi i k / 128}, conjugated for zw8_inv): a[j - 1] at k = j t
+// (j = 1..7), b[j - 1] at k = 8 j (t % 4) (j = 1..3).
+template <typename T>
+struct ZwTw8 {
+    C2<T> a[7], b[3];
+    __device__ __forceinline__ void load(const C2<T>* __restrict__ twg, int t) {
+#pragma unroll
+        for (int j = 1; j < 8; j++) a[j - 1] = twg[(j * t) & 127];
+#pragma unroll
+        for (int j = 1; j < 4; j++) b[j - 1] = twg[(8 * j * (t & 3)) & 127];
+    }
+};
+
+// register r = 4 r2 + q of wave w to entry 256 r + 64 w + l, then register
+// 4 r2 + q of wave w from entry 256 (4 r2 + w) + 64 q + l
+template <typename T>
+__device__ __forceinline__ void zw8_exchange(C2<T>* x, C2<T>* sx, int wave, int lane) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) sx[256 * r + 64 * wave + lane] = x[r];
+    SyncLds::run();
+#pragma unroll
+    for (int r = 0; r < 8; r++) x[r] = sx[256 * (4 * (r >> 2) + wave) + 64 * (r & 3) + lane];
+}
+
+template <typename T>
+__device__ __forceinline__ void zw8_inv(C2<T>* x, const ZwTw8<T>& t, C2<T>* sx, int wave, int lane) {
+    Dft<T, 8, 1>::run(x);
+#pragma unroll
+    for (int r = 1; r < 8; r++) {
+        const C2<T> a = x[r], b = t.a[r - 1];
+        x[r] = mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    }
+    zw8_exchange<T>(x, sx, wave, lane);
+    Dft<T, 4, 1>::run(x);
+    Dft<T, 4, 1>::run(x + 4);
+    zw_twiddle<true, T>(x, t.b);
+    zw_twiddle<true, T>(x + 4, t.b);
+    xw_swap2<5, 4, T>(x, lane);
+    xw_swap2<5, 4, T>(x + 4, lane);
+    Dft<T, 4, 1>::run(x);
+    Dft<T, 4, 1>::run(x + 4);
+}
+
+template <typename T>
+__device__ __forceinline__ void zw8_fwd(C2<T>* x, const ZwTw8<T>& t, C2<T>* sx, int wave, int lane) {
+    Dft<T, 4, -1>::run(x);
+    Dft<T, 4, -1>::run(x + 4);
+    xw_swap2<5, 4, T>(x, lane);
+    xw_swap2<5, 4, T>(x + 4, lane);
+    zw_twiddle<false, T>(x, t.b);
+    zw_twiddle<false, T>(x + 4, t.b);
+    Dft<T, 4, -1>::run(x);
+    Dft<T, 4, -1>::run(x + 4);
+    zw8_exchange<T>(x, sx, wave, lane);
+#pragma unroll
+    for (int r = 1; r < 8; r++) x[r] = cmul<T>(x[r], t.a[r - 1]);
+    Dft<T, 8, -1>::run(x);
+}
+
 // C of the half-spectrum layout with its inputs streamed into LDS (FP64).
 // The same arithmetic as k_ns_cross_zh; what changes is how the six input
 // fields arrive.  Each block loops over column tiles (a persistent grid of
@@ -517,7 +586,8 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
 
 template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double, int RT = 0,
           int LAUX = 0>
-__global__ __launch_bounds__(P::TPL* LPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+__global__ __launch_bounds__(P::TPL* LPB)
+__attribute__((amdgpu_waves_per_eu(RT == 4 ? 2 : sizeof(T) == 4 ? 4 : 1)))
 void k_ns_cross_zd(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ twg,
                                                              const C2<T>* __restrict__ twng, int nx,
@@ -536,8 +606,10 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     constexpr int RPI = 64 / LPR;            // rows of a slot one wave instruction fills
     constexpr int IPW = M / (RPI * NW);      // slot loads per wave (and per lane) per field
     constexpr unsigned ROWB = LPB * VB;      // bytes per slot row
-    static_assert(R0 == 4 && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == 4 && P::nb(P::S - 1) == 1,
-                  "the slot reads assume one radix-4 butterfly per thread in the first and last stage");
+    static_assert(R0 == E && (E == 4 || E == 8) && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == R0 &&
+                      P::nb(P::S - 1) == 1,
+                  "the slot reads assume one butterfly of all a thread's values in the first and last stage");
+    static_assert((E == 8) == (RT == 4), "eight values per thread: the one-exchange transforms zw8_inv, zw8_fwd");
     static_assert(NTH % 64 == 0 && LPB % VPL == 0 && 64 % LPR == 0 && M % (RPI * NW) == 0 &&
                       (R0 - 1) * Q * ROWB < 65536,
                   "slot geometry");
@@ -564,22 +636,32 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     static_assert(RT == 0 || IL, "register twiddles: interleaved exchange");
     static_assert(RT != 3 || (M == 128 && P::TPL == 32 && E == 4 && LPB == 8),
                   "the one-exchange transforms: 128-point lines, eight per block of four waves");
+    static_assert(RT != 4 || (M == 128 && P::TPL == 16 && LPB == 16 && HALF),
+                  "the one-exchange transforms of eight values: 128-point lines, sixteen per block of four waves");
     // RT >= 1: this thread's packing factors e^{-2 pi i k/nz}, k = tid + r Q
     // (the positions of its first-stage inputs and of its last-stage outputs);
     // RT == 2: also its stage twiddles, loaded once for every tile; RT == 3:
-    // the transforms with one LDS exchange (zw_inv, zw_fwd) and their twiddles
-    C2<T> twp[RT >= 1 ? 4 : 1], tws[RT == 2 ? rt_size<P>() : 1];
+    // the transforms with one LDS exchange (zw_inv, zw_fwd) and their twiddles;
+    // RT == 4: the same with eight values per thread (zw8_inv, zw8_fwd)
+    C2<T> twp[RT >= 1 ? E : 1], tws[RT == 2 ? rt_size<P>() : 1];
     ZwTw<T> zt;
+    ZwTw8<T> zt8;
     if constexpr (RT >= 1 && HALF) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) twp[r] = twng[tid + r * Q];
+        for (int r = 0; r < E; r++) twp[r] = twng[tid + r * Q];
     }
     if constexpr (RT == 2) rt_load<P, 0, T>(tws, tid, twg);
     if constexpr (RT == 3) zt.load(twg, tid);
+    if constexpr (RT == 4) zt8.load(twg, tid);
     auto idx = [&](int pos) { return IL ? pos * LPB + c : pos * CS + c; };
     auto fft = [&](auto sign, C2<T>* v) {
         constexpr int SG = decltype(sign)::value;
-        if constexpr (RT == 3) {
+        if constexpr (RT == 4) {
+            if constexpr (SG > 0)
+                zw8_inv<T>(v, zt8, sx, wave, lane);
+            else
+                zw8_fwd<T>(v, zt8, sx, wave, lane);
+        } else if constexpr (RT == 3) {
             if constexpr (SG > 0)
                 zw_inv<T>(v, zt, sx, wave, lane);
             else
@@ -647,7 +729,7 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     auto inv = [&](int sl, C2<T>* x) {
         const unsigned base = lds_offset(&slot[sl][0]);
         const unsigned own = base + (unsigned)(tid * LPB + c) * VB;
-        v2 av[4];
+        v2 av[E];
         if constexpr (!HALF) {
             // the full spectrum: this thread's first-stage values as they are
             read4(own, av);
@@ -655,10 +737,16 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
             for (int r = 0; r < 4; r++) x[r] = mk<T>(av[r].x, av[r].y);
         } else {
             const unsigned mir = base + (unsigned)((tid == 0 ? Q : Q - tid) * LPB + c) * VB;
-            v2 mv[4];  // X[M - k] for r = 0..3
-            read8(own, mir, av, mv);
+            v2 mv[E];  // X[M - k] for r = 0..E-1
+            if constexpr (E == 8) {
+                // rows tid + r Q (r = 0..7) and their mirrors (Q - tid) + (7 - r) Q
+                read8(own, mir + 4u * Q * ROWB, av, mv);
+                read8(own + 4u * Q * ROWB, mir, av + 4, mv + 4);
+            } else {
+                read8(own, mir, av, mv);
+            }
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < E; r++) {
                 const int pos = tid + r * Q;
                 const C2<T> a = mk<T>(av[r].x, av[r].y);
                 const C2<T> b = pos == 0 ? mk<T>(T(0), T(0)) : mk<T>(mv[r].x, mv[r].y);
@@ -3296,6 +3384,10 @@ int launch_fourier_xw(sh_ns_s* s) {
 // rows, which this layout of eight lines per four waves does not give: 212 us
 // against 138 with 8)
 #define SH_CROSSH_F64_ZW_VARIANTS(X) X(256, W128, 8)
+// FP32 with the one-exchange transforms of eight values per thread (zw8_inv,
+// zw8_fwd): sixteen columns on four waves, nontemporal loads, plain and
+// nontemporal stores: X(nz, plan, columns per block)
+#define SH_CROSSH_F32_ZW8_VARIANTS(X) X(256, Q128, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -3458,6 +3550,11 @@ const NsVariants& crossh_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 2, 2>);                                 \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed interleaved exchange twiddles in registers nontemporal loads and stores"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 2, 2>);
+#define SH_VZW8(n_, P_, C_)                                                                                 \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms of eight values nontemporal loads"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 4, 2>);                                \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms of eight values nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 4, 2>);
 #define SH_VZW(n_, P_, C_)                                                                                  \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms nontemporal loads");     \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 3, 2>);                                 \
@@ -3486,7 +3583,9 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F32_LDS_VARIANTS(SH_VD)
             SH_CROSSH_F32_LNT_VARIANTS(SH_VDL)
             SH_CROSSH_F32_RTL_VARIANTS(SH_VRL)
+            SH_CROSSH_F32_ZW8_VARIANTS(SH_VZW8)
         }
+#undef SH_VZW8
 #undef SH_VZW
 #undef SH_VRL
 #undef SH_VR
