@@ -303,8 +303,9 @@ def main():
     a = parse()
     TUNE = not a.no_tune
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: the multi-GPU slab all-to-all is not part of this "
-              "build (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the multi-GPU slab all-to-all (the reference's rput "
+              "transposes, src/Scalar.cpp:486-503, 582-596) is not part of this library; run with "
+              "--gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     import spectral_upcxx_amd as S
