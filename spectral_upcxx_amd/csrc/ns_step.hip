@@ -557,6 +557,75 @@ __device__ __forceinline__ void zw8_fwd(C2<T>* x, const ZwTw8<T>& t, C2<T>* sx, 
     Dft<T, 8, -1>::run(x);
 }
 
+// The same for sixteen 256-point lines (one per column c = lane % 16) on
+// eight waves: thread (wave w, lane l) is line position t = 4 w + l / 16 and
+// register r holds position t + 32 r.  Radix 8 over the registers; one LDS
+// exchange transposing the register index with the wave index; radix 8; the
+// swap of register bits 1, 0 with lane bits 5, 4; radix 4.
+//   zs_inv: natural input, sign +1, output x[n] at
+//     n = w + 8 (4 (r >> 2) + ((l >> 4) & 3)) + 64 (r & 3);
+//   zs_fwd: the inverse process (that order in, sign -1, natural out).
+// Twiddles (e^{-2 pi i k / 256}, conjugated for zs_inv): a[j - 1] at k = j t,
+// b[j - 1] at k = 8 j (t % 4) (j = 1..7).
+template <typename T>
+struct ZsTw {
+    C2<T> a[7], b[7];
+    __device__ __forceinline__ void load(const C2<T>* __restrict__ twg, int t) {
+#pragma unroll
+        for (int j = 1; j < 8; j++) {
+            a[j - 1] = twg[(j * t) & 255];
+            b[j - 1] = twg[(8 * j * (t & 3)) & 255];
+        }
+    }
+};
+
+// x[1..7] times w[0..6] (CONJ: times their conjugates)
+template <bool CONJ, typename T>
+__device__ __forceinline__ void zs_twiddle(C2<T>* x, const C2<T>* w) {
+#pragma unroll
+    for (int r = 1; r < 8; r++) {
+        const C2<T> a = x[r], b = w[r - 1];
+        x[r] = CONJ ? mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y) : cmul<T>(a, b);
+    }
+}
+
+// register r of wave w to entry 512 r + 64 w + l, then register r from entry
+// 512 w + 64 r + l
+template <typename T>
+__device__ __forceinline__ void zs_exchange(C2<T>* x, C2<T>* sx, int wave, int lane) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) sx[512 * r + 64 * wave + lane] = x[r];
+    SyncLds::run();
+#pragma unroll
+    for (int r = 0; r < 8; r++) x[r] = sx[512 * wave + 64 * r + lane];
+}
+
+template <typename T>
+__device__ __forceinline__ void zs_inv(C2<T>* x, const ZsTw<T>& t, C2<T>* sx, int wave, int lane) {
+    Dft<T, 8, 1>::run(x);
+    zs_twiddle<true, T>(x, t.a);
+    zs_exchange<T>(x, sx, wave, lane);
+    Dft<T, 8, 1>::run(x);
+    zs_twiddle<true, T>(x, t.b);
+    xw_swap2<5, 4, T>(x, lane);
+    xw_swap2<5, 4, T>(x + 4, lane);
+    Dft<T, 4, 1>::run(x);
+    Dft<T, 4, 1>::run(x + 4);
+}
+
+template <typename T>
+__device__ __forceinline__ void zs_fwd(C2<T>* x, const ZsTw<T>& t, C2<T>* sx, int wave, int lane) {
+    Dft<T, 4, -1>::run(x);
+    Dft<T, 4, -1>::run(x + 4);
+    xw_swap2<5, 4, T>(x, lane);
+    xw_swap2<5, 4, T>(x + 4, lane);
+    zs_twiddle<false, T>(x, t.b);
+    Dft<T, 8, -1>::run(x);
+    zs_exchange<T>(x, sx, wave, lane);
+    zs_twiddle<false, T>(x, t.a);
+    Dft<T, 8, -1>::run(x);
+}
+
 // C of the half-spectrum layout with its inputs streamed into LDS (FP64).
 // The same arithmetic as k_ns_cross_zh; what changes is how the six input
 // fields arrive.  Each block loops over column tiles (a persistent grid of
@@ -587,7 +656,7 @@ __device__ __forceinline__ unsigned lds_offset(const T* p) {
 template <class P, int LPB, bool NT = false, bool IL = false, bool HALF = true, typename T = double, int RT = 0,
           int LAUX = 0>
 __global__ __launch_bounds__(P::TPL* LPB)
-__attribute__((amdgpu_waves_per_eu(RT == 4 ? 2 : sizeof(T) == 4 ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu(RT >= 4 ? 2 : sizeof(T) == 4 ? 4 : 1)))
 void k_ns_cross_zd(C2<T>* __restrict__ W,
                                                              const C2<T>* __restrict__ twg,
                                                              const C2<T>* __restrict__ twng, int nx,
@@ -609,7 +678,8 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     static_assert(R0 == E && (E == 4 || E == 8) && P::nb(0) == 1 && P::full(0) && P::radix(P::S - 1) == R0 &&
                       P::nb(P::S - 1) == 1,
                   "the slot reads assume one butterfly of all a thread's values in the first and last stage");
-    static_assert((E == 8) == (RT == 4), "eight values per thread: the one-exchange transforms zw8_inv, zw8_fwd");
+    static_assert((E == 8) == (RT == 4 || RT == 5),
+                  "eight values per thread: the one-exchange transforms zw8_inv, zw8_fwd, zs_inv, zs_fwd");
     static_assert(NTH % 64 == 0 && LPB % VPL == 0 && 64 % LPR == 0 && M % (RPI * NW) == 0 &&
                       (R0 - 1) * Q * ROWB < 65536,
                   "slot geometry");
@@ -638,14 +708,18 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
                   "the one-exchange transforms: 128-point lines, eight per block of four waves");
     static_assert(RT != 4 || (M == 128 && P::TPL == 16 && LPB == 16 && HALF),
                   "the one-exchange transforms of eight values: 128-point lines, sixteen per block of four waves");
+    static_assert(RT != 5 || (M == 256 && P::TPL == 32 && LPB == 16 && HALF),
+                  "the one-exchange transforms of eight values: 256-point lines, sixteen per block of eight waves");
     // RT >= 1: this thread's packing factors e^{-2 pi i k/nz}, k = tid + r Q
     // (the positions of its first-stage inputs and of its last-stage outputs);
     // RT == 2: also its stage twiddles, loaded once for every tile; RT == 3:
     // the transforms with one LDS exchange (zw_inv, zw_fwd) and their twiddles;
-    // RT == 4: the same with eight values per thread (zw8_inv, zw8_fwd)
+    // RT == 4: the same with eight values per thread (zw8_inv, zw8_fwd); RT ==
+    // 5: for 256-point lines (zs_inv, zs_fwd)
     C2<T> twp[RT >= 1 ? E : 1], tws[RT == 2 ? rt_size<P>() : 1];
     ZwTw<T> zt;
     ZwTw8<T> zt8;
+    ZsTw<T> zts;
     if constexpr (RT >= 1 && HALF) {
 #pragma unroll
         for (int r = 0; r < E; r++) twp[r] = twng[tid + r * Q];
@@ -653,10 +727,16 @@ void k_ns_cross_zd(C2<T>* __restrict__ W,
     if constexpr (RT == 2) rt_load<P, 0, T>(tws, tid, twg);
     if constexpr (RT == 3) zt.load(twg, tid);
     if constexpr (RT == 4) zt8.load(twg, tid);
+    if constexpr (RT == 5) zts.load(twg, tid);
     auto idx = [&](int pos) { return IL ? pos * LPB + c : pos * CS + c; };
     auto fft = [&](auto sign, C2<T>* v) {
         constexpr int SG = decltype(sign)::value;
-        if constexpr (RT == 4) {
+        if constexpr (RT == 5) {
+            if constexpr (SG > 0)
+                zs_inv<T>(v, zts, sx, wave, lane);
+            else
+                zs_fwd<T>(v, zts, sx, wave, lane);
+        } else if constexpr (RT == 4) {
             if constexpr (SG > 0)
                 zw8_inv<T>(v, zt8, sx, wave, lane);
             else
@@ -3388,6 +3468,8 @@ int launch_fourier_xw(sh_ns_s* s) {
 // zw8_fwd): sixteen columns on four waves, nontemporal loads, plain and
 // nontemporal stores: X(nz, plan, columns per block)
 #define SH_CROSSH_F32_ZW8_VARIANTS(X) X(256, Q128, 16)
+// the same for 256-point packed lines (zs_inv, zs_fwd): sixteen columns on eight waves
+#define SH_CROSSH_F32_ZS_VARIANTS(X) X(512, Q256, 16)
 #define SH_CROSSH_F32_VARIANTS(X)                                                                 \
     X(16, P8, 16) X(24, Q12, 16) X(32, P16, 16) X(48, Q24, 16) X(64, Q32, 16) X(64, Q32, 8)       \
     X(64, Q32, 32) X(96, Q48, 16)                                                                 \
@@ -3555,6 +3637,11 @@ const NsVariants& crossh_variants(int n) {
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 4, 2>);                                \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms of eight values nontemporal loads and stores"); \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 4, 2>);
+#define SH_VZS(n_, P_, C_)                                                                                  \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms of eight values nontemporal loads"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 5, 2>);                                \
+    m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms of eight values nontemporal loads and stores"); \
+    m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, true, true, true, 5, 2>);
 #define SH_VZW(n_, P_, C_)                                                                                  \
     m[n_].names.push_back(#P_ " half columns=" #C_ " LDS-streamed one-exchange transforms nontemporal loads");     \
     m[n_].fns.push_back(&launch_cross_zd<T, P_, C_, false, true, true, 3, 2>);                                 \
@@ -3584,7 +3671,9 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F32_LNT_VARIANTS(SH_VDL)
             SH_CROSSH_F32_RTL_VARIANTS(SH_VRL)
             SH_CROSSH_F32_ZW8_VARIANTS(SH_VZW8)
+            SH_CROSSH_F32_ZS_VARIANTS(SH_VZS)
         }
+#undef SH_VZS
 #undef SH_VZW8
 #undef SH_VZW
 #undef SH_VRL
