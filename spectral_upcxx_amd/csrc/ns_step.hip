@@ -3458,6 +3458,9 @@ int launch_fourier_xw(sh_ns_s* s) {
 // FP32 with the stage twiddles and packing factors in registers, nontemporal
 // loads (at 256^3: 127-129 us against 139-141; 8 columns per block ran 211-256)
 #define SH_CROSSH_F32_RTL_VARIANTS(X) X(256, W128, 16)
+// the same for the 64^3 boxes of the ensembles (packed lines of 32; no plan of
+// 48 has one butterfly of all a thread's values in its first and last stage)
+#define SH_CROSSH_F32_RTL_SMALL_VARIANTS(X) X(64, Q32, 16) X(64, Q32, 32)
 // FP64 with the transforms of one LDS exchange (zw_inv, zw_fwd), nontemporal
 // loads, plain and nontemporal stores: X(nz, plan, columns per block) (at 256^3:
 // 221-222 us against 240; FP32 needs 16 columns per block for whole 128-byte
@@ -3670,6 +3673,7 @@ const NsVariants& crossh_variants(int n) {
             SH_CROSSH_F32_LDS_VARIANTS(SH_VD)
             SH_CROSSH_F32_LNT_VARIANTS(SH_VDL)
             SH_CROSSH_F32_RTL_VARIANTS(SH_VRL)
+            SH_CROSSH_F32_RTL_SMALL_VARIANTS(SH_VRL)
             SH_CROSSH_F32_ZW8_VARIANTS(SH_VZW8)
             SH_CROSSH_F32_ZS_VARIANTS(SH_VZS)
         }
